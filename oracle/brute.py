@@ -1,0 +1,84 @@
+"""Pure-Python naive Jacobi evaluation for tiny inputs.  Test infrastructure only.
+
+Each program is its rule set written as Python set comprehensions.  The
+recursive step iterates F_{i+1} = F_i u T(F_i) from the base-stratum result
+F_0 and records |T(F_i) - F_i| per iteration, stopping after the first empty
+difference -- the delta sequence the reference's semi-naive loop produces
+(iteration 0 reads FULL for DELTA, every other atom reads FULL; see the
+oracle.cpp header for the reference citations).  It shares no code with the
+C++ oracle, so agreement of the two pins the restatement.
+"""
+
+from __future__ import annotations
+
+
+def _fix(full, step, max_iters=None):
+  deltas = []
+  it = 0
+  while max_iters is None or it < max_iters:
+    new = step(full) - full
+    deltas.append(len(new))
+    full = full | new
+    it += 1
+    if not new:
+      break
+  return full, deltas
+
+
+def tc(arc):
+  edge = {(int(a), int(b)) for a, b in arc}
+  succ = {}
+  for a, b in edge:
+    succ.setdefault(a, set()).add(b)
+  path0 = set(edge)
+  path, deltas = _fix(path0, lambda p: {(x, z) for (x, y) in p for z in succ.get(y, ())})
+  return {"Edge": edge, "Path": path}, {"Path": deltas}
+
+
+def sg(arc):
+  arc = {(int(a), int(b)) for a, b in arc}
+  kids = {}
+  for p, x in arc:
+    kids.setdefault(p, set()).add(x)
+  base = {(x, y) for p, ks in kids.items() for x in ks for y in ks if x != y}
+  sgr, deltas = _fix(base, lambda s: {(x, y) for (a, b) in s
+                                     for x in kids.get(a, ()) for y in kids.get(b, ())})
+  return {"Sg": sgr}, {"Sg": deltas}
+
+
+def andersen(addr, assign, load, store):
+  addr = {(int(a), int(b)) for a, b in addr}
+  assign = {(int(a), int(b)) for a, b in assign}
+  load = {(int(a), int(b)) for a, b in load}
+  store = {(int(a), int(b)) for a, b in store}
+
+  def step(pt):
+    by_v = {}
+    for v, o in pt:
+      by_v.setdefault(v, set()).add(o)
+    out = {(v, o) for (v, w) in assign for o in by_v.get(w, ())}
+    out |= {(x, o) for (x, y) in load for a in by_v.get(y, ()) for o in by_v.get(a, ())}
+    out |= {(a, o) for (x, y) in store for a in by_v.get(x, ()) for o in by_v.get(y, ())}
+    return out
+
+  pt, deltas = _fix(set(addr), step)
+  return {"PointsTo": pt}, {"PointsTo": deltas}
+
+
+def triangle(rows):
+  kn = {(int(a), int(b)) for a, b in rows}
+  succ = {}
+  for a, b in kn:
+    succ.setdefault(a, set()).add(b)
+  tri = {(a, b, c) for (a, b) in kn for c in succ.get(b, ()) if c in succ.get(a, ())}
+  return {"Knows": kn, "Triangle": tri}, {"Triangle": [len(tri)]}
+
+
+def reach(edge, src):
+  edge = {(int(a), int(b)) for a, b in edge}
+  succ = {}
+  for a, b in edge:
+    succ.setdefault(a, set()).add(b)
+  r0 = {int(s) for s in src}
+  r, deltas = _fix(r0, lambda r: {z for y in r for z in succ.get(y, ())})
+  return {"Reach": r}, {"Reach": deltas}

@@ -1,0 +1,122 @@
+"""ctypes front of the C++ oracle (``liborc.so``).  Test infrastructure only."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+
+class _Input(ctypes.Structure):
+  _fields_ = [
+    ("name", ctypes.c_char_p),
+    ("rows", ctypes.POINTER(ctypes.c_uint32)),
+    ("n", ctypes.c_uint64),
+    ("arity", ctypes.c_uint32),
+  ]
+
+
+def build() -> str:
+  """Compile liborc.so from oracle.cpp (make)."""
+  subprocess.run(["make", "-s", "-C", _HERE], check=True)
+  return _LIB_PATH
+
+
+def _load():
+  global _lib
+  if _lib is None:
+    if not os.path.exists(_LIB_PATH):
+      build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    lib.orc_run.restype = ctypes.c_void_p
+    lib.orc_run.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Input), ctypes.c_int,
+                            ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
+    lib.orc_error.restype = ctypes.c_char_p
+    lib.orc_error.argtypes = [ctypes.c_void_p]
+    lib.orc_words.restype = ctypes.c_uint64
+    lib.orc_words.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    lib.orc_export.restype = ctypes.c_int
+    lib.orc_export.argtypes = [ctypes.c_void_p, ctypes.c_char_p,
+                               ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64]
+    lib.orc_num_deltas.restype = ctypes.c_uint64
+    lib.orc_num_deltas.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    lib.orc_deltas.restype = ctypes.c_int
+    lib.orc_deltas.argtypes = [ctypes.c_void_p, ctypes.c_char_p,
+                               ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
+    for f in ("orc_iterations", "orc_count"):
+      getattr(lib, f).restype = ctypes.c_uint64
+      getattr(lib, f).argtypes = [ctypes.c_void_p]
+    lib.orc_seconds.restype = ctypes.c_double
+    lib.orc_seconds.argtypes = [ctypes.c_void_p]
+    lib.orc_free.restype = None
+    lib.orc_free.argtypes = [ctypes.c_void_p]
+    _lib = lib
+  return _lib
+
+
+ARITY = {"Triangle": 3, "Reach": 1}
+
+
+class OracleResult:
+  """Sorted relation contents, per-iteration deltas and timing of one run."""
+
+  def __init__(self, relations, deltas, iterations, count, seconds):
+    self.relations = relations      # name -> (n, arity) uint32, lexicographically sorted
+    self.deltas = deltas            # name -> list[int], |DELTA| per iteration
+    self.iterations = iterations
+    self.count = count
+    self.seconds = seconds
+
+  def size(self, rel: str) -> int:
+    return int(self.relations[rel].shape[0])
+
+
+def run(program: str, inputs: dict[str, np.ndarray], max_iters: int = 0, threads: int = 0,
+        key_range: tuple[int, int] | None = None,
+        relations: tuple[str, ...] = ("Edge", "Path", "Sg", "PointsTo", "Knows", "Triangle",
+                                      "Reach")) -> OracleResult:
+  """Run ``program`` ("tc", "sg", "andersen", "triangle", "triangle_count",
+  "reach") on ``inputs`` (relation name -> (n, arity) uint32 rows)."""
+  lib = _load()
+  keep = []
+  arr = (_Input * max(len(inputs), 1))()
+  for i, (name, rows) in enumerate(inputs.items()):
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    if rows.ndim == 1:
+      rows = rows.reshape(-1, 1)
+    keep.append(rows)
+    arr[i].name = name.encode()
+    arr[i].rows = rows.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    arr[i].n = rows.shape[0]
+    arr[i].arity = rows.shape[1]
+  lo, hi = key_range if key_range else (0, 0)
+  h = lib.orc_run(program.encode(), arr, len(inputs), max_iters, threads, lo, hi)
+  if not h:
+    raise ValueError(f"unknown oracle program {program!r}")
+  try:
+    err = lib.orc_error(h)
+    if err:
+      raise ValueError(err.decode())
+    rels, deltas = {}, {}
+    for name in relations:
+      nw = lib.orc_words(h, name.encode())
+      arity = ARITY.get(name, 2)
+      buf = np.empty(nw, dtype=np.uint32)
+      if lib.orc_export(h, name.encode(), buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                        nw) == 0:
+        rels[name] = buf.reshape(-1, arity)
+      nd = lib.orc_num_deltas(h, name.encode())
+      if nd:
+        d = np.empty(nd, dtype=np.uint64)
+        lib.orc_deltas(h, name.encode(), d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), nd)
+        deltas[name] = [int(x) for x in d]
+    return OracleResult(rels, deltas, int(lib.orc_iterations(h)), int(lib.orc_count(h)),
+                        float(lib.orc_seconds(h)))
+  finally:
+    lib.orc_free(h)

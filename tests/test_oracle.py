@@ -1,0 +1,126 @@
+"""Pin the C++ oracle (oracle/oracle.cpp) before trusting it.
+
+Checks, none of which retype the oracle's own loops:
+* brute force: pure-Python naive Jacobi iteration (oracle/brute.py) on tiny
+  seeded inputs, for every program -- final sets AND per-iteration deltas;
+* closed form (TC): |DELTA_i| equals the number of ordered pairs whose
+  shortest-path distance is exactly i+2 (all-pairs BFS with scipy);
+* closed form (triangles): sum(A .* (A @ A)) over the oriented adjacency;
+* invariants: FULL = disjoint union of deltas, deltas end with 0.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+
+import oracle
+from oracle import brute
+from srdatalog_hip import datasets as ds
+
+
+def as_set(rows):
+  return set(map(tuple, np.asarray(rows).tolist()))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+@pytest.mark.parametrize("n,m", [(12, 20), (40, 90), (80, 200)])
+def test_tc_matches_brute(seed, n, m):
+  e = ds.tc_edges(seed, n, m)
+  r = oracle.run("tc", {"ArcInput": e})
+  b, bd = brute.tc(e)
+  assert as_set(r.relations["Path"]) == b["Path"]
+  assert r.deltas["Path"] == bd["Path"]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tc_deltas_are_bfs_distance_histogram(seed):
+  n, m = 300, 900
+  e = ds.tc_edges(seed, n, m)
+  a = sp.csr_matrix((np.ones(m), (e[:, 0], e[:, 1])), shape=(n, n))
+  dist = csg.shortest_path(a, method="D", unweighted=True)
+  # Path after the base step = distance-1 pairs (and cycles give dist(x,x)
+  # via walks; shortest_path puts 0 on the diagonal, so handle x->x walks
+  # separately with the shortest cycle through x).
+  d = dist.copy()
+  np.fill_diagonal(d, np.inf)
+  cyc = np.full(n, np.inf)
+  for x in range(n):
+    succ = e[e[:, 0] == x, 1]
+    if succ.size:
+      cyc[x] = 1 + dist[succ, x].min()
+  d[np.arange(n), np.arange(n)] = cyc
+  finite = d[np.isfinite(d)].astype(np.int64)
+  hist = np.bincount(finite)
+  r = oracle.run("tc", {"ArcInput": e})
+  want = [int(hist[k]) if k < hist.size else 0 for k in range(2, 2 + len(r.deltas["Path"]))]
+  assert r.deltas["Path"] == want
+  assert r.deltas["Path"][-1] == 0
+  assert sum(r.deltas["Path"]) + m == r.size("Path") == finite.size
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 5])
+@pytest.mark.parametrize("n", [30, 120])
+def test_sg_matches_brute(seed, n):
+  e = ds.sg_forest_edges(np.random.default_rng(seed), n)
+  r = oracle.run("sg", {"Arc": e})
+  b, bd = brute.sg(e)
+  assert as_set(r.relations["Sg"]) == b["Sg"]
+  assert r.deltas["Sg"] == bd["Sg"]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_andersen_matches_brute(seed):
+  a = ds.andersen_inputs(seed, n_vars=300, n_objs=80, sizes=(120, 400, 120, 120))
+  r = oracle.run("andersen", a)
+  b, bd = brute.andersen(a["AddressOf"], a["Assign"], a["Load"], a["Store"])
+  assert as_set(r.relations["PointsTo"]) == b["PointsTo"]
+  assert r.deltas["PointsTo"] == bd["PointsTo"]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("scale,m", [(6, 300), (9, 3000)])
+def test_triangle_matches_brute_and_closed_form(seed, scale, m):
+  e = ds.triangle_edges(seed, scale, m)
+  r = oracle.run("triangle", {"KnowsInput": e})
+  b, _ = brute.triangle(e)
+  assert as_set(r.relations["Triangle"]) == b["Triangle"]
+  n = 1 << scale
+  a = sp.csr_matrix((np.ones(e.shape[0]), (e[:, 0], e[:, 1])), shape=(n, n))
+  closed = int(a.multiply(a @ a).sum())
+  assert r.count == closed == len(b["Triangle"])
+  rc = oracle.run("triangle_count", {"KnowsInput": e})
+  assert rc.count == closed and rc.relations["Triangle"].shape[0] == 0
+  # Orientation low->high makes every listed triangle strictly increasing.
+  t = r.relations["Triangle"]
+  assert np.all(t[:, 0] < t[:, 1]) and np.all(t[:, 1] < t[:, 2])
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_reach_matches_brute_and_bfs(seed):
+  n = 200
+  e = ds.tc_edges(seed, n, 300)
+  src = np.array([seed * 7 % n], dtype=np.uint32)
+  r = oracle.run("reach", {"Edge": e, "Src": src})
+  b, bd = brute.reach(e, src)
+  assert set(r.relations["Reach"].ravel().tolist()) == b["Reach"]
+  assert r.deltas["Reach"] == bd["Reach"]
+  a = sp.csr_matrix((np.ones(e.shape[0]), (e[:, 0], e[:, 1])), shape=(n, n))
+  lv = csg.breadth_first_order(a, int(src[0]), return_predecessors=False)
+  assert len(lv) == r.size("Reach")
+
+
+def test_max_iters_truncates():
+  e = ds.tc_edges(0, 40, 90)
+  full = oracle.run("tc", {"ArcInput": e})
+  part = oracle.run("tc", {"ArcInput": e}, max_iters=2)
+  assert part.deltas["Path"] == full.deltas["Path"][:2]
+  assert part.iterations == 2
+
+
+def test_sorted_unique_exports():
+  e = ds.tc_edges(1, 50, 120)
+  r = oracle.run("tc", {"ArcInput": e})
+  p = r.relations["Path"].astype(np.uint64)
+  k = (p[:, 0] << np.uint64(32)) | p[:, 1]
+  assert np.all(k[1:] > k[:-1])
