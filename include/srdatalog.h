@@ -1,0 +1,84 @@
+/* Drop-in C ABI of a compiled srdatalog program (one shared library per
+ * program, produced by srdatalog_hip.build_project + compile_jit_project).
+ *
+ * The first six symbols are exactly the ones the reference's generated
+ * shim exports and its Python driver binds with ctypes
+ * (src/srdatalog/codegen/jit/main_file.py:656-781, bound in
+ * examples/run_benchmark.py:195-203): same names, C signatures and return
+ * codes.  The remaining symbols are additions for what the reference's shim
+ * cannot report (device-side results, per-iteration deltas, in-memory
+ * input); they do not change the six.
+ *
+ * Threading: one program instance per library; calls are not re-entrant.
+ * Device: HIP device 0 (gfx950).
+ */
+#ifndef SRDATALOG_H
+#define SRDATALOG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces main_file.py:685-691.  Initialise device 0 and the memory pool
+ * and parse the embedded plan.  0 ok, 1 std::exception (message on stderr),
+ * 2 other error. */
+int srdatalog_init(void);
+
+/* Replaces main_file.py:693-729.  Load one input relation from a CSV/TSV
+ * file (delimiter auto-detected: tab, comma or spaces; one tuple per line).
+ * 0 ok, 1 null argument, 2 unknown or non-input relation, 3 parse/IO error. */
+int srdatalog_load_csv(const char* rel_name, const char* path);
+
+/* Replaces main_file.py:731-741.  Load every relation declared with an
+ * input_file from data_dir/input_file.  0 ok, 1 null, 2 error. */
+int srdatalog_load_all(const char* data_dir);
+
+/* Replaces main_file.py:743-753.  Run the program to fixpoint (max_iters
+ * caps each recursive step's iterations; 0 = unlimited).  Prints the
+ * reference's " >>>>>>>>>>>>>>>>> Rel : N" size lines for print_size
+ * relations (silenced by SRDATALOG_QUIET=1).  0 ok, 1 nothing loaded,
+ * 2 error. */
+int srdatalog_run(unsigned long long max_iters);
+
+/* Replaces main_file.py:755-770.  FULL size of `rel_name`.  Deliberate
+ * deviation: after srdatalog_run this is the DEVICE result (the reference
+ * reads its host DB, which reports 0 for derived relations).  Before a run
+ * it is the loaded input size.  For a count-only (.with_count()) rule's
+ * head it is the count.  0 for unknown relations. */
+unsigned long long srdatalog_size(const char* rel_name);
+
+/* Replaces main_file.py:772-775.  Free all host and device state; 0. */
+int srdatalog_shutdown(void);
+
+/* ---- additions ---------------------------------------------------------- */
+
+/* Copy the FULL tuple set of `rel_name`, sorted lexicographically over the
+ * relation's columns, row-major into out (cap_rows rows).  *n_rows = rows in
+ * the relation (out may be NULL to query).  0 ok, 1 null, 2 unknown
+ * relation, 3 error (e.g. cap too small). */
+int srdatalog_export(const char* rel_name, uint32_t* out, uint64_t cap_rows, uint64_t* n_rows);
+
+/* Per-iteration |DELTA| of the last step that computed rel_name; returns the
+ * number of entries (written up to cap), -1 for an unknown relation. */
+int srdatalog_delta_sizes(const char* rel_name, unsigned long long* out, unsigned long long cap);
+
+/* Number of iterations recorded for rel_name (length of its delta vector). */
+unsigned long long srdatalog_num_iterations(const char* rel_name);
+
+/* Load an input relation from host memory (row-major n x arity u32). */
+int srdatalog_load_rows(const char* rel_name, const uint32_t* rows, uint64_t n, uint32_t arity);
+
+/* Copy loaded inputs to device memory now (srdatalog_run does it lazily);
+ * later runs reuse the resident copies. */
+int srdatalog_prepare(void);
+
+/* Wall seconds of the last srdatalog_run's fixpoint (inputs resident). */
+double srdatalog_run_seconds(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRDATALOG_H */

@@ -1,0 +1,62 @@
+/* C ABI of the MI355X set primitives that the semi-naive loop is built from.
+ *
+ * These entry points expose, one by one, the device operations behind the
+ * reference's MIR maintenance ops so they can be parity-tested and timed in
+ * isolation.  Buffers are HOST row-major arrays (n rows x k u32 columns);
+ * each call uploads, runs on the current HIP device and downloads.  All
+ * functions return 0 on success and nonzero on error (message via
+ * srd_last_error()).
+ *
+ * Reference counterparts (runtime/generalized_datalog/ in harp-lab/srdatalog-python):
+ *   srd_prim_sort_dedup      gpu/device_sorted_array_index.ipp:348 build_take_ownership
+ *                            (sort gpu/sort.h:297, dedup gpu/deduplication.h:178,
+ *                             root_unique_values gpu/deduplication.h:457)
+ *   srd_prim_set_difference  gpu/device_sorted_array_index.ipp:966 set_difference_update
+ *   srd_prim_merge           gpu/device_sorted_array_index.ipp:709 merge
+ *   srd_prim_exclusive_scan  thrust::exclusive_scan as used by
+ *                            src/srdatalog/codegen/jit/complete_runner.py:1072-1112
+ */
+#ifndef SRDATALOG_PRIMS_H
+#define SRDATALOG_PRIMS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Initialise the device pool on `device`. */
+int srd_prim_init(int device);
+
+/* Lexicographically sort and deduplicate n rows of k columns.
+ * out_rows: capacity n*k words; *out_n = distinct rows.
+ * root_vals / root_offs (capacity n and n+1, may be NULL): distinct column-0
+ * values and their first-row offsets, root_offs[r] = *out_n; *out_r = r. */
+int srd_prim_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint32_t* out_rows,
+                        uint64_t* out_n, uint32_t* root_vals, uint32_t* root_offs,
+                        uint64_t* out_r);
+
+/* out = a - b for sorted duplicate-free inputs of arity k (capacity na*k). */
+int srd_prim_set_difference(const uint32_t* a, uint64_t na, const uint32_t* b, uint64_t nb,
+                            uint32_t k, uint32_t* out, uint64_t* out_n);
+
+/* out = a u b for sorted, duplicate-free, DISJOINT inputs (capacity (na+nb)*k). */
+int srd_prim_merge(const uint32_t* a, uint64_t na, const uint32_t* b, uint64_t nb, uint32_t k,
+                   uint32_t* out, uint64_t* out_n);
+
+/* out[0..n] = exclusive prefix sums of in[0..n), out[n] = total. */
+int srd_prim_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out);
+
+/* Device-resident timing of sort+dedup: the rows are uploaded once, then
+ * sorted `reps` times; *ms_per_rep = mean wall time per sort (events). */
+int srd_prim_time_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint32_t reps,
+                             double* ms_per_rep);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* srd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRDATALOG_PRIMS_H */

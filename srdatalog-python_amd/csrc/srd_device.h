@@ -1,0 +1,168 @@
+// Device-side building blocks shared by the runtime primitives and the
+// generated per-rule join kernels: wave64 intrinsics, sorted-column search,
+// and the POD index view handed to kernels.
+//
+// Index layout in HBM (one relation version, one column order):
+//   cols       k u32 columns, column c at cols + c*stride, rows sorted
+//              lexicographically and duplicate-free;
+//   root_vals  the r distinct values of column 0, ascending;
+//   root_offs  r+1 row offsets: rows with column 0 == root_vals[g] are
+//              [root_offs[g], root_offs[g+1]).
+// This is the sorted-array column index of the reference
+// (runtime/generalized_datalog/gpu/device_sorted_array_index.h:619-643)
+// plus a CSR offset array so a depth-0 key lookup is one search over the
+// distinct keys instead of two over the whole column.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace srd {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+struct IndexView {
+  const uint32_t* cols;
+  const uint32_t* root_vals;
+  const uint32_t* root_offs;
+  uint64_t stride;
+  uint32_t n;      // rows
+  uint32_t r;      // distinct column-0 values
+  uint32_t arity;
+  uint32_t pad_;
+};
+
+struct Rng {
+  uint32_t b, e;
+};
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of m strictly below this lane.
+__device__ __forceinline__ uint32_t popc_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    T t = __shfl_up(v, o, kWave);
+    if (l >= uint32_t(o)) v += t;
+  }
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    T t = __shfl_xor(v, o, kWave);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan for kBlock threads; returns this thread's
+// exclusive prefix and writes the block total to *total.  `wsum` must hold
+// kWavesPerBlock entries of shared memory.
+template <class T>
+__device__ __forceinline__ T block_excl_scan(T v, T* wsum, T* total) {
+  const uint32_t l = lane_id();
+  const uint32_t w = threadIdx.x / kWave;
+  T inc = wave_incl_scan(v);
+  if (l == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  T base = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kWavesPerBlock; ++i) {
+    T s = wsum[i];
+    if (uint32_t(i) < w) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+
+// First position p in [lo,hi) with a[p] >= v (hi if none).
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ a, uint32_t lo,
+                                                uint32_t hi, uint32_t v) {
+  uint32_t n = hi - lo;
+  while (n > 0) {
+    uint32_t half = n >> 1;
+    uint32_t mid = lo + half;
+    if (a[mid] < v) {
+      lo = mid + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  return lo;
+}
+
+// First position p in [lo,hi) with a[p] > v (hi if none).
+__device__ __forceinline__ uint32_t upper_bound(const uint32_t* __restrict__ a, uint32_t lo,
+                                                uint32_t hi, uint32_t v) {
+  uint32_t n = hi - lo;
+  while (n > 0) {
+    uint32_t half = n >> 1;
+    uint32_t mid = lo + half;
+    if (a[mid] <= v) {
+      lo = mid + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  return lo;
+}
+
+__device__ __forceinline__ const uint32_t* col(const IndexView& v, int c) {
+  return v.cols + uint64_t(c) * v.stride;
+}
+
+// Rows whose column 0 equals key (empty range if absent), via root arrays.
+__device__ __forceinline__ Rng root_find(const IndexView& v, uint32_t key) {
+  uint32_t g = lower_bound(v.root_vals, 0, v.r, key);
+  if (g < v.r && v.root_vals[g] == key) return Rng{v.root_offs[g], v.root_offs[g + 1]};
+  return Rng{0, 0};
+}
+
+// Rows within r whose column `depth` equals key; `last` says column `depth`
+// is the index's last column, where values within a fixed prefix are unique.
+__device__ __forceinline__ Rng child_find(const IndexView& v, int depth, Rng r, uint32_t key,
+                                          bool last) {
+  const uint32_t* c = col(v, depth);
+  uint32_t lo = lower_bound(c, r.b, r.e, key);
+  if (lo >= r.e || c[lo] != key) return Rng{0, 0};
+  if (last) return Rng{lo, lo + 1};
+  return Rng{lo, upper_bound(c, lo + 1, r.e, key)};
+}
+
+// Lexicographic comparison of row i of a against row j of b over k columns.
+__device__ __forceinline__ int cmp_rows(const uint32_t* a, uint64_t sa, uint64_t i,
+                                        const uint32_t* b, uint64_t sb, uint64_t j, int k) {
+  for (int c = 0; c < k; ++c) {
+    uint32_t x = a[uint64_t(c) * sa + i], y = b[uint64_t(c) * sb + j];
+    if (x != y) return x < y ? -1 : 1;
+  }
+  return 0;
+}
+
+}  // namespace srd

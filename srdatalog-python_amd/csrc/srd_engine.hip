@@ -1,0 +1,1031 @@
+// MIR executor of the MI355X runtime.
+//
+// A program arrives as text: relation declarations followed by the MIR
+// program in the reference's S-expression form (the format of the
+// reference's tests/fixtures/**/*.mir.sexpr goldens), plus a table of
+// generated per-rule pipeline kernels (srd_pipeline.h).  The engine keeps
+// every relation version (FULL / DELTA / NEW) as sorted u32 column indexes
+// in HBM and executes the MIR ops with the device primitives of
+// srd_prims.hip.
+//
+// Semantics follow the reference host orchestration:
+//   recursive step loop   src/srdatalog/codegen/jit/orchestrator_jit.py:600-739
+//                         (stop at iter>0 when the summed canonical DELTA
+//                         size is 0)
+//   non-recursive step    orchestrator_jit.py:747-855 (one pass)
+//   iteration-0 DELTA     reads FULL (complete_runner.py:834-838)
+//   rebuild-index         gpu_mir_helpers.h:203 (sort+dedup NEW)
+//   compute-delta-index   gpu_mir_helpers.h:87 (DELTA = NEW - FULL)
+//   merge-index           gpu_mir_helpers.h:135 (FULL u= DELTA)
+//   rebuild-index-from-index  gpu_mir_helpers.h:250
+//   clear-relation        gpu_mir_helpers.h:307
+//   count-only rules      orchestrator_jit.py:156-172 (count printed, no
+//                         materialisation)
+//   CSV input             runtime/io.h:178-395 (delimiter auto-detect)
+#include <fcntl.h>
+#include <hip/hip_runtime.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "srd_engine.h"
+#include "srd_pipeline.h"
+#include "srd_runtime.h"
+
+namespace srd {
+
+// ============================================================ S-expr ====
+struct SNode {
+  bool atom = false;
+  bool quoted = false;
+  std::string text;
+  std::vector<SNode> kids;
+
+  bool is_sym(const char* s) const { return atom && !quoted && text == s; }
+  const SNode* kw(const char* key) const {  // value after :key
+    for (size_t i = 0; i + 1 < kids.size(); ++i)
+      if (kids[i].atom && !kids[i].quoted && kids[i].text == key) return &kids[i + 1];
+    return nullptr;
+  }
+  std::string head() const { return (!atom && !kids.empty() && kids[0].atom) ? kids[0].text : ""; }
+};
+
+class SParser {
+ public:
+  explicit SParser(const std::string& t) : s_(t) {}
+  std::vector<SNode> parse_all() {
+    std::vector<SNode> out;
+    skip();
+    while (i_ < s_.size()) {
+      out.push_back(parse());
+      skip();
+    }
+    return out;
+  }
+
+ private:
+  const std::string& s_;
+  size_t i_ = 0;
+  void skip() {
+    while (i_ < s_.size()) {
+      if (isspace((unsigned char)s_[i_])) {
+        ++i_;
+      } else if (s_[i_] == ';') {
+        while (i_ < s_.size() && s_[i_] != '\n') ++i_;
+      } else {
+        break;
+      }
+    }
+  }
+  SNode parse() {
+    skip();
+    if (i_ >= s_.size()) fail("mir: unexpected end of program text");
+    SNode n;
+    if (s_[i_] == '(') {
+      ++i_;
+      skip();
+      while (i_ < s_.size() && s_[i_] != ')') {
+        n.kids.push_back(parse());
+        skip();
+      }
+      if (i_ >= s_.size()) fail("mir: unbalanced parenthesis");
+      ++i_;
+      return n;
+    }
+    if (s_[i_] == ')') fail("mir: unexpected ')'");
+    n.atom = true;
+    if (s_[i_] == '"') {
+      n.quoted = true;
+      ++i_;
+      while (i_ < s_.size() && s_[i_] != '"') {
+        if (s_[i_] == '\\' && i_ + 1 < s_.size()) ++i_;
+        n.text.push_back(s_[i_++]);
+      }
+      ++i_;
+      return n;
+    }
+    while (i_ < s_.size() && !isspace((unsigned char)s_[i_]) && s_[i_] != '(' && s_[i_] != ')')
+      n.text.push_back(s_[i_++]);
+    return n;
+  }
+};
+
+static std::vector<uint32_t> int_list(const SNode* n) {
+  std::vector<uint32_t> v;
+  if (!n) return v;
+  if (n->atom) {
+    v.push_back(uint32_t(std::stoul(n->text)));
+    return v;
+  }
+  for (auto& k : n->kids) v.push_back(uint32_t(std::stoul(k.text)));
+  return v;
+}
+
+static uint32_t parse_ver(const SNode* n) {
+  if (!n) fail("mir: missing :ver");
+  if (n->text == "FULL") return SRD_FULL;
+  if (n->text == "DELTA") return SRD_DELTA;
+  if (n->text == "NEW") return SRD_NEW;
+  fail("mir: unknown version " + n->text);
+}
+
+// ============================================================= model ====
+enum class OpKind {
+  Pipeline,
+  Parallel,
+  RebuildIndex,
+  ClearRelation,
+  CheckSize,
+  ComputeDelta,
+  MergeIndex,
+  RebuildFromIndex,
+  Reconstruct,
+  Hook,
+};
+
+struct Op {
+  OpKind kind;
+  int rel = -1;
+  std::vector<uint32_t> index, index2;
+  uint32_t ver = SRD_FULL;
+  std::string rule;
+  const SrdPipelineDesc* pipe = nullptr;
+  std::vector<Op> kids;
+};
+
+struct Step {
+  bool recursive = false;
+  std::vector<Op> ops;
+  std::vector<int> canonical_rels;  // relations with compute-delta in the plan
+};
+
+using Order = std::vector<uint32_t>;
+
+struct Versioned {
+  Index idx;
+  uint64_t version = 0;
+};
+
+struct Relation {
+  std::string name;
+  uint32_t arity = 0;
+  std::string input_file;
+  bool print_size = false;
+  Order canonical;
+  bool canonical_set = false;
+  // FULL: indexes with the FULL version they reflect.
+  std::map<Order, Versioned> full;
+  uint64_t full_version = 0;   // deltas merged into the canonical FULL index
+  uint64_t delta_seq = 0;      // version FULL reaches once the current DELTA is merged
+  std::map<Order, Index> delta;
+  std::map<Order, Index> newv;
+  // NEW raw rows, columns in raw_order
+  DevBuf<uint32_t> raw;
+  uint64_t raw_n = 0, raw_cap = 0;
+  Order raw_order;
+  // input data (EDB)
+  std::vector<uint32_t> host_rows;  // row-major
+  uint64_t host_n = 0;
+  bool has_input = false;
+  DevBuf<uint32_t> dev_input;       // columns, stride dev_stride
+  uint64_t dev_stride = 0;
+  bool dev_input_valid = false;
+  std::vector<uint32_t> input_max;
+  // results
+  std::vector<uint64_t> deltas;     // per-iteration |DELTA| of the last step computing it
+  int deltas_step = -1;
+  uint64_t count = 0;               // count-only result
+  bool counted = false;
+};
+
+}  // namespace srd
+
+struct srd_engine {
+  std::vector<srd::Relation> rels;
+  std::map<std::string, int> rel_by_name;
+  std::vector<srd::Step> steps;
+  std::map<std::string, const SrdPipelineDesc*> pipes;
+  hipStream_t s = nullptr;
+  srd::Workspace ws;
+  std::string err;
+  bool quiet = false;
+  bool ran = false;
+  uint64_t iterations = 0;         // iterations of the last recursive step
+  double run_seconds = 0;
+  std::vector<std::pair<std::string, uint64_t>> counts;  // rule -> count (count-only)
+  uint32_t grain_override = 0;
+  // per-pipeline scratch, grown on demand
+  srd::DevBuf<uint32_t> key_rng, key_aux, counts_buf, work_ctr;
+  srd::DevBuf<uint64_t> key_work, work_start, offsets;
+  uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
+};
+
+namespace srd {
+
+// =========================================================== parsing ====
+static int rel_id(srd_engine& e, const std::string& name) {
+  auto it = e.rel_by_name.find(name);
+  if (it == e.rel_by_name.end()) fail("mir: unknown relation " + name);
+  return it->second;
+}
+
+static void parse_index_ref(srd_engine& e, const SNode* n, int& rel, Order& idx) {
+  if (!n || n->atom || n->kids.empty()) fail("mir: bad :index form");
+  rel = rel_id(e, n->kids[0].text);
+  idx.clear();
+  for (size_t i = 1; i < n->kids.size(); ++i) idx.push_back(uint32_t(std::stoul(n->kids[i].text)));
+}
+
+static Op parse_op(srd_engine& e, const SNode& n) {
+  Op op;
+  std::string h = n.head();
+  if (h == "execute-pipeline") {
+    op.kind = OpKind::Pipeline;
+    const SNode* r = n.kw(":rule");
+    if (!r) fail("mir: execute-pipeline without :rule");
+    op.rule = r->text;
+    auto it = e.pipes.find(op.rule);
+    if (it == e.pipes.end()) fail("mir: no compiled kernel for rule " + op.rule);
+    op.pipe = it->second;
+  } else if (h == "parallel-group") {
+    op.kind = OpKind::Parallel;
+    for (size_t i = 1; i < n.kids.size(); ++i)
+      if (!n.kids[i].atom) op.kids.push_back(parse_op(e, n.kids[i]));
+  } else if (h == "rebuild-index") {
+    op.kind = OpKind::RebuildIndex;
+    parse_index_ref(e, n.kw(":index"), op.rel, op.index);
+    op.ver = parse_ver(n.kw(":ver"));
+  } else if (h == "clear-relation") {
+    op.kind = OpKind::ClearRelation;
+    op.rel = rel_id(e, n.kw(":schema")->text);
+    op.ver = parse_ver(n.kw(":ver"));
+  } else if (h == "check-size") {
+    op.kind = OpKind::CheckSize;
+    op.rel = rel_id(e, n.kw(":schema")->text);
+  } else if (h == "compute-delta-index") {
+    op.kind = OpKind::ComputeDelta;
+    op.rel = rel_id(e, n.kw(":schema")->text);
+    op.index = int_list(n.kw(":canonical-index"));
+  } else if (h == "merge-index") {
+    op.kind = OpKind::MergeIndex;
+    parse_index_ref(e, n.kw(":index"), op.rel, op.index);
+  } else if (h == "rebuild-index-from-index") {
+    op.kind = OpKind::RebuildFromIndex;
+    int r2;
+    parse_index_ref(e, n.kw(":source"), op.rel, op.index);
+    parse_index_ref(e, n.kw(":target"), r2, op.index2);
+    if (r2 != op.rel) fail("mir: rebuild-index-from-index across relations");
+    op.ver = parse_ver(n.kw(":ver"));
+  } else if (h == "post-stratum-reconstruct-intern-cols") {
+    op.kind = OpKind::Reconstruct;
+    op.rel = rel_id(e, n.kw(":rel")->text);
+    op.index = int_list(n.kw(":canonical-index"));
+  } else if (h == "inject-cpp-hook") {
+    op.kind = OpKind::Hook;
+  } else {
+    fail("mir: unsupported op " + h);
+  }
+  return op;
+}
+
+static void collect_canonical(srd_engine& e, const std::vector<Op>& ops, Step& st) {
+  for (auto& op : ops) {
+    if (op.kind == OpKind::ComputeDelta) {
+      if (std::find(st.canonical_rels.begin(), st.canonical_rels.end(), op.rel) ==
+          st.canonical_rels.end())
+        st.canonical_rels.push_back(op.rel);
+      Relation& r = e.rels[op.rel];
+      if (!r.canonical_set) {
+        r.canonical = op.index;
+        r.canonical_set = true;
+      }
+    }
+    if (op.kind == OpKind::Parallel) collect_canonical(e, op.kids, st);
+  }
+}
+
+static void parse_program(srd_engine& e, const std::string& text) {
+  auto top = SParser(text).parse_all();
+  for (auto& n : top) {
+    if (n.head() == "relations") {
+      for (size_t i = 1; i < n.kids.size(); ++i) {
+        const SNode& d = n.kids[i];
+        Relation r;
+        r.name = d.kw(":name")->text;
+        r.arity = uint32_t(std::stoul(d.kw(":arity")->text));
+        if (auto* f = d.kw(":input-file")) r.input_file = f->text;
+        if (auto* p = d.kw(":print-size")) r.print_size = p->text == "true";
+        for (uint32_t c = 0; c < r.arity; ++c) r.canonical.push_back(c);
+        e.rel_by_name[r.name] = int(e.rels.size());
+        e.rels.push_back(std::move(r));
+      }
+    }
+  }
+  for (auto& n : top) {
+    if (n.head() != "program") continue;
+    for (size_t i = 1; i < n.kids.size(); ++i) {
+      const SNode& sn = n.kids[i];
+      if (sn.head() != "step") fail("mir: expected (step ...)");
+      Step st;
+      if (auto* r = sn.kw(":recursive")) st.recursive = r->text == "true";
+      for (size_t j = 1; j < sn.kids.size(); ++j) {
+        const SNode& body = sn.kids[j];
+        if (body.atom) continue;
+        if (body.head() == "fixpoint-plan") {
+          for (size_t k = 1; k < body.kids.size(); ++k)
+            if (!body.kids[k].atom) st.ops.push_back(parse_op(e, body.kids[k]));
+        } else {
+          st.ops.push_back(parse_op(e, body));
+        }
+      }
+      collect_canonical(e, st.ops, st);
+      e.steps.push_back(std::move(st));
+    }
+  }
+}
+
+// ========================================================= relations ====
+static Order perm_between(const Order& from, const Order& to) {
+  // out column c of `to` order = column perm[c] of `from` order
+  Order p(to.size());
+  for (size_t c = 0; c < to.size(); ++c) {
+    auto it = std::find(from.begin(), from.end(), to[c]);
+    if (it == from.end()) fail("index orders are not permutations of each other");
+    p[c] = uint32_t(it - from.begin());
+  }
+  return p;
+}
+
+static Index empty_index(uint32_t arity) {
+  Index x;
+  x.arity = arity;
+  x.col_max.assign(arity, 0);
+  return x;
+}
+
+static void upload_input(srd_engine& e, Relation& r) {
+  if (r.dev_input_valid || !r.has_input) return;
+  const uint64_t n = r.host_n, k = r.arity;
+  r.dev_stride = round_stride(n);
+  std::vector<uint32_t> t(r.dev_stride * k, 0);
+  r.input_max.assign(k, 0);
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint64_t c = 0; c < k; ++c) {
+      uint32_t v = r.host_rows[i * k + c];
+      t[c * r.dev_stride + i] = v;
+      r.input_max[c] = std::max(r.input_max[c], v);
+    }
+  r.dev_input.alloc(std::max<uint64_t>(t.size(), 1), e.s);
+  if (!t.empty())
+    SRD_CHECK(hipMemcpyAsync(r.dev_input.p, t.data(), t.size() * 4, hipMemcpyHostToDevice, e.s));
+  SRD_CHECK(hipStreamSynchronize(e.s));
+  r.dev_input_valid = true;
+}
+
+// FULL index in `order`, current with the relation's FULL version.
+static const Index& full_index(srd_engine& e, Relation& r, const Order& order) {
+  auto it = r.full.find(order);
+  if (it != r.full.end() && it->second.version == r.full_version) return it->second.idx;
+  Versioned v;
+  v.version = r.full_version;
+  auto cit = r.full.find(r.canonical);
+  if (order != r.canonical && cit != r.full.end() && cit->second.version == r.full_version) {
+    build_index_from(cit->second.idx, perm_between(r.canonical, order), v.idx, e.ws, e.s);
+  } else if (r.has_input && r.full_version == 0) {
+    upload_input(e, r);
+    std::vector<const uint32_t*> in(r.arity);
+    std::vector<uint32_t> mx(r.arity);
+    for (uint32_t c = 0; c < r.arity; ++c) {
+      in[c] = r.dev_input.p + uint64_t(order[c]) * r.dev_stride;
+      mx[c] = r.input_max[order[c]];
+    }
+    if (r.host_n)
+      build_index(in.data(), r.arity, r.host_n, v.idx, e.ws, e.s, &mx);
+    else
+      v.idx = empty_index(r.arity);
+  } else if (r.full_version == 0 && !r.has_input) {
+    v.idx = empty_index(r.arity);
+  } else {
+    fail("relation " + r.name + ": FULL index requested before its canonical index exists");
+  }
+  auto& slot = r.full[order];
+  slot = std::move(v);
+  return slot.idx;
+}
+
+static const Index& delta_index(srd_engine& e, Relation& r, const Order& order) {
+  auto it = r.delta.find(order);
+  if (it != r.delta.end()) return it->second;
+  auto cit = r.delta.find(r.canonical);
+  Index x;
+  if (cit != r.delta.end())
+    build_index_from(cit->second, perm_between(r.canonical, order), x, e.ws, e.s);
+  else
+    x = empty_index(r.arity);
+  auto& slot = r.delta[order];
+  slot = std::move(x);
+  return slot;
+}
+
+static const Index& new_index(srd_engine& e, Relation& r, const Order& order) {
+  auto it = r.newv.find(order);
+  if (it != r.newv.end()) return it->second;
+  Index x;
+  if (r.raw_n > 0) {
+    Order p = perm_between(r.raw_order, order);
+    std::vector<const uint32_t*> in(r.arity);
+    for (uint32_t c = 0; c < r.arity; ++c) in[c] = r.raw.p + uint64_t(p[c]) * r.raw_cap;
+    build_index(in.data(), r.arity, r.raw_n, x, e.ws, e.s);
+  } else if (!r.newv.empty()) {
+    auto& any = *r.newv.begin();
+    build_index_from(any.second, perm_between(any.first, order), x, e.ws, e.s);
+  } else {
+    x = empty_index(r.arity);
+  }
+  auto& slot = r.newv[order];
+  slot = std::move(x);
+  return slot;
+}
+
+// Make room for `extra` more NEW rows (columns in `order`); returns base row.
+static uint64_t reserve_new(srd_engine& e, Relation& r, const Order& order, uint64_t extra) {
+  if (r.raw_n == 0 && r.raw_order != order) r.raw_order = order;
+  if (r.raw_order != order) fail("relation " + r.name + ": pipelines insert with different dedup indexes");
+  uint64_t need = r.raw_n + extra;
+  if (need > r.raw_cap) {
+    uint64_t cap = std::max<uint64_t>(round_stride(need + need / 2), 1024);
+    DevBuf<uint32_t> nb(cap * r.arity, e.s);
+    if (r.raw_n)
+      for (uint32_t c = 0; c < r.arity; ++c)
+        SRD_CHECK(hipMemcpyAsync(nb.p + uint64_t(c) * cap, r.raw.p + uint64_t(c) * r.raw_cap,
+                                 r.raw_n * 4, hipMemcpyDeviceToDevice, e.s));
+    r.raw = std::move(nb);
+    r.raw_cap = cap;
+  }
+  uint64_t base = r.raw_n;
+  r.raw_n = need;
+  return base;
+}
+
+static void clear_new(Relation& r) {
+  r.newv.clear();
+  r.raw.release();
+  r.raw_n = r.raw_cap = 0;
+}
+
+// ============================================================ pipelines ====
+static uint64_t read_u64(srd_engine& e, const uint64_t* dev) {
+  uint64_t* h = reinterpret_cast<uint64_t*>(pinned_words(e.ws, 2));
+  SRD_CHECK(hipMemcpyAsync(h, dev, 8, hipMemcpyDeviceToHost, e.s));
+  SRD_CHECK(hipStreamSynchronize(e.s));
+  return h[0];
+}
+
+static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t iteration) {
+  SrdLaunchArgs a;
+  std::memset(&a, 0, sizeof(a));
+  // Resolve views (iteration 0 reads FULL wherever the rule reads DELTA).
+  for (uint32_t v = 0; v < d->n_views; ++v) {
+    const SrdViewSpec& vs = d->views[v];
+    Relation& r = e.rels[rel_id(e, vs.rel)];
+    Order order(vs.cols, vs.cols + vs.ncols);
+    const Index* x;
+    if (vs.ver == SRD_FULL || (vs.ver == SRD_DELTA && iteration == 0))
+      x = &full_index(e, r, order);
+    else if (vs.ver == SRD_DELTA)
+      x = &delta_index(e, r, order);
+    else
+      x = &new_index(e, r, order);
+    a.views.v[v] = x->view();
+  }
+  a.grain = e.grain_override ? e.grain_override : d->default_grain;
+  if (d->root_kind == SRD_ROOT_JOIN) {
+    uint32_t drv = d->root_view[0];
+    for (uint32_t i = 0; i < d->n_root_src; ++i) {
+      uint32_t v = d->root_view[i];
+      if (a.views.v[v].r < a.views.v[drv].r) drv = v;
+    }
+    a.drv_view = drv;
+    a.n_root = a.views.v[drv].r;
+  } else {
+    a.drv_view = d->root_view[0];
+    a.n_root = a.views.v[a.drv_view].n ? 1 : 0;
+  }
+  uint64_t total = 0;
+  if (a.n_root > 0) {
+    const uint64_t nk = a.n_root;
+    e.key_rng.reserve(nk * 2 * SRD_MAX_VIEWS, e.s);
+    e.key_aux.reserve(nk, e.s);
+    e.key_work.reserve(nk, e.s);
+    e.work_start.reserve(nk + 1, e.s);
+    e.work_ctr.reserve(2, e.s);
+    a.key_rng = e.key_rng.p;
+    a.key_aux = e.key_aux.p;
+    a.key_work = e.key_work.p;
+    a.work_start = e.work_start.p;
+    a.work_ctr = e.work_ctr.p;
+    {
+      std::string nm = std::string("join.root:") + d->rule;
+      timing_begin(nm.c_str(), e.s);
+      d->root(&a, e.s);
+      SRD_CHECK(hipGetLastError());
+      timing_end(e.s);
+    }
+    exclusive_scan_u64(a.key_work, a.work_start, nk, e.ws, e.s);
+    a.total_work = read_u64(e, a.work_start + nk);
+    a.n_slices = (a.total_work + a.grain - 1) / a.grain;
+    const uint64_t slices = a.n_slices;
+    if (slices > 0) {
+      e.counts_buf.reserve(slices, e.s);
+      e.offsets.reserve(slices + 1, e.s);
+      a.counts = e.counts_buf.p;
+      a.offsets = e.offsets.p;
+      a.grid_blocks = uint32_t(std::min<uint64_t>((slices + 3) / 4, 2048));
+      fill_u32(a.work_ctr, 0, 2, e.s);
+      {
+        std::string nm = std::string("join.count:") + d->rule;
+        timing_begin(nm.c_str(), e.s);
+        d->count(&a, e.s);
+        SRD_CHECK(hipGetLastError());
+        timing_end(e.s);
+      }
+      exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
+      total = read_u64(e, a.offsets + slices);
+      if (total > 0 && !d->count_only) {
+        for (uint32_t k = 0; k < d->n_dest; ++k) {
+          const SrdDestSpec& ds = d->dest[k];
+          Relation& r = e.rels[rel_id(e, ds.rel)];
+          Order order(ds.cols, ds.cols + ds.arity);
+          uint64_t base = reserve_new(e, r, order, total);
+          a.out[k].cols = r.raw.p;
+          a.out[k].stride = r.raw_cap;
+          a.out[k].base = base;
+        }
+        fill_u32(a.work_ctr, 0, 2, e.s);
+        std::string nm = std::string("join.write:") + d->rule;
+        timing_begin(nm.c_str(), e.s);
+        d->write(&a, e.s);
+        SRD_CHECK(hipGetLastError());
+        timing_end(e.s);
+      }
+    }
+  }
+  e.join_probes += total;
+  if (d->count_only) {
+    e.counts.emplace_back(d->rule, total);
+    for (uint32_t k = 0; k < d->n_dest; ++k) {
+      Relation& r = e.rels[rel_id(e, d->dest[k].rel)];
+      r.count += total;
+      r.counted = true;
+    }
+    if (!e.quiet) std::cout << " >>>>>>>>>>>>>>>>> " << d->rule << " count: " << total << std::endl;
+  }
+}
+
+// =========================================================== MIR ops ====
+static void exec_ops(srd_engine& e, const std::vector<Op>& ops, uint64_t iteration, int step_no);
+
+static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no) {
+  switch (op.kind) {
+    case OpKind::Pipeline:
+      run_pipeline(e, op.pipe, iteration);
+      break;
+    case OpKind::Parallel:
+      exec_ops(e, op.kids, iteration, step_no);
+      break;
+    case OpKind::RebuildIndex: {
+      Relation& r = e.rels[op.rel];
+      if (op.ver == SRD_NEW) {
+        r.newv.erase(op.index);
+        new_index(e, r, op.index);
+        // the sorted index now owns the NEW rows
+        r.raw.release();
+        r.raw_n = r.raw_cap = 0;
+      } else if (op.ver == SRD_DELTA) {
+        r.delta.erase(op.index);
+        delta_index(e, r, op.index);
+      } else {
+        full_index(e, r, op.index);
+      }
+      break;
+    }
+    case OpKind::ClearRelation: {
+      Relation& r = e.rels[op.rel];
+      if (op.ver == SRD_NEW) clear_new(r);
+      else if (op.ver == SRD_DELTA) r.delta.clear();
+      else {
+        r.full.clear();
+        r.full_version = r.delta_seq = 0;
+      }
+      break;
+    }
+    case OpKind::CheckSize:
+      break;
+    case OpKind::ComputeDelta: {
+      Relation& r = e.rels[op.rel];
+      const Index& nw = new_index(e, r, op.index);
+      const Index& fl = full_index(e, r, op.index);
+      Index d;
+      if (fl.n == 0)
+        copy_index(nw, d, e.s);
+      else
+        set_difference(nw, fl, d, e.ws, e.s);
+      r.delta.clear();
+      if (r.deltas_step != step_no) {
+        r.deltas.clear();
+        r.deltas_step = step_no;
+      }
+      r.deltas.push_back(d.n);
+      r.delta[op.index] = std::move(d);
+      r.delta_seq = r.full_version + 1;
+      break;
+    }
+    case OpKind::MergeIndex: {
+      Relation& r = e.rels[op.rel];
+      const uint64_t target = r.delta_seq;
+      if (target == 0) break;  // nothing computed yet
+      auto it = r.full.find(op.index);
+      if (it != r.full.end() && it->second.version == target) break;
+      const Index& dl = delta_index(e, r, op.index);
+      Versioned v;
+      v.version = target;
+      if (it != r.full.end() && it->second.version == target - 1) {
+        merge_disjoint(it->second.idx, dl, v.idx, e.ws, e.s);
+      } else if (op.index != r.canonical) {
+        auto cit = r.full.find(r.canonical);
+        if (cit != r.full.end() && cit->second.version == target) {
+          build_index_from(cit->second.idx, perm_between(r.canonical, op.index), v.idx, e.ws, e.s);
+        } else {
+          const Index& base = full_index(e, r, op.index);  // at target - 1
+          merge_disjoint(base, dl, v.idx, e.ws, e.s);
+        }
+      } else {
+        const Index& base = full_index(e, r, op.index);
+        merge_disjoint(base, dl, v.idx, e.ws, e.s);
+      }
+      r.full[op.index] = std::move(v);
+      if (op.index == r.canonical) r.full_version = target;
+      break;
+    }
+    case OpKind::RebuildFromIndex: {
+      Relation& r = e.rels[op.rel];
+      Index x;
+      if (op.ver == SRD_DELTA) {
+        const Index& src = delta_index(e, r, op.index);
+        build_index_from(src, perm_between(op.index, op.index2), x, e.ws, e.s);
+        r.delta[op.index2] = std::move(x);
+      } else if (op.ver == SRD_NEW) {
+        const Index& src = new_index(e, r, op.index);
+        build_index_from(src, perm_between(op.index, op.index2), x, e.ws, e.s);
+        r.newv[op.index2] = std::move(x);
+      } else {
+        full_index(e, r, op.index2);
+      }
+      break;
+    }
+    case OpKind::Reconstruct:
+    case OpKind::Hook:
+      break;
+  }
+}
+
+static void exec_ops(srd_engine& e, const std::vector<Op>& ops, uint64_t iteration, int step_no) {
+  for (auto& op : ops) exec_op(e, op, iteration, step_no);
+}
+
+static void reset_state(srd_engine& e) {
+  for (auto& r : e.rels) {
+    r.full.clear();
+    r.delta.clear();
+    clear_new(r);
+    r.full_version = r.delta_seq = 0;
+    r.deltas.clear();
+    r.deltas_step = -1;
+    r.count = 0;
+    r.counted = false;
+  }
+  e.counts.clear();
+  e.iterations = 0;
+  e.join_probes = 0;
+}
+
+static void run_program(srd_engine& e, uint64_t max_iters) {
+  reset_state(e);
+  for (auto& r : e.rels) upload_input(e, r);
+  auto t0 = std::chrono::steady_clock::now();
+  for (size_t si = 0; si < e.steps.size(); ++si) {
+    const Step& st = e.steps[si];
+    auto ts = std::chrono::steady_clock::now();
+    if (!st.recursive) {
+      exec_ops(e, st.ops, 0, int(si));
+    } else {
+      uint64_t it = 0;
+      for (; it < max_iters; ++it) {
+        if (it > 0) {
+          uint64_t tot = 0;
+          for (int ri : st.canonical_rels) {
+            Relation& r = e.rels[ri];
+            auto d = r.delta.find(r.canonical);
+            if (d != r.delta.end()) tot += d->second.n;
+          }
+          if (std::getenv("SRDATALOG_PRINT_DELTA"))
+            std::cerr << "[iter " << it << "] delta=" << tot << std::endl;
+          if (tot == 0) break;
+        }
+        exec_ops(e, st.ops, it, int(si));
+      }
+      e.iterations = it;
+    }
+    SRD_CHECK(hipStreamSynchronize(e.s));
+    if (!e.quiet && std::getenv("SRDATALOG_PRINT_STEPS")) {
+      double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
+      std::cout << "[Step " << si << "] " << ms << " ms" << std::endl;
+    }
+  }
+  SRD_CHECK(hipStreamSynchronize(e.s));
+  e.run_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  e.ran = true;
+  if (!e.quiet)
+    for (auto& r : e.rels)
+      if (r.print_size) {
+        auto it = r.full.find(r.canonical);
+        uint64_t n = it == r.full.end() ? 0 : it->second.idx.n;
+        std::cout << " >>>>>>>>>>>>>>>>> " << r.name << " : " << n << std::endl;
+      }
+}
+
+// =============================================================== CSV ====
+static void load_csv_file(Relation& r, const std::string& path) {
+  int fd = open(path.c_str(), O_RDONLY);
+  if (fd < 0) fail("could not open file: " + path);
+  struct stat sb;
+  if (fstat(fd, &sb) != 0) {
+    close(fd);
+    fail("could not stat: " + path);
+  }
+  size_t size = size_t(sb.st_size);
+  r.host_rows.clear();
+  r.host_n = 0;
+  r.has_input = true;
+  r.dev_input_valid = false;
+  if (size == 0) {
+    close(fd);
+    return;
+  }
+  char* m = (char*)mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+  close(fd);
+  if (m == MAP_FAILED) fail("could not mmap: " + path);
+  const char* p = m;
+  const char* end = m + size;
+  // delimiter from the first line: tab, else comma, else runs of spaces
+  const char* le = p;
+  while (le < end && *le != '\n') ++le;
+  char delim = ' ';
+  bool tabs = std::find(p, le, '\t') != le, commas = std::find(p, le, ',') != le;
+  if (tabs) delim = '\t';
+  else if (commas) delim = ',';
+  r.host_rows.reserve(size / 8);
+  std::vector<uint32_t> row(r.arity);
+  while (p < end) {
+    while (p < end && (*p == '\n' || *p == '\r')) ++p;
+    if (p >= end) break;
+    bool ok = true;
+    for (uint32_t c = 0; c < r.arity; ++c) {
+      while (p < end && (*p == ' ' || *p == '\t') && *p != delim) ++p;
+      if (delim == ' ' || delim == '\t')
+        while (p < end && (*p == ' ' || *p == '\t')) ++p;
+      bool neg = false;
+      if (p < end && *p == '-') {
+        neg = true;
+        ++p;
+      }
+      uint64_t v = 0;
+      const char* st = p;
+      while (p < end && *p >= '0' && *p <= '9') v = v * 10 + uint64_t(*p++ - '0');
+      if (p == st) ok = false;
+      row[c] = neg ? uint32_t(-int64_t(v)) : uint32_t(v);
+      while (p < end && (*p == ' ' || *p == '\t') && delim != ' ' && delim != '\t') ++p;
+      if (p < end && *p == delim) ++p;
+    }
+    while (p < end && *p != '\n') ++p;
+    if (p < end) ++p;
+    if (ok) {
+      r.host_rows.insert(r.host_rows.end(), row.begin(), row.end());
+      r.host_n++;
+    }
+  }
+  munmap(m, size);
+}
+
+}  // namespace srd
+
+// ============================================================ C ABI ====
+using namespace srd;
+
+template <class F>
+static int eguard(srd_engine* e, int code_on_error, F&& f) {
+  if (!e) return 1;
+  try {
+    f();
+    e->err.clear();
+    return 0;
+  } catch (const std::exception& ex) {
+    e->err = ex.what();
+    std::cerr << "srdatalog: " << ex.what() << std::endl;
+    return code_on_error;
+  } catch (...) {
+    e->err = "unknown error";
+    return code_on_error;
+  }
+}
+
+extern "C" {
+
+srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, uint32_t n) {
+  auto* e = new srd_engine();
+  try {
+    pool_init(0);
+    SRD_CHECK(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
+    const char* q = std::getenv("SRDATALOG_QUIET");
+    e->quiet = q && *q && *q != '0';
+    if (const char* g = std::getenv("SRDATALOG_GRAIN")) e->grain_override = uint32_t(std::atoi(g));
+    for (uint32_t i = 0; i < n; ++i) e->pipes[pipes[i].rule] = &pipes[i];
+    parse_program(*e, text);
+  } catch (const std::exception& ex) {
+    std::cerr << "srdatalog_init: " << ex.what() << std::endl;
+    e->err = ex.what();
+  }
+  return e;
+}
+
+void srd_engine_destroy(srd_engine* e) {
+  if (!e) return;
+  hipStream_t s = e->s;
+  {
+    std::unique_ptr<srd_engine> hold(e);
+    for (auto& r : e->rels) {
+      r.full.clear();
+      r.delta.clear();
+      clear_new(r);
+      r.dev_input.release();
+    }
+  }
+  if (s) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
+}
+
+const char* srd_engine_last_error(srd_engine* e) { return e ? e->err.c_str() : "no engine"; }
+
+int srd_engine_ok(srd_engine* e) { return e && e->err.empty() ? 1 : 0; }
+
+int srd_engine_load_csv(srd_engine* e, const char* rel, const char* path) {
+  if (!e || !rel || !path) return 1;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end() || e->rels[it->second].input_file.empty()) {
+    std::cerr << "srdatalog_load_csv: unknown or non-input relation " << rel << std::endl;
+    return 2;
+  }
+  return eguard(e, 3, [&] { load_csv_file(e->rels[it->second], path); });
+}
+
+int srd_engine_load_all(srd_engine* e, const char* dir) {
+  if (!e || !dir) return 1;
+  return eguard(e, 2, [&] {
+    for (auto& r : e->rels)
+      if (!r.input_file.empty()) load_csv_file(r, std::string(dir) + "/" + r.input_file);
+  });
+}
+
+int srd_engine_load_rows(srd_engine* e, const char* rel, const uint32_t* rows, uint64_t n,
+                         uint32_t arity) {
+  if (!e || !rel || (n && !rows)) return 1;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return 2;
+  Relation& r = e->rels[it->second];
+  if (arity != r.arity) return 2;
+  return eguard(e, 3, [&] {
+    r.host_rows.assign(rows, rows + n * arity);
+    r.host_n = n;
+    r.has_input = true;
+    r.dev_input_valid = false;
+  });
+}
+
+int srd_engine_prepare(srd_engine* e) {
+  return eguard(e, 2, [&] {
+    for (auto& r : e->rels) upload_input(*e, r);
+  });
+}
+
+int srd_engine_run(srd_engine* e, unsigned long long max_iters) {
+  if (!e) return 1;
+  if (!e->err.empty() && e->steps.empty()) return 2;
+  bool any_input = false;
+  for (auto& r : e->rels) any_input |= r.has_input;
+  bool any_decl = false;
+  for (auto& r : e->rels) any_decl |= !r.input_file.empty();
+  if (any_decl && !any_input) return 1;
+  return eguard(e, 2, [&] { run_program(*e, max_iters ? max_iters : uint64_t(2147483647)); });
+}
+
+unsigned long long srd_engine_size(srd_engine* e, const char* rel) {
+  if (!e || !rel) return 0;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return 0;
+  Relation& r = e->rels[it->second];
+  if (r.counted) return r.count;
+  auto f = r.full.find(r.canonical);
+  if (f != r.full.end()) return f->second.idx.n;
+  if (!e->ran && r.has_input) return r.host_n;
+  return 0;
+}
+
+int srd_engine_export(srd_engine* e, const char* rel, uint32_t* out, uint64_t cap_rows,
+                      uint64_t* n_rows) {
+  if (!e || !rel || !n_rows) return 1;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return 2;
+  Relation& r = e->rels[it->second];
+  return eguard(e, 3, [&] {
+    *n_rows = 0;
+    auto f = r.full.find(r.canonical);
+    if (f == r.full.end() || f->second.idx.n == 0) return;
+    const Index* x = &f->second.idx;
+    Index tmp;
+    Order ident(r.arity);
+    for (uint32_t c = 0; c < r.arity; ++c) ident[c] = c;
+    if (r.canonical != ident) {
+      build_index_from(*x, perm_between(r.canonical, ident), tmp, e->ws, e->s);
+      x = &tmp;
+    }
+    *n_rows = x->n;
+    if (!out) return;
+    if (cap_rows < x->n) fail("srdatalog_export: buffer too small");
+    std::vector<uint32_t> t(x->stride * x->arity);
+    SRD_CHECK(hipMemcpyAsync(t.data(), x->cols.p, t.size() * 4, hipMemcpyDeviceToHost, e->s));
+    SRD_CHECK(hipStreamSynchronize(e->s));
+    for (uint64_t i = 0; i < x->n; ++i)
+      for (uint32_t c = 0; c < x->arity; ++c) out[i * x->arity + c] = t[c * x->stride + i];
+  });
+}
+
+unsigned long long srd_engine_num_iterations(srd_engine* e, const char* rel) {
+  if (!e || !rel) return 0;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return 0;
+  return e->rels[it->second].deltas.size();
+}
+
+int srd_engine_delta_sizes(srd_engine* e, const char* rel, unsigned long long* out,
+                           unsigned long long cap) {
+  if (!e || !rel) return -1;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return -1;
+  auto& d = e->rels[it->second].deltas;
+  for (uint64_t i = 0; i < d.size() && i < cap; ++i) out[i] = d[i];
+  return int(d.size());
+}
+
+int srd_engine_shutdown(srd_engine* e) {
+  if (!e) return 0;
+  for (auto& r : e->rels) {
+    r.full.clear();
+    r.delta.clear();
+    clear_new(r);
+    r.host_rows.clear();
+    r.host_rows.shrink_to_fit();
+    r.host_n = 0;
+    r.has_input = false;
+    r.dev_input.release();
+    r.dev_input_valid = false;
+  }
+  e->ran = false;
+  return 0;
+}
+
+double srd_engine_run_seconds(srd_engine* e) { return e ? e->run_seconds : 0.0; }
+
+unsigned long long srd_engine_join_tuples(srd_engine* e) { return e ? e->join_probes : 0; }
+
+void srd_engine_set_quiet(srd_engine* e, int quiet) {
+  if (e) e->quiet = quiet != 0;
+}
+
+void srd_engine_set_grain(srd_engine* e, unsigned grain) {
+  if (e) e->grain_override = grain;
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// Device helpers used by the generated per-rule join kernels.
+//
+// Execution model of a generated kernel (see srd_pipeline.h for the launch
+// protocol): a wave64 owns one slice = (root key, [lo,hi) of the first
+// nested level's items).  Items are processed 64 at a time, one per lane.
+// If the rule has a second nested level, the lanes that produced a binding
+// publish it in LDS and the wave walks the concatenation of their level-2
+// iteration spaces 64 items at a time (load-balanced search over a 64-entry
+// prefix array), so a lane with a huge level-2 range does not serialize the
+// wave.  Every emitted tuple is one lane's result for one item; ballot +
+// mbcnt give its position, so emission never diverges.
+//
+// Counterparts in the reference: the warp-cooperative WCOJ body emitted by
+// src/srdatalog/codegen/jit/pipeline.py:360 / root.py / instructions.py over
+// the search primitives of runtime/generalized_datalog/gpu/search.h and
+// IntersectView (gpu/device_sorted_array_index.ipp:1284-1487).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "srd_device.h"
+#include "srd_pipeline.h"
+
+namespace srdj {
+
+using srd::IndexView;
+using srd::Rng;
+
+__device__ __forceinline__ const uint32_t* colp(const IndexView& v, int c) {
+  return v.cols + uint64_t(c) * v.stride;
+}
+
+// Largest k in [0, n) with start[k] <= x (start has n+1 ascending entries).
+__device__ __forceinline__ uint32_t find_slot(const uint64_t* __restrict__ start, uint32_t n,
+                                              uint64_t x) {
+  uint32_t lo = 0, hi = n;  // answer in [lo, hi)
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (start[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Equal range of the value at driver position pos inside [b,e) of column
+// col (values within [b,e) sorted); `last` = column is the index's last.
+__device__ __forceinline__ Rng run_at(const uint32_t* __restrict__ col, uint32_t pos, uint32_t e,
+                                      uint32_t v, bool last) {
+  if (last) return Rng{pos, pos + 1};
+  return Rng{pos, srd::upper_bound(col, pos + 1, e, v)};
+}
+
+// Owner lane of flattened item j given the wave's 65-entry exclusive
+// prefix array (pref[64] = total).
+__device__ __forceinline__ uint32_t owner_of(const uint64_t* pref, uint64_t j) {
+  uint32_t lo = 0, hi = 64;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (pref[mid] <= j) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Persistent-wave slice dispenser: lane 0 takes the next slice id.
+__device__ __forceinline__ uint64_t next_slice(uint32_t* ctr) {
+  uint64_t s = 0;
+  if (srd::lane_id() == 0) s = atomicAdd(ctr, 1u);
+  return __shfl(s, 0, srd::kWave);
+}
+
+}  // namespace srdj

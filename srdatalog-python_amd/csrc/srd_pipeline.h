@@ -1,0 +1,111 @@
+// Contract between the runtime's MIR executor and the per-rule join kernels
+// that the Python emitter generates for a program.
+//
+// A pipeline (one `execute-pipeline` MIR node) runs as three launches:
+//   root   one thread per distinct key of the driver root source: look the
+//          key up in every root source, record the ranges, and size the
+//          first nested level's iteration space (key_work);
+//   count  the per-key work is flattened into one global item space (an
+//          exclusive scan of key_work); persistent waves pull fixed-size
+//          slices of it from an atomic counter, so a slice may span many
+//          small keys or a fraction of one huge key, and count the tuples
+//          each slice emits;
+//   write  the same traversal, writing each slice's tuples at its
+//          exclusive-scan offset in the destination NEW buffer.
+// Output order therefore depends only on the item order, never on which
+// wave ran a slice.
+//
+// Reference counterpart: the JitRunner_<rule> phases setup / launch_count /
+// scan_and_resize / launch_materialize of
+// src/srdatalog/codegen/jit/complete_runner.py:929-1523 and the block-group
+// work partition of complete_runner.py:377-736.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "srd_device.h"
+
+#define SRD_MAX_VIEWS 8
+#define SRD_MAX_COLS 8
+#define SRD_MAX_DESTS 2
+
+enum SrdVersion : uint32_t { SRD_FULL = 0, SRD_DELTA = 1, SRD_NEW = 2 };
+enum SrdRootKind : uint32_t { SRD_ROOT_JOIN = 0, SRD_ROOT_SCAN = 1 };
+
+struct SrdViewSpec {
+  const char* rel;
+  uint32_t ncols;
+  uint32_t cols[SRD_MAX_COLS];  // index column order
+  uint32_t ver;                 // SrdVersion
+};
+
+struct SrdDestSpec {
+  const char* rel;
+  uint32_t arity;
+  uint32_t cols[SRD_MAX_COLS];  // dedup index (column order written to NEW)
+};
+
+struct SrdViews {
+  srd::IndexView v[SRD_MAX_VIEWS];
+};
+
+struct SrdOut {
+  uint32_t* cols;   // NEW buffer, column c at cols + c*stride
+  uint64_t stride;
+  uint64_t base;    // first row this launch writes
+};
+
+struct SrdLaunchArgs {
+  SrdViews views;
+  uint32_t n_root;             // driver root keys (join) or 1 (scan)
+  uint32_t drv_view;           // view whose root_vals enumerate the keys
+  uint32_t grain;              // level-1 items per slice
+  uint32_t grid_blocks;        // persistent blocks for count / write
+  uint64_t n_slices;           // ceil(total work / grain)
+  uint64_t total_work;
+  uint32_t* key_rng;           // per key: 2 u32 per root slot
+  uint32_t* key_aux;           // per key: level-1 driver choice
+  uint64_t* key_work;          // per key: level-1 item count
+  uint64_t* work_start;        // exclusive scan of key_work, n_root+1 entries
+  uint32_t* counts;            // per slice tuple count
+  uint64_t* offsets;           // exclusive scan of counts
+  uint32_t* work_ctr;          // atomic slice counter (zeroed before each launch)
+  SrdOut out[SRD_MAX_DESTS];
+};
+
+typedef void (*SrdLaunchFn)(const SrdLaunchArgs* args, hipStream_t stream);
+
+struct SrdPipelineDesc {
+  const char* rule;
+  uint32_t n_views;
+  SrdViewSpec views[SRD_MAX_VIEWS];
+  uint32_t root_kind;                // SrdRootKind
+  uint32_t n_root_src;               // root sources (join) / 1 (scan)
+  uint32_t root_view[SRD_MAX_VIEWS];
+  uint32_t n_dest;
+  SrdDestSpec dest[SRD_MAX_DESTS];
+  uint32_t count_only;               // rule carries .with_count()
+  uint32_t default_grain;
+  SrdLaunchFn root;
+  SrdLaunchFn count;
+  SrdLaunchFn write;
+};
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct srd_engine;
+
+// Engine entry points used by the generated extern "C" shim of a program.
+// program_text: relation declarations followed by the MIR program in the
+// reference's S-expression form.
+struct srd_engine* srd_engine_create(const char* program_text, const SrdPipelineDesc* pipelines,
+                                     uint32_t n_pipelines);
+void srd_engine_destroy(struct srd_engine* e);
+const char* srd_engine_last_error(struct srd_engine* e);
+
+#ifdef __cplusplus
+}
+#endif

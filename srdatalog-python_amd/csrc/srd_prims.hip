@@ -1,0 +1,1215 @@
+// Stream-ordered set primitives of the MI355X runtime (see srd_runtime.h).
+//
+// Design (wave64, gfx950):
+//  * Every streaming kernel works on 4096-element tiles: 256 threads = 4
+//    waves, each wave owns 1024 consecutive elements and reads them as 16
+//    coalesced 64-lane rows ("wave-striped"), so element order inside a
+//    wave is (item, lane) and stable ranks come from one ballot per row.
+//  * sort = LSD radix over the tightly packed key (columns concatenated at
+//    their significant bit widths, column 0 most significant) with 8-bit
+//    digits.  The pack kernel also builds every pass's digit histogram, and
+//    each pass is ONE kernel: tile ids are taken in launch order from an
+//    atomic counter, each tile ranks its keys with a 8-ballot digit match,
+//    publishes its per-digit count and resolves its global offset with a
+//    decoupled look-back over earlier tiles, then writes its keys from an
+//    LDS-sorted copy so that each digit run is a coalesced store.
+//  * dedup unpacks the sorted keys straight into the index columns and
+//    emits the distinct column-0 values with their row offsets in the same
+//    pass.
+//  * set difference and merge stage, per 4096-row tile, the matching
+//    window of the other sorted input in LDS and binary-search it there.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "srd_runtime.h"
+
+namespace srd {
+
+void fail(const std::string& msg) { throw std::runtime_error(msg); }
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ================================================================ pool ====
+namespace {
+std::mutex g_pool_mu;
+PoolStats g_stats;
+bool g_pool_ready = false;
+
+uint64_t env_bytes(const char* a, const char* b) {
+  const char* v = std::getenv(a);
+  if (!v) v = std::getenv(b);
+  if (!v || !*v) return 0;
+  return std::strtoull(v, nullptr, 10);
+}
+}  // namespace
+
+void pool_init(int device) {
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  if (g_pool_ready) return;
+  SRD_CHECK(hipSetDevice(device));
+  hipMemPool_t pool;
+  SRD_CHECK(hipDeviceGetDefaultMemPool(&pool, device));
+  uint64_t thr = ~uint64_t(0);
+  SRD_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+  g_stats.budget = env_bytes("SRDATALOG_POOL_MAX_SIZE", "SRDATALOG_RMM_POOL_MAX_SIZE");
+  g_pool_ready = true;
+}
+
+void* pool_alloc(uint64_t bytes, hipStream_t s) {
+  if (!g_pool_ready) pool_init(0);
+  {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    if (g_stats.budget && g_stats.in_use + bytes > g_stats.budget)
+      fail("srdatalog pool: allocation of " + std::to_string(bytes) + " bytes exceeds the budget of " +
+           std::to_string(g_stats.budget) + " bytes (" + std::to_string(g_stats.in_use) + " in use)");
+    g_stats.in_use += bytes;
+    g_stats.peak = std::max(g_stats.peak, g_stats.in_use);
+    g_stats.allocs++;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMallocAsync(&p, bytes, s);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    g_stats.in_use -= bytes;
+    fail("srdatalog pool: hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  }
+  return p;
+}
+
+void pool_free(void* p, uint64_t bytes, hipStream_t s) {
+  if (!p) return;
+  (void)hipFreeAsync(p, s);
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  g_stats.in_use -= bytes;
+}
+
+PoolStats pool_stats() {
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  return g_stats;
+}
+
+void pool_set_budget(uint64_t bytes) {
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  g_stats.budget = bytes;
+}
+
+void pool_reset_peak() {
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  g_stats.peak = g_stats.in_use;
+}
+
+Workspace::~Workspace() {
+  if (pinned) (void)hipHostFree(pinned);
+}
+
+uint32_t* pinned_words(Workspace& ws, uint64_t words) {
+  if (words > ws.pinned_words) {
+    if (ws.pinned) SRD_CHECK(hipHostFree(ws.pinned));
+    uint64_t w = std::max<uint64_t>(words, 1024);
+    SRD_CHECK(hipHostMalloc((void**)&ws.pinned, w * 4, hipHostMallocDefault));
+    ws.pinned_words = w;
+  }
+  return ws.pinned;
+}
+
+// ============================================================== timing ====
+namespace {
+struct PendingTime {
+  std::string name;
+  hipEvent_t a, b;
+  double bytes;
+};
+bool g_timing = false;
+std::vector<PendingTime> g_pending;
+std::vector<hipEvent_t> g_free_events;
+std::map<std::string, KernelTime> g_times;
+std::string g_cur_name;
+hipEvent_t g_cur_start;
+
+hipEvent_t get_event() {
+  if (!g_free_events.empty()) {
+    hipEvent_t e = g_free_events.back();
+    g_free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  SRD_CHECK(hipEventCreate(&e));
+  return e;
+}
+}  // namespace
+
+void timing_enable(bool on) { g_timing = on; }
+bool timing_enabled() { return g_timing; }
+
+void timing_begin(const char* name, hipStream_t s) {
+  if (!g_timing) return;
+  g_cur_name = name;
+  g_cur_start = get_event();
+  SRD_CHECK(hipEventRecord(g_cur_start, s));
+}
+
+void timing_end(hipStream_t s, double alg_bytes) {
+  if (!g_timing) return;
+  hipEvent_t e = get_event();
+  SRD_CHECK(hipEventRecord(e, s));
+  g_pending.push_back({g_cur_name, g_cur_start, e, alg_bytes});
+}
+
+std::vector<KernelTime> timing_collect() {
+  for (auto& p : g_pending) {
+    SRD_CHECK(hipEventSynchronize(p.b));
+    float ms = 0;
+    SRD_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& t = g_times[p.name];
+    t.name = p.name;
+    t.launches++;
+    t.ms += ms;
+    t.bytes += p.bytes;
+    g_free_events.push_back(p.a);
+    g_free_events.push_back(p.b);
+  }
+  g_pending.clear();
+  std::vector<KernelTime> out;
+  for (auto& kv : g_times) out.push_back(kv.second);
+  return out;
+}
+
+void timing_reset() {
+  timing_collect();
+  g_times.clear();
+}
+
+struct TimeScope {
+  hipStream_t s;
+  double bytes;
+  TimeScope(const char* n, hipStream_t st, double b = 0) : s(st), bytes(b) { timing_begin(n, st); }
+  ~TimeScope() { timing_end(s, bytes); }
+};
+
+// ================================================================ fill ====
+template <class T>
+__global__ void fill_kernel(T* p, T v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+static uint32_t grid_for(uint64_t n, uint32_t per_block, uint32_t cap = 65535u * 16u) {
+  uint64_t g = (n + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  return uint32_t(std::min<uint64_t>(g, cap));
+}
+
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  fill_kernel<uint32_t><<<grid_for(n, 1024, 8192), 256, 0, s>>>(p, v, n);
+  SRD_CHECK(hipGetLastError());
+}
+
+void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  fill_kernel<uint64_t><<<grid_for(n, 1024, 8192), 256, 0, s>>>(p, v, n);
+  SRD_CHECK(hipGetLastError());
+}
+
+// ================================================================ scan ====
+constexpr int kItems = 16;
+constexpr uint32_t kTile = kBlock * kItems;  // 4096
+
+// Position of (item i, lane l) of wave w inside a tile.
+__device__ __forceinline__ uint32_t tile_pos(uint32_t w, uint32_t i, uint32_t l) {
+  return w * (kWave * kItems) + i * kWave + l;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void scan_reduce_kernel(const T* __restrict__ in, uint64_t n,
+                                                             uint64_t* __restrict__ partial) {
+  __shared__ uint64_t wsum[kWavesPerBlock];
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + uint64_t(i) * kBlock + threadIdx.x;
+    if (idx < n) s += uint64_t(in[idx]);
+  }
+  s = wave_sum(s);
+  if (lane_id() == 0) wsum[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) t += wsum[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of one tile per block; carry[blockIdx] (if non-null) is
+// added.  The block owning the last element writes out[n] = total.
+template <class T>
+__global__ __launch_bounds__(kBlock) void scan_tile_kernel(const T* __restrict__ in, uint64_t n,
+                                                           const uint64_t* __restrict__ carry,
+                                                           uint64_t* __restrict__ out) {
+  __shared__ uint64_t wtot[kWavesPerBlock];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint64_t v[kItems];
+  uint64_t run = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + tile_pos(w, i, l);
+    v[i] = idx < n ? uint64_t(in[idx]) : 0;
+    run += v[i];
+  }
+  run = wave_sum(run);
+  if (l == 0) wtot[w] = run;
+  __syncthreads();
+  uint64_t off = carry ? carry[blockIdx.x] : 0;
+  uint64_t total = off;
+  for (uint32_t j = 0; j < kWavesPerBlock; ++j) {
+    if (j < w) off += wtot[j];
+    total += wtot[j];
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t inc = wave_incl_scan(v[i]);
+    uint64_t idx = base + tile_pos(w, i, l);
+    if (idx < n) out[idx] = off + inc - v[i];
+    off += __shfl(inc, kWave - 1, kWave);
+  }
+  uint64_t last_block = (n == 0) ? 0 : (n - 1) / kTile;
+  if (blockIdx.x == last_block && threadIdx.x == 0) out[n] = total;
+}
+
+template <class T>
+static void scan_impl(const T* in, uint64_t* out, uint64_t n, hipStream_t s) {
+  if (n <= kTile) {
+    scan_tile_kernel<T><<<1, kBlock, 0, s>>>(in, n, nullptr, out);
+    SRD_CHECK(hipGetLastError());
+    return;
+  }
+  uint64_t tiles = (n + kTile - 1) / kTile;
+  DevBuf<uint64_t> partial(tiles, s), carry(tiles + 1, s);
+  scan_reduce_kernel<T><<<uint32_t(tiles), kBlock, 0, s>>>(in, n, partial.p);
+  SRD_CHECK(hipGetLastError());
+  scan_impl<uint64_t>(partial.p, carry.p, tiles, s);
+  scan_tile_kernel<T><<<uint32_t(tiles), kBlock, 0, s>>>(in, n, carry.p, out);
+  SRD_CHECK(hipGetLastError());
+}
+
+void exclusive_scan_u32(const uint32_t* in, uint64_t* out, uint64_t n, Workspace&, hipStream_t s) {
+  TimeScope t("scan", s, double(n) * 12);
+  scan_impl<uint32_t>(in, out, n, s);
+}
+
+void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace&, hipStream_t s) {
+  TimeScope t("scan", s, double(n) * 16);
+  scan_impl<uint64_t>(in, out, n, s);
+}
+
+// ======================================================== sort: packing ====
+constexpr int kMaxCols = 8;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = 16;
+
+struct PackSpec {
+  const uint32_t* in[kMaxCols];
+  uint32_t shift[kMaxCols];  // bit offset of column c in the 128-bit key
+  uint32_t k;
+  uint32_t passes;
+};
+
+__global__ __launch_bounds__(kBlock) void col_max_kernel(PackSpec spec, uint64_t n,
+                                                         uint32_t* __restrict__ out_max) {
+  for (uint32_t c = 0; c < spec.k; ++c) {
+    uint32_t m = 0;
+    for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * kBlock)
+      m = max(m, spec.in[c][i]);
+    m = wave_max(m);
+    if (lane_id() == 0 && m) atomicMax(&out_max[c], m);
+  }
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t digit_of(uint64_t lo, uint64_t hi, uint32_t shift) {
+  if (W == 1) return uint32_t(lo >> shift) & (kRadix - 1);
+  if (shift >= 64) return uint32_t(hi >> (shift - 64)) & (kRadix - 1);
+  uint64_t v = lo >> shift;
+  if (shift > 56) v |= hi << (64 - shift);
+  return uint32_t(v) & (kRadix - 1);
+}
+
+// Pack columns into W-word keys and accumulate every pass's digit histogram.
+template <int W>
+__global__ __launch_bounds__(kBlock) void pack_hist_kernel(PackSpec spec, uint64_t n,
+                                                           uint64_t* __restrict__ lo_out,
+                                                           uint64_t* __restrict__ hi_out,
+                                                           uint32_t* __restrict__ hist,
+                                                           uint32_t* __restrict__ tile_ctr) {
+  __shared__ uint32_t h[kMaxPasses * kRadix];
+  for (uint32_t i = threadIdx.x; i < spec.passes * kRadix; i += kBlock) h[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kMaxPasses) tile_ctr[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t c = 0; c < spec.k; ++c) {
+      uint64_t v = spec.in[c][i];
+      uint32_t s = spec.shift[c];
+      if (s >= 64) {
+        hi |= v << (s - 64);
+      } else {
+        lo |= v << s;
+        if (W == 2 && s > 32) hi |= v >> (64 - s);
+      }
+    }
+    lo_out[i] = lo;
+    if (W == 2) hi_out[i] = hi;
+    for (uint32_t p = 0; p < spec.passes; ++p)
+      atomicAdd(&h[p * kRadix + digit_of<W>(lo, hi, p * kRadixBits)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < spec.passes * kRadix; i += kBlock)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// ====================================================== sort: onesweep ====
+// Look-back word: [63:48] epoch, [47:46] flag (1 = tile aggregate,
+// 2 = inclusive prefix), [45:0] count.
+constexpr uint64_t kLbAgg = 1, kLbPrefix = 2;
+constexpr uint64_t kLbMask = (uint64_t(1) << 46) - 1;
+constexpr uint32_t kSpinLimit = 1u << 26;
+
+__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint64_t flag, uint64_t v) {
+  return (uint64_t(epoch) << 48) | (flag << 46) | v;
+}
+
+__device__ __forceinline__ void lb_store(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t lb_load(uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lanes (among `valid`) whose digit equals this lane's digit.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+  uint64_t m = ballot(valid);
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    uint64_t bb = ballot((d >> b) & 1u);
+    m &= ((d >> b) & 1u) ? bb : ~bb;
+  }
+  return m;
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+    const uint64_t* __restrict__ in_lo, const uint64_t* __restrict__ in_hi,
+    uint64_t* __restrict__ out_lo, uint64_t* __restrict__ out_hi, uint32_t n, uint32_t shift,
+    const uint32_t* __restrict__ bin_start, uint64_t* __restrict__ lookback,
+    uint32_t* __restrict__ tile_ctr, uint32_t epoch, uint32_t* __restrict__ err) {
+  constexpr int ITEMS = (W == 1) ? 16 : 8;
+  constexpr uint32_t TILE = kBlock * ITEMS;
+  __shared__ uint64_t s_keys[W][TILE];
+  __shared__ uint32_t s_wcnt[kWavesPerBlock][kRadix];
+  __shared__ uint32_t s_local[kRadix];
+  __shared__ uint64_t s_global[kRadix];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wsum[kWavesPerBlock];
+
+  const uint32_t tid = threadIdx.x, w = tid / kWave, l = lane_id();
+  if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  for (uint32_t i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&s_wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t base = uint64_t(tile) * TILE;
+  const uint32_t count = uint32_t(min<uint64_t>(TILE, n - base));
+
+  uint64_t klo[ITEMS], khi[ITEMS];
+  uint32_t rank[ITEMS], dig[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint32_t p = w * (kWave * ITEMS) + i * kWave + l;
+    bool valid = p < count;
+    klo[i] = valid ? in_lo[base + p] : 0;
+    khi[i] = (W == 2 && valid) ? in_hi[base + p] : 0;
+    uint32_t d = valid ? digit_of<W>(klo[i], khi[i], shift) : 0;
+    dig[i] = d;
+    uint64_t peers = match_digit(d, valid);
+    uint32_t before = valid ? s_wcnt[w][d] : 0;
+    rank[i] = before + popc_below(peers);
+    if (valid && popc_below(peers) == 0) s_wcnt[w][d] = before + __popcll(peers);
+  }
+  __syncthreads();
+
+  // Thread d owns digit d: per-wave exclusive offsets and the tile total.
+  const uint32_t d = tid;
+  uint32_t tot = 0;
+#pragma unroll
+  for (int j = 0; j < kWavesPerBlock; ++j) {
+    uint32_t c = s_wcnt[j][d];
+    s_wcnt[j][d] = tot;
+    tot += c;
+  }
+  uint64_t* my_lb = lookback + uint64_t(tile) * kRadix + d;
+  uint64_t excl = 0;
+  if (tile == 0) {
+    lb_store(my_lb, lb_pack(epoch, kLbPrefix, tot));
+  } else {
+    lb_store(my_lb, lb_pack(epoch, kLbAgg, tot));
+    int64_t t = int64_t(tile) - 1;
+    uint32_t spins = 0;
+    while (t >= 0) {
+      uint64_t v = lb_load(lookback + uint64_t(t) * kRadix + d);
+      uint64_t flag = (v >> 46) & 3;
+      if (uint32_t(v >> 48) != epoch || flag == 0) {
+        if (++spins > kSpinLimit) {
+          atomicOr(err, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += v & kLbMask;
+      if (flag == kLbPrefix) break;
+      --t;
+    }
+    lb_store(my_lb, lb_pack(epoch, kLbPrefix, excl + tot));
+  }
+  // Local start of digit d inside the LDS-sorted tile.
+  uint32_t wtotal;
+  uint32_t lstart = block_excl_scan<uint32_t>(tot, s_wsum, &wtotal);
+  s_local[d] = lstart;
+  s_global[d] = uint64_t(bin_start[d]) + excl - lstart;
+  __syncthreads();
+
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint32_t p = w * (kWave * ITEMS) + i * kWave + l;
+    if (p < count) {
+      uint32_t lp = s_local[dig[i]] + s_wcnt[w][dig[i]] + rank[i];
+      s_keys[0][lp] = klo[i];
+      if (W == 2) s_keys[W - 1][lp] = khi[i];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint32_t p = i * kBlock + tid;
+    if (p < count) {
+      uint64_t lo = s_keys[0][p];
+      uint64_t hi = (W == 2) ? s_keys[W - 1][p] : 0;
+      uint64_t g = s_global[digit_of<W>(lo, hi, shift)] + p;
+      if (g < n) {  // always true when the look-back resolved; err reports otherwise
+        out_lo[g] = lo;
+        if (W == 2) out_hi[g] = hi;
+      }
+    }
+  }
+}
+
+__global__ void bin_scan_kernel(const uint32_t* __restrict__ hist, uint32_t passes,
+                                uint32_t* __restrict__ starts) {
+  // one block of kRadix threads per pass
+  __shared__ uint32_t wsum[kWavesPerBlock];
+  uint32_t p = blockIdx.x;
+  uint32_t v = hist[p * kRadix + threadIdx.x];
+  uint32_t tot;
+  uint32_t ex = block_excl_scan<uint32_t>(v, wsum, &tot);
+  starts[p * kRadix + threadIdx.x] = ex;
+}
+
+// ========================================================= sort: dedup ====
+template <int W>
+__device__ __forceinline__ void load_key(const uint64_t* lo, const uint64_t* hi, uint64_t i,
+                                         uint64_t& a, uint64_t& b) {
+  a = lo[i];
+  b = (W == 2) ? hi[i] : 0;
+}
+
+// Column 0 of the packed key: bits [s0, s0+32).
+template <int W>
+__device__ __forceinline__ uint32_t key_col(uint64_t lo, uint64_t hi, uint32_t s, uint32_t bits) {
+  if (bits == 0) return 0;
+  uint64_t v;
+  if (W == 1 || s + bits <= 64) v = lo >> s;
+  else if (s >= 64) v = hi >> (s - 64);
+  else v = (lo >> s) | (hi << (64 - s));
+  return bits >= 32 ? uint32_t(v) : uint32_t(v) & ((1u << bits) - 1u);
+}
+
+struct UnpackSpec {
+  uint32_t shift[kMaxCols];
+  uint32_t bits[kMaxCols];
+  uint32_t k;
+};
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void dedup_count_kernel(const uint64_t* __restrict__ lo,
+                                                             const uint64_t* __restrict__ hi,
+                                                             uint64_t n, UnpackSpec spec,
+                                                             uint32_t* __restrict__ tile_u,
+                                                             uint32_t* __restrict__ tile_r) {
+  __shared__ uint32_t su[kWavesPerBlock], sr[kWavesPerBlock];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint32_t cu = 0, cr = 0;
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + tile_pos(w, i, l);
+    bool fu = false, fr = false;
+    if (idx < n) {
+      uint64_t a, b, pa = 0, pb = 0;
+      load_key<W>(lo, hi, idx, a, b);
+      if (idx > 0) load_key<W>(lo, hi, idx - 1, pa, pb);
+      fu = idx == 0 || a != pa || b != pb;
+      fr = idx == 0 || key_col<W>(a, b, spec.shift[0], spec.bits[0]) !=
+                           key_col<W>(pa, pb, spec.shift[0], spec.bits[0]);
+    }
+    cu += __popcll(ballot(fu));
+    cr += __popcll(ballot(fr));
+  }
+  if (l == 0) su[w] = cu, sr[w] = cr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t a = 0, b = 0;
+    for (int j = 0; j < kWavesPerBlock; ++j) a += su[j], b += sr[j];
+    tile_u[blockIdx.x] = a;
+    tile_r[blockIdx.x] = b;
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void dedup_write_kernel(
+    const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi, uint64_t n, UnpackSpec spec,
+    const uint64_t* __restrict__ uoff, const uint64_t* __restrict__ roff,
+    uint32_t* __restrict__ cols, uint64_t stride, uint32_t* __restrict__ root_vals,
+    uint32_t* __restrict__ root_offs) {
+  __shared__ uint32_t su[kWavesPerBlock], sr[kWavesPerBlock];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint64_t a[kItems], b[kItems];
+  uint64_t mu[kItems], mr[kItems];
+  uint32_t cu = 0, cr = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + tile_pos(w, i, l);
+    bool fu = false, fr = false;
+    a[i] = b[i] = 0;
+    if (idx < n) {
+      uint64_t pa = 0, pb = 0;
+      load_key<W>(lo, hi, idx, a[i], b[i]);
+      if (idx > 0) load_key<W>(lo, hi, idx - 1, pa, pb);
+      fu = idx == 0 || a[i] != pa || b[i] != pb;
+      fr = idx == 0 || key_col<W>(a[i], b[i], spec.shift[0], spec.bits[0]) !=
+                           key_col<W>(pa, pb, spec.shift[0], spec.bits[0]);
+    }
+    mu[i] = ballot(fu);
+    mr[i] = ballot(fr);
+    cu += __popcll(mu[i]);
+    cr += __popcll(mr[i]);
+  }
+  if (l == 0) su[w] = cu, sr[w] = cr;
+  __syncthreads();
+  uint64_t pu = uoff[blockIdx.x], pr = roff[blockIdx.x];
+  for (uint32_t j = 0; j < w; ++j) pu += su[j], pr += sr[j];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if ((mu[i] >> l) & 1) {
+      uint64_t row = pu + popc_below(mu[i]);
+      for (uint32_t c = 0; c < spec.k; ++c)
+        cols[uint64_t(c) * stride + row] = key_col<W>(a[i], b[i], spec.shift[c], spec.bits[c]);
+      if ((mr[i] >> l) & 1) {
+        uint64_t g = pr + popc_below(mr[i]);
+        root_vals[g] = key_col<W>(a[i], b[i], spec.shift[0], spec.bits[0]);
+        root_offs[g] = uint32_t(row);
+      }
+    }
+    pu += __popcll(mu[i]);
+    pr += __popcll(mr[i]);
+  }
+}
+
+// ==================================================== sort: host driver ====
+static uint32_t bit_width(uint32_t v) { return v ? 32 - __builtin_clz(v) : 0; }
+
+void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace& ws,
+                 hipStream_t s, const std::vector<uint32_t>* col_max) {
+  if (k == 0 || k > uint32_t(kMaxCols)) fail("build_index: arity must be in [1, 8]");
+  if (n >= (uint64_t(1) << 32)) fail("build_index: more than 2^32-1 rows");
+  out = Index();
+  out.arity = k;
+  out.col_max.assign(k, 0);
+  if (n == 0) return;
+
+  PackSpec ps{};
+  ps.k = k;
+  for (uint32_t c = 0; c < k; ++c) ps.in[c] = in[c];
+
+  // Scratch layout in ws.small: [hist P*R][starts P*R][tile ctr P][err 1]
+  // [col max 8][dedup tile counts 2*tiles].
+  const uint64_t dtiles = (n + kTile - 1) / kTile;
+  ws.small.reserve(uint64_t(2 * kMaxPasses) * kRadix + kMaxPasses + 1 + kMaxCols + 2 * dtiles, s);
+  uint32_t* hist = ws.small.p;
+  uint32_t* starts = hist + kMaxPasses * kRadix;
+  uint32_t* ctr = starts + kMaxPasses * kRadix;
+  uint32_t* err = ctr + kMaxPasses;
+  uint32_t* dmax = err + 1;
+  uint32_t* tu = dmax + kMaxCols;
+  uint32_t* tr = tu + dtiles;
+
+  // 1. column maxima -> packed bit layout
+  std::vector<uint32_t> mx(k, 0);
+  uint32_t* hp = nullptr;
+  if (col_max && col_max->size() == k) {
+    mx = *col_max;
+  } else {
+    fill_u32(dmax, 0, k, s);
+    {
+      TimeScope t("sort.colmax", s, double(n) * 4 * k);
+      col_max_kernel<<<grid_for(n, kBlock * 16, 2048), kBlock, 0, s>>>(ps, n, dmax);
+      SRD_CHECK(hipGetLastError());
+    }
+    hp = pinned_words(ws, k);
+    SRD_CHECK(hipMemcpyAsync(hp, dmax, k * 4, hipMemcpyDeviceToHost, s));
+    SRD_CHECK(hipStreamSynchronize(s));
+    for (uint32_t c = 0; c < k; ++c) mx[c] = hp[c];
+  }
+  UnpackSpec us{};
+  us.k = k;
+  uint32_t total_bits = 0;
+  for (int c = int(k) - 1; c >= 0; --c) {
+    uint32_t b = bit_width(mx[c]);
+    us.shift[c] = total_bits;
+    us.bits[c] = b;
+    ps.shift[c] = total_bits;
+    total_bits += b;
+  }
+  if (total_bits > 128) fail("build_index: packed key wider than 128 bits");
+  const int W = total_bits > 64 ? 2 : 1;
+  const uint32_t passes = (total_bits + kRadixBits - 1) / kRadixBits;
+  ps.passes = passes;
+
+  for (int b = 0; b < 2; ++b)
+    for (int wd = 0; wd < W; ++wd) ws.keys[b][wd].reserve(n, s);
+  fill_u32(hist, 0, uint64_t(kMaxPasses) * kRadix, s);
+  fill_u32(err, 0, 1, s);
+
+  // 2. pack + histograms
+  {
+    TimeScope t("sort.pack", s, double(n) * (4.0 * k + 8.0 * W));
+    uint32_t g = grid_for(n, kBlock * 8, 4096);
+    if (W == 1)
+      pack_hist_kernel<1><<<g, kBlock, 0, s>>>(ps, n, ws.keys[0][0].p, nullptr, hist, ctr);
+    else
+      pack_hist_kernel<2><<<g, kBlock, 0, s>>>(ps, n, ws.keys[0][0].p, ws.keys[0][1].p, hist, ctr);
+    SRD_CHECK(hipGetLastError());
+  }
+  int cur = 0;
+  if (passes > 0) {
+    bin_scan_kernel<<<passes, kRadix, 0, s>>>(hist, passes, starts);
+    SRD_CHECK(hipGetLastError());
+    // Skip passes whose digit is constant over all keys.
+    hp = pinned_words(ws, uint64_t(passes) * kRadix);
+    SRD_CHECK(hipMemcpyAsync(hp, hist, uint64_t(passes) * kRadix * 4, hipMemcpyDeviceToHost, s));
+    SRD_CHECK(hipStreamSynchronize(s));
+    std::vector<bool> trivial(passes, false);
+    for (uint32_t p = 0; p < passes; ++p)
+      for (int d = 0; d < kRadix; ++d)
+        if (hp[p * kRadix + d] == n) trivial[p] = true;
+
+    const uint32_t items = (W == 1) ? 16 : 8;
+    const uint64_t tile = uint64_t(kBlock) * items;
+    const uint64_t tiles = (n + tile - 1) / tile;
+    uint64_t lb_words = tiles * kRadix;
+    if (lb_words > ws.lookback.n) {
+      ws.lookback.alloc(lb_words + lb_words / 4, s);
+      fill_u64(ws.lookback.p, 0, ws.lookback.n, s);
+      ws.epoch = 0;
+    }
+    for (uint32_t p = 0; p < passes; ++p) {
+      if (trivial[p]) continue;
+      if (++ws.epoch >= 0xFFFF) {
+        fill_u64(ws.lookback.p, 0, ws.lookback.n, s);
+        ws.epoch = 1;
+      }
+      TimeScope t("sort.onesweep", s, double(n) * 16.0 * W);
+      const uint64_t* ilo = ws.keys[cur][0].p;
+      const uint64_t* ihi = W == 2 ? ws.keys[cur][1].p : nullptr;
+      uint64_t* olo = ws.keys[cur ^ 1][0].p;
+      uint64_t* ohi = W == 2 ? ws.keys[cur ^ 1][1].p : nullptr;
+      if (W == 1)
+        onesweep_kernel<1><<<uint32_t(tiles), kBlock, 0, s>>>(ilo, ihi, olo, ohi, uint32_t(n),
+                                                               p * kRadixBits, starts + p * kRadix,
+                                                               ws.lookback.p, ctr + p, ws.epoch, err);
+      else
+        onesweep_kernel<2><<<uint32_t(tiles), kBlock, 0, s>>>(ilo, ihi, olo, ohi, uint32_t(n),
+                                                               p * kRadixBits, starts + p * kRadix,
+                                                               ws.lookback.p, ctr + p, ws.epoch, err);
+      SRD_CHECK(hipGetLastError());
+      cur ^= 1;
+    }
+  }
+
+  // 3. dedup + unpack + roots
+  const uint64_t tiles = dtiles;
+  ws.tile_scan.reserve(2 * (tiles + 1), s);
+  uint64_t* su = ws.tile_scan.p;
+  uint64_t* sr = su + tiles + 1;
+  const uint64_t* klo = ws.keys[cur][0].p;
+  const uint64_t* khi = W == 2 ? ws.keys[cur][1].p : nullptr;
+  {
+    TimeScope t("sort.dedup_count", s, double(n) * 8 * W);
+    if (W == 1)
+      dedup_count_kernel<1><<<uint32_t(tiles), kBlock, 0, s>>>(klo, khi, n, us, tu, tr);
+    else
+      dedup_count_kernel<2><<<uint32_t(tiles), kBlock, 0, s>>>(klo, khi, n, us, tu, tr);
+    SRD_CHECK(hipGetLastError());
+  }
+  scan_impl<uint32_t>(tu, su, tiles, s);
+  scan_impl<uint32_t>(tr, sr, tiles, s);
+  uint64_t* h64 = reinterpret_cast<uint64_t*>(pinned_words(ws, 8));
+  SRD_CHECK(hipMemcpyAsync(&h64[0], su + tiles, 8, hipMemcpyDeviceToHost, s));
+  SRD_CHECK(hipMemcpyAsync(&h64[1], sr + tiles, 8, hipMemcpyDeviceToHost, s));
+  SRD_CHECK(hipMemcpyAsync(&h64[2], err, 4, hipMemcpyDeviceToHost, s));
+  SRD_CHECK(hipStreamSynchronize(s));
+  if (uint32_t(h64[2]) != 0) fail("build_index: radix sort look-back did not resolve");
+  const uint64_t u = h64[0], r = h64[1];
+  out.n = u;
+  out.r = uint32_t(r);
+  out.stride = round_stride(u);
+  out.cols.alloc(out.stride * k, s);
+  out.root_vals.alloc(r, s);
+  out.root_offs.alloc(r + 1, s);
+  {
+    TimeScope t("sort.dedup_write", s, double(n) * 8 * W + double(u) * 4 * k + double(r) * 8);
+    if (W == 1)
+      dedup_write_kernel<1><<<uint32_t(tiles), kBlock, 0, s>>>(klo, khi, n, us, su, sr, out.cols.p,
+                                                               out.stride, out.root_vals.p,
+                                                               out.root_offs.p);
+    else
+      dedup_write_kernel<2><<<uint32_t(tiles), kBlock, 0, s>>>(klo, khi, n, us, su, sr, out.cols.p,
+                                                               out.stride, out.root_vals.p,
+                                                               out.root_offs.p);
+    SRD_CHECK(hipGetLastError());
+  }
+  fill_u32(out.root_offs.p + r, uint32_t(u), 1, s);
+  out.col_max = mx;
+}
+
+void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index& out,
+                      Workspace& ws, hipStream_t s) {
+  if (perm.size() != src.arity) fail("build_index_from: permutation arity mismatch");
+  std::vector<const uint32_t*> in(src.arity);
+  std::vector<uint32_t> mx(src.arity);
+  for (uint32_t c = 0; c < src.arity; ++c) {
+    in[c] = src.col(perm[c]);
+    mx[c] = src.col_max.size() == src.arity ? src.col_max[perm[c]] : 0;
+  }
+  if (src.n == 0) {
+    out = Index();
+    out.arity = src.arity;
+    out.col_max.assign(src.arity, 0);
+    return;
+  }
+  bool known = src.col_max.size() == src.arity;
+  build_index(in.data(), src.arity, src.n, out, ws, s, known ? &mx : nullptr);
+}
+
+// ============================================================== roots ====
+__global__ __launch_bounds__(kBlock) void root_count_kernel(const uint32_t* __restrict__ c0,
+                                                            uint64_t n,
+                                                            uint32_t* __restrict__ tile_r) {
+  __shared__ uint32_t sr[kWavesPerBlock];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint32_t cr = 0;
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + tile_pos(w, i, l);
+    bool f = idx < n && (idx == 0 || c0[idx] != c0[idx - 1]);
+    cr += __popcll(ballot(f));
+  }
+  if (l == 0) sr[w] = cr;
+  __syncthreads();
+  if (threadIdx.x == 0) tile_r[blockIdx.x] = sr[0] + sr[1] + sr[2] + sr[3];
+}
+
+__global__ __launch_bounds__(kBlock) void root_write_kernel(const uint32_t* __restrict__ c0,
+                                                            uint64_t n,
+                                                            const uint64_t* __restrict__ roff,
+                                                            uint32_t* __restrict__ root_vals,
+                                                            uint32_t* __restrict__ root_offs) {
+  __shared__ uint32_t sr[kWavesPerBlock];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint64_t m[kItems];
+  uint32_t cr = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + tile_pos(w, i, l);
+    bool f = idx < n && (idx == 0 || c0[idx] != c0[idx - 1]);
+    m[i] = ballot(f);
+    cr += __popcll(m[i]);
+  }
+  if (l == 0) sr[w] = cr;
+  __syncthreads();
+  uint64_t pr = roff[blockIdx.x];
+  for (uint32_t j = 0; j < w; ++j) pr += sr[j];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if ((m[i] >> l) & 1) {
+      uint64_t idx = base + tile_pos(w, i, l);
+      uint64_t g = pr + popc_below(m[i]);
+      root_vals[g] = c0[idx];
+      root_offs[g] = uint32_t(idx);
+    }
+    pr += __popcll(m[i]);
+  }
+}
+
+// Build root arrays of an index whose columns are filled.
+void build_roots(Index& x, Workspace& ws, hipStream_t s) {
+  x.root_vals.release();
+  x.root_offs.release();
+  x.r = 0;
+  if (x.n == 0) return;
+  const uint64_t tiles = (x.n + kTile - 1) / kTile;
+  DevBuf<uint32_t> tr(tiles, s);
+  DevBuf<uint64_t> sr(tiles + 1, s);
+  {
+    TimeScope t("roots", s, double(x.n) * 8);
+    root_count_kernel<<<uint32_t(tiles), kBlock, 0, s>>>(x.col(0), x.n, tr.p);
+    SRD_CHECK(hipGetLastError());
+  }
+  scan_impl<uint32_t>(tr.p, sr.p, tiles, s);
+  uint64_t* h64 = reinterpret_cast<uint64_t*>(pinned_words(ws, 2));
+  SRD_CHECK(hipMemcpyAsync(h64, sr.p + tiles, 8, hipMemcpyDeviceToHost, s));
+  SRD_CHECK(hipStreamSynchronize(s));
+  x.r = uint32_t(h64[0]);
+  x.root_vals.alloc(x.r, s);
+  x.root_offs.alloc(x.r + 1, s);
+  {
+    TimeScope t("roots", s, double(x.r) * 8);
+    root_write_kernel<<<uint32_t(tiles), kBlock, 0, s>>>(x.col(0), x.n, sr.p, x.root_vals.p,
+                                                         x.root_offs.p);
+    SRD_CHECK(hipGetLastError());
+  }
+  fill_u32(x.root_offs.p + x.r, uint32_t(x.n), 1, s);
+}
+
+// ============================================ windowed lexicographic search ====
+// Stage capacity (rows) of the other input's window in LDS.
+constexpr uint32_t kStageRows = 2048;
+
+template <int K>
+struct RowK {
+  uint32_t v[K];
+};
+
+template <int K>
+__device__ __forceinline__ RowK<K> load_row(const uint32_t* cols, uint64_t stride, uint64_t i) {
+  RowK<K> r;
+#pragma unroll
+  for (int c = 0; c < K; ++c) r.v[c] = cols[uint64_t(c) * stride + i];
+  return r;
+}
+
+template <int K>
+__device__ __forceinline__ bool row_less(const RowK<K>& a, const RowK<K>& b) {
+#pragma unroll
+  for (int c = 0; c < K; ++c)
+    if (a.v[c] != b.v[c]) return a.v[c] < b.v[c];
+  return false;
+}
+
+template <int K>
+__device__ __forceinline__ bool row_eq(const RowK<K>& a, const RowK<K>& b) {
+#pragma unroll
+  for (int c = 0; c < K; ++c)
+    if (a.v[c] != b.v[c]) return false;
+  return true;
+}
+
+// First row in [lo,hi) of (cols,stride) not less than key.
+template <int K>
+__device__ __forceinline__ uint64_t lex_lower_bound(const uint32_t* cols, uint64_t stride,
+                                                    uint64_t lo, uint64_t hi, const RowK<K>& key) {
+  uint64_t n = hi - lo;
+  while (n > 0) {
+    uint64_t half = n >> 1, mid = lo + half;
+    if (row_less<K>(load_row<K>(cols, stride, mid), key)) {
+      lo = mid + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  return lo;
+}
+
+// First row in [lo,hi) greater than key.
+template <int K>
+__device__ __forceinline__ uint64_t lex_upper_bound(const uint32_t* cols, uint64_t stride,
+                                                    uint64_t lo, uint64_t hi, const RowK<K>& key) {
+  uint64_t n = hi - lo;
+  while (n > 0) {
+    uint64_t half = n >> 1, mid = lo + half;
+    if (!row_less<K>(key, load_row<K>(cols, stride, mid))) {
+      lo = mid + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  return lo;
+}
+
+// Per tile of `a` (kTile rows): window of `b` covering the tile's key range,
+// staged in LDS when it fits.  mode 0: flags[i] = a[i] not in b (set
+// difference); mode 1: scatter a[i] to out at i + #(b < a[i]) (merge).
+template <int K, int MODE>
+__global__ __launch_bounds__(kBlock) void window_kernel(
+    const uint32_t* __restrict__ a, uint64_t sa, uint64_t na, const uint32_t* __restrict__ b,
+    uint64_t sb, uint64_t nb, uint8_t* __restrict__ flags, uint32_t* __restrict__ tile_cnt,
+    uint32_t* __restrict__ out, uint64_t so) {
+  __shared__ uint32_t stage[K][kStageRows];
+  __shared__ uint64_t s_lo, s_hi;
+  __shared__ uint32_t s_cnt[kWavesPerBlock];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  const uint64_t last = min<uint64_t>(base + kTile, na) - 1;
+  if (threadIdx.x == 0) {
+    s_lo = lex_lower_bound<K>(b, sb, 0, nb, load_row<K>(a, sa, base));
+    s_hi = lex_upper_bound<K>(b, sb, s_lo, nb, load_row<K>(a, sa, last));
+  }
+  __syncthreads();
+  const uint64_t wlo = s_lo, whi = s_hi;
+  const bool staged = whi - wlo <= kStageRows;
+  if (staged) {
+    for (uint64_t j = threadIdx.x; j < whi - wlo; j += kBlock)
+#pragma unroll
+      for (int c = 0; c < K; ++c) stage[c][j] = b[uint64_t(c) * sb + wlo + j];
+  }
+  __syncthreads();
+  uint32_t cnt = 0;
+#pragma unroll 4
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + tile_pos(w, i, l);
+    bool keep = false;
+    if (idx < na) {
+      RowK<K> key = load_row<K>(a, sa, idx);
+      uint64_t pos;
+      if (staged) {
+        uint64_t lo = 0, n = whi - wlo;
+        while (n > 0) {
+          uint64_t half = n >> 1, mid = lo + half;
+          RowK<K> m;
+#pragma unroll
+          for (int c = 0; c < K; ++c) m.v[c] = stage[c][mid];
+          if (row_less<K>(m, key)) {
+            lo = mid + 1;
+            n -= half + 1;
+          } else {
+            n = half;
+          }
+        }
+        pos = wlo + lo;
+      } else {
+        pos = lex_lower_bound<K>(b, sb, wlo, whi, key);
+      }
+      if (MODE == 0) {
+        bool found = false;
+        if (pos < whi) {
+          RowK<K> m;
+          if (staged) {
+#pragma unroll
+            for (int c = 0; c < K; ++c) m.v[c] = stage[c][pos - wlo];
+          } else {
+            m = load_row<K>(b, sb, pos);
+          }
+          found = row_eq<K>(m, key);
+        }
+        keep = !found;
+        flags[idx] = keep ? 1 : 0;
+      } else {
+        uint64_t o = idx + pos;
+#pragma unroll
+        for (int c = 0; c < K; ++c) out[uint64_t(c) * so + o] = key.v[c];
+      }
+    }
+    if (MODE == 0) cnt += __popcll(ballot(keep));
+  }
+  if (MODE == 0) {
+    if (l == 0) s_cnt[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void compact_kernel(const uint32_t* __restrict__ a,
+                                                         uint64_t sa, uint64_t na,
+                                                         const uint8_t* __restrict__ flags,
+                                                         const uint64_t* __restrict__ toff,
+                                                         uint32_t* __restrict__ out,
+                                                         uint64_t so) {
+  __shared__ uint32_t s_cnt[kWavesPerBlock];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint64_t m[kItems];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + tile_pos(w, i, l);
+    m[i] = ballot(idx < na && flags[idx]);
+    cnt += __popcll(m[i]);
+  }
+  if (l == 0) s_cnt[w] = cnt;
+  __syncthreads();
+  uint64_t p = toff[blockIdx.x];
+  for (uint32_t j = 0; j < w; ++j) p += s_cnt[j];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if ((m[i] >> l) & 1) {
+      uint64_t idx = base + tile_pos(w, i, l);
+      uint64_t o = p + popc_below(m[i]);
+#pragma unroll
+      for (int c = 0; c < K; ++c) out[uint64_t(c) * so + o] = a[uint64_t(c) * sa + idx];
+    }
+    p += __popcll(m[i]);
+  }
+}
+
+template <int K>
+static void set_difference_k(const Index& a, const Index& b, Index& out, Workspace& ws,
+                             hipStream_t s) {
+  const uint64_t tiles = (a.n + kTile - 1) / kTile;
+  ws.flags.reserve(a.n, s);
+  DevBuf<uint32_t> tc(tiles, s);
+  DevBuf<uint64_t> to(tiles + 1, s);
+  {
+    TimeScope t("setdiff.probe", s, double(a.n + b.n) * 4 * K + double(a.n));
+    window_kernel<K, 0><<<uint32_t(tiles), kBlock, 0, s>>>(a.cols.p, a.stride, a.n, b.cols.p,
+                                                           b.stride, b.n, ws.flags.p, tc.p,
+                                                           nullptr, 0);
+    SRD_CHECK(hipGetLastError());
+  }
+  scan_impl<uint32_t>(tc.p, to.p, tiles, s);
+  uint64_t* h64 = reinterpret_cast<uint64_t*>(pinned_words(ws, 2));
+  SRD_CHECK(hipMemcpyAsync(h64, to.p + tiles, 8, hipMemcpyDeviceToHost, s));
+  SRD_CHECK(hipStreamSynchronize(s));
+  const uint64_t d = h64[0];
+  out = Index();
+  out.arity = K;
+  out.n = d;
+  out.stride = round_stride(d);
+  out.col_max = a.col_max;
+  if (d == 0) return;
+  out.cols.alloc(out.stride * K, s);
+  {
+    TimeScope t("setdiff.compact", s, double(a.n) * (1 + 4 * K) + double(d) * 4 * K);
+    compact_kernel<K><<<uint32_t(tiles), kBlock, 0, s>>>(a.cols.p, a.stride, a.n, ws.flags.p,
+                                                         to.p, out.cols.p, out.stride);
+    SRD_CHECK(hipGetLastError());
+  }
+  build_roots(out, ws, s);
+}
+
+void copy_index(const Index& a, Index& out, hipStream_t s) {
+  out = Index();
+  out.arity = a.arity;
+  out.n = a.n;
+  out.stride = a.stride;
+  out.r = a.r;
+  out.col_max = a.col_max;
+  if (a.n == 0) return;
+  out.cols.alloc(a.stride * a.arity, s);
+  out.root_vals.alloc(a.r, s);
+  out.root_offs.alloc(a.r + 1, s);
+  SRD_CHECK(hipMemcpyAsync(out.cols.p, a.cols.p, a.stride * a.arity * 4, hipMemcpyDeviceToDevice, s));
+  SRD_CHECK(hipMemcpyAsync(out.root_vals.p, a.root_vals.p, uint64_t(a.r) * 4,
+                           hipMemcpyDeviceToDevice, s));
+  SRD_CHECK(hipMemcpyAsync(out.root_offs.p, a.root_offs.p, uint64_t(a.r + 1) * 4,
+                           hipMemcpyDeviceToDevice, s));
+}
+
+void set_difference(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s) {
+  if (a.arity != b.arity && b.n) fail("set_difference: arity mismatch");
+  if (a.n == 0) {
+    out = Index();
+    out.arity = a.arity;
+    out.col_max = a.col_max;
+    return;
+  }
+  if (b.n == 0) {
+    copy_index(a, out, s);
+    return;
+  }
+  switch (a.arity) {
+    case 1: set_difference_k<1>(a, b, out, ws, s); break;
+    case 2: set_difference_k<2>(a, b, out, ws, s); break;
+    case 3: set_difference_k<3>(a, b, out, ws, s); break;
+    case 4: set_difference_k<4>(a, b, out, ws, s); break;
+    default: fail("set_difference: arity > 4 not supported");
+  }
+}
+
+template <int K>
+static void merge_k(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s) {
+  out = Index();
+  out.arity = K;
+  out.n = a.n + b.n;
+  out.stride = round_stride(out.n);
+  out.cols.alloc(out.stride * K, s);
+  out.col_max.assign(K, 0);
+  for (int c = 0; c < K; ++c)
+    out.col_max[c] = std::max(a.col_max.size() == K ? a.col_max[c] : 0u,
+                              b.col_max.size() == K ? b.col_max[c] : 0u);
+  {
+    TimeScope t("merge.scatter", s, double(out.n) * 8 * K);
+    if (a.n) {
+      uint64_t tiles = (a.n + kTile - 1) / kTile;
+      window_kernel<K, 1><<<uint32_t(tiles), kBlock, 0, s>>>(a.cols.p, a.stride, a.n, b.cols.p,
+                                                             b.stride, b.n, nullptr, nullptr,
+                                                             out.cols.p, out.stride);
+      SRD_CHECK(hipGetLastError());
+    }
+    if (b.n) {
+      uint64_t tiles = (b.n + kTile - 1) / kTile;
+      window_kernel<K, 1><<<uint32_t(tiles), kBlock, 0, s>>>(b.cols.p, b.stride, b.n, a.cols.p,
+                                                             a.stride, a.n, nullptr, nullptr,
+                                                             out.cols.p, out.stride);
+      SRD_CHECK(hipGetLastError());
+    }
+  }
+  build_roots(out, ws, s);
+}
+
+void merge_disjoint(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s) {
+  if (a.n == 0) {
+    copy_index(b, out, s);
+    return;
+  }
+  if (b.n == 0) {
+    copy_index(a, out, s);
+    return;
+  }
+  if (a.arity != b.arity) fail("merge_disjoint: arity mismatch");
+  switch (a.arity) {
+    case 1: merge_k<1>(a, b, out, ws, s); break;
+    case 2: merge_k<2>(a, b, out, ws, s); break;
+    case 3: merge_k<3>(a, b, out, ws, s); break;
+    case 4: merge_k<4>(a, b, out, ws, s); break;
+    default: fail("merge_disjoint: arity > 4 not supported");
+  }
+}
+
+}  // namespace srd

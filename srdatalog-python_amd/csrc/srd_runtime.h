@@ -1,0 +1,179 @@
+// Host-side API of the MI355X semi-naive runtime: device memory pool,
+// owned sorted indexes and the stream-ordered set primitives the
+// fixpoint loop is built from (sort+dedup, set difference, merge, scan).
+//
+// Counterparts in the reference runtime (runtime/generalized_datalog/):
+//   Pool            gpu/device_array_rmm_impl.h:53-160 (RMM pool)
+//   Index           gpu/device_sorted_array_index.h:337-643
+//   build_index     gpu/device_sorted_array_index.ipp:263-553 + gpu/sort.h:297
+//                   + gpu/deduplication.h:178-499
+//   set_difference  gpu/device_sorted_array_index.ipp:966-1140
+//   merge_disjoint  gpu/device_sorted_array_index.ipp:709-850
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "srd_device.h"
+
+namespace srd {
+
+[[noreturn]] void fail(const std::string& msg);
+void check(hipError_t e, const char* what);
+#define SRD_CHECK(x) ::srd::check((x), #x)
+
+// ------------------------------------------------------------------ pool --
+// Stream-ordered pool over hipMallocAsync.  Memory returned to the pool is
+// cached (release threshold = max) so steady-state iterations do not call
+// into the driver.  `budget` (bytes, 0 = unlimited) makes allocations beyond
+// it throw, which is how admission control sees memory pressure.
+struct PoolStats {
+  uint64_t in_use = 0;
+  uint64_t peak = 0;
+  uint64_t budget = 0;
+  uint64_t allocs = 0;
+};
+
+void pool_init(int device);
+void* pool_alloc(uint64_t bytes, hipStream_t s);
+void pool_free(void* p, uint64_t bytes, hipStream_t s);
+PoolStats pool_stats();
+void pool_set_budget(uint64_t bytes);
+void pool_reset_peak();
+
+// Owned device buffer allocated from the pool.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  uint64_t n = 0;
+  hipStream_t s = nullptr;
+  DevBuf() = default;
+  DevBuf(uint64_t count, hipStream_t st) { alloc(count, st); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s) { o.p = nullptr, o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p, n = o.n, s = o.s;
+      o.p = nullptr, o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(uint64_t count, hipStream_t st) {
+    release();
+    s = st;
+    n = count;
+    p = count ? static_cast<T*>(pool_alloc(count * sizeof(T), st)) : nullptr;
+  }
+  // Grow to at least `count` elements (contents not preserved).
+  void reserve(uint64_t count, hipStream_t st) {
+    if (count > n) alloc(count + count / 4, st);
+  }
+  void release() {
+    if (p) pool_free(p, n * sizeof(T), s);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// ----------------------------------------------------------------- index --
+// Sorted, duplicate-free k-column index with root (distinct column-0)
+// arrays; see srd_device.h for the layout.
+struct Index {
+  uint32_t arity = 0;
+  uint64_t n = 0;
+  uint64_t stride = 0;
+  uint32_t r = 0;
+  DevBuf<uint32_t> cols;       // arity * stride
+  DevBuf<uint32_t> root_vals;  // r
+  DevBuf<uint32_t> root_offs;  // r + 1
+  std::vector<uint32_t> col_max;  // per-column maximum (0 when empty)
+
+  IndexView view() const {
+    IndexView v;
+    v.cols = cols.p;
+    v.root_vals = root_vals.p;
+    v.root_offs = root_offs.p;
+    v.stride = stride;
+    v.n = uint32_t(n);
+    v.r = r;
+    v.arity = arity;
+    v.pad_ = 0;
+    return v;
+  }
+  const uint32_t* col(uint32_t c) const { return cols.p + uint64_t(c) * stride; }
+  uint32_t* col(uint32_t c) { return cols.p + uint64_t(c) * stride; }
+};
+
+inline uint64_t round_stride(uint64_t n) { return (n + 63) & ~uint64_t(63); }
+
+// Reusable scratch for the primitives (grown on demand, kept across calls).
+struct Workspace {
+  DevBuf<uint64_t> keys[2][2];   // [ping/pong][word]
+  DevBuf<uint64_t> lookback;     // onesweep inter-tile state
+  DevBuf<uint32_t> small;        // histograms, counters, tile totals
+  DevBuf<uint64_t> tile_scan;    // per-tile offsets
+  DevBuf<uint8_t> flags;
+  DevBuf<uint32_t> hostbuf_dev;  // staging for values read back
+  uint32_t epoch = 0;            // lookback epoch (avoids per-pass clears)
+  uint32_t* pinned = nullptr;    // pinned host words for read-backs
+  uint64_t pinned_words = 0;
+  ~Workspace();
+};
+
+uint32_t* pinned_words(Workspace& ws, uint64_t words);
+
+// Sort + dedup the k input columns (column c at in[c], n rows, taken in
+// the given order) into `out` (replaced).  `col_max` may give known upper
+// bounds per column (empty: computed on the device).
+void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace& ws,
+                 hipStream_t s, const std::vector<uint32_t>* col_max = nullptr);
+
+// Same, with the source columns permuted: out column c = src column perm[c].
+void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index& out,
+                      Workspace& ws, hipStream_t s);
+
+// out = a - b (both sorted and duplicate-free, same arity and column order).
+void set_difference(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s);
+
+// out = a u b for disjoint sorted inputs.
+void merge_disjoint(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s);
+
+// (Re)build root_vals / root_offs of an index whose columns are sorted.
+void build_roots(Index& x, Workspace& ws, hipStream_t s);
+
+// Deep copy.
+void copy_index(const Index& a, Index& out, hipStream_t s);
+
+// out[0..n] = exclusive prefix sums of in[0..n) with out[n] = total.
+void exclusive_scan_u32(const uint32_t* in, uint64_t* out, uint64_t n, Workspace& ws,
+                        hipStream_t s);
+void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace& ws,
+                        hipStream_t s);
+
+// Fill n words with value (a kernel, safe under stream capture).
+void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+void fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
+
+// Kernel timing hooks: when enabled, every primitive kernel launch is
+// bracketed by events and its duration accumulated under its name.
+struct KernelTime {
+  std::string name;
+  uint64_t launches = 0;
+  double ms = 0;
+  double bytes = 0;  // algorithmic bytes attributed by the caller
+};
+void timing_enable(bool on);
+bool timing_enabled();
+void timing_begin(const char* name, hipStream_t s);
+void timing_end(hipStream_t s, double alg_bytes = 0);
+std::vector<KernelTime> timing_collect();
+void timing_reset();
+
+}  // namespace srd

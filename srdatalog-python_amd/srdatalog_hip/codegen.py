@@ -1,0 +1,675 @@
+"""MIR -> HIP (gfx950) emitter.
+
+For every ``execute-pipeline`` of a MIR program this emits one root kernel
+and one slice kernel (instantiated for count and write), plus the host
+launchers and the ``SrdPipelineDesc`` table that the runtime's MIR executor
+drives (csrc/srd_pipeline.h).  The program's MIR text is embedded verbatim
+and the six ``srdatalog_*`` entry points of the reference's generated shim
+(src/srdatalog/codegen/jit/main_file.py:656-781) are emitted on top of the
+runtime engine.
+
+Kernel shape (wave64, see csrc/srd_join.h):
+
+* level 0  -- the root column-join (or a scan): one thread per distinct key
+  of the smallest root source finds the key in every root source and sizes
+  level 1 (``key_work``);
+* level 1  -- a column-join (candidates from the smallest source range, the
+  others binary-searched) or a cartesian product (mixed-radix decode),
+  flattened over all keys: one item per lane;
+* level 2  -- optional second column-join or cartesian product: each lane's
+  binding is published in LDS and the wave walks the concatenated level-2
+  spaces 64 items at a time.
+
+This mirrors the join semantics of the reference's emitted kernels
+(src/srdatalog/codegen/jit/pipeline.py:360, root.py, instructions.py):
+leapfrog intersection of sorted column ranges, prefix lookups, cartesian
+expansion, filters and insert-into, with a wave64-native work layout.
+"""
+
+from __future__ import annotations
+
+import re
+from dataclasses import dataclass, field
+
+from . import mir as M
+
+VER_CODE = {M.FULL: "SRD_FULL", M.DELTA: "SRD_DELTA", M.NEW: "SRD_NEW"}
+
+
+class CodegenError(NotImplementedError):
+  pass
+
+
+def cident(s: str) -> str:
+  return re.sub(r"[^A-Za-z0-9_]", "_", s)
+
+
+@dataclass
+class Rg:
+  """A row range of one view at a column depth (C++ expressions)."""
+
+  view: int
+  depth: int
+  b: str
+  e: str
+  full0: bool = False     # the whole index at depth 0 (distinct values = root_vals)
+  uniform: bool = False   # wave-uniform (not per lane)
+
+
+@dataclass
+class PipeInfo:
+  ep: M.ExecutePipeline
+  views: list[tuple[str, tuple[int, ...], str]] = field(default_factory=list)
+
+  def view_id(self, rel: str, index, ver: str) -> int:
+    key = (rel, tuple(index), ver)
+    if key not in self.views:
+      if len(self.views) >= 8:
+        raise CodegenError("more than 8 distinct source indexes in one pipeline")
+      self.views.append(key)
+    return self.views.index(key)
+
+
+class PipelineGen:
+  """Emits the kernels of one pipeline."""
+
+  def __init__(self, ep: M.ExecutePipeline, arity: dict[str, int], uid: int):
+    self.ep = ep
+    self.arity = arity
+    self.ns = f"srdp_{uid}_{cident(ep.rule)}"
+    self.info = PipeInfo(ep)
+    self.ops = list(ep.ops)
+    for op in self.ops:
+      if isinstance(op, M.Negation):
+        raise CodegenError(f"{ep.rule}: negation is not supported by the HIP emitter yet")
+      if not isinstance(op, (M.Scan, M.ColumnJoin, M.CartesianJoin, M.Filter, M.InsertInto)):
+        raise CodegenError(f"{ep.rule}: unsupported op {type(op).__name__}")
+    self.root = self.ops[0]
+    if not isinstance(self.root, (M.Scan, M.ColumnJoin)):
+      raise CodegenError(f"{ep.rule}: pipeline must start with a scan or a column-join")
+    self.levels = [op for op in self.ops[1:] if isinstance(op, (M.ColumnJoin, M.CartesianJoin))]
+    if isinstance(self.root, M.Scan) and self.levels:
+      raise CodegenError(f"{ep.rule}: joins after a scan root are not supported")
+    if len(self.levels) > 2:
+      raise CodegenError(f"{ep.rule}: more than two nested join levels")
+    for i, lv in enumerate(self.levels):
+      if isinstance(lv, M.CartesianJoin) and i != len(self.levels) - 1:
+        raise CodegenError(f"{ep.rule}: cartesian join must be the last join level")
+    self.filters = [op for op in self.ops if isinstance(op, M.Filter)]
+    self.dests = [op for op in self.ops if isinstance(op, M.InsertInto)]
+    if not self.dests:
+      raise CodegenError(f"{ep.rule}: pipeline without insert-into")
+    if len(self.dests) > 2:
+      raise CodegenError(f"{ep.rule}: more than two insert-into")
+    # view ids in source order
+    for s in ep.sources():
+      self.info.view_id(s.rel, s.index, s.ver)
+
+  # ------------------------------------------------------------ helpers --
+  def vid(self, s) -> int:
+    return self.info.view_id(s.rel, s.index, s.ver)
+
+  def view_arity(self, v: int) -> int:
+    rel, index, _ = self.info.views[v]
+    return len(index)
+
+  @staticmethod
+  def var(name: str) -> str:
+    return "v_" + cident(name)
+
+  def col_expr(self, v: int, depth: int) -> str:
+    return f"srdj::colp(V.v[{v}], {depth})"
+
+  def size_expr(self, r: Rg, rows: bool) -> str:
+    """Iteration size: distinct values (column join) or rows (cartesian)."""
+    if r.full0:
+      return f"V.v[{r.view}].n" if rows else f"V.v[{r.view}].r"
+    return f"({r.e} - {r.b})"
+
+  def find_expr(self, r: Rg, val: str) -> str:
+    """Rng of rows in r whose column r.depth equals val."""
+    if r.full0:
+      return f"srd::root_find(V.v[{r.view}], {val})"
+    last = "true" if r.depth == self.view_arity(r.view) - 1 else "false"
+    return f"srd::child_find(V.v[{r.view}], {r.depth}, srd::Rng{{{r.b}, {r.e}}}, {val}, {last})"
+
+  # Filters are applied as soon as all their vars are bound.
+  def filters_ready(self, bound: set[str], done: set[int]) -> list[tuple[int, M.Filter]]:
+    out = []
+    for i, f in enumerate(self.filters):
+      if i not in done and all(v in bound for v in f.vars):
+        out.append((i, f))
+    return out
+
+  def filter_code(self, f: M.Filter) -> str:
+    params = ", ".join(f"uint32_t {cident(v)}" for v in f.vars)
+    args = ", ".join(self.var(v) for v in f.vars)
+    return f"([&]({params}) -> bool {{ {f.code} }})({args})"
+
+  def emit_code(self, cond: str, ind: str) -> str:
+    lines = [f"{ind}{{",
+             f"{ind}  const uint64_t m_ = srd::ballot({cond});",
+             f"{ind}  if (MODE == 1 && ({cond})) {{",
+             f"{ind}    const uint64_t o_ = opos + srd::popc_below(m_);"]
+    for di, d in enumerate(self.dests):
+      for c, relcol in enumerate(d.index):
+        term = d.terms[relcol]
+        lines.append(f"{ind}    A.out[{di}].cols[{c} * A.out[{di}].stride + A.out[{di}].base + o_]"
+                     f" = {self.var(term)};")
+    lines += [f"{ind}  }}",
+              f"{ind}  const uint32_t c_ = __popcll(m_);",
+              f"{ind}  opos += c_;",
+              f"{ind}  cnt += c_;",
+              f"{ind}}}"]
+    return "\n".join(lines)
+
+  # ------------------------------------------------------------ ranges ---
+  def resolve(self, reg: dict, v: int, prefix: tuple, code: list[str], ind: str, tag: str) -> Rg:
+    """Range of view v with the given bound prefix, narrowing if needed."""
+    key = (v, prefix)
+    if key in reg:
+      return reg[key]
+    if not prefix:
+      r = Rg(v, 0, "0u", f"V.v[{v}].n", full0=True, uniform=True)
+      reg[key] = r
+      return r
+    parent = self.resolve(reg, v, prefix[:-1], code, ind, tag)
+    nm = f"nr_{tag}_{v}_{len(prefix)}"
+    code.append(f"{ind}const srd::Rng {nm} = {self.find_expr(parent, self.var(prefix[-1]))};")
+    r = Rg(v, len(prefix), f"{nm}.b", f"{nm}.e")
+    reg[key] = r
+    return r
+
+  # -------------------------------------------------------------- root ---
+  def root_slots(self):
+    """(view, prefix) ranges computed by the root kernel and stored per key."""
+    if isinstance(self.root, M.Scan):
+      return []
+    v0 = self.root.var
+    slots = []
+    for s in self.root.sources:
+      key = (self.vid(s), (v0,))
+      if key not in slots:
+        slots.append(key)
+    return slots
+
+  def gen_root(self) -> str:
+    out = [f"__global__ __launch_bounds__(256) void root_k(SrdLaunchArgs A) {{",
+           "  const SrdViews& V = A.views;",
+           "  const uint32_t k = blockIdx.x * 256u + threadIdx.x;",
+           "  if (k >= A.n_root) return;"]
+    if isinstance(self.root, M.Scan):
+      out += [f"  A.key_work[k] = V.v[{self.vid(self.root)}].n;", "  A.key_aux[k] = 0;", "}"]
+      return "\n".join(out)
+    v0 = self.root.var
+    slots = self.root_slots()
+    out += ["  const srd::IndexView& D = V.v[A.drv_view];",
+            "  const uint32_t key = D.root_vals[k];",
+            f"  const uint32_t {self.var(v0)} = key;",
+            "  bool ok = true;"]
+    reg = {}
+    for si, (v, pre) in enumerate(slots):
+      out.append(f"  const srd::Rng rs{si} = (A.drv_view == {v}u) ? srd::Rng{{D.root_offs[k], "
+                 f"D.root_offs[k + 1]}} : srd::root_find(V.v[{v}], key);")
+      out.append(f"  ok = ok && rs{si}.e > rs{si}.b;")
+      reg[(v, pre)] = Rg(v, 1, f"rs{si}.b", f"rs{si}.e")
+    out.append("  uint64_t work = 0;")
+    out.append("  uint32_t drv = 0;")
+    out.append("  if (ok) {")
+    body: list[str] = []
+    if not self.levels:
+      body.append("    work = 1;")
+    else:
+      lv = self.levels[0]
+      rngs = []
+      for s in lv.sources:
+        rngs.append(self.resolve(reg, self.vid(s), tuple(s.prefix), body, "    ", "r"))
+      if isinstance(lv, M.ColumnJoin):
+        body.append(f"    uint64_t best = {self.size_expr(rngs[0], False)};")
+        for j in range(1, len(rngs)):
+          body.append(f"    {{ const uint64_t z = {self.size_expr(rngs[j], False)}; "
+                      f"if (z < best) {{ best = z; drv = {j}; }} }}")
+        body.append("    work = best;")
+      else:
+        body.append("    work = 1;")
+        for r in rngs:
+          body.append(f"    work *= uint64_t({self.size_expr(r, True)});")
+    out += body
+    out.append("  }")
+    for si in range(len(slots)):
+      out.append(f"  A.key_rng[uint64_t(k) * {2 * len(slots)} + {2 * si}] = rs{si}.b;")
+      out.append(f"  A.key_rng[uint64_t(k) * {2 * len(slots)} + {2 * si + 1}] = rs{si}.e;")
+    out += ["  A.key_aux[k] = drv;", "  A.key_work[k] = work;", "}"]
+    return "\n".join(out)
+
+  # -------------------------------------------------------- level code ---
+  def gen_cj_candidate(self, lv: M.ColumnJoin, rngs: list[Rg], drv_expr: str, item: str,
+                       hit: str, ind: str, tag: str) -> tuple[list[str], dict]:
+    """Code evaluating one candidate of a column join; returns code and the
+    child ranges registered for (view, prefix + var)."""
+    var = self.var(lv.var)
+    code = [f"{ind}uint32_t {var} = 0;"]
+    child = {}
+    for j, (s, r) in enumerate(zip(lv.sources, rngs)):
+      code.append(f"{ind}uint32_t {tag}c{j}b = 0, {tag}c{j}e = 0;")
+      child[(self.vid(s), tuple(s.prefix) + (lv.var,))] = Rg(self.vid(s), len(s.prefix) + 1,
+                                                             f"{tag}c{j}b", f"{tag}c{j}e")
+    code.append(f"{ind}switch ({drv_expr}) {{")
+    for j, r in enumerate(rngs):
+      code.append(f"{ind}  case {j}: {{")
+      i2 = ind + "    "
+      if r.full0:
+        code.append(f"{i2}const uint32_t pos_ = uint32_t({item});")
+        code.append(f"{i2}const uint32_t val_ = V.v[{r.view}].root_vals[pos_];")
+        code.append(f"{i2}const srd::Rng cd_ = srd::Rng{{V.v[{r.view}].root_offs[pos_], "
+                    f"V.v[{r.view}].root_offs[pos_ + 1]}};")
+        code.append(f"{i2}const bool first_ = true;")
+      else:
+        col = self.col_expr(r.view, r.depth)
+        last = r.depth == self.view_arity(r.view) - 1
+        code.append(f"{i2}const uint32_t* col_ = {col};")
+        code.append(f"{i2}const uint32_t pos_ = {r.b} + uint32_t({item});")
+        code.append(f"{i2}const uint32_t val_ = col_[pos_];")
+        if last:
+          code.append(f"{i2}const bool first_ = true;")
+          code.append(f"{i2}const srd::Rng cd_ = srd::Rng{{pos_, pos_ + 1}};")
+        else:
+          code.append(f"{i2}const bool first_ = pos_ == {r.b} || col_[pos_ - 1] != val_;")
+          code.append(f"{i2}const srd::Rng cd_ = srdj::run_at(col_, pos_, {r.e}, val_, false);")
+      code.append(f"{i2}if (first_) {{")
+      i3 = i2 + "  "
+      code.append(f"{i3}bool ok_ = true;")
+      for m, rm in enumerate(rngs):
+        if m == j:
+          continue
+        code.append(f"{i3}const srd::Rng cf{m}_ = ok_ ? {self.find_expr(rm, 'val_')} : srd::Rng{{0, 0}};")
+        code.append(f"{i3}ok_ = ok_ && cf{m}_.e > cf{m}_.b;")
+      code.append(f"{i3}if (ok_) {{")
+      code.append(f"{i3}  {hit} = true;")
+      code.append(f"{i3}  {var} = val_;")
+      for m in range(len(rngs)):
+        src = "cd_" if m == j else f"cf{m}_"
+        code.append(f"{i3}  {tag}c{m}b = {src}.b; {tag}c{m}e = {src}.e;")
+      code.append(f"{i3}}}")
+      code.append(f"{i2}}}")
+      code.append(f"{i2}break;")
+      code.append(f"{ind}  }}")
+    code.append(f"{ind}}}")
+    return code, child
+
+  def gen_cart(self, lv: M.CartesianJoin, rngs: list[Rg], item: str, guard: str, ind: str,
+               tag: str) -> list[str]:
+    """Mixed-radix decode of a flattened cartesian item (last source fastest);
+    reads happen only under `guard`, where every range is non-empty."""
+    if not lv.var_from_source or len(lv.var_from_source) != len(rngs):
+      raise CodegenError(f"{self.ep.rule}: cartesian join without :var-from-source")
+    code = []
+    for vs in lv.var_from_source:
+      for vname in vs:
+        code.append(f"{ind}uint32_t {self.var(vname)} = 0;")
+    code.append(f"{ind}if ({guard}) {{")
+    i2 = ind + "  "
+    code.append(f"{i2}uint64_t rem_{tag} = {item};")
+    for j in reversed(range(len(rngs))):
+      z = f"uint64_t({self.size_expr(rngs[j], True)})"
+      code.append(f"{i2}const uint32_t off{tag}{j} = uint32_t(rem_{tag} % {z});")
+      code.append(f"{i2}rem_{tag} /= {z};")
+    for j, (vs, r) in enumerate(zip(lv.var_from_source, rngs)):
+      code.append(f"{i2}const uint32_t row{tag}{j} = {r.b} + off{tag}{j};")
+      for t, vname in enumerate(vs):
+        code.append(f"{i2}{self.var(vname)} = {self.col_expr(r.view, r.depth + t)}[row{tag}{j}];")
+    code.append(f"{ind}}}")
+    return code
+
+  # ------------------------------------------------------- slice kernel ---
+  def gen_run(self) -> str:
+    o = ["template <int MODE>",
+         "__global__ __launch_bounds__(256) void run_k(SrdLaunchArgs A) {",
+         "  const SrdViews& V = A.views;",
+         "  const uint32_t lane = srd::lane_id();"]
+    two = len(self.levels) == 2
+    nfr = 0
+    if two:
+      o += ["  const uint32_t wv = threadIdx.x / 64u;",
+            "  __shared__ uint64_t pref_s[4][65];"]
+    slots = self.root_slots()
+    ns = len(slots)
+    o += ["  for (;;) {",
+          "    const uint64_t slice = srdj::next_slice(A.work_ctr);",
+          "    if (slice >= A.n_slices) break;",
+          "    const uint64_t it0 = slice * A.grain;",
+          "    const uint64_t it1 = min(it0 + A.grain, A.total_work);",
+          "    uint64_t opos = MODE ? A.offsets[slice] : 0;",
+          "    uint32_t cnt = 0;",
+          "    for (uint64_t base = it0; base < it1; base += 64) {",
+          "      const uint64_t item = base + lane;",
+          "      const bool act = item < it1;",
+          "      const uint64_t last_item = min(base + 63, it1 - 1);",
+          "      const uint32_t klo = srdj::find_slot(A.work_start, A.n_root, base);",
+          "      const uint32_t khi = srdj::find_slot(A.work_start, A.n_root, last_item) + 1;",
+          "      uint32_t k = klo;",
+          "      if (act && khi - klo > 1) {",
+          "        uint32_t lo = klo, hi = khi;",
+          "        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1;"
+          " if (A.work_start[mid] <= item) lo = mid; else hi = mid; }",
+          "        k = lo;",
+          "      }",
+          "      const uint64_t li = act ? item - A.work_start[k] : 0;"]
+    ind = "      "
+    bound: set[str] = set()
+    done: set[int] = set()
+    reg: dict = {}
+    if isinstance(self.root, M.Scan):
+      sv = self.vid(self.root)
+      o.append(f"{ind}const uint32_t row0 = uint32_t(li);")
+      for t, vname in enumerate(self.root.vars):
+        o.append(f"{ind}const uint32_t {self.var(vname)} = act ? {self.col_expr(sv, t)}[row0] : 0u;")
+        bound.add(vname)
+      o.append(f"{ind}bool h1 = act;")
+    else:
+      v0 = self.root.var
+      o.append(f"{ind}const uint32_t {self.var(v0)} = act ? V.v[A.drv_view].root_vals[k] : 0u;")
+      bound.add(v0)
+      for si, (v, pre) in enumerate(slots):
+        o.append(f"{ind}const uint32_t rs{si}b = act ? A.key_rng[uint64_t(k) * {2 * ns} + {2 * si}] : 0u;")
+        o.append(f"{ind}const uint32_t rs{si}e = act ? A.key_rng[uint64_t(k) * {2 * ns} + {2 * si + 1}] : 0u;")
+        reg[(v, pre)] = Rg(v, 1, f"rs{si}b", f"rs{si}e")
+      o.append(f"{ind}const uint32_t l1drv = act ? A.key_aux[k] : 0u;")
+      o.append(f"{ind}bool h1 = false;")
+      if not self.levels:
+        o.append(f"{ind}h1 = act;")
+      else:
+        lv = self.levels[0]
+        rngs = [self.resolve(reg, self.vid(s), tuple(s.prefix), o, ind, "a") for s in lv.sources]
+        if isinstance(lv, M.ColumnJoin):
+          o.append(f"{ind}if (act) {{")
+          code, child = self.gen_cj_candidate(lv, rngs, "l1drv", "li", "h1", ind + "  ", "l1")
+          # declarations must be visible after the if: hoist them
+          decls = [c for c in code if c.strip().startswith("uint32_t ")]
+          rest = [c for c in code if not c.strip().startswith("uint32_t ")]
+          o[-1:-1] = [d.replace(ind + "  ", ind, 1) for d in decls]
+          o += rest
+          o.append(f"{ind}}}")
+          reg.update(child)
+          bound.add(lv.var)
+        else:
+          o.append(f"{ind}h1 = act;")
+          o += self.gen_cart(lv, rngs, "li", "act", ind, "a")
+          for j, vs in enumerate(lv.var_from_source):
+            bound.update(vs)
+    for i, f in self.filters_ready(bound, done):
+      o.append(f"{ind}h1 = h1 && {self.filter_code(f)};")
+      done.add(i)
+    if not two:
+      if len(done) != len(self.filters):
+        raise CodegenError(f"{self.ep.rule}: filter over unbound variables")
+      o.append(self.emit_code("h1", ind))
+    else:
+      lv2 = self.levels[1]
+      l1var = self.levels[0].var
+      # frame: level-1 var + ranges of level-2 sources
+      rngs2 = [self.resolve(reg, self.vid(s), tuple(s.prefix), o, ind, "b") for s in lv2.sources]
+      frame = [self.var(l1var)]
+      per_lane = []
+      for j, r in enumerate(rngs2):
+        if r.full0:
+          per_lane.append(None)
+        else:
+          per_lane.append((len(frame), len(frame) + 1))
+          frame += [r.b, r.e]
+      # vars bound at level 0 that are per-lane (root var) and needed later
+      v0name = None if isinstance(self.root, M.Scan) else self.root.var
+      if v0name is not None:
+        frame.append(self.var(v0name))
+      nfr = len(frame)
+      o.insert(5, f"  __shared__ uint32_t fr_s[4][{nfr + 1}][64];")
+      # level-2 size per lane
+      o.append(f"{ind}uint64_t s2 = 0;")
+      o.append(f"{ind}uint32_t d2 = 0;")
+      o.append(f"{ind}if (h1) {{")
+      if isinstance(lv2, M.ColumnJoin):
+        o.append(f"{ind}  s2 = {self.size_expr(rngs2[0], False)};")
+        for j in range(1, len(rngs2)):
+          o.append(f"{ind}  {{ const uint64_t z = {self.size_expr(rngs2[j], False)}; "
+                   f"if (z < s2) {{ s2 = z; d2 = {j}; }} }}")
+      else:
+        o.append(f"{ind}  s2 = 1;")
+        for r in rngs2:
+          o.append(f"{ind}  s2 *= uint64_t({self.size_expr(r, True)});")
+      o.append(f"{ind}}}")
+      o.append(f"{ind}const uint64_t inc2 = srd::wave_incl_scan<uint64_t>(s2);")
+      o.append(f"{ind}const uint64_t tot2 = __shfl(inc2, 63, 64);")
+      o.append(f"{ind}pref_s[wv][lane + 1] = inc2;")
+      o.append(f"{ind}if (lane == 0) pref_s[wv][0] = 0;")
+      for fi, expr in enumerate(frame):
+        o.append(f"{ind}fr_s[wv][{fi}][lane] = {expr};")
+      o.append(f"{ind}fr_s[wv][{nfr}][lane] = d2;")
+      o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
+      o.append(f"{ind}__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
+      o.append(f"{ind}for (uint64_t j0 = 0; j0 < tot2; j0 += 64) {{")
+      i2 = ind + "  "
+      o.append(f"{i2}const uint64_t jj = j0 + lane;")
+      o.append(f"{i2}const bool act2 = jj < tot2;")
+      o.append(f"{i2}const uint32_t own = act2 ? srdj::owner_of(pref_s[wv], jj) : 0u;")
+      o.append(f"{i2}const uint64_t loc = act2 ? jj - pref_s[wv][own] : 0;")
+      # reload frame values of the owner lane under fresh names
+      o.append(f"{i2}const uint32_t {self.var(l1var)}_o = fr_s[wv][0][own];")
+      rngs2o = []
+      for j, r in enumerate(rngs2):
+        if per_lane[j] is None:
+          rngs2o.append(r)
+        else:
+          bi, ei = per_lane[j]
+          o.append(f"{i2}const uint32_t f{j}b = fr_s[wv][{bi}][own];")
+          o.append(f"{i2}const uint32_t f{j}e = fr_s[wv][{ei}][own];")
+          rngs2o.append(Rg(r.view, r.depth, f"f{j}b", f"f{j}e"))
+      if v0name is not None:
+        o.append(f"{i2}const uint32_t {self.var(v0name)}_o = fr_s[wv][{nfr - 1}][own];")
+      o.append(f"{i2}const uint32_t d2o = fr_s[wv][{nfr}][own];")
+      # shadow the bound vars with the owner's values inside this scope
+      o.append(f"{i2}{{")
+      i3 = i2 + "  "
+      o.append(f"{i3}const uint32_t {self.var(l1var)} = {self.var(l1var)}_o;")
+      if v0name is not None:
+        o.append(f"{i3}const uint32_t {self.var(v0name)} = {self.var(v0name)}_o;")
+      o.append(f"{i3}bool h2 = false;")
+      if isinstance(lv2, M.ColumnJoin):
+        o.append(f"{i3}if (act2) {{")
+        code, child = self.gen_cj_candidate(lv2, rngs2o, "d2o", "loc", "h2", i3 + "  ", "l2")
+        decls = [c for c in code if c.strip().startswith("uint32_t ")]
+        rest = [c for c in code if not c.strip().startswith("uint32_t ")]
+        o[-1:-1] = [d.replace(i3 + "  ", i3, 1) for d in decls]
+        o += rest
+        o.append(f"{i3}}}")
+        bound.add(lv2.var)
+      else:
+        o.append(f"{i3}h2 = act2;")
+        o += self.gen_cart(lv2, rngs2o, "loc", "act2", i3, "b")
+        for vs in lv2.var_from_source:
+          bound.update(vs)
+      for i, f in self.filters_ready(bound, done):
+        o.append(f"{i3}h2 = h2 && {self.filter_code(f)};")
+        done.add(i)
+      if len(done) != len(self.filters):
+        raise CodegenError(f"{self.ep.rule}: filter over unbound variables")
+      o.append(self.emit_code("h2", i3))
+      o.append(f"{i2}}}")
+      o.append(f"{ind}}}")
+      o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
+    o += ["    }",
+          "    if (MODE == 0 && lane == 0) A.counts[slice] = cnt;",
+          "  }",
+          "}"]
+    return "\n".join(o)
+
+  # --------------------------------------------------------- assembly ----
+  def gen_kernels(self) -> str:
+    out = [f"namespace {self.ns} {{", self.gen_root(), "", self.gen_run(), "",
+           "void launch_root(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  root_k<<<(a->n_root + 255u) / 256u, 256, 0, s>>>(*a);",
+           "}",
+           "void launch_count(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<0><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}",
+           "void launch_write(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<1><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}",
+           f"}}  // namespace {self.ns}", ""]
+    return "\n".join(out)
+
+  def gen_desc(self) -> str:
+    views = []
+    for rel, index, ver in self.info.views:
+      cols = ", ".join(str(c) for c in index)
+      views.append(f"{{\"{rel}\", {len(index)}, {{{cols}}}, {VER_CODE[ver]}}}")
+    if isinstance(self.root, M.Scan):
+      kind, roots = "SRD_ROOT_SCAN", [self.vid(self.root)]
+    else:
+      kind, roots = "SRD_ROOT_JOIN", [self.vid(s) for s in self.root.sources]
+    dests = []
+    for d in self.dests:
+      cols = ", ".join(str(c) for c in d.index)
+      dests.append(f"{{\"{d.rel}\", {len(d.index)}, {{{cols}}}}}")
+    grain = 256 if len(self.levels) == 2 else 4096
+    return ("  {" + f"\"{self.ep.rule}\", {len(views)}, {{{', '.join(views)}}}, {kind}, "
+            f"{len(roots)}, {{{', '.join(str(r) for r in roots)}}}, {len(dests)}, "
+            f"{{{', '.join(dests)}}}, {1 if self.ep.count else 0}, {grain}, "
+            f"&{self.ns}::launch_root, &{self.ns}::launch_count, &{self.ns}::launch_write" + "}")
+
+
+@dataclass
+class RelDecl:
+  name: str
+  arity: int
+  input_file: str = ""
+  print_size: bool = False
+
+
+def program_text(decls: list[RelDecl], prog: M.Program) -> str:
+  lines = ["(relations"]
+  for d in decls:
+    s = f"  (relation :name {d.name} :arity {d.arity}"
+    if d.input_file:
+      s += f' :input-file "{d.input_file}"'
+    if d.print_size:
+      s += " :print-size true"
+    lines.append(s + ")")
+  lines.append(")")
+  return "\n".join(lines) + "\n" + M.print_program(prog) + "\n"
+
+
+SHIM = r"""
+// ======== extern "C" shim (same symbols as the reference's generated shim) ========
+static srd_engine* g_eng = nullptr;
+
+static bool ensure_engine() {
+  if (!g_eng) g_eng = srd_engine_create(k_program, k_pipes, sizeof(k_pipes) / sizeof(k_pipes[0]));
+  return g_eng && srd_engine_ok(g_eng);
+}
+
+extern "C" {
+
+int srdatalog_init(void) {
+  try {
+    return ensure_engine() ? 0 : 1;
+  } catch (...) {
+    return 2;
+  }
+}
+
+int srdatalog_load_csv(const char* rel_name, const char* path) {
+  if (!rel_name || !path) return 1;
+  if (!ensure_engine()) return 3;
+  return srd_engine_load_csv(g_eng, rel_name, path);
+}
+
+int srdatalog_load_all(const char* data_dir) {
+  if (!data_dir) return 1;
+  if (!ensure_engine()) return 2;
+  return srd_engine_load_all(g_eng, data_dir);
+}
+
+int srdatalog_run(unsigned long long max_iters) {
+  if (!g_eng) return 1;
+  return srd_engine_run(g_eng, max_iters);
+}
+
+unsigned long long srdatalog_size(const char* rel_name) {
+  if (!g_eng || !rel_name) return 0;
+  return srd_engine_size(g_eng, rel_name);
+}
+
+int srdatalog_shutdown(void) {
+  if (g_eng) {
+    srd_engine_destroy(g_eng);
+    g_eng = nullptr;
+  }
+  return 0;
+}
+
+int srdatalog_export(const char* rel_name, uint32_t* out, uint64_t cap_rows, uint64_t* n_rows) {
+  if (!rel_name || !n_rows) return 1;
+  if (!g_eng) return 2;
+  return srd_engine_export(g_eng, rel_name, out, cap_rows, n_rows);
+}
+
+int srdatalog_delta_sizes(const char* rel_name, unsigned long long* out, unsigned long long cap) {
+  if (!g_eng || !rel_name) return -1;
+  return srd_engine_delta_sizes(g_eng, rel_name, out, cap);
+}
+
+unsigned long long srdatalog_num_iterations(const char* rel_name) {
+  if (!g_eng || !rel_name) return 0;
+  return srd_engine_num_iterations(g_eng, rel_name);
+}
+
+int srdatalog_load_rows(const char* rel_name, const uint32_t* rows, uint64_t n, uint32_t arity) {
+  if (!rel_name) return 1;
+  if (!ensure_engine()) return 3;
+  return srd_engine_load_rows(g_eng, rel_name, rows, n, arity);
+}
+
+int srdatalog_prepare(void) {
+  if (!ensure_engine()) return 2;
+  return srd_engine_prepare(g_eng);
+}
+
+double srdatalog_run_seconds(void) { return g_eng ? srd_engine_run_seconds(g_eng) : 0.0; }
+
+unsigned long long srdatalog_join_tuples(void) { return g_eng ? srd_engine_join_tuples(g_eng) : 0; }
+
+void srdatalog_set_grain(unsigned grain) {
+  if (ensure_engine()) srd_engine_set_grain(g_eng, grain);
+}
+
+}  // extern "C"
+"""
+
+
+def generate_program_source(name: str, decls: list[RelDecl], prog: M.Program) -> str:
+  """Complete HIP translation unit of a program."""
+  arity = {d.name: d.arity for d in decls}
+  gens = [PipelineGen(ep, arity, i) for i, ep in enumerate(prog.pipelines())]
+  seen = set()
+  for g in gens:
+    if g.ep.rule in seen:
+      raise CodegenError(f"duplicate pipeline rule name {g.ep.rule}")
+    seen.add(g.ep.rule)
+  text = program_text(decls, prog)
+  if ")SRDMIR\"" in text:
+    raise CodegenError("program text contains the raw-string delimiter")
+  out = [f"// Generated by srdatalog_hip.codegen for program {name}. Do not edit.",
+         "#include <hip/hip_runtime.h>",
+         "#include <stdint.h>",
+         '#include "srd_join.h"',
+         '#include "srd_engine.h"',
+         '#include "srd_pipeline.h"',
+         ""]
+  for g in gens:
+    out.append(g.gen_kernels())
+  out.append("static const SrdPipelineDesc k_pipes[] = {")
+  out.append(",\n".join(g.gen_desc() for g in gens) if gens else "")
+  out.append("};")
+  out.append(f'static const char* k_program = R"SRDMIR({text})SRDMIR";')
+  out.append(SHIM)
+  return "\n".join(out)

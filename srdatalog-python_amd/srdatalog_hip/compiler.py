@@ -1,0 +1,633 @@
+"""DSL Program -> MIR, following the reference's planning semantics.
+
+The plan this produces is the contract the backend executes, so it must be
+the plan the reference would run: same strata, semi-naive variants, join
+order, column indexes and maintenance op order.  The stages restate the
+reference pipeline (src/srdatalog/hir/__init__.py compile_to_mir):
+
+  stratify         Tarjan SCCs over head->body IDB edges, visited in rule
+                   order with sorted neighbours; a recursive SCC's base
+                   rules become their own non-recursive stratum first;
+                   consecutive independent non-recursive strata fuse
+                   (hir/stratify.py:225-254)
+  variants         base variant per non-recursive rule; one DELTA variant
+                   per SCC atom of a recursive rule (hir/semi_naive.py:47)
+  planning         clause order (filters first, then the delta atom, then
+                   max overlap with bound vars), var order (delta atom's
+                   join vars, other join vars, free vars), user plans,
+                   per-atom access order / index columns (hir/plan.py)
+  index selection  required indexes per SCC member and a canonical index,
+                   preferring one read as FULL (hir/index.py:104)
+  lowering         column-join per join var, one cartesian-join for the
+                   free vars, filters, insert-into; maintenance op order
+                   (hir/lower.py:487-926)
+  MIR passes       rebuilds before reconstruct, source reorder by clause
+                   order then prefixed-first (mir/passes.py:104-340)
+
+The result prints byte-identically to the reference's MIR goldens
+(tests/test_compiler_golden.py).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+from . import mir as M
+from .codegen import RelDecl
+from .dsl import Atom, Filter, Negation, Program, Rule
+
+
+class CompileError(ValueError):
+  pass
+
+
+# -------------------------------------------------------------- strata ----
+@dataclass
+class Stratum:
+  members: set[str]
+  recursive: bool
+  rules: list[Rule]
+  base_variants: list["Variant"] = field(default_factory=list)
+  rec_variants: list["Variant"] = field(default_factory=list)
+  required: dict[str, list[list[int]]] = field(default_factory=dict)
+  canonical: dict[str, list[int]] = field(default_factory=dict)
+
+
+def _body_rels(rule: Rule) -> set[str]:
+  out = set()
+  for c in rule.body:
+    if isinstance(c, Atom):
+      out.add(c.rel)
+    elif isinstance(c, Negation):
+      out.add(c.atom.rel)
+  return out
+
+
+def stratify(rules: list[Rule]) -> list[Stratum]:
+  idbs = {h.rel for r in rules for h in r.heads}
+  deps: dict[str, set[str]] = {rel: set() for rel in idbs}
+  for r in rules:
+    b = _body_rels(r) & idbs
+    for h in r.heads:
+      deps[h.rel] |= b
+
+  # Tarjan, iterative over an explicit stack; emits SCCs bottom-up.
+  order, low, on, stack, sccs = {}, {}, set(), [], []
+  counter = 0
+
+  def visit(root: str) -> None:
+    nonlocal counter
+    work = [(root, iter(sorted(deps[root])))]
+    order[root] = low[root] = counter
+    counter += 1
+    stack.append(root)
+    on.add(root)
+    while work:
+      v, it = work[-1]
+      advanced = False
+      for w in it:
+        if w not in order:
+          order[w] = low[w] = counter
+          counter += 1
+          stack.append(w)
+          on.add(w)
+          work.append((w, iter(sorted(deps[w]))))
+          advanced = True
+          break
+        if w in on:
+          low[v] = min(low[v], order[w])
+      if advanced:
+        continue
+      work.pop()
+      if work:
+        u = work[-1][0]
+        low[u] = min(low[u], low[v])
+      if low[v] == order[v]:
+        comp = set()
+        while True:
+          w = stack.pop()
+          on.discard(w)
+          comp.add(w)
+          if w == v:
+            break
+        sccs.append(comp)
+
+  for r in rules:
+    for h in r.heads:
+      if h.rel not in order:
+        visit(h.rel)
+
+  strata: list[Stratum] = []
+  for comp in sccs:
+    comp_rules = [r for r in rules if any(h.rel in comp for h in r.heads)]
+    if not comp_rules:
+      continue
+    rec = len(comp) > 1 or any(rel in deps[rel] for rel in comp)
+    if rec:
+      base = [r for r in comp_rules if not (_body_rels(r) & comp)]
+      recr = [r for r in comp_rules if _body_rels(r) & comp]
+      if base:
+        strata.append(Stratum(set(comp), False, base))
+      if recr:
+        strata.append(Stratum(set(comp), True, recr))
+    else:
+      strata.append(Stratum(set(comp), False, comp_rules))
+  return _fuse(strata)
+
+
+def _fuse(strata: list[Stratum]) -> list[Stratum]:
+  rec_members = set().union(*[s.members for s in strata if s.recursive]) if strata else set()
+
+  def fusable(s):
+    return not s.recursive and not (s.members & rec_members)
+
+  out, i = [], 0
+  while i < len(strata):
+    cur = strata[i]
+    if not fusable(cur):
+      out.append(cur)
+      i += 1
+      continue
+    produced = set(cur.members)
+    j = i + 1
+    while j < len(strata) and fusable(strata[j]):
+      nxt = strata[j]
+      if any(_body_rels(r) & produced for r in nxt.rules):
+        break
+      cur.members |= nxt.members
+      cur.rules.extend(nxt.rules)
+      produced = set(cur.members)
+      j += 1
+    out.append(cur)
+    i = j
+  return out
+
+
+# ------------------------------------------------------------ planning ----
+@dataclass
+class Access:
+  rel: str
+  ver: str
+  order: list[str]        # variables in access order
+  index: list[int]        # column permutation
+  prefix_len: int
+  clause: int
+
+
+@dataclass
+class Variant:
+  rule: Rule
+  delta: int
+  versions: list[str]
+  clause_order: list[int] = field(default_factory=list)
+  var_order: list[str] = field(default_factory=list)
+  join_vars: set[str] = field(default_factory=set)
+  atoms: list[Access] = field(default_factory=list)
+  negs: list[Access] = field(default_factory=list)
+
+
+def _clause_vars(c) -> list[str]:
+  if isinstance(c, Atom):
+    return list(c.args)
+  if isinstance(c, Negation):
+    return list(c.atom.args)
+  if isinstance(c, Filter):
+    return list(c.vars)
+  return []
+
+
+def _needs(c) -> set[str]:
+  if isinstance(c, (Negation, Filter)):
+    return set(_clause_vars(c))
+  return set()
+
+
+def _binds(c) -> set[str]:
+  return set(c.args) if isinstance(c, Atom) else set()
+
+
+def _join_vars(rule: Rule) -> set[str]:
+  seen: dict[str, int] = {}
+  for c in rule.body:
+    if isinstance(c, Atom):
+      for v in c.args:
+        seen[v] = seen.get(v, 0) + 1
+  return {v for v, n in seen.items() if n > 1}
+
+
+def _heuristic_clause_order(rule: Rule, delta: int) -> list[int]:
+  bound: set[str] = set()
+  left = set(range(len(rule.body)))
+  out = []
+  while left:
+    ready = sorted(i for i in left if _needs(rule.body[i]) <= bound)
+    if not ready:
+      atoms = [i for i in sorted(left) if isinstance(rule.body[i], Atom)]
+      ready = [atoms[0] if atoms else min(left)]
+    pick = next((i for i in ready if isinstance(rule.body[i], Filter)), -1)
+    if pick < 0 and delta in ready:
+      pick = delta
+    if pick < 0:
+      best = -1
+      for i in ready:
+        c = rule.body[i]
+        ov = sum(1 for v in _clause_vars(c) if v in bound) if isinstance(c, Atom) else 0
+        if ov > best:
+          best, pick = ov, i
+    out.append(pick)
+    left.discard(pick)
+    bound |= _binds(rule.body[pick])
+  return out
+
+
+def _var_order(rule: Rule, clause_order: list[int], jv: set[str], delta: int) -> list[str]:
+  out: list[str] = []
+
+  def add(vs, want_join: bool):
+    for v in vs:
+      if (v in jv) == want_join and v not in out:
+        out.append(v)
+
+  if 0 <= delta < len(rule.body):
+    add(_clause_vars(rule.body[delta]), True)
+  for i in clause_order:
+    add(_clause_vars(rule.body[i]), True)
+  for i in clause_order:
+    add(_clause_vars(rule.body[i]), False)
+  return out
+
+
+def _clause_order_from_vars(rule: Rule, var_order: list[str], delta: int) -> list[int]:
+  out, left, bound = [], set(range(len(rule.body))), set()
+  for v in var_order:
+    if v in bound:
+      continue
+    cands = sorted(i for i in left if _needs(rule.body[i]) <= bound
+                   and v in _clause_vars(rule.body[i]))
+    if not cands:
+      continue
+    pick = delta if delta in cands else cands[0]
+    out.append(pick)
+    left.discard(pick)
+    bound |= set(_clause_vars(rule.body[pick]))
+  while left:
+    ready = [i for i in sorted(left) if _needs(rule.body[i]) <= bound]
+    filt = [i for i in ready if isinstance(rule.body[i], Filter)]
+    pick = filt[0] if filt else (ready[0] if ready else min(left))
+    out.append(pick)
+    left.discard(pick)
+    bound |= set(_clause_vars(rule.body[pick]))
+  return out
+
+
+def _access(atom: Atom, ver: str, jv: set[str], var_order: list[str], clause: int,
+            negated: bool) -> Access:
+  cvars = set(atom.args)
+  order = [v for v in var_order if v in cvars] + sorted(cvars - set(var_order))
+  if negated:
+    prefix_len = sum(1 for v in order if not v.startswith("_"))
+  else:
+    prefix_len = sum(1 for v in order if v in jv)
+  index = []
+  for v in order:
+    index.append(atom.args.index(v))
+  index += [c for c in range(len(atom.args)) if c not in index]
+  return Access(atom.rel, ver, order, index, prefix_len, clause)
+
+
+def plan_variant(v: Variant) -> None:
+  rule = v.rule
+  jv = _join_vars(rule)
+  plan = next((p for p in rule.plans if p.delta == v.delta), None)
+  if plan is not None and plan.var_order:
+    v.var_order = list(plan.var_order)
+    v.clause_order = (list(plan.clause_order) if plan.clause_order
+                      else _clause_order_from_vars(rule, v.var_order, v.delta))
+  else:
+    v.clause_order = _heuristic_clause_order(rule, v.delta)
+    v.var_order = _var_order(rule, v.clause_order, jv, v.delta)
+  v.join_vars = jv
+  for k, c in enumerate(rule.body):
+    if isinstance(c, Atom):
+      v.atoms.append(_access(c, v.versions[k], jv, v.var_order, k, False))
+    elif isinstance(c, Negation):
+      v.negs.append(_access(c.atom, M.FULL, jv, v.var_order, k, True))
+
+
+# ------------------------------------------------------ index selection ----
+def _complete(idx, arity: int) -> list[int]:
+  out = list(idx)
+  out += [c for c in range(arity) if c not in out]
+  return out
+
+
+def _indices_of(variants: list[Variant], ver: str | None) -> dict[str, set[tuple]]:
+  out: dict[str, set[tuple]] = {}
+  for v in variants:
+    for a in v.atoms + v.negs:
+      if ver is None or a.ver == ver:
+        out.setdefault(a.rel, set()).add(tuple(a.index))
+  return out
+
+
+def select_indices(strata: list[Stratum], arity: dict[str, int]) -> None:
+  global_map: dict[str, list[list[int]]] = {}
+  for s in strata:
+    allv = _indices_of(s.base_variants + s.rec_variants, None)
+    for rel in sorted(allv):
+      lst = global_map.setdefault(rel, [])
+      for t in sorted(allv[rel]):
+        if list(t) not in lst:
+          lst.append(list(t))
+  for s in strata:
+    if s.recursive:
+      dl = _indices_of(s.rec_variants, M.DELTA)
+      fl = _indices_of(s.rec_variants, M.FULL)
+      groups = [dl, fl]
+    else:
+      fl = {}
+      groups = [_indices_of(s.base_variants, None)]
+    for rel in sorted(s.members):
+      ar = arity[rel]
+      idxs: list[list[int]] = []
+      for g in groups:
+        for t in sorted(g.get(rel, ())):
+          c = _complete(t, ar)
+          if c not in idxs:
+            idxs.append(c)
+      if not idxs:
+        for t in global_map.get(rel, []):
+          c = _complete(t, ar)
+          if c not in idxs:
+            idxs.append(c)
+      if not idxs:
+        idxs.append(list(range(ar)))
+      s.required[rel] = idxs
+      canon = None
+      if s.recursive and rel in fl:
+        canon = next((i for i in idxs if len(i) == ar and tuple(i) in fl[rel]), None)
+      if canon is None:
+        canon = next((i for i in idxs if len(i) == ar), list(range(ar)))
+      s.canonical[rel] = list(canon)
+
+
+# ------------------------------------------------------------- lowering ----
+def _lower_body(v: Variant) -> list:
+  ops: list = []
+  if len(v.atoms) == 1:
+    a = v.atoms[0]
+    ops.append(M.Scan(list(a.order), a.rel, list(a.index), a.ver, list(a.order[:a.prefix_len])))
+  elif len(v.atoms) > 1:
+    bound: list[str] = []
+    chosen: dict[int, list[int]] = {}
+    for var in v.var_order:
+      if var not in v.join_vars:
+        continue
+      srcs = []
+      for a in v.atoms:
+        if var in a.order:
+          chosen[a.clause] = list(a.index)
+          srcs.append(M.ColumnSource(a.rel, list(a.index), a.ver,
+                                     [x for x in a.order if x in bound], clause_idx=a.clause))
+      if srcs:
+        ops.append(M.ColumnJoin(var, srcs))
+        bound.append(var)
+    pos_vars = set().union(*[set(a.order) for a in v.atoms])
+    neg_only = {x for a in v.negs for x in a.order if x not in pos_vars}
+    free = [x for x in v.var_order if x not in v.join_vars and x not in neg_only]
+    for c in v.rule.body:
+      if isinstance(c, Atom):
+        for x in c.args:
+          if x not in v.join_vars and x not in free and x not in neg_only:
+            free.append(x)
+    if free:
+      fset = set(free)
+      srcs, vfs = [], []
+      for a in v.atoms:
+        if not any(x in fset for x in a.order):
+          continue
+        if a.clause in chosen:
+          idx = list(chosen[a.clause])
+        else:
+          idx = []
+          for b in bound:
+            if b in a.order:
+              col = a.index[a.order.index(b)]
+              if col not in idx:
+                idx.append(col)
+          idx += [c for c in a.index if c not in idx]
+        col_var = {a.index[i]: x for i, x in enumerate(a.order) if i < len(a.index)}
+        prefix = []
+        for col in idx:
+          x = col_var.get(col)
+          if x is not None and x in bound and x not in prefix:
+            prefix.append(x)
+        mine = []
+        for col in idx:
+          x = col_var.get(col)
+          if x is not None and x in fset and x not in prefix and x not in mine:
+            mine.append(x)
+        srcs.append(M.ColumnSource(a.rel, idx, a.ver, prefix, clause_idx=a.clause))
+        vfs.append(mine)
+      if srcs:
+        ops.append(M.CartesianJoin(free, vfs, srcs))
+  for a in v.negs:
+    ops.append(M.Negation(a.rel, M.FULL, list(a.index), list(a.order[:a.prefix_len])))
+  for c in v.rule.body:
+    if isinstance(c, Filter):
+      ops.append(M.Filter(list(c.vars), c.code))
+  return ops
+
+
+def _inserts(rule: Rule, s: Stratum) -> list:
+  out = []
+  for h in rule.heads:
+    canon = s.canonical.get(h.rel, list(range(len(h.args))))
+    out.append(M.InsertInto(h.rel, M.NEW, list(canon), list(h.args)))
+  return out
+
+
+def _pipeline(v: Variant, s: Stratum) -> M.ExecutePipeline:
+  name = v.rule.name or ""
+  if v.delta >= 0:
+    name = f"{name}_D{v.delta}"
+  ep = M.ExecutePipeline(name, _lower_body(v) + _inserts(v.rule, s), count=v.rule.count)
+  ep.clause_order = list(v.clause_order)
+  return ep
+
+
+def _simple_maintenance(rel: str, idxs, canon) -> list:
+  ops = [M.RebuildIndex(rel, list(canon), M.NEW), M.CheckSize(rel, M.NEW),
+         M.ComputeDeltaIndex(rel, list(canon)), M.ClearRelation(rel, M.NEW)]
+  for idx in idxs:
+    if list(idx) != list(canon):
+      ops.append(M.RebuildIndexFromIndex(rel, list(canon), list(idx), M.DELTA))
+    ops.append(M.MergeIndex(rel, list(idx)))
+  return ops
+
+
+def _loop_maintenance(rel: str, idxs, canon, full_needed: set[tuple]) -> list:
+  ops = [M.RebuildIndex(rel, list(canon), M.NEW), M.ClearRelation(rel, M.DELTA),
+         M.CheckSize(rel, M.NEW), M.ComputeDeltaIndex(rel, list(canon)),
+         M.ClearRelation(rel, M.NEW)]
+  for idx in idxs:
+    if list(idx) != list(canon):
+      ops.append(M.RebuildIndexFromIndex(rel, list(canon), list(idx), M.DELTA))
+    if tuple(idx) == tuple(canon) or tuple(idx) in full_needed:
+      ops.append(M.MergeIndex(rel, list(idx)))
+  return ops
+
+
+def lower(strata: list[Stratum], arity: dict[str, int]) -> list[M.Step]:
+  steps: list[M.Step] = []
+  for s in strata:
+    if s.recursive:
+      pipes = [_pipeline(v, s) for v in s.rec_variants]
+      hooks = [M.InjectCppHook(v.rule.name or "", v.rule.debug_code)
+               for v in s.rec_variants if v.rule.debug_code]
+      ops: list = []
+      if len(pipes) > 1:
+        ops.append(M.ParallelGroup(pipes))
+      elif pipes:
+        ops.append(pipes[0])
+      ops += hooks
+      full_reads: dict[str, set[tuple]] = {}
+      for v in s.rec_variants:
+        for a in v.atoms:
+          if a.ver == M.FULL:
+            full_reads.setdefault(a.rel, set()).add(tuple(_complete(a.index, arity[a.rel])))
+      for rel in sorted(s.members):
+        if rel in s.required:
+          ops += _loop_maintenance(rel, s.required[rel], s.canonical[rel],
+                                   full_reads.get(rel, set()))
+      if ops:
+        steps.append(M.Step(True, M.FixpointPlan(ops)))
+        for rel in sorted(s.members):
+          if rel in s.canonical:
+            steps.append(M.Step(False, M.PostStratumReconstruct(rel, list(s.canonical[rel]))))
+    else:
+      pipes, maint, modified = [], [], []
+      for v in s.base_variants:
+        pipes.append(_pipeline(v, s))
+        for h in v.rule.heads:
+          if h.rel not in modified:
+            modified.append(h.rel)
+          if h.rel in s.required:
+            maint += _simple_maintenance(h.rel, s.required[h.rel], s.canonical[h.rel])
+      ops = []
+      if len(pipes) > 1:
+        ops.append(M.ParallelGroup(pipes))
+      elif pipes:
+        ops.append(pipes[0])
+      ops += maint
+      if ops:
+        steps.append(M.Step(False, M.FixpointPlan(ops)))
+        for rel in modified:
+          if rel in s.canonical:
+            steps.append(M.Step(False, M.PostStratumReconstruct(rel, list(s.canonical[rel]))))
+        for v in s.base_variants:
+          if v.rule.debug_code:
+            steps.append(M.Step(False, M.InjectCppHook(v.rule.name or "", v.rule.debug_code)))
+  return steps
+
+
+# ----------------------------------------------------------- MIR passes ----
+def _plan_pipelines(body) -> list[M.ExecutePipeline]:
+  out = []
+  if isinstance(body, M.FixpointPlan):
+    for op in body.ops:
+      if isinstance(op, M.ExecutePipeline):
+        out.append(op)
+      elif isinstance(op, M.ParallelGroup):
+        out.extend(o for o in op.ops if isinstance(o, M.ExecutePipeline))
+  return out
+
+
+def _needed_indices(body, rel: str, out: set) -> None:
+  for ep in _plan_pipelines(body):
+    for op in ep.ops:
+      if isinstance(op, (M.ColumnJoin, M.CartesianJoin)):
+        for src in op.sources:
+          if src.rel == rel and src.ver in (M.FULL, M.DELTA):
+            out.add(tuple(src.index))
+
+
+def _rebuilds_before_reconstruct(steps: list[M.Step]) -> list[M.Step]:
+  out = []
+  for i, st in enumerate(steps):
+    out.append(st)
+    if not isinstance(st.body, M.PostStratumReconstruct):
+      continue
+    rel = st.body.rel
+    merged: set[tuple] = set()
+    for j in range(i - 1, -1, -1):
+      b = steps[j].body
+      if isinstance(b, M.FixpointPlan) and any(
+          d.rel == rel for ep in _plan_pipelines(b) for d in ep.dests()):
+        merged = {tuple(o.index) for o in b.ops if isinstance(o, M.MergeIndex) and o.rel == rel}
+        break
+    needed: set[tuple] = set()
+    for j in range(i + 1, len(steps)):
+      _needed_indices(steps[j].body, rel, needed)
+    for idx in sorted(needed - merged):
+      out.append(M.Step(False, M.RebuildIndex(rel, list(idx), M.FULL)))
+  return out
+
+
+def _reorder_sources(steps: list[M.Step]) -> None:
+  for st in steps:
+    for ep in _plan_pipelines(st.body):
+      order = getattr(ep, "clause_order", [])
+
+      def pos(src):
+        return order.index(src.clause_idx) if src.clause_idx in order else len(order)
+
+      for op in ep.ops:
+        if isinstance(op, M.ColumnJoin) and order:
+          op.sources.sort(key=pos)
+        elif isinstance(op, M.CartesianJoin) and order:
+          pairs = sorted(zip(op.sources, op.var_from_source), key=lambda p: pos(p[0]))
+          op.sources = [p[0] for p in pairs]
+          op.var_from_source = [p[1] for p in pairs]
+      for op in ep.ops:
+        if isinstance(op, (M.ColumnJoin, M.CartesianJoin)) and len(op.sources) > 1:
+          if op.sources[0].prefix or not any(s.prefix for s in op.sources[1:]):
+            continue
+          if isinstance(op, M.ColumnJoin):
+            op.sources.sort(key=lambda s: 0 if s.prefix else 1)
+          else:
+            pairs = sorted(zip(op.sources, op.var_from_source), key=lambda p: 0 if p[0].prefix else 1)
+            op.sources = [p[0] for p in pairs]
+            op.var_from_source = [p[1] for p in pairs]
+
+
+# ---------------------------------------------------------------- entry ----
+def relation_decls(program: Program) -> list[RelDecl]:
+  return [RelDecl(r.name, r.arity, r.input_file, r.print_size) for r in program.relations]
+
+
+def compile_to_mir(program: Program) -> M.Program:
+  arity = {r.name: r.arity for r in program.relations}
+  for r in program.rules:
+    for c in r.body:
+      if not isinstance(c, (Atom, Negation, Filter)):
+        raise CompileError(f"rule {r.name}: unsupported body clause {type(c).__name__}")
+  strata = stratify(list(program.rules))
+  for s in strata:
+    if s.recursive:
+      for r in s.rules:
+        for i, c in enumerate(r.body):
+          if isinstance(c, Atom) and c.rel in s.members:
+            vers = [M.FULL] * len(r.body)
+            vers[i] = M.DELTA
+            s.rec_variants.append(Variant(r, i, vers))
+    else:
+      for r in s.rules:
+        s.base_variants.append(Variant(r, -1, [M.FULL] * len(r.body)))
+    for v in s.base_variants + s.rec_variants:
+      plan_variant(v)
+  select_indices(strata, arity)
+  steps = lower(strata, arity)
+  steps = _rebuilds_before_reconstruct(steps)
+  _reorder_sources(steps)
+  return M.Program(steps)

@@ -1,0 +1,135 @@
+"""The benchmark programs, written in the DSL as the reference's examples
+write them (examples/tc.py, examples/sg.py, examples/andersen.py,
+examples/lsqb_triangle_count.py), plus single-source reachability.
+
+``build_all()`` compiles every program into ``_lib/programs/lib<name>.so``;
+``load(name)`` returns its ctypes binding (runtime.ProgramLib).
+"""
+
+from __future__ import annotations
+
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+from . import build as B
+from . import native
+from .dsl import Filter, PlanEntry, Program, Relation, Var
+from .runtime import ProgramLib
+
+PROG_DIR = os.path.join(native.LIB_DIR, "programs")
+
+
+def tc_program() -> Program:
+  """Transitive closure (examples/tc.py)."""
+  x, y, z = Var("x"), Var("y"), Var("z")
+  arc = Relation("ArcInput", 2, input_file="Arc.csv")
+  edge = Relation("Edge", 2)
+  path = Relation("Path", 2, print_size=True)
+  return Program(rules=[
+    (edge(x, y) <= arc(x, y)).named("EdgeLoad"),
+    (path(x, y) <= edge(x, y)).named("TCBase"),
+    (path(x, z) <= path(x, y) & edge(y, z)).named("TCRec"),
+  ])
+
+
+def sg_program() -> Program:
+  """Same generation (examples/sg.py)."""
+  p, q, x, y = Var("p"), Var("q"), Var("x"), Var("y")
+  arc = Relation("Arc", 2, input_file="Arc.csv")
+  sg = Relation("Sg", 2, print_size=True)
+  return Program(rules=[
+    (sg(x, y) <= arc(p, x) & arc(p, y) & Filter(("x", "y"), "return x != y;"))
+    .named("SGBase").with_plan(var_order=["p", "x", "y"]),
+    (sg(x, y) <= arc(p, x) & sg(p, q) & arc(q, y))
+    .named("SGRec").with_plan(var_order=["p", "q", "x", "y"]),
+  ])
+
+
+def andersen_program() -> Program:
+  """Andersen points-to (examples/andersen.py; the benchmark's four rules
+  with variables renamed)."""
+  w, x, y, z = Var("w"), Var("x"), Var("y"), Var("z")
+  addr = Relation("AddressOf", 2, input_file="addressOf.csv")
+  assign = Relation("Assign", 2, input_file="assign.csv")
+  load = Relation("Load", 2, input_file="load.csv")
+  store = Relation("Store", 2, input_file="store.csv")
+  pt = Relation("PointsTo", 2, print_size=True)
+  return Program(rules=[
+    (pt(y, x) <= addr(y, x)).named("Base").with_plan(var_order=["y", "x"]),
+    (pt(y, x) <= pt(z, x) & assign(y, z)).named("Assign")
+    .with_plan(var_order=["z", "x", "y"]),
+    (pt(y, w) <= pt(x, z) & load(y, x) & pt(z, w)).named("Load")
+    .with_plan(delta=0, var_order=["x", "z", "y", "w"])
+    .with_plan(delta=2, var_order=["z", "x", "y", "w"]),
+    (pt(z, w) <= pt(y, z) & store(y, x) & pt(x, w)).named("Store")
+    .with_plan(delta=0, var_order=["y", "x", "z", "w"])
+    .with_plan(delta=2, var_order=["x", "y", "w", "z"]),
+  ])
+
+
+def triangle_program(count_only: bool) -> Program:
+  """Triangle listing / count (examples/lsqb_triangle_count.py).  With the
+  input oriented low->high, every result has a < b < c."""
+  a, b, c, x, y = Var("a"), Var("b"), Var("c"), Var("x"), Var("y")
+  k_in = Relation("KnowsInput", 2, input_file="Person_knows_Person.csv")
+  knows = Relation("Knows", 2)
+  tri = Relation("Triangle", 3, print_size=not count_only)
+  rule = (tri(a, b, c) <= knows(a, b) & knows(b, c) & knows(a, c)).named("TriangleJoin") \
+      .with_plan(var_order=["a", "b", "c"])
+  if count_only:
+    rule = rule.with_count()
+  return Program(rules=[(knows(x, y) <= k_in(x, y)).named("KnowsLoad"), rule])
+
+
+def reach_program() -> Program:
+  """Single-source reachability: Reach(y) <= Src(y); Reach(z) <= Reach(y), Edge(y,z)."""
+  y, z = Var("y"), Var("z")
+  src = Relation("Src", 1, input_file="Src.csv")
+  edge = Relation("Edge", 2, input_file="Edge.csv")
+  reach = Relation("Reach", 1, print_size=True)
+  return Program(rules=[
+    (reach(y) <= src(y)).named("ReachBase"),
+    (reach(z) <= reach(y) & edge(y, z)).named("ReachRec"),
+  ])
+
+
+PROGRAMS = {
+  "tc": tc_program,
+  "sg": sg_program,
+  "andersen": andersen_program,
+  "triangle": lambda: triangle_program(False),
+  "triangle_count": lambda: triangle_program(True),
+  "reach": reach_program,
+}
+
+
+def so_path(name: str) -> str:
+  return os.path.join(PROG_DIR, f"lib{name}.so")
+
+
+def build_one(name: str) -> str:
+  prog = PROGRAMS[name]()
+  proj = B.build_project(prog, name, cache_base=os.path.join(PROG_DIR, "src"))
+  res = B.compile_jit_project(proj, out_path=so_path(name))
+  if not res.ok():
+    raise RuntimeError(f"compiling program {name} failed:\n{res.stderr[-6000:]}")
+  return res.artifact
+
+
+def build_all(jobs: int = 6) -> list[str]:
+  os.makedirs(PROG_DIR, exist_ok=True)
+  with ThreadPoolExecutor(max_workers=jobs) as ex:
+    return list(ex.map(build_one, PROGRAMS))
+
+
+_loaded: dict[str, ProgramLib] = {}
+
+
+def load(name: str) -> ProgramLib:
+  """ctypes binding of a prebuilt program library (build_all() first)."""
+  if name not in _loaded:
+    path = so_path(name)
+    if not os.path.exists(path):
+      raise RuntimeError(f"{path} is not built; run __graft_entry__.build() first")
+    _loaded[name] = ProgramLib(path)
+  return _loaded[name]

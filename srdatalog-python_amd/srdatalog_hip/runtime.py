@@ -1,0 +1,116 @@
+"""ctypes binding of a compiled program (include/srdatalog.h).
+
+``ProgramLib`` binds exactly the reference's six shim symbols with the
+reference's argument/return types (examples/run_benchmark.py:195-203) plus
+the additions this backend exports.  Loading requires the HIP runtime; there
+is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class SrdatalogError(RuntimeError):
+  pass
+
+
+class ProgramLib:
+  def __init__(self, so_path: str):
+    native.load_runtime()
+    lib = ctypes.CDLL(so_path, mode=ctypes.RTLD_GLOBAL)
+    lib.srdatalog_init.restype = ctypes.c_int
+    lib.srdatalog_load_csv.restype = ctypes.c_int
+    lib.srdatalog_load_csv.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    lib.srdatalog_load_all.restype = ctypes.c_int
+    lib.srdatalog_load_all.argtypes = [ctypes.c_char_p]
+    lib.srdatalog_run.restype = ctypes.c_int
+    lib.srdatalog_run.argtypes = [ctypes.c_ulonglong]
+    lib.srdatalog_size.restype = ctypes.c_ulonglong
+    lib.srdatalog_size.argtypes = [ctypes.c_char_p]
+    lib.srdatalog_shutdown.restype = ctypes.c_int
+    lib.srdatalog_export.restype = ctypes.c_int
+    lib.srdatalog_export.argtypes = [ctypes.c_char_p, _u32p, ctypes.c_uint64, _u64p]
+    lib.srdatalog_delta_sizes.restype = ctypes.c_int
+    lib.srdatalog_delta_sizes.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_ulonglong),
+                                          ctypes.c_ulonglong]
+    lib.srdatalog_num_iterations.restype = ctypes.c_ulonglong
+    lib.srdatalog_num_iterations.argtypes = [ctypes.c_char_p]
+    lib.srdatalog_load_rows.restype = ctypes.c_int
+    lib.srdatalog_load_rows.argtypes = [ctypes.c_char_p, _u32p, ctypes.c_uint64, ctypes.c_uint32]
+    lib.srdatalog_prepare.restype = ctypes.c_int
+    lib.srdatalog_run_seconds.restype = ctypes.c_double
+    lib.srdatalog_join_tuples.restype = ctypes.c_ulonglong
+    lib.srdatalog_set_grain.argtypes = [ctypes.c_uint]
+    lib.srdatalog_set_grain.restype = None
+    self.lib = lib
+    self.path = so_path
+
+  # -- the reference's six ---------------------------------------------------
+  def init(self) -> int:
+    return self.lib.srdatalog_init()
+
+  def load_csv(self, rel: str, path: str) -> int:
+    return self.lib.srdatalog_load_csv(rel.encode(), path.encode())
+
+  def load_all(self, data_dir: str) -> int:
+    return self.lib.srdatalog_load_all(data_dir.encode())
+
+  def run(self, max_iters: int = 0) -> int:
+    return self.lib.srdatalog_run(max_iters)
+
+  def size(self, rel: str) -> int:
+    return int(self.lib.srdatalog_size(rel.encode()))
+
+  def shutdown(self) -> int:
+    return self.lib.srdatalog_shutdown()
+
+  # -- additions ---------------------------------------------------------------
+  def load_rows(self, rel: str, rows: np.ndarray) -> None:
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    if rows.ndim == 1:
+      rows = rows.reshape(-1, 1)
+    rc = self.lib.srdatalog_load_rows(rel.encode(), rows.ctypes.data_as(_u32p), rows.shape[0],
+                                      rows.shape[1])
+    if rc != 0:
+      raise SrdatalogError(f"srdatalog_load_rows({rel}) returned {rc}")
+
+  def prepare(self) -> None:
+    rc = self.lib.srdatalog_prepare()
+    if rc != 0:
+      raise SrdatalogError(f"srdatalog_prepare returned {rc}")
+
+  def export(self, rel: str, arity: int) -> np.ndarray:
+    n = ctypes.c_uint64()
+    rc = self.lib.srdatalog_export(rel.encode(), None, 0, ctypes.byref(n))
+    if rc != 0:
+      raise SrdatalogError(f"srdatalog_export({rel}) returned {rc}")
+    out = np.empty((max(n.value, 1), arity), dtype=np.uint32)
+    rc = self.lib.srdatalog_export(rel.encode(), out.ctypes.data_as(_u32p), n.value, ctypes.byref(n))
+    if rc != 0:
+      raise SrdatalogError(f"srdatalog_export({rel}) returned {rc}")
+    return out[:n.value]
+
+  def delta_sizes(self, rel: str) -> list[int]:
+    cnt = self.lib.srdatalog_delta_sizes(rel.encode(), None, 0)
+    if cnt < 0:
+      raise SrdatalogError(f"unknown relation {rel}")
+    buf = (ctypes.c_ulonglong * max(cnt, 1))()
+    self.lib.srdatalog_delta_sizes(rel.encode(), buf, cnt)
+    return [int(buf[i]) for i in range(cnt)]
+
+  def run_seconds(self) -> float:
+    return float(self.lib.srdatalog_run_seconds())
+
+  def join_tuples(self) -> int:
+    return int(self.lib.srdatalog_join_tuples())
+
+  def set_grain(self, grain: int) -> None:
+    self.lib.srdatalog_set_grain(grain)

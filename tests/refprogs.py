@@ -1,0 +1,82 @@
+"""The reference test suite's example programs, written in this package's
+DSL (same rules, names and plans as the reference's
+tests/test_integration_*.py / test_hir_mir_tc_e2e.py), paired with the MIR
+goldens those tests check (tests/golden/mir/)."""
+
+from srdatalog_hip.dsl import Filter, PlanEntry, Program, Relation, Var
+
+
+def tc() -> Program:
+  X, Y, Z = Var("x"), Var("y"), Var("z")
+  arc, edge, path = Relation("ArcInput", 2), Relation("Edge", 2), Relation("Path", 2)
+  return Program(rules=[
+    (edge(X, Y) <= arc(X, Y)).named("EdgeLoad"),
+    (path(X, Y) <= edge(X, Y)).named("TCBase"),
+    (path(X, Z) <= path(X, Y) & edge(Y, Z)).named("TCRec"),
+  ])
+
+
+def tc_2level_bg() -> Program:
+  X, Y, Z = Var("x"), Var("y"), Var("z")
+  arc, edge, path = Relation("ArcInput", 2), Relation("Edge", 2), Relation("Path", 2)
+  return Program(rules=[
+    (edge(X, Y) <= arc(X, Y)).named("EdgeLoad"),
+    (path(X, Y) <= edge(X, Y)).named("TCBase"),
+    (path(X, Z) <= path(X, Y) & edge(Y, Z)).named("TCRec")
+    .with_plan(delta=0, block_group=True, var_order=["y", "x", "z"]),
+  ])
+
+
+def sg() -> Program:
+  P, Q, X, Y = Var("p"), Var("q"), Var("x"), Var("y")
+  arc, sgr = Relation("Arc", 2), Relation("Sg", 2)
+  return Program(rules=[
+    (sgr(X, Y) <= arc(P, X) & arc(P, Y) & Filter(vars=("x", "y"), code="return x != y;"))
+    .named("SGBase").with_plan(var_order=["p", "x", "y"]),
+    (sgr(X, Y) <= arc(P, X) & sgr(P, Q) & arc(Q, Y))
+    .named("SGRec").with_plan(var_order=["p", "q", "x", "y"]),
+  ])
+
+
+def andersen() -> Program:
+  X, Y, Z, W = Var("x"), Var("y"), Var("z"), Var("w")
+  ao, asgn = Relation("AddressOf", 2), Relation("Assign", 2)
+  load_r, store_r, pt = Relation("Load", 2), Relation("Store", 2), Relation("PointsTo", 2)
+  return Program(rules=[
+    (pt(Y, X) <= ao(Y, X)).named("Base").with_plan(var_order=["y", "x"]),
+    (pt(Y, X) <= pt(Z, X) & asgn(Y, Z)).named("Assign")
+    .with_plan(delta=0, clause_order=[0, 1], var_order=["z", "x", "y"]),
+    (pt(Y, W) <= pt(X, Z) & load_r(Y, X) & pt(Z, W)).named("Load").with_plans([
+      PlanEntry(delta=0, clause_order=(0, 1, 2), var_order=("x", "z", "y", "w")),
+      PlanEntry(delta=2, clause_order=(2, 0, 1), var_order=("z", "x", "y", "w")),
+    ]),
+    (pt(Z, W) <= pt(Y, Z) & store_r(Y, X) & pt(X, W)).named("Store").with_plans([
+      PlanEntry(delta=0, clause_order=(0, 1, 2), var_order=("y", "x", "z", "w")),
+      PlanEntry(delta=2, clause_order=(2, 1, 0), var_order=("x", "y", "w", "z")),
+    ]).with_inject_cpp("debug"),
+  ])
+
+
+def lsqb_triangle_count() -> Program:
+  A, B, C, X, Y = Var("a"), Var("b"), Var("c"), Var("x"), Var("y")
+  k_in, knows, tri = Relation("KnowsInput", 2), Relation("Knows", 2), Relation("Triangle", 3)
+  return Program(rules=[
+    (knows(X, Y) <= k_in(X, Y)).named("KnowsLoad"),
+    (tri(A, B, C) <= knows(A, B) & knows(B, C) & knows(A, C)).named("TriangleJoin")
+    .with_plan(var_order=["a", "b", "c"]).with_count(),
+  ])
+
+
+def triangle() -> Program:
+  X, Y, Z, H, F = Var("x"), Var("y"), Var("z"), Var("h"), Var("f")
+  rr, sr, tr, zr = Relation("RRel", 2), Relation("SRel", 3), Relation("TRel", 3), Relation("ZRel", 3)
+  return Program(rules=[
+    (zr(X, Y, Z) <= rr(X, Y) & sr(Y, Z, H) & tr(Z, X, F)).named("Triangle")
+    .with_plan(var_order=["x", "y", "z"]),
+  ])
+
+
+GOLDEN = {
+  "tc": tc, "tc_2level_bg": tc_2level_bg, "sg": sg, "andersen": andersen,
+  "lsqb_triangle_count": lsqb_triangle_count, "triangle": triangle,
+}

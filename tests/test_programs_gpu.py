@@ -1,0 +1,198 @@
+"""End-to-end parity of compiled programs (through the srdatalog_* C ABI)
+against the CPU oracle: final sorted tuple sets AND per-iteration delta
+sizes, bit-exact."""
+
+import numpy as np
+import pytest
+
+import oracle
+from srdatalog_hip import datasets as ds
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def progs():
+  from srdatalog_hip import programs
+  return programs
+
+
+def run_prog(progs, name, inputs, max_iters=0):
+  lib = progs.load(name)
+  assert lib.init() == 0
+  for rel, rows in inputs.items():
+    lib.load_rows(rel, rows)
+  rc = lib.run(max_iters)
+  assert rc == 0, f"srdatalog_run returned {rc}"
+  return lib
+
+
+def check_rel(lib, orc, rel, arity):
+  got = lib.export(rel, arity)
+  want = orc.relations[rel]
+  assert got.shape == want.shape, f"{rel}: {got.shape} vs {want.shape}"
+  np.testing.assert_array_equal(got, want)
+  assert lib.size(rel) == want.shape[0]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tc_config(progs, seed):
+  e = ds.tc_edges(seed, 1000, 5000)
+  orc = oracle.run("tc", {"ArcInput": e})
+  lib = run_prog(progs, "tc", {"ArcInput": e})
+  try:
+    check_rel(lib, orc, "Path", 2)
+    check_rel(lib, orc, "Edge", 2)
+    assert lib.delta_sizes("Path") == orc.deltas["Path"]
+  finally:
+    lib.shutdown()
+
+
+@pytest.mark.parametrize("n,m,seed", [(10, 15, 3), (60, 150, 4), (400, 600, 5), (3000, 6000, 6)])
+def test_tc_small(progs, n, m, seed):
+  e = ds.tc_edges(seed, n, m)
+  orc = oracle.run("tc", {"ArcInput": e})
+  lib = run_prog(progs, "tc", {"ArcInput": e})
+  try:
+    check_rel(lib, orc, "Path", 2)
+    assert lib.delta_sizes("Path") == orc.deltas["Path"]
+  finally:
+    lib.shutdown()
+
+
+def test_tc_duplicates_and_loops(progs):
+  # duplicate input rows and self-loops are legal input
+  e = np.array([[1, 2], [1, 2], [2, 3], [3, 3], [3, 1], [5, 6]], dtype=np.uint32)
+  orc = oracle.run("tc", {"ArcInput": e})
+  lib = run_prog(progs, "tc", {"ArcInput": e})
+  try:
+    check_rel(lib, orc, "Path", 2)
+    assert lib.delta_sizes("Path") == orc.deltas["Path"]
+  finally:
+    lib.shutdown()
+
+
+def test_tc_max_iters(progs):
+  e = ds.tc_edges(0, 1000, 5000)
+  orc = oracle.run("tc", {"ArcInput": e}, max_iters=3)
+  lib = run_prog(progs, "tc", {"ArcInput": e}, max_iters=3)
+  try:
+    check_rel(lib, orc, "Path", 2)
+    assert lib.delta_sizes("Path") == orc.deltas["Path"]
+  finally:
+    lib.shutdown()
+
+
+def test_tc_empty(progs):
+  e = np.zeros((0, 2), dtype=np.uint32)
+  lib = run_prog(progs, "tc", {"ArcInput": e})
+  try:
+    assert lib.size("Path") == 0
+    assert lib.export("Path", 2).shape == (0, 2)
+  finally:
+    lib.shutdown()
+
+
+def test_tc_large_ids(progs):
+  rng = np.random.default_rng(9)
+  ids = rng.choice(np.arange(2**32 - 5000, 2**32, dtype=np.uint64), size=300, replace=False)
+  src = rng.choice(ids, 500)
+  dst = rng.choice(ids, 500)
+  e = np.stack([src, dst], 1).astype(np.uint32)
+  orc = oracle.run("tc", {"ArcInput": e})
+  lib = run_prog(progs, "tc", {"ArcInput": e})
+  try:
+    check_rel(lib, orc, "Path", 2)
+    assert lib.delta_sizes("Path") == orc.deltas["Path"]
+  finally:
+    lib.shutdown()
+
+
+@pytest.mark.parametrize("n,seed", [(50, 0), (2000, 1), (20000, 2)])
+def test_sg(progs, n, seed):
+  e = ds.sg_forest_edges(np.random.default_rng(seed), n)
+  orc = oracle.run("sg", {"Arc": e})
+  lib = run_prog(progs, "sg", {"Arc": e})
+  try:
+    check_rel(lib, orc, "Sg", 2)
+    assert lib.delta_sizes("Sg") == orc.deltas["Sg"]
+  finally:
+    lib.shutdown()
+
+
+@pytest.mark.parametrize("scale_div,seed", [(1000, 0), (100, 1), (100, 2), (30, 0)])
+def test_andersen(progs, scale_div, seed):
+  f = scale_div
+  a = ds.andersen_inputs(seed, n_vars=200_000 // f, n_objs=50_000 // f,
+                         sizes=(100_000 // f, 400_000 // f, 100_000 // f, 100_000 // f))
+  orc = oracle.run("andersen", a)
+  lib = run_prog(progs, "andersen", a)
+  try:
+    check_rel(lib, orc, "PointsTo", 2)
+    assert lib.delta_sizes("PointsTo") == orc.deltas["PointsTo"]
+  finally:
+    lib.shutdown()
+
+
+@pytest.mark.parametrize("scale,m,seed", [(8, 2000, 0), (12, 60_000, 1), (16, 1_000_000, 2)])
+def test_triangle_listing(progs, scale, m, seed):
+  e = ds.triangle_edges(seed, scale, m)
+  orc = oracle.run("triangle", {"KnowsInput": e})
+  lib = run_prog(progs, "triangle", {"KnowsInput": e})
+  try:
+    check_rel(lib, orc, "Triangle", 3)
+    check_rel(lib, orc, "Knows", 2)
+    assert lib.delta_sizes("Triangle") == orc.deltas["Triangle"]
+  finally:
+    lib.shutdown()
+
+
+@pytest.mark.parametrize("scale,m,seed", [(12, 60_000, 1), (16, 1_000_000, 2)])
+def test_triangle_count(progs, scale, m, seed):
+  e = ds.triangle_edges(seed, scale, m)
+  orc = oracle.run("triangle_count", {"KnowsInput": e})
+  lib = run_prog(progs, "triangle_count", {"KnowsInput": e})
+  try:
+    assert lib.size("Triangle") == orc.count
+  finally:
+    lib.shutdown()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_reach(progs, seed):
+  e = ds.tc_edges(seed, 5000, 12000)
+  src = np.array([[seed * 17]], dtype=np.uint32)
+  orc = oracle.run("reach", {"Edge": e, "Src": src})
+  lib = run_prog(progs, "reach", {"Edge": e, "Src": src})
+  try:
+    check_rel(lib, orc, "Reach", 1)
+    assert lib.delta_sizes("Reach") == orc.deltas["Reach"]
+  finally:
+    lib.shutdown()
+
+
+def test_repeat_runs_identical(progs):
+  e = ds.tc_edges(1, 1000, 5000)
+  lib = run_prog(progs, "tc", {"ArcInput": e})
+  try:
+    first = lib.export("Path", 2)
+    d1 = lib.delta_sizes("Path")
+    assert lib.run(0) == 0
+    np.testing.assert_array_equal(lib.export("Path", 2), first)
+    assert lib.delta_sizes("Path") == d1
+  finally:
+    lib.shutdown()
+
+
+def test_csv_load_all(progs, tmp_path):
+  e = ds.tc_edges(2, 300, 700)
+  ds.write_csv(str(tmp_path / "Arc.csv"), e, delimiter="\t")
+  orc = oracle.run("tc", {"ArcInput": e})
+  lib = progs.load("tc")
+  assert lib.init() == 0
+  try:
+    assert lib.load_all(str(tmp_path)) == 0
+    assert lib.run(0) == 0
+    check_rel(lib, orc, "Path", 2)
+  finally:
+    lib.shutdown()
