@@ -52,6 +52,15 @@ int srd_prim_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out);
 int srd_prim_time_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint32_t reps,
                              double* ms_per_rep);
 
+/* Kernel timing: while enabled, every primitive and join launch on the
+ * runtime's streams is bracketed by HIP events.  srd_timing_report writes a
+ * JSON array [{"name", "launches", "ms", "bytes"}] (ms = summed duration,
+ * bytes = algorithmic bytes the primitive attributes to those launches)
+ * into buf; returns the length needed (excluding NUL). */
+int srd_timing_enable(int on);
+uint64_t srd_timing_report(char* buf, uint64_t cap);
+int srd_timing_reset(void);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* srd_last_error(void);
 

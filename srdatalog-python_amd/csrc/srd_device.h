@@ -49,6 +49,14 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t m) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Value of v in `lane`, as a wave-uniform (scalar) result; v_readlane does
+// not depend on EXEC, unlike a __shfl through ds_bpermute.
+__device__ __forceinline__ uint64_t read_lane_u64(uint64_t v, int lane) {
+  uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
+  uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), lane);
+  return (uint64_t(hi) << 32) | lo;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
   const uint32_t l = lane_id();

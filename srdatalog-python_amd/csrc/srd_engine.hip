@@ -229,6 +229,7 @@ struct srd_engine {
   srd::DevBuf<uint32_t> key_rng, key_aux, counts_buf, work_ctr;
   srd::DevBuf<uint64_t> key_work, work_start, offsets;
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
+  bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
 };
 
 namespace srd {
@@ -545,6 +546,11 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
     exclusive_scan_u64(a.key_work, a.work_start, nk, e.ws, e.s);
     a.total_work = read_u64(e, a.work_start + nk);
     a.n_slices = (a.total_work + a.grain - 1) / a.grain;
+    if (e.trace) {
+      std::fprintf(stderr, "[srd]   %s: keys %u drv %u work %llu slices %llu\n", d->rule, a.n_root,
+                   a.drv_view, (unsigned long long)a.total_work, (unsigned long long)a.n_slices);
+      std::fflush(stderr);
+    }
     const uint64_t slices = a.n_slices;
     if (slices > 0) {
       e.counts_buf.reserve(slices, e.s);
@@ -562,6 +568,10 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
       }
       exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
       total = read_u64(e, a.offsets + slices);
+      if (e.trace) {
+        std::fprintf(stderr, "[srd]   %s: counted %llu tuples\n", d->rule, (unsigned long long)total);
+        std::fflush(stderr);
+      }
       if (total > 0 && !d->count_only) {
         for (uint32_t k = 0; k < d->n_dest; ++k) {
           const SrdDestSpec& ds = d->dest[k];
@@ -596,7 +606,29 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
 // =========================================================== MIR ops ====
 static void exec_ops(srd_engine& e, const std::vector<Op>& ops, uint64_t iteration, int step_no);
 
+static const char* op_name(OpKind k) {
+  switch (k) {
+    case OpKind::Pipeline: return "execute-pipeline";
+    case OpKind::Parallel: return "parallel-group";
+    case OpKind::RebuildIndex: return "rebuild-index";
+    case OpKind::ClearRelation: return "clear-relation";
+    case OpKind::CheckSize: return "check-size";
+    case OpKind::ComputeDelta: return "compute-delta-index";
+    case OpKind::MergeIndex: return "merge-index";
+    case OpKind::RebuildFromIndex: return "rebuild-index-from-index";
+    case OpKind::Reconstruct: return "reconstruct";
+    case OpKind::Hook: return "hook";
+  }
+  return "?";
+}
+
 static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no) {
+  if (e.trace) {
+    std::fprintf(stderr, "[srd] step %d iter %llu %s %s\n", step_no, (unsigned long long)iteration,
+                 op_name(op.kind), op.kind == OpKind::Pipeline ? op.rule.c_str()
+                                   : (op.rel >= 0 ? e.rels[op.rel].name.c_str() : ""));
+    std::fflush(stderr);
+  }
   switch (op.kind) {
     case OpKind::Pipeline:
       run_pipeline(e, op.pipe, iteration);
@@ -647,6 +679,11 @@ static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no
         r.deltas_step = step_no;
       }
       r.deltas.push_back(d.n);
+      if (e.trace) {
+        std::fprintf(stderr, "[srd]   delta %s = %llu (new %llu, full %llu)\n", r.name.c_str(),
+                     (unsigned long long)d.n, (unsigned long long)nw.n, (unsigned long long)fl.n);
+        std::fflush(stderr);
+      }
       r.delta[op.index] = std::move(d);
       r.delta_seq = r.full_version + 1;
       break;
@@ -857,6 +894,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
   try {
     pool_init(0);
     SRD_CHECK(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
+    const char* tr = std::getenv("SRDATALOG_TRACE");
+    e->trace = tr && *tr && *tr != '0';
     const char* q = std::getenv("SRDATALOG_QUIET");
     e->quiet = q && *q && *q != '0';
     if (const char* g = std::getenv("SRDATALOG_GRAIN")) e->grain_override = uint32_t(std::atoi(g));

@@ -63,11 +63,23 @@ __device__ __forceinline__ uint32_t owner_of(const uint64_t* pref, uint64_t j) {
   return lo;
 }
 
-// Persistent-wave slice dispenser: lane 0 takes the next slice id.
-__device__ __forceinline__ uint64_t next_slice(uint32_t* ctr) {
-  uint64_t s = 0;
-  if (srd::lane_id() == 0) s = atomicAdd(ctr, 1u);
-  return __shfl(s, 0, srd::kWave);
+// Persistent-wave slice dispenser.  Every lane executes the atomic (lane 0
+// adds 1, the others 0) and the wave takes lane 0's result with v_readlane,
+// a scalar.  There must be no lane-dependent branch around the atomic: with
+// `if (lane == 0) s = atomicAdd(...)` the compiler threads the lanes that
+// skip the branch (whose s is the constant 0) straight back to the
+// broadcast, forming a second loop in which lanes 1..63 re-read lane 0's
+// stale register and never fetch a new slice.
+__device__ __forceinline__ uint32_t next_slice(uint32_t* ctr) {
+  const uint32_t s = atomicAdd(ctr, srd::lane_id() == 0 ? 1u : 0u);
+  return __builtin_amdgcn_readlane(s, 0);
+}
+
+// Wave-uniform copy of a 64-bit value held by `lane`.
+__device__ __forceinline__ uint64_t uniform_from(uint64_t v, int lane) {
+  uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
+  uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), lane);
+  return (uint64_t(hi) << 32) | lo;
 }
 
 }  // namespace srdj

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(const T* __restrict__
     uint64_t inc = wave_incl_scan(v[i]);
     uint64_t idx = base + tile_pos(w, i, l);
     if (idx < n) out[idx] = off + inc - v[i];
-    off += __shfl(inc, kWave - 1, kWave);
+    off += read_lane_u64(inc, kWave - 1);
   }
   uint64_t last_block = (n == 0) ? 0 : (n - 1) / kTile;
   if (blockIdx.x == last_block && threadIdx.x == 0) out[n] = total;
@@ -326,16 +326,31 @@ struct PackSpec {
   uint32_t passes;
 };
 
+// Per-column maxima plus an "out of order" flag: out_max[kMaxCols] is set
+// when some row is lexicographically smaller than its predecessor.
 __global__ __launch_bounds__(kBlock) void col_max_kernel(PackSpec spec, uint64_t n,
                                                          uint32_t* __restrict__ out_max) {
-  for (uint32_t c = 0; c < spec.k; ++c) {
-    uint32_t m = 0;
-    for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
-         i += uint64_t(gridDim.x) * kBlock)
-      m = max(m, spec.in[c][i]);
-    m = wave_max(m);
-    if (lane_id() == 0 && m) atomicMax(&out_max[c], m);
+  uint32_t m[kMaxCols];
+  for (uint32_t c = 0; c < spec.k; ++c) m[c] = 0;
+  bool unsorted = false;
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    int order = 0;  // compare row i with row i-1
+    for (uint32_t c = 0; c < spec.k; ++c) {
+      uint32_t v = spec.in[c][i];
+      m[c] = max(m[c], v);
+      if (i > 0 && order == 0) {
+        uint32_t p = spec.in[c][i - 1];
+        order = v < p ? -1 : (v > p ? 1 : 0);
+      }
+    }
+    unsorted |= order < 0;
   }
+  for (uint32_t c = 0; c < spec.k; ++c) {
+    uint32_t v = wave_max(m[c]);
+    if (lane_id() == 0 && v) atomicMax(&out_max[c], v);
+  }
+  if (ballot(unsorted) && lane_id() == 0) atomicOr(&out_max[kMaxCols], 1u);
 }
 
 template <int W>
@@ -657,31 +672,33 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
   // Scratch layout in ws.small: [hist P*R][starts P*R][tile ctr P][err 1]
   // [col max 8][dedup tile counts 2*tiles].
   const uint64_t dtiles = (n + kTile - 1) / kTile;
-  ws.small.reserve(uint64_t(2 * kMaxPasses) * kRadix + kMaxPasses + 1 + kMaxCols + 2 * dtiles, s);
+  ws.small.reserve(uint64_t(2 * kMaxPasses) * kRadix + kMaxPasses + 1 + kMaxCols + 1 + 2 * dtiles, s);
   uint32_t* hist = ws.small.p;
   uint32_t* starts = hist + kMaxPasses * kRadix;
   uint32_t* ctr = starts + kMaxPasses * kRadix;
   uint32_t* err = ctr + kMaxPasses;
   uint32_t* dmax = err + 1;
-  uint32_t* tu = dmax + kMaxCols;
+  uint32_t* tu = dmax + kMaxCols + 1;
   uint32_t* tr = tu + dtiles;
 
-  // 1. column maxima -> packed bit layout
+  // 1. column maxima -> packed bit layout; input already in order -> no passes
   std::vector<uint32_t> mx(k, 0);
+  bool already_sorted = false;
   uint32_t* hp = nullptr;
   if (col_max && col_max->size() == k) {
     mx = *col_max;
   } else {
-    fill_u32(dmax, 0, k, s);
+    fill_u32(dmax, 0, kMaxCols + 1, s);
     {
       TimeScope t("sort.colmax", s, double(n) * 4 * k);
       col_max_kernel<<<grid_for(n, kBlock * 16, 2048), kBlock, 0, s>>>(ps, n, dmax);
       SRD_CHECK(hipGetLastError());
     }
-    hp = pinned_words(ws, k);
-    SRD_CHECK(hipMemcpyAsync(hp, dmax, k * 4, hipMemcpyDeviceToHost, s));
+    hp = pinned_words(ws, kMaxCols + 1);
+    SRD_CHECK(hipMemcpyAsync(hp, dmax, (kMaxCols + 1) * 4, hipMemcpyDeviceToHost, s));
     SRD_CHECK(hipStreamSynchronize(s));
     for (uint32_t c = 0; c < k; ++c) mx[c] = hp[c];
+    already_sorted = hp[kMaxCols] == 0;
   }
   UnpackSpec us{};
   us.k = k;
@@ -724,7 +741,7 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
     std::vector<bool> trivial(passes, false);
     for (uint32_t p = 0; p < passes; ++p)
       for (int d = 0; d < kRadix; ++d)
-        if (hp[p * kRadix + d] == n) trivial[p] = true;
+        if (hp[p * kRadix + d] == n || already_sorted) trivial[p] = true;
 
     const uint32_t items = (W == 1) ? 16 : 8;
     const uint64_t tile = uint64_t(kBlock) * items;

@@ -1,6 +1,8 @@
 // C ABI over the set primitives (declared in include/srdatalog_prims.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -170,6 +172,38 @@ int srd_prim_time_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint3
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   });
+}
+
+int srd_timing_enable(int on) {
+  srd::timing_enable(on != 0);
+  return 0;
+}
+
+uint64_t srd_timing_report(char* buf, uint64_t cap) {
+  std::string js = "[";
+  try {
+    auto v = srd::timing_collect();
+    for (size_t i = 0; i < v.size(); ++i) {
+      char tmp[512];
+      std::snprintf(tmp, sizeof(tmp), "%s{\"name\": \"%s\", \"launches\": %llu, \"ms\": %.6f, \"bytes\": %.1f}",
+                    i ? ", " : "", v[i].name.c_str(), (unsigned long long)v[i].launches, v[i].ms,
+                    v[i].bytes);
+      js += tmp;
+    }
+  } catch (const std::exception& e) {
+    g_err = e.what();
+  }
+  js += "]";
+  if (buf && cap) {
+    size_t n = std::min<size_t>(js.size(), cap - 1);
+    std::memcpy(buf, js.data(), n);
+    buf[n] = 0;
+  }
+  return js.size();
+}
+
+int srd_timing_reset(void) {
+  return guard([&] { srd::timing_reset(); });
 }
 
 }  // extern "C"

@@ -335,7 +335,7 @@ class PipelineGen:
     slots = self.root_slots()
     ns = len(slots)
     o += ["  for (;;) {",
-          "    const uint64_t slice = srdj::next_slice(A.work_ctr);",
+          "    const uint64_t slice = srdj::next_slice(A.work_ctr);  // wave-uniform",
           "    if (slice >= A.n_slices) break;",
           "    const uint64_t it0 = slice * A.grain;",
           "    const uint64_t it1 = min(it0 + A.grain, A.total_work);",
@@ -438,7 +438,7 @@ class PipelineGen:
           o.append(f"{ind}  s2 *= uint64_t({self.size_expr(r, True)});")
       o.append(f"{ind}}}")
       o.append(f"{ind}const uint64_t inc2 = srd::wave_incl_scan<uint64_t>(s2);")
-      o.append(f"{ind}const uint64_t tot2 = __shfl(inc2, 63, 64);")
+      o.append(f"{ind}const uint64_t tot2 = srdj::uniform_from(inc2, 63);")
       o.append(f"{ind}pref_s[wv][lane + 1] = inc2;")
       o.append(f"{ind}if (lane == 0) pref_s[wv][0] = 0;")
       for fi, expr in enumerate(frame):

@@ -104,6 +104,27 @@ def _bind_prims(lib) -> None:
   lib.srd_prim_exclusive_scan.argtypes = [_u32p, ctypes.c_uint64, _u64p]
   lib.srd_prim_time_sort_dedup.argtypes = [_u32p, ctypes.c_uint64, ctypes.c_uint32,
                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+  lib.srd_timing_enable.argtypes = [ctypes.c_int]
+  lib.srd_timing_report.restype = ctypes.c_uint64
+  lib.srd_timing_report.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
+
+
+def timing_enable(on: bool = True) -> None:
+  load_runtime().srd_timing_enable(1 if on else 0)
+
+
+def timing_reset() -> None:
+  load_runtime().srd_timing_reset()
+
+
+def timing_report() -> list[dict]:
+  """Per-kernel-name launches / summed ms / algorithmic bytes since the last reset."""
+  import json
+  lib = load_runtime()
+  n = lib.srd_timing_report(None, 0)
+  buf = ctypes.create_string_buffer(n + 1)
+  lib.srd_timing_report(buf, n + 1)
+  return json.loads(buf.value.decode())
 
 
 def _ptr32(a: np.ndarray):

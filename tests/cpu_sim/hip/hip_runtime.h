@@ -1,0 +1,46 @@
+// Host-only stand-in for <hip/hip_runtime.h>, used to compile the runtime's
+// HOST logic (srd_engine.hip) with g++ for CPU tests.  Device code paths are
+// never executed in this build: the device-side helpers compile to stubs.
+#pragma once
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+
+#define __device__
+#define __host__
+#define __global__
+#define __shared__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+
+typedef int hipError_t;
+typedef void* hipStream_t;
+typedef void* hipEvent_t;
+typedef void* hipMemPool_t;
+enum { hipSuccess = 0 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice,
+                     hipMemcpyHostToHost };
+enum { hipStreamNonBlocking = 1, hipHostMallocDefault = 0 };
+enum hipMemPoolAttr { hipMemPoolAttrReleaseThreshold };
+
+inline const char* hipGetErrorString(hipError_t) { return "cpu-sim error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)1; return 0; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  if (n) std::memmove(d, s, n);
+  return hipSuccess;
+}
+
+struct srd_dim3 { unsigned x = 0, y = 0, z = 0; };
+static srd_dim3 threadIdx, blockIdx, blockDim, gridDim;
+
+inline unsigned __builtin_amdgcn_mbcnt_hi(unsigned, unsigned) { std::abort(); }
+inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned, unsigned) { std::abort(); }
+inline unsigned long long __ballot(int) { std::abort(); }
+template <class T> inline T __shfl_up(T, int, int) { std::abort(); }
+template <class T> inline T __shfl_xor(T, int, int) { std::abort(); }
+template <class T> inline T __shfl(T, int, int) { std::abort(); }
+inline void __syncthreads() { std::abort(); }

@@ -1,0 +1,139 @@
+// CPU implementations of the runtime primitives declared in srd_runtime.h,
+// used only to exercise the engine's host logic on a machine without a GPU.
+// Semantics are the definitions (lexicographic sort + dedup, set
+// difference, disjoint union), independent of the device kernels.
+#include <algorithm>
+#include <cstdio>
+#include <stdexcept>
+#include <vector>
+
+#include "srd_runtime.h"
+
+namespace srd {
+
+void fail(const std::string& msg) { throw std::runtime_error(msg); }
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(what);
+}
+
+static PoolStats g_stats;
+void pool_init(int) {}
+void* pool_alloc(uint64_t bytes, hipStream_t) {
+  g_stats.in_use += bytes;
+  g_stats.peak = std::max(g_stats.peak, g_stats.in_use);
+  return std::malloc(bytes ? bytes : 1);
+}
+void pool_free(void* p, uint64_t bytes, hipStream_t) {
+  g_stats.in_use -= bytes;
+  std::free(p);
+}
+PoolStats pool_stats() { return g_stats; }
+void pool_set_budget(uint64_t b) { g_stats.budget = b; }
+void pool_reset_peak() { g_stats.peak = g_stats.in_use; }
+
+Workspace::~Workspace() { std::free(pinned); }
+uint32_t* pinned_words(Workspace& ws, uint64_t words) {
+  if (words > ws.pinned_words) {
+    std::free(ws.pinned);
+    ws.pinned = static_cast<uint32_t*>(std::malloc(words * 4));
+    ws.pinned_words = words;
+  }
+  return ws.pinned;
+}
+
+using Row = std::vector<uint32_t>;
+
+static std::vector<Row> rows_of(const Index& x) {
+  std::vector<Row> out(x.n, Row(x.arity));
+  for (uint64_t i = 0; i < x.n; ++i)
+    for (uint32_t c = 0; c < x.arity; ++c) out[i][c] = x.col(c)[i];
+  return out;
+}
+
+static void fill_index(Index& out, uint32_t k, const std::vector<Row>& rows, hipStream_t s) {
+  out = Index();
+  out.arity = k;
+  out.n = rows.size();
+  out.stride = round_stride(out.n);
+  out.col_max.assign(k, 0);
+  if (!out.n) return;
+  out.cols.alloc(out.stride * k, s);
+  for (uint64_t i = 0; i < out.n; ++i)
+    for (uint32_t c = 0; c < k; ++c) {
+      out.col(c)[i] = rows[i][c];
+      out.col_max[c] = std::max(out.col_max[c], rows[i][c]);
+    }
+  Workspace ws;
+  build_roots(out, ws, s);
+}
+
+void build_roots(Index& x, Workspace&, hipStream_t s) {
+  x.root_vals.release();
+  x.root_offs.release();
+  x.r = 0;
+  if (!x.n) return;
+  std::vector<uint32_t> v, o;
+  for (uint64_t i = 0; i < x.n; ++i)
+    if (i == 0 || x.col(0)[i] != x.col(0)[i - 1]) v.push_back(x.col(0)[i]), o.push_back(uint32_t(i));
+  o.push_back(uint32_t(x.n));
+  x.r = uint32_t(v.size());
+  x.root_vals.alloc(v.size(), s);
+  x.root_offs.alloc(o.size(), s);
+  std::copy(v.begin(), v.end(), x.root_vals.p);
+  std::copy(o.begin(), o.end(), x.root_offs.p);
+}
+
+void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace&,
+                 hipStream_t s, const std::vector<uint32_t>*) {
+  std::vector<Row> rows(n, Row(k));
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint32_t c = 0; c < k; ++c) rows[i][c] = in[c][i];
+  std::sort(rows.begin(), rows.end());
+  rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+  fill_index(out, k, rows, s);
+}
+
+void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index& out,
+                      Workspace& ws, hipStream_t s) {
+  std::vector<const uint32_t*> in(src.arity);
+  for (uint32_t c = 0; c < src.arity; ++c) in[c] = src.n ? src.col(perm[c]) : nullptr;
+  build_index(in.data(), src.arity, src.n, out, ws, s, nullptr);
+}
+
+void set_difference(const Index& a, const Index& b, Index& out, Workspace&, hipStream_t s) {
+  auto ra = rows_of(a), rb = rows_of(b);
+  std::vector<Row> d;
+  std::set_difference(ra.begin(), ra.end(), rb.begin(), rb.end(), std::back_inserter(d));
+  fill_index(out, a.arity, d, s);
+}
+
+void merge_disjoint(const Index& a, const Index& b, Index& out, Workspace&, hipStream_t s) {
+  auto ra = rows_of(a), rb = rows_of(b);
+  std::vector<Row> m;
+  std::merge(ra.begin(), ra.end(), rb.begin(), rb.end(), std::back_inserter(m));
+  fill_index(out, a.arity ? a.arity : b.arity, m, s);
+}
+
+void copy_index(const Index& a, Index& out, hipStream_t s) { fill_index(out, a.arity, rows_of(a), s); }
+
+void exclusive_scan_u32(const uint32_t* in, uint64_t* out, uint64_t n, Workspace&, hipStream_t) {
+  uint64_t t = 0;
+  for (uint64_t i = 0; i < n; ++i) out[i] = t, t += in[i];
+  out[n] = t;
+}
+void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace&, hipStream_t) {
+  uint64_t t = 0;
+  for (uint64_t i = 0; i < n; ++i) out[i] = t, t += in[i];
+  out[n] = t;
+}
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
+void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
+
+void timing_enable(bool) {}
+bool timing_enabled() { return false; }
+void timing_begin(const char*, hipStream_t) {}
+void timing_end(hipStream_t, double) {}
+std::vector<KernelTime> timing_collect() { return {}; }
+void timing_reset() {}
+
+}  // namespace srd
