@@ -42,6 +42,9 @@ def parse():
                  help="CPU baseline: root keys x in [0, K) of the same graph")
   p.add_argument("--no-cpu-baseline", action="store_true")
   p.add_argument("--no-parity", action="store_true")
+  p.add_argument("--calib-copy-bytes", type=int, default=0,
+                 help="also run the plain streaming copy kernel over this many bytes "
+                      "(profiler counter calibration)")
   return p.parse_args()
 
 
@@ -63,8 +66,15 @@ def main() -> int:
 
   t0 = time.time()
   edges = datasets.triangle_edges(args.seed, args.scale, args.edges)
+  # rows in a seeded random order: the Knows index build then really sorts
+  edges = edges[np.random.default_rng(args.seed + 1).permutation(edges.shape[0])]
   gen_s = time.time() - t0
   log(f"generated {edges.shape[0]} edges in {gen_s:.1f}s")
+
+  if args.calib_copy_bytes:
+    ms = native.calib_copy(args.calib_copy_bytes, reps=1)
+    log(f"calib copy {args.calib_copy_bytes} B: {ms:.3f} ms "
+        f"({2 * args.calib_copy_bytes / ms / 1e6:.1f} GB/s)")
 
   lib = programs.load("triangle")
   if lib.init() != 0:

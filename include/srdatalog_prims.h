@@ -52,6 +52,11 @@ int srd_prim_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out);
 int srd_prim_time_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint32_t reps,
                              double* ms_per_rep);
 
+/* Calibration: `reps` launches of srd_calib_copy_kernel, a plain streaming
+ * copy of `bytes` (read once, written once); *ms_per_rep = mean duration.
+ * Its profiler HBM counters give the scale for the other kernels' counters. */
+int srd_calib_copy(uint64_t bytes, int reps, double* ms_per_rep);
+
 /* Kernel timing: while enabled, every primitive and join launch on the
  * runtime's streams is bracketed by HIP events.  srd_timing_report writes a
  * JSON array [{"name", "launches", "ms", "bytes"}] (ms = summed duration,

@@ -228,6 +228,13 @@ struct srd_engine {
   // per-pipeline scratch, grown on demand
   srd::DevBuf<uint32_t> key_rng, key_aux, counts_buf, work_ctr;
   srd::DevBuf<uint64_t> key_work, work_start, offsets;
+  // fused-mode staging, grown from the largest output each rule produced
+  srd::DevBuf<uint32_t> stage, overflow;
+  srd::DevBuf<uint64_t> recs, stage_ctr;
+  std::map<std::string, uint64_t> stage_hint;
+  bool two_pass = false;           // SRDATALOG_JOIN_MODE=twopass: count + write only
+  bool no_direct = false;          // SRDATALOG_JOIN_MODE=fused: never the direct mode
+  uint64_t stage_cap_env = 0;      // SRDATALOG_STAGE_CAP: fixed staging rows (tests overflow)
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
 };
@@ -406,13 +413,16 @@ static const Index& full_index(srd_engine& e, Relation& r, const Order& order) {
   } else if (r.has_input && r.full_version == 0) {
     upload_input(e, r);
     std::vector<const uint32_t*> in(r.arity);
-    std::vector<uint32_t> mx(r.arity);
+    bool identity = true;
     for (uint32_t c = 0; c < r.arity; ++c) {
       in[c] = r.dev_input.p + uint64_t(order[c]) * r.dev_stride;
-      mx[c] = r.input_max[order[c]];
+      identity = identity && order[c] == c;
     }
+    // Input already sorted and duplicate-free in this order is adopted as is
+    // (the resident input is immutable, so the index may share it).
     if (r.host_n)
-      build_index(in.data(), r.arity, r.host_n, v.idx, e.ws, e.s, &mx);
+      build_index(in.data(), r.arity, r.host_n, v.idx, e.ws, e.s, nullptr,
+                  identity ? &r.dev_input : nullptr, r.dev_stride);
     else
       v.idx = empty_index(r.arity);
   } else if (r.full_version == 0 && !r.has_input) {
@@ -447,7 +457,9 @@ static const Index& new_index(srd_engine& e, Relation& r, const Order& order) {
     Order p = perm_between(r.raw_order, order);
     std::vector<const uint32_t*> in(r.arity);
     for (uint32_t c = 0; c < r.arity; ++c) in[c] = r.raw.p + uint64_t(p[c]) * r.raw_cap;
-    build_index(in.data(), r.arity, r.raw_n, x, e.ws, e.s);
+    // NEW rows that a join emitted already in index order are adopted
+    build_index(in.data(), r.arity, r.raw_n, x, e.ws, e.s, nullptr,
+                r.raw_order == order ? &r.raw : nullptr, r.raw_cap);
   } else if (!r.newv.empty()) {
     auto& any = *r.newv.begin();
     build_index_from(any.second, perm_between(any.first, order), x, e.ws, e.s);
@@ -558,36 +570,91 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
       a.counts = e.counts_buf.p;
       a.offsets = e.offsets.p;
       a.grid_blocks = uint32_t(std::min<uint64_t>((slices + 3) / 4, 2048));
-      fill_u32(a.work_ctr, 0, 2, e.s);
-      {
-        std::string nm = std::string("join.count:") + d->rule;
-        timing_begin(nm.c_str(), e.s);
-        d->count(&a, e.s);
-        SRD_CHECK(hipGetLastError());
-        timing_end(e.s);
-      }
-      exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
-      total = read_u64(e, a.offsets + slices);
-      if (e.trace) {
-        std::fprintf(stderr, "[srd]   %s: counted %llu tuples\n", d->rule, (unsigned long long)total);
-        std::fflush(stderr);
-      }
-      if (total > 0 && !d->count_only) {
+      auto bind_dests = [&](uint64_t n_rows) {
         for (uint32_t k = 0; k < d->n_dest; ++k) {
           const SrdDestSpec& ds = d->dest[k];
           Relation& r = e.rels[rel_id(e, ds.rel)];
           Order order(ds.cols, ds.cols + ds.arity);
-          uint64_t base = reserve_new(e, r, order, total);
+          uint64_t base = reserve_new(e, r, order, n_rows);
           a.out[k].cols = r.raw.p;
           a.out[k].stride = r.raw_cap;
           a.out[k].base = base;
         }
+      };
+      auto launch = [&](SrdLaunchFn fn, const char* phase) {
         fill_u32(a.work_ctr, 0, 2, e.s);
-        std::string nm = std::string("join.write:") + d->rule;
+        std::string nm = std::string("join.") + phase + ":" + d->rule;
         timing_begin(nm.c_str(), e.s);
-        d->write(&a, e.s);
+        fn(&a, e.s);
         SRD_CHECK(hipGetLastError());
         timing_end(e.s);
+      };
+      if (d->direct_ok && !d->count_only && !e.two_pass && !e.no_direct) {
+        // one tuple per item: rows land at base + item, nothing to count
+        total = a.total_work;
+        bind_dests(total);
+        launch(d->direct, "direct");
+      } else if (d->count_only || e.two_pass) {
+        launch(d->count, "count");
+        exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
+        total = read_u64(e, a.offsets + slices);
+        if (total > 0 && !d->count_only) {
+          bind_dests(total);
+          launch(d->write, "write");
+        }
+      } else {
+        // fused: one traversal into staging segments, then gather
+        uint32_t kmax = 0;
+        for (uint32_t k = 0; k < d->n_dest; ++k) kmax = std::max(kmax, 8 * k + d->dest[k].arity);
+        uint64_t cap = std::max<uint64_t>(e.stage_hint[d->rule], uint64_t(1) << 24);
+        cap = std::max<uint64_t>(cap, slices * 2 * SRD_SEG);
+        if (e.stage_cap_env) cap = e.stage_cap_env;
+        if (e.stage.n < cap * kmax) e.stage.alloc(cap * kmax, e.s);
+        if (!e.stage_cap_env) cap = e.stage.n / kmax;
+        const uint64_t rec_cap = slices + cap / (SRD_SEG - 64) + 64;
+        e.recs.reserve(3 * rec_cap, e.s);
+        e.stage_ctr.reserve(2, e.s);
+        e.overflow.reserve(2, e.s);
+        fill_u64(e.stage_ctr.p, 0, 2, e.s);
+        fill_u32(e.overflow.p, 0, 2, e.s);
+        a.stage = e.stage.p;
+        a.stage_cap = cap;
+        a.stage_ctr = reinterpret_cast<unsigned long long*>(e.stage_ctr.p);
+        a.recs = e.recs.p;
+        a.rec_cap = rec_cap;
+        a.rec_ctr = e.overflow.p + 1;
+        a.overflow = e.overflow.p;
+        launch(d->fused, "fused");
+        exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
+        uint32_t* h = pinned_words(e.ws, 4);
+        SRD_CHECK(hipMemcpyAsync(h, a.offsets + slices, 8, hipMemcpyDeviceToHost, e.s));
+        SRD_CHECK(hipMemcpyAsync(h + 2, e.overflow.p, 4, hipMemcpyDeviceToHost, e.s));
+        SRD_CHECK(hipStreamSynchronize(e.s));
+        total = uint64_t(h[0]) | (uint64_t(h[1]) << 32);
+        const bool ovf = h[2] != 0;
+        e.stage_hint[d->rule] = std::max<uint64_t>(e.stage_hint[d->rule],
+                                                   total + total / 4 + slices * SRD_SEG);
+        if (total > 0) {
+          bind_dests(total);
+          if (!ovf) {
+            GatherDest gd[SRD_MAX_DESTS];
+            for (uint32_t k = 0; k < d->n_dest; ++k)
+              gd[k] = GatherDest{a.out[k].cols, a.out[k].stride, a.out[k].base, d->dest[k].arity};
+            gather_segments(a.recs, a.rec_ctr, rec_cap, a.stage, cap, a.offsets, gd, d->n_dest,
+                            total, e.s);
+          } else {
+            launch(d->write, "write");
+          }
+        }
+        if (e.trace) {
+          std::fprintf(stderr, "[srd]   %s: fused %llu tuples%s\n", d->rule,
+                       (unsigned long long)total, ovf ? " (staging overflow: write pass)" : "");
+          std::fflush(stderr);
+        }
+      }
+      if (e.trace) {
+        std::fprintf(stderr, "[srd]   %s: %llu tuples\n", d->rule, (unsigned long long)total);
+        std::fflush(stderr);
       }
     }
   }
@@ -899,6 +966,11 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     const char* q = std::getenv("SRDATALOG_QUIET");
     e->quiet = q && *q && *q != '0';
     if (const char* g = std::getenv("SRDATALOG_GRAIN")) e->grain_override = uint32_t(std::atoi(g));
+    if (const char* c = std::getenv("SRDATALOG_STAGE_CAP")) e->stage_cap_env = std::strtoull(c, nullptr, 10);
+    if (const char* m = std::getenv("SRDATALOG_JOIN_MODE")) {
+      e->two_pass = std::string(m) == "twopass";
+      e->no_direct = std::string(m) == "fused";
+    }
     for (uint32_t i = 0; i < n; ++i) e->pipes[pipes[i].rule] = &pipes[i];
     parse_program(*e, text);
   } catch (const std::exception& ex) {

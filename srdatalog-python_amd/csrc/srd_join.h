@@ -75,6 +75,35 @@ __device__ __forceinline__ uint32_t next_slice(uint32_t* ctr) {
   return __builtin_amdgcn_readlane(s, 0);
 }
 
+// Fused mode: open a staging segment of SRD_SEG rows for the wave (all lanes
+// take part; lane 0's bump is broadcast with v_readlane).  Returns ~0 when
+// staging is exhausted (the overflow flag is raised; the slice keeps
+// counting, and the host then re-runs the write pass).
+__device__ __forceinline__ uint64_t seg_open(const SrdLaunchArgs& A) {
+  const unsigned long long r =
+      atomicAdd(A.stage_ctr, srd::lane_id() == 0 ? (unsigned long long)SRD_SEG : 0ull);
+  const uint64_t s = srd::read_lane_u64(r, 0);
+  const bool ovf = s + SRD_SEG > A.stage_cap;
+  atomicOr(A.overflow, (ovf && srd::lane_id() == 0) ? 1u : 0u);
+  return ovf ? ~uint64_t(0) : s;
+}
+
+// Fused mode: log segment [start, start+used) as rows local..local+used of
+// `slice`'s output.
+__device__ __forceinline__ void seg_close(const SrdLaunchArgs& A, uint64_t slice, uint64_t start,
+                                          uint32_t used, uint32_t local) {
+  if (start == ~uint64_t(0) || used == 0) return;
+  const uint32_t r = atomicAdd(A.rec_ctr, srd::lane_id() == 0 ? 1u : 0u);
+  const uint32_t idx = __builtin_amdgcn_readlane(r, 0);
+  const bool ovf = idx >= A.rec_cap;
+  atomicOr(A.overflow, (ovf && srd::lane_id() == 0) ? 1u : 0u);
+  if (!ovf && srd::lane_id() == 0) {
+    A.recs[3ull * idx] = slice;
+    A.recs[3ull * idx + 1] = start;
+    A.recs[3ull * idx + 2] = (uint64_t(local) << 32) | used;
+  }
+}
+
 // Wave-uniform copy of a 64-bit value held by `lane`.
 __device__ __forceinline__ uint64_t uniform_from(uint64_t v, int lane) {
   uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);

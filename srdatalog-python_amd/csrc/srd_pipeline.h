@@ -1,19 +1,27 @@
 // Contract between the runtime's MIR executor and the per-rule join kernels
 // that the Python emitter generates for a program.
 //
-// A pipeline (one `execute-pipeline` MIR node) runs as three launches:
+// A pipeline (one `execute-pipeline` MIR node) runs as:
 //   root   one thread per distinct key of the driver root source: look the
 //          key up in every root source, record the ranges, and size the
 //          first nested level's iteration space (key_work);
-//   count  the per-key work is flattened into one global item space (an
-//          exclusive scan of key_work); persistent waves pull fixed-size
-//          slices of it from an atomic counter, so a slice may span many
-//          small keys or a fraction of one huge key, and count the tuples
-//          each slice emits;
-//   write  the same traversal, writing each slice's tuples at its
-//          exclusive-scan offset in the destination NEW buffer.
-// Output order therefore depends only on the item order, never on which
-// wave ran a slice.
+//   then the per-key work is flattened into one global item space (an
+//   exclusive scan of key_work) cut into fixed-size slices that persistent
+//   waves pull from an atomic counter, so a slice may span many small keys
+//   or a fraction of one huge key.  The traversal runs in one of four modes:
+//   direct (3)  the rule emits exactly one tuple per item (last level is a
+//               filter-free cartesian product or scan): tuple i goes to row
+//               base+i of NEW, one pass, no counting;
+//   fused  (2)  one pass; each slice appends its tuples to segments of a
+//               staging buffer (bump-allocated with an atomic) and logs
+//               (slice, local offset, source, count) records; after the
+//               scan of the per-slice counts a gather kernel moves every
+//               segment to its slice's offset in NEW;
+//   count  (0)  counts only (count-only rules; fallback when staging
+//               overflows);
+//   write  (1)  re-runs the traversal writing at the scanned offsets.
+// Output order depends only on the item order, never on which wave ran a
+// slice, so every mode yields the same NEW rows in the same order.
 //
 // Reference counterpart: the JitRunner_<rule> phases setup / launch_count /
 // scan_and_resize / launch_materialize of
@@ -72,7 +80,17 @@ struct SrdLaunchArgs {
   uint64_t* offsets;           // exclusive scan of counts
   uint32_t* work_ctr;          // atomic slice counter (zeroed before each launch)
   SrdOut out[SRD_MAX_DESTS];
+  // fused mode staging (column c of dest d at stage + (d*SRD_MAX_COLS + c)*stage_cap)
+  uint32_t* stage;
+  uint64_t stage_cap;          // tuples
+  unsigned long long* stage_ctr;   // bump allocator (tuples)
+  uint64_t* recs;              // 3 u64 per segment record
+  uint64_t rec_cap;
+  uint32_t* rec_ctr;
+  uint32_t* overflow;          // set when staging or the record log ran out
 };
+
+#define SRD_SEG 1024u          // staging segment size (tuples)
 
 typedef void (*SrdLaunchFn)(const SrdLaunchArgs* args, hipStream_t stream);
 
@@ -87,9 +105,12 @@ struct SrdPipelineDesc {
   SrdDestSpec dest[SRD_MAX_DESTS];
   uint32_t count_only;               // rule carries .with_count()
   uint32_t default_grain;
+  uint32_t direct_ok;                // one tuple per level-1 item (mode 3 valid)
   SrdLaunchFn root;
   SrdLaunchFn count;
   SrdLaunchFn write;
+  SrdLaunchFn fused;
+  SrdLaunchFn direct;
 };
 
 #ifdef __cplusplus

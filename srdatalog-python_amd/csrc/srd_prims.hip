@@ -220,6 +220,91 @@ void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
   SRD_CHECK(hipGetLastError());
 }
 
+// ============================================================ gather ====
+struct GatherArgs {
+  const uint64_t* recs;
+  const uint32_t* rec_ctr;
+  uint64_t rec_cap;
+  const uint32_t* stage;
+  uint64_t stage_stride;
+  const uint64_t* offsets;
+  GatherDest d[2];
+  uint32_t nd;
+};
+
+// One wave per segment record; lanes copy consecutive rows (coalesced).
+__global__ __launch_bounds__(kBlock) void gather_segments_kernel(GatherArgs g) {
+  const uint64_t nrec = min<uint64_t>(*g.rec_ctr, g.rec_cap);
+  const uint32_t l = lane_id();
+  const uint64_t waves = uint64_t(gridDim.x) * kWavesPerBlock;
+  for (uint64_t r = uint64_t(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave; r < nrec;
+       r += waves) {
+    const uint64_t slice = g.recs[3 * r], src = g.recs[3 * r + 1], lc = g.recs[3 * r + 2];
+    const uint64_t cnt = lc & 0xFFFFFFFFu, local = lc >> 32;
+    const uint64_t dst0 = g.offsets[slice] + local;
+    for (uint32_t d = 0; d < g.nd; ++d) {
+      const GatherDest& D = g.d[d];
+      for (uint32_t c = 0; c < D.k; ++c) {
+        const uint32_t* sc = g.stage + (uint64_t(d) * 8 + c) * g.stage_stride + src;
+        uint32_t* dc = D.cols + uint64_t(c) * D.stride + D.base + dst0;
+        for (uint64_t j = l; j < cnt; j += kWave) dc[j] = sc[j];
+      }
+    }
+  }
+}
+
+void gather_segments(const uint64_t* recs, const uint32_t* rec_ctr, uint64_t rec_cap,
+                     const uint32_t* stage, uint64_t stage_stride, const uint64_t* offsets,
+                     const GatherDest* dests, uint32_t n_dests, uint64_t total, hipStream_t s) {
+  if (n_dests > 2) fail("gather_segments: at most two destinations");
+  GatherArgs g{};
+  g.recs = recs;
+  g.rec_ctr = rec_ctr;
+  g.rec_cap = rec_cap;
+  g.stage = stage;
+  g.stage_stride = stage_stride;
+  g.offsets = offsets;
+  g.nd = n_dests;
+  double bytes = 0;
+  for (uint32_t d = 0; d < n_dests; ++d) {
+    g.d[d] = dests[d];
+    bytes += 8.0 * double(total) * dests[d].k;
+  }
+  TimeScope t("join.gather", s, bytes);
+  gather_segments_kernel<<<2048, kBlock, 0, s>>>(g);
+  SRD_CHECK(hipGetLastError());
+}
+
+// ========================================================= calibration ====
+// Plain streaming copy: reads and writes exactly n words, so profiler HBM
+// counters read on it give the bytes-per-counter-unit scale.
+__global__ __launch_bounds__(kBlock) void srd_calib_copy_kernel(const uint4* __restrict__ src,
+                                                                uint4* __restrict__ dst,
+                                                                uint64_t n16) {
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n16;
+       i += uint64_t(gridDim.x) * kBlock)
+    dst[i] = src[i];
+}
+
+void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep) {
+  const uint64_t n16 = (bytes + 15) / 16;
+  DevBuf<uint4> a(n16, s), b(n16, s);
+  hipEvent_t e0, e1;
+  SRD_CHECK(hipEventCreate(&e0));
+  SRD_CHECK(hipEventCreate(&e1));
+  const uint32_t grid = 256u * 8u;
+  srd_calib_copy_kernel<<<grid, kBlock, 0, s>>>(a.p, b.p, n16);  // warm-up (first touch)
+  SRD_CHECK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) srd_calib_copy_kernel<<<grid, kBlock, 0, s>>>(a.p, b.p, n16);
+  SRD_CHECK(hipEventRecord(e1, s));
+  SRD_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  SRD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_rep = reps ? ms / reps : 0;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
 // ================================================================ scan ====
 constexpr int kItems = 16;
 constexpr uint32_t kTile = kBlock * kItems;  // 4096
@@ -326,13 +411,14 @@ struct PackSpec {
   uint32_t passes;
 };
 
-// Per-column maxima plus an "out of order" flag: out_max[kMaxCols] is set
-// when some row is lexicographically smaller than its predecessor.
+// Per-column maxima plus order flags in out_max[kMaxCols]: bit 0 = some row
+// is lexicographically smaller than its predecessor, bit 1 = some row equals
+// its predecessor.  0 means strictly increasing: already a valid index.
 __global__ __launch_bounds__(kBlock) void col_max_kernel(PackSpec spec, uint64_t n,
                                                          uint32_t* __restrict__ out_max) {
   uint32_t m[kMaxCols];
   for (uint32_t c = 0; c < spec.k; ++c) m[c] = 0;
-  bool unsorted = false;
+  bool unsorted = false, dup = false;
   for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
        i += uint64_t(gridDim.x) * kBlock) {
     int order = 0;  // compare row i with row i-1
@@ -345,12 +431,14 @@ __global__ __launch_bounds__(kBlock) void col_max_kernel(PackSpec spec, uint64_t
       }
     }
     unsorted |= order < 0;
+    dup |= (i > 0 && order == 0);
   }
   for (uint32_t c = 0; c < spec.k; ++c) {
     uint32_t v = wave_max(m[c]);
     if (lane_id() == 0 && v) atomicMax(&out_max[c], v);
   }
-  if (ballot(unsorted) && lane_id() == 0) atomicOr(&out_max[kMaxCols], 1u);
+  const uint32_t fl = (ballot(unsorted) ? 1u : 0u) | (ballot(dup) ? 2u : 0u);
+  if (fl && lane_id() == 0) atomicOr(&out_max[kMaxCols], fl);
 }
 
 template <int W>
@@ -657,7 +745,8 @@ __global__ __launch_bounds__(kBlock) void dedup_write_kernel(
 static uint32_t bit_width(uint32_t v) { return v ? 32 - __builtin_clz(v) : 0; }
 
 void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace& ws,
-                 hipStream_t s, const std::vector<uint32_t>* col_max) {
+                 hipStream_t s, const std::vector<uint32_t>* col_max, const DevBuf<uint32_t>* adopt,
+                 uint64_t adopt_stride) {
   if (k == 0 || k > uint32_t(kMaxCols)) fail("build_index: arity must be in [1, 8]");
   if (n >= (uint64_t(1) << 32)) fail("build_index: more than 2^32-1 rows");
   out = Index();
@@ -698,7 +787,24 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
     SRD_CHECK(hipMemcpyAsync(hp, dmax, (kMaxCols + 1) * 4, hipMemcpyDeviceToHost, s));
     SRD_CHECK(hipStreamSynchronize(s));
     for (uint32_t c = 0; c < k; ++c) mx[c] = hp[c];
-    already_sorted = hp[kMaxCols] == 0;
+    already_sorted = (hp[kMaxCols] & 1u) == 0;
+    if (hp[kMaxCols] == 0) {
+      // strictly increasing: the rows are already a sorted, duplicate-free index
+      out.n = n;
+      out.col_max = mx;
+      if (adopt && adopt->p) {
+        out.cols = *adopt;
+        out.stride = adopt_stride;
+      } else {
+        out.stride = round_stride(n);
+        out.cols.alloc(out.stride * k, s);
+        for (uint32_t c = 0; c < k; ++c)
+          SRD_CHECK(hipMemcpyAsync(out.cols.p + uint64_t(c) * out.stride, in[c], n * 4,
+                                   hipMemcpyDeviceToDevice, s));
+      }
+      build_roots(out, ws, s);
+      return;
+    }
   }
   UnpackSpec us{};
   us.k = k;
@@ -1140,22 +1246,16 @@ static void set_difference_k(const Index& a, const Index& b, Index& out, Workspa
   build_roots(out, ws, s);
 }
 
-void copy_index(const Index& a, Index& out, hipStream_t s) {
-  out = Index();
+void copy_index(const Index& a, Index& out, hipStream_t) {
+  if (&a == &out) return;
   out.arity = a.arity;
   out.n = a.n;
   out.stride = a.stride;
   out.r = a.r;
   out.col_max = a.col_max;
-  if (a.n == 0) return;
-  out.cols.alloc(a.stride * a.arity, s);
-  out.root_vals.alloc(a.r, s);
-  out.root_offs.alloc(a.r + 1, s);
-  SRD_CHECK(hipMemcpyAsync(out.cols.p, a.cols.p, a.stride * a.arity * 4, hipMemcpyDeviceToDevice, s));
-  SRD_CHECK(hipMemcpyAsync(out.root_vals.p, a.root_vals.p, uint64_t(a.r) * 4,
-                           hipMemcpyDeviceToDevice, s));
-  SRD_CHECK(hipMemcpyAsync(out.root_offs.p, a.root_offs.p, uint64_t(a.r + 1) * 4,
-                           hipMemcpyDeviceToDevice, s));
+  out.cols = a.cols;
+  out.root_vals = a.root_vals;
+  out.root_offs = a.root_offs;
 }
 
 void set_difference(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s) {

@@ -174,6 +174,13 @@ int srd_prim_time_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint3
   });
 }
 
+int srd_calib_copy(uint64_t bytes, int reps, double* ms_per_rep) {
+  return guard([&] {
+    Ctx& c = ctx();
+    srd::calib_copy(bytes, reps, c.s, ms_per_rep);
+  });
+}
+
 int srd_timing_enable(int on) {
   srd::timing_enable(on != 0);
   return 0;

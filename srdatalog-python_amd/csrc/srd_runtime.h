@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -45,38 +46,33 @@ PoolStats pool_stats();
 void pool_set_budget(uint64_t bytes);
 void pool_reset_peak();
 
-// Owned device buffer allocated from the pool.
+// Device buffer allocated from the pool.  Copies share the allocation (the
+// last owner returns it to the pool on its stream): indexes are immutable
+// once built, so FULL / DELTA / NEW can share columns without copying.
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   uint64_t n = 0;
   hipStream_t s = nullptr;
+  std::shared_ptr<void> owner;
   DevBuf() = default;
   DevBuf(uint64_t count, hipStream_t st) { alloc(count, st); }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s) { o.p = nullptr, o.n = 0; }
-  DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) {
-      release();
-      p = o.p, n = o.n, s = o.s;
-      o.p = nullptr, o.n = 0;
-    }
-    return *this;
-  }
-  ~DevBuf() { release(); }
   void alloc(uint64_t count, hipStream_t st) {
     release();
     s = st;
     n = count;
-    p = count ? static_cast<T*>(pool_alloc(count * sizeof(T), st)) : nullptr;
+    if (count) {
+      p = static_cast<T*>(pool_alloc(count * sizeof(T), st));
+      const uint64_t bytes = count * sizeof(T);
+      owner = std::shared_ptr<void>(p, [bytes, st](void* q) { pool_free(q, bytes, st); });
+    }
   }
   // Grow to at least `count` elements (contents not preserved).
   void reserve(uint64_t count, hipStream_t st) {
     if (count > n) alloc(count + count / 4, st);
   }
   void release() {
-    if (p) pool_free(p, n * sizeof(T), s);
+    owner.reset();
     p = nullptr;
     n = 0;
   }
@@ -131,9 +127,13 @@ uint32_t* pinned_words(Workspace& ws, uint64_t words);
 
 // Sort + dedup the k input columns (column c at in[c], n rows, taken in
 // the given order) into `out` (replaced).  `col_max` may give known upper
-// bounds per column (empty: computed on the device).
+// bounds per column (empty: computed on the device, together with an
+// order check).  Rows already strictly increasing skip the sort entirely;
+// if `adopt` holds those same columns (column c at adopt->p + c*stride) the
+// index takes the buffer over without copying.
 void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace& ws,
-                 hipStream_t s, const std::vector<uint32_t>* col_max = nullptr);
+                 hipStream_t s, const std::vector<uint32_t>* col_max = nullptr,
+                 const DevBuf<uint32_t>* adopt = nullptr, uint64_t adopt_stride = 0);
 
 // Same, with the source columns permuted: out column c = src column perm[c].
 void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index& out,
@@ -148,7 +148,7 @@ void merge_disjoint(const Index& a, const Index& b, Index& out, Workspace& ws, h
 // (Re)build root_vals / root_offs of an index whose columns are sorted.
 void build_roots(Index& x, Workspace& ws, hipStream_t s);
 
-// Deep copy.
+// Copy (shares the immutable column / root buffers).
 void copy_index(const Index& a, Index& out, hipStream_t s);
 
 // out[0..n] = exclusive prefix sums of in[0..n) with out[n] = total.
@@ -160,6 +160,23 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace
 // Fill n words with value (a kernel, safe under stream capture).
 void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 void fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
+
+// Move staged join segments to their final rows: record i (3 u64:
+// slice, source row, local_offset<<32 | count) copies `count` rows of the k
+// staging columns (stride stage_stride, dest d's columns starting at column
+// d*8) to row out_base + offsets[slice] + local_offset of each dest.
+struct GatherDest {
+  uint32_t* cols;
+  uint64_t stride;
+  uint64_t base;
+  uint32_t k;
+};
+void gather_segments(const uint64_t* recs, const uint32_t* rec_ctr, uint64_t rec_cap,
+                     const uint32_t* stage, uint64_t stage_stride, const uint64_t* offsets,
+                     const GatherDest* dests, uint32_t n_dests, uint64_t total, hipStream_t s);
+
+// Streaming copy of `bytes` (reps launches) for bandwidth / counter calibration.
+void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep);
 
 // Kernel timing hooks: when enabled, every primitive kernel launch is
 // bracketed by events and its duration accumulated under its name.

@@ -147,21 +147,57 @@ class PipelineGen:
     return f"([&]({params}) -> bool {{ {f.code} }})({args})"
 
   def emit_code(self, cond: str, ind: str) -> str:
+    """Wave-cooperative emission of this lane's tuple when `cond` holds.
+
+    MODE 0 counts, 1 writes at the slice's scanned offset, 2 appends to the
+    slice's staging segment, 3 writes at the item's own row (direct)."""
     lines = [f"{ind}{{",
              f"{ind}  const uint64_t m_ = srd::ballot({cond});",
+             f"{ind}  const uint32_t c_ = __popcll(m_);",
              f"{ind}  if (MODE == 1 && ({cond})) {{",
              f"{ind}    const uint64_t o_ = opos + srd::popc_below(m_);"]
     for di, d in enumerate(self.dests):
       for c, relcol in enumerate(d.index):
-        term = d.terms[relcol]
         lines.append(f"{ind}    A.out[{di}].cols[{c} * A.out[{di}].stride + A.out[{di}].base + o_]"
-                     f" = {self.var(term)};")
+                     f" = {self.var(d.terms[relcol])};")
     lines += [f"{ind}  }}",
-              f"{ind}  const uint32_t c_ = __popcll(m_);",
+              f"{ind}  if (MODE == 3 && ({cond})) {{"]
+    for di, d in enumerate(self.dests):
+      for c, relcol in enumerate(d.index):
+        lines.append(f"{ind}    A.out[{di}].cols[{c} * A.out[{di}].stride + A.out[{di}].base + item]"
+                     f" = {self.var(d.terms[relcol])};")
+    lines += [f"{ind}  }}",
+              f"{ind}  if (MODE == 2 && c_) {{",
+              f"{ind}    if (seg_start == ~uint64_t(0) || seg_used + c_ > SRD_SEG) {{",
+              f"{ind}      srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
+              f"{ind}      seg_start = srdj::seg_open(A);",
+              f"{ind}      seg_used = 0;",
+              f"{ind}      seg_local = cnt;",
+              f"{ind}    }}",
+              f"{ind}    if (seg_start != ~uint64_t(0) && ({cond})) {{",
+              f"{ind}      const uint64_t o_ = seg_start + seg_used + srd::popc_below(m_);"]
+    for di, d in enumerate(self.dests):
+      for c, relcol in enumerate(d.index):
+        lines.append(f"{ind}      A.stage[{di * 8 + c}ull * A.stage_cap + o_] = {self.var(d.terms[relcol])};")
+    lines += [f"{ind}    }}",
+              f"{ind}    seg_used += c_;",
+              f"{ind}  }}",
               f"{ind}  opos += c_;",
               f"{ind}  cnt += c_;",
               f"{ind}}}"]
     return "\n".join(lines)
+
+  def direct_ok(self) -> bool:
+    """Every level-1 item yields exactly one tuple: no filters, and either a
+    scan root, a root join with no nested level, or a cartesian last level
+    directly under the root."""
+    if self.filters:
+      return False
+    if isinstance(self.root, M.Scan):
+      return True
+    if not self.levels:
+      return True
+    return len(self.levels) == 1 and isinstance(self.levels[0], M.CartesianJoin)
 
   # ------------------------------------------------------------ ranges ---
   def resolve(self, reg: dict, v: int, prefix: tuple, code: list[str], ind: str, tag: str) -> Rg:
@@ -339,8 +375,10 @@ class PipelineGen:
           "    if (slice >= A.n_slices) break;",
           "    const uint64_t it0 = slice * A.grain;",
           "    const uint64_t it1 = min(it0 + A.grain, A.total_work);",
-          "    uint64_t opos = MODE ? A.offsets[slice] : 0;",
+          "    uint64_t opos = MODE == 1 ? A.offsets[slice] : 0;",
           "    uint32_t cnt = 0;",
+          "    uint64_t seg_start = ~uint64_t(0);",
+          "    uint32_t seg_used = 0, seg_local = 0;",
           "    for (uint64_t base = it0; base < it1; base += 64) {",
           "      const uint64_t item = base + lane;",
           "      const bool act = item < it1;",
@@ -497,7 +535,8 @@ class PipelineGen:
       o.append(f"{ind}}}")
       o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
     o += ["    }",
-          "    if (MODE == 0 && lane == 0) A.counts[slice] = cnt;",
+          "    if (MODE == 2) srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
+          "    if ((MODE == 0 || MODE == 2) && lane == 0) A.counts[slice] = cnt;",
           "  }",
           "}"]
     return "\n".join(o)
@@ -513,6 +552,12 @@ class PipelineGen:
            "}",
            "void launch_write(const SrdLaunchArgs* a, hipStream_t s) {",
            "  run_k<1><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}",
+           "void launch_fused(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<2><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}",
+           "void launch_direct(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<3><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}",
            f"}}  // namespace {self.ns}", ""]
     return "\n".join(out)
@@ -534,7 +579,9 @@ class PipelineGen:
     return ("  {" + f"\"{self.ep.rule}\", {len(views)}, {{{', '.join(views)}}}, {kind}, "
             f"{len(roots)}, {{{', '.join(str(r) for r in roots)}}}, {len(dests)}, "
             f"{{{', '.join(dests)}}}, {1 if self.ep.count else 0}, {grain}, "
-            f"&{self.ns}::launch_root, &{self.ns}::launch_count, &{self.ns}::launch_write" + "}")
+            f"{1 if self.direct_ok() else 0}, "
+            f"&{self.ns}::launch_root, &{self.ns}::launch_count, &{self.ns}::launch_write, "
+            f"&{self.ns}::launch_fused, &{self.ns}::launch_direct" + "}")
 
 
 @dataclass

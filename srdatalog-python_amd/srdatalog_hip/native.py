@@ -104,9 +104,18 @@ def _bind_prims(lib) -> None:
   lib.srd_prim_exclusive_scan.argtypes = [_u32p, ctypes.c_uint64, _u64p]
   lib.srd_prim_time_sort_dedup.argtypes = [_u32p, ctypes.c_uint64, ctypes.c_uint32,
                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+  lib.srd_calib_copy.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
   lib.srd_timing_enable.argtypes = [ctypes.c_int]
   lib.srd_timing_report.restype = ctypes.c_uint64
   lib.srd_timing_report.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
+
+
+def calib_copy(nbytes: int, reps: int = 5) -> float:
+  """ms per launch of the plain streaming copy kernel over nbytes."""
+  lib = load_runtime()
+  ms = ctypes.c_double()
+  _check(lib, lib.srd_calib_copy(nbytes, reps, ctypes.byref(ms)), "calib_copy")
+  return ms.value
 
 
 def timing_enable(on: bool = True) -> None:

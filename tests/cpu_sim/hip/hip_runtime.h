@@ -44,3 +44,4 @@ template <class T> inline T __shfl_up(T, int, int) { std::abort(); }
 template <class T> inline T __shfl_xor(T, int, int) { std::abort(); }
 template <class T> inline T __shfl(T, int, int) { std::abort(); }
 inline void __syncthreads() { std::abort(); }
+inline unsigned __builtin_amdgcn_readlane(unsigned, int) { std::abort(); }

@@ -84,7 +84,7 @@ void build_roots(Index& x, Workspace&, hipStream_t s) {
 }
 
 void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace&,
-                 hipStream_t s, const std::vector<uint32_t>*) {
+                 hipStream_t s, const std::vector<uint32_t>*, const DevBuf<uint32_t>*, uint64_t) {
   std::vector<Row> rows(n, Row(k));
   for (uint64_t i = 0; i < n; ++i)
     for (uint32_t c = 0; c < k; ++c) rows[i][c] = in[c][i];
@@ -97,7 +97,7 @@ void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index
                       Workspace& ws, hipStream_t s) {
   std::vector<const uint32_t*> in(src.arity);
   for (uint32_t c = 0; c < src.arity; ++c) in[c] = src.n ? src.col(perm[c]) : nullptr;
-  build_index(in.data(), src.arity, src.n, out, ws, s, nullptr);
+  build_index(in.data(), src.arity, src.n, out, ws, s, nullptr, nullptr, 0);
 }
 
 void set_difference(const Index& a, const Index& b, Index& out, Workspace&, hipStream_t s) {
@@ -114,7 +114,7 @@ void merge_disjoint(const Index& a, const Index& b, Index& out, Workspace&, hipS
   fill_index(out, a.arity ? a.arity : b.arity, m, s);
 }
 
-void copy_index(const Index& a, Index& out, hipStream_t s) { fill_index(out, a.arity, rows_of(a), s); }
+void copy_index(const Index& a, Index& out, hipStream_t) { out = a; }
 
 void exclusive_scan_u32(const uint32_t* in, uint64_t* out, uint64_t n, Workspace&, hipStream_t) {
   uint64_t t = 0;
@@ -128,6 +128,21 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace
 }
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
 void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
+
+void gather_segments(const uint64_t* recs, const uint32_t* rec_ctr, uint64_t rec_cap,
+                     const uint32_t* stage, uint64_t stage_stride, const uint64_t* offsets,
+                     const GatherDest* dests, uint32_t n_dests, uint64_t, hipStream_t) {
+  uint64_t n = std::min<uint64_t>(*rec_ctr, rec_cap);
+  for (uint64_t r = 0; r < n; ++r) {
+    uint64_t slice = recs[3 * r], src = recs[3 * r + 1], lc = recs[3 * r + 2];
+    uint64_t cnt = lc & 0xFFFFFFFFu, local = lc >> 32;
+    for (uint32_t d = 0; d < n_dests; ++d)
+      for (uint32_t c = 0; c < dests[d].k; ++c)
+        for (uint64_t j = 0; j < cnt; ++j)
+          dests[d].cols[c * dests[d].stride + dests[d].base + offsets[slice] + local + j] =
+              stage[(uint64_t(d) * 8 + c) * stage_stride + src + j];
+  }
+}
 
 void timing_enable(bool) {}
 bool timing_enabled() { return false; }

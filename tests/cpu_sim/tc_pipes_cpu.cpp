@@ -19,6 +19,51 @@ srd::Rng cpu_root_find(const srd::IndexView& v, uint32_t key) {
   return {v.root_offs[g], v.root_offs[g + 1]};
 }
 
+// Emission shared by the modes: MODE 0 count, 1 write at the slice offset,
+// 2 stage into SRD_SEG-row segments + records, 3 direct (row = item).
+struct Emit {
+  const SrdLaunchArgs* a;
+  int mode;
+  uint64_t slice = 0, opos = 0, seg = ~0ull, item = 0;
+  uint32_t cnt = 0, used = 0, local = 0;
+  void close() {
+    if (seg == ~0ull || used == 0) return;
+    uint32_t idx = (*a->rec_ctr)++;
+    if (idx >= a->rec_cap) { *a->overflow = 1; return; }
+    a->recs[3 * idx] = slice;
+    a->recs[3 * idx + 1] = seg;
+    a->recs[3 * idx + 2] = (uint64_t(local) << 32) | used;
+  }
+  void put(uint32_t c0, uint32_t c1) {
+    if (mode == 1) {
+      a->out[0].cols[a->out[0].base + opos] = c0;
+      a->out[0].cols[a->out[0].stride + a->out[0].base + opos] = c1;
+    } else if (mode == 3) {
+      a->out[0].cols[a->out[0].base + item] = c0;
+      a->out[0].cols[a->out[0].stride + a->out[0].base + item] = c1;
+    } else if (mode == 2) {
+      if (seg == ~0ull || used + 1 > SRD_SEG) {
+        close();
+        uint64_t s = *a->stage_ctr;
+        *a->stage_ctr += SRD_SEG;
+        seg = (s + SRD_SEG > a->stage_cap) ? (*a->overflow = 1, ~0ull) : s;
+        used = 0;
+        local = cnt;
+      }
+      if (seg != ~0ull) {
+        a->stage[seg + used] = c0;
+        a->stage[a->stage_cap + seg + used] = c1;
+      }
+      ++used;
+    }
+    ++opos, ++cnt;
+  }
+  void end() {
+    if (mode == 2) close();
+    if (mode == 0 || mode == 2) a->counts[slice] = cnt;
+  }
+};
+
 // ---- scans: key_work[0] = rows; items are rows; swap = write (y, x)
 template <bool SWAP>
 void scan_root(const SrdLaunchArgs* a, hipStream_t) {
@@ -26,21 +71,20 @@ void scan_root(const SrdLaunchArgs* a, hipStream_t) {
   a->key_aux[0] = 0;
 }
 
-template <bool SWAP, bool WRITE>
+template <bool SWAP, int MODE>
 void scan_run(const SrdLaunchArgs* a, hipStream_t) {
   const srd::IndexView& v = a->views.v[0];
   for (uint64_t s = 0; s < a->n_slices; ++s) {
     uint64_t lo = s * a->grain, hi = std::min<uint64_t>(lo + a->grain, a->total_work);
-    if (!WRITE) {
-      a->counts[s] = uint32_t(hi - lo);
-      continue;
-    }
-    uint64_t o = a->offsets[s];
-    for (uint64_t i = lo; i < hi; ++i, ++o) {
+    Emit em{a, MODE};
+    em.slice = s;
+    em.opos = MODE == 1 ? a->offsets[s] : 0;
+    for (uint64_t i = lo; i < hi; ++i) {
       uint32_t x = colp(v, 0)[i], y = colp(v, 1)[i];
-      a->out[0].cols[a->out[0].base + o] = SWAP ? y : x;
-      a->out[0].cols[a->out[0].stride + a->out[0].base + o] = SWAP ? x : y;
+      em.item = i;
+      em.put(SWAP ? y : x, SWAP ? x : y);
     }
+    em.end();
   }
 }
 
@@ -60,40 +104,39 @@ void rec_root(const SrdLaunchArgs* a, hipStream_t) {
   }
 }
 
-template <bool WRITE>
+template <int MODE>
 void rec_run(const SrdLaunchArgs* a, hipStream_t) {
   for (uint64_t s = 0; s < a->n_slices; ++s) {
     uint64_t lo = s * a->grain, hi = std::min<uint64_t>(lo + a->grain, a->total_work);
-    uint64_t o = WRITE ? a->offsets[s] : 0;
-    uint32_t cnt = 0;
+    Emit em{a, MODE};
+    em.slice = s;
+    em.opos = MODE == 1 ? a->offsets[s] : 0;
     for (uint64_t it = lo; it < hi; ++it) {
       uint32_t k = uint32_t(std::upper_bound(a->work_start, a->work_start + a->n_root + 1, it) -
                             a->work_start) - 1;
       uint64_t li = it - a->work_start[k];
-      uint32_t b0 = a->key_rng[k * 4], e0 = a->key_rng[k * 4 + 1];
+      uint32_t b0 = a->key_rng[k * 4];
       uint32_t b1 = a->key_rng[k * 4 + 2], e1 = a->key_rng[k * 4 + 3];
       uint32_t d1 = e1 - b1;
       uint32_t x = colp(a->views.v[0], 1)[b0 + li / d1];
       uint32_t z = colp(a->views.v[1], 1)[b1 + li % d1];
-      (void)e0;
-      if (WRITE) {
-        a->out[0].cols[a->out[0].base + o] = z;
-        a->out[0].cols[a->out[0].stride + a->out[0].base + o] = x;
-        ++o;
-      }
-      ++cnt;
+      em.item = it;
+      em.put(z, x);
     }
-    if (!WRITE) a->counts[s] = cnt;
+    em.end();
   }
 }
 
 const SrdPipelineDesc k_pipes[] = {
   {"EdgeLoad", 1, {{"ArcInput", 2, {0, 1}, SRD_FULL}}, SRD_ROOT_SCAN, 1, {0}, 1,
-   {{"Edge", 2, {0, 1}}}, 0, 4096, &scan_root<false>, &scan_run<false, false>, &scan_run<false, true>},
+   {{"Edge", 2, {0, 1}}}, 0, 4096, 1, &scan_root<false>, &scan_run<false, 0>, &scan_run<false, 1>,
+   &scan_run<false, 2>, &scan_run<false, 3>},
   {"TCBase", 1, {{"Edge", 2, {0, 1}, SRD_FULL}}, SRD_ROOT_SCAN, 1, {0}, 1,
-   {{"Path", 2, {1, 0}}}, 0, 4096, &scan_root<true>, &scan_run<true, false>, &scan_run<true, true>},
+   {{"Path", 2, {1, 0}}}, 0, 4096, 1, &scan_root<true>, &scan_run<true, 0>, &scan_run<true, 1>,
+   &scan_run<true, 2>, &scan_run<true, 3>},
   {"TCRec_D0", 2, {{"Path", 2, {1, 0}, SRD_DELTA}, {"Edge", 2, {0, 1}, SRD_FULL}}, SRD_ROOT_JOIN, 2,
-   {0, 1}, 1, {{"Path", 2, {1, 0}}}, 0, 7, &rec_root, &rec_run<false>, &rec_run<true>},
+   {0, 1}, 1, {{"Path", 2, {1, 0}}}, 0, 7, 1, &rec_root, &rec_run<0>, &rec_run<1>, &rec_run<2>,
+   &rec_run<3>},
 };
 
 }  // namespace

@@ -66,8 +66,14 @@ def run_tc(lib, edges, max_iters=0):
   return out[:n.value], [int(buf[i]) for i in range(cnt)], size
 
 
+@pytest.mark.parametrize("mode", ["direct", "fused", "twopass"])
 @pytest.mark.parametrize("n,m,seed", [(12, 20, 0), (60, 150, 1), (300, 700, 2)])
-def test_engine_tc_matches_oracle(eng, n, m, seed):
+def test_engine_tc_matches_oracle(eng, n, m, seed, mode, monkeypatch):
+  # the engine reads SRDATALOG_JOIN_MODE when it is created
+  if mode != "direct":
+    monkeypatch.setenv("SRDATALOG_JOIN_MODE", mode)
+  else:
+    monkeypatch.delenv("SRDATALOG_JOIN_MODE", raising=False)
   e = ds.tc_edges(seed, n, m)
   want = oracle.run("tc", {"ArcInput": e})
   path, deltas, size = run_tc(eng, e)

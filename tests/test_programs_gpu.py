@@ -196,3 +196,37 @@ def test_csv_load_all(progs, tmp_path):
     check_rel(lib, orc, "Path", 2)
   finally:
     lib.shutdown()
+
+
+# ------------------------------------------------------------------ modes ----
+# The engine picks the direct / fused / two-pass join mode per rule; each is
+# forced here (SRDATALOG_JOIN_MODE is read when the engine is created) and
+# must give the same relation and deltas as the oracle.  SRDATALOG_STAGE_CAP
+# shrinks the fused staging buffer so the overflow -> write-pass fallback runs.
+MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
+         ({"SRDATALOG_JOIN_MODE": "fused"}, "fused"),
+         ({"SRDATALOG_JOIN_MODE": "fused", "SRDATALOG_STAGE_CAP": "3000"}, "fused-overflow"),
+         ({}, "default")]
+
+
+@pytest.mark.parametrize("env,label", MODES, ids=[m[1] for m in MODES])
+def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
+  for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP"):
+    monkeypatch.delenv(k, raising=False)
+  for k, v in env.items():
+    monkeypatch.setenv(k, v)
+  cases = [
+    ("tc", {"ArcInput": ds.tc_edges(7, 800, 3000)}, "Path", 2, "tc"),
+    ("sg", {"Arc": ds.sg_forest_edges(np.random.default_rng(3), 4000)}, "Sg", 2, "sg"),
+    ("triangle", {"KnowsInput": ds.triangle_edges(4, 12, 80_000)}, "Triangle", 3, "triangle"),
+    ("andersen", ds.andersen_inputs(5, n_vars=2000, n_objs=500, sizes=(1000, 4000, 1000, 1000)),
+     "PointsTo", 2, "andersen"),
+  ]
+  for prog, inputs, rel, arity, oname in cases:
+    orc = oracle.run(oname, inputs)
+    lib = run_prog(progs, prog, inputs)
+    try:
+      check_rel(lib, orc, rel, arity)
+      assert lib.delta_sizes(rel) == orc.deltas[rel], (prog, label)
+    finally:
+      lib.shutdown()
