@@ -42,6 +42,8 @@ def parse():
                  help="CPU baseline: root keys x in [0, K) of the same graph")
   p.add_argument("--no-cpu-baseline", action="store_true")
   p.add_argument("--no-parity", action="store_true")
+  p.add_argument("--prim-rows", type=int, default=1 << 26,
+                 help="rows of the sort/set-difference/merge benchmarks (0 = skip)")
   p.add_argument("--calib-copy-bytes", type=int, default=0,
                  help="also run the plain streaming copy kernel over this many bytes "
                       "(profiler counter calibration)")
@@ -50,6 +52,47 @@ def parse():
 
 def log(msg: str) -> None:
   print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def primitives(native, n_sort: int, reps: int) -> dict:
+  """Index-maintenance primitives at full scale on device-generated u32
+  pairs (k=2, w=8 B/tuple): sort+dedup of n_sort unordered rows with 40-bit
+  keys (5 radix passes), set-difference NEW(n) - FULL(2n), merge of FULL(2n)
+  and DELTA(n/2).  Algorithmic bytes per call:
+    sort+dedup      2*n*w*passes + n*w + u*w + n*4 + r*4 (r <= 2^20)
+    set-difference  (|N|+|F|)*w + |D|*w
+    merge           2*(|F|+|D|)*w
+  """
+  w, passes = 8, 5
+  out = {}
+  n = n_sort
+  ms, u = native.prim_bench("sort_dedup", n, 0, 2, reps)
+  alg = 2 * n * w * passes + n * w + u * w + n * 4 + (1 << 20) * 4
+  out["sort_dedup"] = {"rows": n, "distinct": u, "ms": round(ms, 4), "alg_bytes": alg}
+  na, nb = n, 2 * n
+  ms, d = native.prim_bench("set_difference", na, nb, 2, reps)
+  out["set_difference"] = {"new": na, "full": nb, "delta": d, "ms": round(ms, 4),
+                           "alg_bytes": (na + nb) * w + d * w}
+  na, nb = 2 * n, n // 2
+  ms, m = native.prim_bench("merge", na, nb, 2, reps)
+  out["merge"] = {"full": na, "delta": nb, "ms": round(ms, 4), "alg_bytes": 2 * m * w}
+  for v in out.values():
+    gbs = v["alg_bytes"] / (v["ms"] * 1e-3) / 1e9
+    v["GB/s"] = round(gbs, 1)
+    v["frac"] = round(gbs / HBM_PEAK_GBS, 4)
+  # per-kernel split of one more call of each, from HIP events
+  native.timing_reset()
+  native.timing_enable(True)
+  for op, (a, b) in (("sort_dedup", (n, 0)), ("set_difference", (n, 2 * n)),
+                     ("merge", (2 * n, n // 2))):
+    native.prim_bench(op, a, b, 2, 1)
+  native.timing_enable(False)
+  ks = native.timing_report()
+  native.timing_reset()
+  for k in ks:
+    k["GB/s"] = round(k["bytes"] / (k["ms"] * 1e-3) / 1e9, 1) if k["ms"] > 0 else None
+  out["kernels"] = sorted(ks, key=lambda k: -k["ms"])
+  return out
 
 
 def main() -> int:
@@ -125,6 +168,12 @@ def main() -> int:
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": traffic, "avg_ms": round(avg_ms, 4), "alg_bytes": alg}
 
+  prims = None
+  if args.prim_rows:
+    prims = primitives(native, args.prim_rows, reps=max(args.steps, 1))
+    log("primitives: " + ", ".join(f"{k} {v['GB/s']} GB/s" for k, v in prims.items()
+                                    if k != "kernels"))
+
   parity = None
   if not args.no_parity:
     import oracle
@@ -172,6 +221,7 @@ def main() -> int:
                "gen_s": round(gen_s, 2)},
     "roofline": roofline,
     "cpu_baseline": cpu,
+    "primitives": prims,
     "kernels": sorted(kernels, key=lambda k: -k["ms"])[:12],
   }
   print(json.dumps(out))

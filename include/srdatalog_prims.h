@@ -52,6 +52,17 @@ int srd_prim_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out);
 int srd_prim_time_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint32_t reps,
                              double* ms_per_rep);
 
+/* Device-resident benchmark of one primitive on synthetic inputs generated
+ * on the device (k columns; columns 1..k-1 are 20-bit fields of a 64-bit key,
+ * column 0 the rest):
+ *   op 0  sort+dedup of na unordered rows (keys hashed over 2^(20k) values);
+ *   op 1  set-difference NEW - FULL, NEW = {2i : i < na}, FULL = {3j : j < nb};
+ *   op 2  merge of disjoint FULL = {3i : i < na} and DELTA = {3j+1 : j < nb}.
+ * One untimed call, then `reps` timed calls; *ms_per_rep = mean wall time
+ * per call (HIP events on the primitive's stream); *out_n = output rows. */
+int srd_prim_bench(int op, uint64_t na, uint64_t nb, uint32_t k, uint32_t reps,
+                   double* ms_per_rep, uint64_t* out_n);
+
 /* Calibration: `reps` launches of srd_calib_copy_kernel, a plain streaming
  * copy of `bytes` (read once, written once); *ms_per_rep = mean duration.
  * Its profiler HBM counters give the scale for the other kernels' counters. */

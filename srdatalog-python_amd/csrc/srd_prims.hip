@@ -305,6 +305,43 @@ void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep) {
   (void)hipEventDestroy(e1);
 }
 
+// Synthetic relation columns for primitive benchmarks, generated in place.
+// Row i has the 64-bit key  mul*i + off  (range == 0: strictly ascending)
+// or  mix(i + off) % range  (range > 0: unordered, with repeats).  The key is
+// split into k fields: the low k-1 fields take fbits bits each, column 0 the
+// rest, so the lexicographic row order equals the key order.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;  // splitmix64 finaliser
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void pattern_kernel(uint32_t* __restrict__ cols,
+                                                         uint64_t stride, uint64_t n, uint32_t k,
+                                                         uint32_t fbits, uint64_t mul,
+                                                         uint64_t off, uint64_t range) {
+  const uint64_t fmask = (uint64_t(1) << fbits) - 1;
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    uint64_t key = range ? mix64(i + off) % range : mul * i + off;
+    for (int c = int(k) - 1; c > 0; --c) {
+      cols[uint64_t(c) * stride + i] = uint32_t(key & fmask);
+      key >>= fbits;
+    }
+    cols[i] = uint32_t(key);
+  }
+}
+
+void fill_pattern(uint32_t* cols, uint64_t stride, uint64_t n, uint32_t k, uint32_t fbits,
+                  uint64_t mul, uint64_t off, uint64_t range, hipStream_t s) {
+  if (n == 0) return;
+  if (k == 0 || k > 8 || fbits == 0 || fbits > 32)
+    fail("fill_pattern: bad arity or field width");
+  pattern_kernel<<<256u * 8u, kBlock, 0, s>>>(cols, stride, n, k, fbits, mul, off, range);
+  SRD_CHECK(hipGetLastError());
+}
+
 // ================================================================ scan ====
 constexpr int kItems = 16;
 constexpr uint32_t kTile = kBlock * kItems;  // 4096

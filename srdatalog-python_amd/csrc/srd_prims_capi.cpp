@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -169,6 +170,70 @@ int srd_prim_time_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint3
     float ms = 0;
     SRD_CHECK(hipEventElapsedTime(&ms, e0, e1));
     *ms_per_rep = reps ? ms / reps : 0.0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+int srd_prim_bench(int op, uint64_t na, uint64_t nb, uint32_t k, uint32_t reps,
+                   double* ms_per_rep, uint64_t* out_n) {
+  return guard([&] {
+    Ctx& c = ctx();
+    if (k < 1 || k > 4) throw std::runtime_error("srd_prim_bench: arity must be 1..4");
+    const uint32_t fb = 20;                       // width of columns 1..k-1
+    const uint32_t low = fb * (k - 1);
+    auto sorted_index = [&](uint64_t n, uint64_t mul, uint64_t off) {
+      srd::Index x;
+      x.arity = k;
+      x.n = n;
+      x.col_max.assign(k, k > 1 ? (1u << fb) - 1 : 0u);
+      if (n == 0) return x;
+      const unsigned __int128 top = (unsigned __int128)mul * (n - 1) + off;
+      if ((top >> low) > 0xffffffffu) throw std::runtime_error("srd_prim_bench: keys overflow u32");
+      x.col_max[0] = uint32_t(uint64_t(top >> low));
+      x.stride = srd::round_stride(n);
+      x.cols.alloc(x.stride * k, c.s);
+      srd::fill_pattern(x.cols.p, x.stride, n, k, fb, mul, off, 0, c.s);
+      srd::build_roots(x, c.ws, c.s);
+      return x;
+    };
+    srd::Index a, b, o;
+    srd::DevBuf<uint32_t> raw;
+    std::vector<const uint32_t*> in(k);
+    uint64_t stride = srd::round_stride(na);
+    if (op == 0) {
+      raw.alloc(stride * k + 1, c.s);
+      const uint32_t bits = k == 1 ? 32 : low + 20;  // column 0 spans 2^20 values
+      srd::fill_pattern(raw.p, stride, na, k, fb, 0, 0, uint64_t(1) << bits, c.s);
+      for (uint32_t j = 0; j < k; ++j) in[j] = raw.p + j * stride;
+    } else if (op == 1) {
+      a = sorted_index(na, 2, 0);                 // NEW  = {2i}
+      b = sorted_index(nb, 3, 0);                 // FULL = {3j}
+    } else if (op == 2) {
+      a = sorted_index(na, 3, 0);                 // FULL  = {3i}
+      b = sorted_index(nb, 3, 1);                 // DELTA = {3j+1}, disjoint
+    } else {
+      throw std::runtime_error("srd_prim_bench: op must be 0 (sort+dedup), 1 (set-difference) "
+                               "or 2 (merge)");
+    }
+    auto once = [&] {
+      if (op == 0) srd::build_index(in.data(), k, na, o, c.ws, c.s);
+      else if (op == 1) srd::set_difference(a, b, o, c.ws, c.s);
+      else srd::merge_disjoint(a, b, o, c.ws, c.s);
+    };
+    once();  // warm-up: pool growth, first touch
+    SRD_CHECK(hipStreamSynchronize(c.s));
+    hipEvent_t e0, e1;
+    SRD_CHECK(hipEventCreate(&e0));
+    SRD_CHECK(hipEventCreate(&e1));
+    SRD_CHECK(hipEventRecord(e0, c.s));
+    for (uint32_t r = 0; r < reps; ++r) once();
+    SRD_CHECK(hipEventRecord(e1, c.s));
+    SRD_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    SRD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_rep = reps ? ms / reps : 0.0;
+    if (out_n) *out_n = o.n;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   });

@@ -178,6 +178,12 @@ void gather_segments(const uint64_t* recs, const uint32_t* rec_ctr, uint64_t rec
 // Streaming copy of `bytes` (reps launches) for bandwidth / counter calibration.
 void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep);
 
+// Synthetic columns (k fields of one 64-bit key per row, low k-1 fields
+// fbits wide): key = mul*i + off when range == 0 (ascending, distinct for
+// mul > 0), else a hash of i+off reduced mod range.
+void fill_pattern(uint32_t* cols, uint64_t stride, uint64_t n, uint32_t k, uint32_t fbits,
+                  uint64_t mul, uint64_t off, uint64_t range, hipStream_t s);
+
 // Kernel timing hooks: when enabled, every primitive kernel launch is
 // bracketed by events and its duration accumulated under its name.
 struct KernelTime {

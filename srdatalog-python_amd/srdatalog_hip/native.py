@@ -105,6 +105,8 @@ def _bind_prims(lib) -> None:
   lib.srd_prim_time_sort_dedup.argtypes = [_u32p, ctypes.c_uint64, ctypes.c_uint32,
                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
   lib.srd_calib_copy.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+  lib.srd_prim_bench.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), _u64p]
   lib.srd_timing_enable.argtypes = [ctypes.c_int]
   lib.srd_timing_report.restype = ctypes.c_uint64
   lib.srd_timing_report.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
@@ -116,6 +118,20 @@ def calib_copy(nbytes: int, reps: int = 5) -> float:
   ms = ctypes.c_double()
   _check(lib, lib.srd_calib_copy(nbytes, reps, ctypes.byref(ms)), "calib_copy")
   return ms.value
+
+
+PRIM_OPS = {"sort_dedup": 0, "set_difference": 1, "merge": 2}
+
+
+def prim_bench(op: str, na: int, nb: int = 0, k: int = 2, reps: int = 5) -> tuple[float, int]:
+  """(ms per call, output rows) of one primitive on device-generated inputs
+  (see srd_prim_bench in include/srdatalog_prims.h)."""
+  lib = load_runtime()
+  ms = ctypes.c_double()
+  n = ctypes.c_uint64()
+  _check(lib, lib.srd_prim_bench(PRIM_OPS[op], na, nb, k, reps, ctypes.byref(ms), ctypes.byref(n)),
+         "prim_bench")
+  return ms.value, n.value
 
 
 def timing_enable(on: bool = True) -> None:

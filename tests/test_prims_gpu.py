@@ -96,3 +96,30 @@ def test_exclusive_scan(nat, n):
   got = nat.exclusive_scan(x)
   want = np.concatenate([[0], np.cumsum(x, dtype=np.uint64)])
   np.testing.assert_array_equal(got, want)
+
+
+# ------------------------------------------------------ device benchmark ----
+def _splitmix(x):
+  x = (x + np.uint64(0x9E3779B97F4A7C15)).astype(np.uint64)
+  x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)).astype(np.uint64)
+  x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)).astype(np.uint64)
+  return x ^ (x >> np.uint64(31))
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_prim_bench_output_sizes(k):
+  from srdatalog_hip import native
+  n = 300_000
+  # sort+dedup of hashed keys: distinct count from the same hash on the host
+  bits = 32 if k == 1 else 20 * (k - 1) + 20
+  with np.errstate(over="ignore"):
+    keys = _splitmix(np.arange(n, dtype=np.uint64)) % np.uint64(1 << bits)
+  _, got = native.prim_bench("sort_dedup", n, 0, k, reps=2)
+  assert got == np.unique(keys).size
+  # NEW = {2i}, FULL = {3j}: NEW - FULL drops the multiples of 6 both reach
+  na, nb = 200_000, 150_000
+  m = min(2 * (na - 1), 3 * (nb - 1))
+  _, got = native.prim_bench("set_difference", na, nb, k, reps=2)
+  assert got == na - (m // 6 + 1)
+  _, got = native.prim_bench("merge", na, nb, k, reps=2)
+  assert got == na + nb
