@@ -77,6 +77,20 @@ int srdatalog_prepare(void);
 /* Wall seconds of the last srdatalog_run's fixpoint (inputs resident). */
 double srdatalog_run_seconds(void);
 
+/* Instance table of a packed batch (no counterpart in the reference, which
+ * runs one instance per process): bounds[0..n_bounds) ascending, instance g
+ * owns column-0 values [bounds[g], bounds[g+1]).  While set, every DELTA
+ * computation also counts each instance's DELTA rows on the device.
+ * n_bounds = 0 clears the table.  0 ok, 2 error. */
+int srdatalog_set_instances(const uint32_t* bounds, uint64_t n_bounds);
+
+/* Per-instance |DELTA| of the last step that computed rel_name: row i of
+ * out (n_inst entries, n_inst = n_bounds - 1) is iteration i; up to
+ * cap_iters rows are written.  Returns the number of iterations, -1 for an
+ * unknown relation or error. */
+long long srdatalog_instance_delta_sizes(const char* rel_name, unsigned long long* out,
+                                         unsigned long long cap_iters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -205,6 +205,8 @@ struct Relation {
   // results
   std::vector<uint64_t> deltas;     // per-iteration |DELTA| of the last step computing it
   int deltas_step = -1;
+  // with an instance table: per iteration, |DELTA| rows of every instance
+  std::vector<DevBuf<uint64_t>> inst_deltas;
   uint64_t count = 0;               // count-only result
   bool counted = false;
 };
@@ -237,6 +239,9 @@ struct srd_engine {
   uint64_t stage_cap_env = 0;      // SRDATALOG_STAGE_CAP: fixed staging rows (tests overflow)
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
+  // instance table of a packed batch: n_inst+1 ascending column-0 bounds
+  srd::DevBuf<uint32_t> inst_bounds;
+  uint32_t n_inst = 0;
 };
 
 namespace srd {
@@ -743,9 +748,15 @@ static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no
       r.delta.clear();
       if (r.deltas_step != step_no) {
         r.deltas.clear();
+        r.inst_deltas.clear();
         r.deltas_step = step_no;
       }
       r.deltas.push_back(d.n);
+      if (e.n_inst) {
+        DevBuf<uint64_t> per(e.n_inst, e.s);
+        instance_counts(d.n ? d.cols.p : nullptr, d.n, e.inst_bounds.p, e.n_inst, per.p, e.s);
+        r.inst_deltas.push_back(per);
+      }
       if (e.trace) {
         std::fprintf(stderr, "[srd]   delta %s = %llu (new %llu, full %llu)\n", r.name.c_str(),
                      (unsigned long long)d.n, (unsigned long long)nw.n, (unsigned long long)fl.n);
@@ -815,6 +826,7 @@ static void reset_state(srd_engine& e) {
     clear_new(r);
     r.full_version = r.delta_seq = 0;
     r.deltas.clear();
+    r.inst_deltas.clear();
     r.deltas_step = -1;
     r.count = 0;
     r.counted = false;
@@ -1125,6 +1137,40 @@ int srd_engine_shutdown(srd_engine* e) {
   }
   e->ran = false;
   return 0;
+}
+
+int srd_engine_set_instances(srd_engine* e, const uint32_t* bounds, uint64_t n_bounds) {
+  if (!e) return 1;
+  return eguard(e, 2, [&] {
+    if (n_bounds == 1 || n_bounds > (uint64_t(1) << 31)) fail("set_instances: need 0 or >= 2 bounds");
+    for (uint64_t i = 1; i < n_bounds; ++i)
+      if (bounds[i] < bounds[i - 1]) fail("set_instances: bounds must be ascending");
+    e->n_inst = 0;
+    e->inst_bounds.release();
+    if (n_bounds == 0) return;
+    e->inst_bounds.alloc(n_bounds, e->s);
+    SRD_CHECK(hipMemcpyAsync(e->inst_bounds.p, bounds, n_bounds * 4, hipMemcpyHostToDevice, e->s));
+    SRD_CHECK(hipStreamSynchronize(e->s));
+    e->n_inst = uint32_t(n_bounds - 1);
+  });
+}
+
+long long srd_engine_instance_delta_sizes(srd_engine* e, const char* rel, unsigned long long* out,
+                                          unsigned long long cap_iters) {
+  if (!e || !rel) return -1;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return -1;
+  auto& v = e->rels[it->second].inst_deltas;
+  if (out && cap_iters) {
+    const int rc = eguard(e, 2, [&] {
+      for (size_t i = 0; i < v.size() && i < cap_iters; ++i)
+        SRD_CHECK(hipMemcpyAsync(out + i * e->n_inst, v[i].p, uint64_t(e->n_inst) * 8,
+                                 hipMemcpyDeviceToHost, e->s));
+      SRD_CHECK(hipStreamSynchronize(e->s));
+    });
+    if (rc != 0) return -1;
+  }
+  return (long long)v.size();
 }
 
 double srd_engine_run_seconds(srd_engine* e) { return e ? e->run_seconds : 0.0; }

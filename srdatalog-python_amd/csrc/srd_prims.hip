@@ -342,6 +342,29 @@ void fill_pattern(uint32_t* cols, uint64_t stride, uint64_t n, uint32_t k, uint3
   SRD_CHECK(hipGetLastError());
 }
 
+// Rows of a column-0-sorted index falling in each instance's id range
+// [bounds[g], bounds[g+1]): two searches per instance.
+__global__ __launch_bounds__(kBlock) void instance_count_kernel(const uint32_t* __restrict__ col0,
+                                                                uint32_t n,
+                                                                const uint32_t* __restrict__ bounds,
+                                                                uint32_t n_inst,
+                                                                uint64_t* __restrict__ out) {
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= n_inst) return;
+  const uint32_t lo = n ? lower_bound(col0, 0, n, bounds[g]) : 0;
+  const uint32_t hi = n ? lower_bound(col0, lo, n, bounds[g + 1]) : 0;
+  out[g] = hi - lo;
+}
+
+void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
+                     uint64_t* out, hipStream_t s) {
+  if (n_inst == 0) return;
+  if (n > 0xffffffffull) fail("instance_counts: more than 2^32 rows");
+  instance_count_kernel<<<(n_inst + kBlock - 1) / kBlock, kBlock, 0, s>>>(col0, uint32_t(n),
+                                                                           bounds, n_inst, out);
+  SRD_CHECK(hipGetLastError());
+}
+
 // ================================================================ scan ====
 constexpr int kItems = 16;
 constexpr uint32_t kTile = kBlock * kItems;  // 4096

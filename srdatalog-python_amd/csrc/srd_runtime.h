@@ -181,6 +181,11 @@ void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep);
 // Synthetic columns (k fields of one 64-bit key per row, low k-1 fields
 // fbits wide): key = mul*i + off when range == 0 (ascending, distinct for
 // mul > 0), else a hash of i+off reduced mod range.
+// out[g] = rows of the column-0-sorted column col0 (n rows) with value in
+// [bounds[g], bounds[g+1]), g < n_inst (bounds has n_inst+1 entries).
+void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
+                     uint64_t* out, hipStream_t s);
+
 void fill_pattern(uint32_t* cols, uint64_t stride, uint64_t n, uint32_t k, uint32_t fbits,
                   uint64_t mul, uint64_t off, uint64_t range, hipStream_t s);
 

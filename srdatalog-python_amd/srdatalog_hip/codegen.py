@@ -689,6 +689,17 @@ void srdatalog_set_grain(unsigned grain) {
   if (ensure_engine()) srd_engine_set_grain(g_eng, grain);
 }
 
+int srdatalog_set_instances(const uint32_t* bounds, uint64_t n_bounds) {
+  if (!ensure_engine()) return 2;
+  return srd_engine_set_instances(g_eng, bounds, n_bounds);
+}
+
+long long srdatalog_instance_delta_sizes(const char* rel_name, unsigned long long* out,
+                                         unsigned long long cap_iters) {
+  if (!g_eng) return -1;
+  return srd_engine_instance_delta_sizes(g_eng, rel_name, out, cap_iters);
+}
+
 }  // extern "C"
 """
 

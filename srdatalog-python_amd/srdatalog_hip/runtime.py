@@ -50,6 +50,12 @@ class ProgramLib:
     lib.srdatalog_join_tuples.restype = ctypes.c_ulonglong
     lib.srdatalog_set_grain.argtypes = [ctypes.c_uint]
     lib.srdatalog_set_grain.restype = None
+    lib.srdatalog_set_instances.argtypes = [_u32p, ctypes.c_uint64]
+    lib.srdatalog_set_instances.restype = ctypes.c_int
+    lib.srdatalog_instance_delta_sizes.argtypes = [ctypes.c_char_p,
+                                                   ctypes.POINTER(ctypes.c_ulonglong),
+                                                   ctypes.c_ulonglong]
+    lib.srdatalog_instance_delta_sizes.restype = ctypes.c_longlong
     self.lib = lib
     self.path = so_path
 
@@ -114,3 +120,23 @@ class ProgramLib:
 
   def set_grain(self, grain: int) -> None:
     self.lib.srdatalog_set_grain(grain)
+
+  def set_instances(self, bounds: np.ndarray) -> None:
+    """Instance table of a packed batch (ascending column-0 bounds)."""
+    b = np.ascontiguousarray(bounds, dtype=np.uint32)
+    rc = self.lib.srdatalog_set_instances(b.ctypes.data_as(_u32p), b.size)
+    if rc != 0:
+      raise SrdatalogError(f"srdatalog_set_instances returned {rc}")
+
+  def instance_delta_sizes(self, rel: str, n_inst: int) -> np.ndarray:
+    """(iterations, n_inst) per-instance |DELTA| of rel."""
+    it = self.lib.srdatalog_instance_delta_sizes(rel.encode(), None, 0)
+    if it < 0:
+      raise SrdatalogError(f"srdatalog_instance_delta_sizes({rel}) failed")
+    out = np.zeros((max(it, 1), max(n_inst, 1)), dtype=np.uint64)
+    if it:
+      rc = self.lib.srdatalog_instance_delta_sizes(
+          rel.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)), it)
+      if rc != it:
+        raise SrdatalogError(f"srdatalog_instance_delta_sizes({rel}) returned {rc}")
+    return out[:it, :n_inst].astype(np.int64)

@@ -129,6 +129,15 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
 void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
 
+void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
+                     uint64_t* out, hipStream_t) {
+  for (uint32_t g = 0; g < n_inst; ++g) {
+    const uint32_t* lo = col0 ? std::lower_bound(col0, col0 + n, bounds[g]) : nullptr;
+    const uint32_t* hi = col0 ? std::lower_bound(lo, col0 + n, bounds[g + 1]) : nullptr;
+    out[g] = uint64_t(hi - lo);
+  }
+}
+
 void gather_segments(const uint64_t* recs, const uint32_t* rec_ctr, uint64_t rec_cap,
                      const uint32_t* stage, uint64_t stage_stride, const uint64_t* offsets,
                      const GatherDest* dests, uint32_t n_dests, uint64_t, hipStream_t) {
