@@ -42,9 +42,19 @@ def parse():
                  help="CPU baseline: root keys x in [0, K) of the same graph")
   p.add_argument("--no-cpu-baseline", action="store_true")
   p.add_argument("--no-parity", action="store_true")
-  p.add_argument("--workload", choices=("triangle", "batch"), default="triangle",
+  p.add_argument("--workload", choices=("triangle", "batch", "serve"), default="triangle",
                  help="triangle: the headline triangle-listing fixpoint; batch: the uneven "
-                      "512-instance TC/SG batch, batched vs plain per-instance")
+                      "512-instance TC/SG batch, batched vs plain per-instance; serve: the "
+                      "Poisson request stream, continuous batching vs plain serial")
+  p.add_argument("--serve-rate", type=float, default=200.0)
+  p.add_argument("--serve-seconds", type=float, default=60.0)
+  p.add_argument("--serve-cap", type=float, default=2e8,
+                 help="serve: per-request cap on the estimated output tuples")
+  p.add_argument("--serve-budget", type=float, default=200e9)
+  p.add_argument("--serve-plain-cap", type=float, default=60.0,
+                 help="serve: wall seconds after which the plain serial baseline starts "
+                      "no new request")
+  p.add_argument("--serve-scales", type=str, default="16,18,20,22")
   p.add_argument("--batch-instances", type=int, default=512)
   p.add_argument("--batch-cap", type=float, default=1e8,
                  help="batch: per-instance cap on the estimated output tuples")
@@ -175,6 +185,66 @@ def bench_batch(args) -> dict:
   }
 
 
+def bench_serve(args) -> dict:
+  """Poisson arrivals against resident R-MAT graphs (edge factor 16), mix
+  60% reach / 20% triangle count on a vertex range / 20% TC on an induced
+  subgraph; served with continuous batching, then the same stream served
+  serially by the plain path (bounded by --serve-plain-cap)."""
+  import copy
+
+  from srdatalog_hip import serve
+
+  t0 = time.time()
+  scales = tuple(int(x) for x in args.serve_scales.split(","))
+  graphs = serve.make_graphs(args.seed, scales=scales)
+  reqs = serve.make_requests(args.seed, graphs, rate=args.serve_rate, seconds=args.serve_seconds)
+  log(f"serve: {len(graphs)} graphs ({', '.join(str(g.edges.shape[0]) for g in graphs)} edges) "
+      f"and {len(reqs)} requests in {time.time() - t0:.1f}s")
+  plain_reqs = copy.deepcopy(reqs)
+
+  srv = serve.Server(graphs, budget_bytes=args.serve_budget, cap_tuples=args.serve_cap)
+  try:
+    wall_s = srv.run(reqs)
+  finally:
+    srv.close()
+  served = serve.latency_stats(reqs)
+  log(f"serve: served {served.get('completed')} in {wall_s:.1f}s, batches {srv.batches}")
+
+  keep = [p for p, r in zip(plain_reqs, reqs) if not r.rejected]
+  wall_p = serve.run_plain(keep, graphs, wall_cap=args.serve_plain_cap)
+  plain = serve.latency_stats(keep)
+  log(f"serve: plain served {plain.get('completed')} in {wall_p:.1f}s")
+
+  by_id = {r.rid: r for r in reqs}
+  compared = mismatched = 0
+  for p in keep:
+    if p.t_done >= 0:
+      compared += 1
+      mismatched += not serve.same_result(by_id[p.rid], p)
+  kinds = {}
+  for k in serve.KIND_NAMES:
+    kinds[k] = serve.latency_stats([r for r in reqs if r.kind == k])
+  rejected = [r for r in reqs if r.rejected]
+  return {
+    "metric": "queries/s served (continuous batching) with p50/p99 latency",
+    "value": round(served["completed"] / wall_s, 3),
+    "unit": "queries/s",
+    "p50_ms": served.get("p50_ms"), "p99_ms": served.get("p99_ms"),
+    "plain_value": round(plain["completed"] / wall_p, 3) if plain.get("completed") else 0.0,
+    "plain_p50_ms": plain.get("p50_ms"), "plain_p99_ms": plain.get("p99_ms"),
+    "plain_completed": plain.get("completed", 0),
+    "plain_unstarted": len(keep) - plain.get("completed", 0),
+    "speedup_vs_plain": (round((served["completed"] / wall_s) / (plain["completed"] / wall_p), 3)
+                         if plain.get("completed") else None),
+    "requests": len(reqs), "completed": served["completed"], "rejected": len(rejected),
+    "rejected_examples": [{"rid": r.rid, "kind": r.kind, "graph_scale": graphs[r.graph].scale,
+                           "reason": r.rejected} for r in rejected[:5]],
+    "by_kind": kinds, "batches": srv.batches,
+    "results_compared": compared, "results_identical": mismatched == 0,
+    "served_wall_s": round(wall_s, 3), "plain_wall_s": round(wall_p, 3),
+  }
+
+
 def main() -> int:
   args = parse()
   if args.gpus != 1:
@@ -191,6 +261,16 @@ def main() -> int:
                            "cap_tuples": args.batch_cap, "budget_bytes": args.batch_budget}})
     print(json.dumps(res))
     return 0 if res["outputs_identical"] else 1
+  if args.workload == "serve":
+    res = bench_serve(args)
+    res.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "u32", "data": "synthetic R-MAT + Poisson stream (seeded), in-process",
+                "config": {"workload": f"serving: Poisson {args.serve_rate} req/s for "
+                                       f"{args.serve_seconds} s, R-MAT scales {args.serve_scales}, "
+                                       "edge factor 16; 60% reach / 20% tri-count / 20% TC",
+                           "budget_bytes": args.serve_budget, "cap_tuples": args.serve_cap}})
+    print(json.dumps(res))
+    return 0 if res["results_identical"] else 1
 
   import numpy as np
 

@@ -93,6 +93,22 @@ def reach_program() -> Program:
   ])
 
 
+def reach_multi_program() -> Program:
+  """Many reachability queries in one fixpoint, the query id as column 0:
+  ReachQ(q,y) <= SrcQ(q,y); ReachQ(q,z) <= ReachQ(q,y), Edge(y,z).  Queries
+  never mix (q is carried unchanged), so ReachQ restricted to q is the
+  single-source result of q's sources."""
+  q, y, z = Var("q"), Var("y"), Var("z")
+  srcq = Relation("SrcQ", 2, input_file="SrcQ.csv")
+  edge = Relation("Edge", 2, input_file="Edge.csv")
+  reachq = Relation("ReachQ", 2)
+  return Program(rules=[
+    (reachq(q, y) <= srcq(q, y)).named("ReachQBase"),
+    (reachq(q, z) <= reachq(q, y) & edge(y, z)).named("ReachQRec")
+    .with_plan(var_order=["y", "q", "z"]),
+  ])
+
+
 PROGRAMS = {
   "tc": tc_program,
   "sg": sg_program,
@@ -100,6 +116,7 @@ PROGRAMS = {
   "triangle": lambda: triangle_program(False),
   "triangle_count": lambda: triangle_program(True),
   "reach": reach_program,
+  "reach_multi": reach_multi_program,
 }
 
 
