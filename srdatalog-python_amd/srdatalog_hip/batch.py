@@ -148,12 +148,14 @@ def split(rows: np.ndarray, bounds: np.ndarray) -> list[np.ndarray]:
 
 def trim_deltas(col: np.ndarray) -> list[int]:
   """An instance's own delta sequence from its column of the packed
-  per-iteration counts: it stops at its first empty DELTA after iteration 0,
-  as a run of the instance alone would."""
+  per-iteration counts.  Run alone, its loop checks at the top of iteration
+  k > 0 whether iteration k-1 produced any DELTA
+  (codegen/jit/orchestrator_jit.py:659-695), so its sequence ends with its
+  first empty DELTA, iteration 0 included."""
   out = []
-  for k, v in enumerate(col.tolist()):
+  for v in col.tolist():
     out.append(int(v))
-    if k >= 1 and v == 0:
+    if v == 0:
       break
   return out
 

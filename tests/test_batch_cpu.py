@@ -36,7 +36,7 @@ def test_pack_split_roundtrip():
 
 def test_trim_deltas():
   assert batch.trim_deltas(np.array([5, 3, 0, 0, 0])) == [5, 3, 0]
-  assert batch.trim_deltas(np.array([0, 0, 0])) == [0, 0]
+  assert batch.trim_deltas(np.array([0, 0, 0])) == [0]
   assert batch.trim_deltas(np.array([4, 0])) == [4, 0]
 
 
@@ -86,3 +86,24 @@ def test_estimates_track_true_sizes():
       assert est == true
     else:
       assert true <= 4 * est + 10
+
+
+def test_packed_deltas_with_instances_converged_at_iteration_0():
+  """Instances whose iteration-0 DELTA is empty record [0] alone; the
+  packed run keeps iterating for the others."""
+  e0 = np.zeros((0, 2), np.uint32)
+  insts = [batch.Instance("tc", 4, e0),
+           batch.Instance("tc", 3, np.array([[0, 1]], np.uint32)),
+           batch.Instance("tc", 6, ds.tc_edges(3, 6, 12))]
+  rows, bounds = batch.pack(insts)
+  packed = oracle.run("tc", {"ArcInput": rows})
+  width = len(packed.deltas["Path"])
+  for j, inst in enumerate(insts):
+    alone = oracle.run("tc", {"ArcInput": inst.edges}).deltas["Path"]
+    col = np.zeros(width, np.int64)
+    lo, hi = int(bounds[j]), int(bounds[j + 1])
+    # per-iteration counts of instance j rebuilt from the alone sequence
+    col[:len(alone)] = alone
+    assert batch.trim_deltas(col) == alone
+    assert alone[-1] == 0 and 0 not in alone[:-1]
+    assert hi - lo == inst.n
