@@ -104,6 +104,67 @@ __device__ __forceinline__ void seg_close(const SrdLaunchArgs& A, uint64_t slice
   }
 }
 
+// Cooperative lower bound: for every lane with `on`, the first position p in
+// [b, e) of col with col[p] >= v (e if none).  Consecutive `on` lanes with the
+// same `seg` form a run; the lanes of a run must share [b, e) and hold
+// non-decreasing v (a run is one owner's candidates, read in column order).
+// The first lane of each run searches lower_bound(v_first) and the last one
+// upper_bound(v_last) over the whole range; every other lane of the run then
+// searches only the window between those two positions, where its answer
+// lies.  A divergent full-range search costs one cache-line lookup per lane
+// and step; the window searches of one run touch a few shared lines instead.
+// Must be called with all 64 lanes active (wave-uniform control flow).
+__device__ __forceinline__ uint32_t seg_lower_bound(const uint32_t* __restrict__ col, uint32_t b,
+                                                    uint32_t e, uint32_t v, bool on, uint32_t seg) {
+  const uint32_t lane = srd::lane_id();
+  const uint64_t onm = srd::ballot(on);
+  const uint32_t seg_p = __shfl_up(seg, 1, 64);
+  const uint32_t seg_n = __shfl_down(seg, 1, 64);
+  const bool on_p = lane > 0 && ((onm >> (lane - 1)) & 1ull);
+  const bool on_n = lane < 63 && ((onm >> (lane + 1)) & 1ull);
+  const bool first = on && !(on_p && seg_p == seg);
+  const bool last = on && !(on_n && seg_n == seg);
+  // run ends: first lanes lower_bound(v), last-only lanes upper_bound(v)
+  const bool upper = last && !first;
+  uint32_t lo = b;
+  uint32_t n = (first || last) ? e - b : 0u;
+  while (n > 0) {
+    const uint32_t half = n >> 1;
+    const uint32_t mid = lo + half;
+    const uint32_t x = col[mid];
+    if (upper ? x <= v : x < v) {
+      lo = mid + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  const uint64_t fm = srd::ballot(first);
+  const uint64_t lm = srd::ballot(last);
+  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const uint64_t below = fm & le;
+  const uint32_t src_f = below ? 63u - uint32_t(__clzll((long long)below)) : lane;
+  const uint64_t above = lm >> lane;
+  const uint32_t src_l = above ? lane + uint32_t(__ffsll((unsigned long long)above)) - 1u : lane;
+  const uint32_t w_lo = __shfl(lo, int(src_f), 64);
+  const uint32_t w_hi = __shfl(lo, int(src_l), 64);
+  if (!on || first) return lo;
+  // w_lo <= answer <= w_hi for every lane strictly inside or at the end of a run
+  uint32_t l2 = w_lo;
+  uint32_t n2 = w_hi - w_lo;
+  while (n2 > 0) {
+    const uint32_t half = n2 >> 1;
+    const uint32_t mid = l2 + half;
+    if (col[mid] < v) {
+      l2 = mid + 1;
+      n2 -= half + 1;
+    } else {
+      n2 = half;
+    }
+  }
+  return l2;
+}
+
 // Wave-uniform copy of a 64-bit value held by `lane`.
 __device__ __forceinline__ uint64_t uniform_from(uint64_t v, int lane) {
   uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);

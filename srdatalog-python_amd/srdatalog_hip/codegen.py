@@ -280,20 +280,27 @@ class PipelineGen:
 
   # -------------------------------------------------------- level code ---
   def gen_cj_candidate(self, lv: M.ColumnJoin, rngs: list[Rg], drv_expr: str, item: str,
-                       hit: str, ind: str, tag: str) -> tuple[list[str], dict]:
-    """Code evaluating one candidate of a column join; returns code and the
-    child ranges registered for (view, prefix + var)."""
+                       hit: str, ind: str, tag: str, active: str, seg: str) -> tuple[list[str], dict]:
+    """Code evaluating one candidate of a column join per lane; returns code
+    (to be placed in wave-uniform control flow) and the child ranges
+    registered for (view, prefix + var).
+
+    Each active lane takes its item's value from the driving source
+    (`drv_expr`, uniform within a run of lanes with equal `seg`); the other
+    sources are then searched with srdj::seg_lower_bound, which needs every
+    lane of the wave to take part."""
     var = self.var(lv.var)
-    code = [f"{ind}uint32_t {var} = 0;"]
+    code = [f"{ind}uint32_t {var} = 0;", f"{ind}bool {tag}on = false;"]
     child = {}
     for j, (s, r) in enumerate(zip(lv.sources, rngs)):
       code.append(f"{ind}uint32_t {tag}c{j}b = 0, {tag}c{j}e = 0;")
       child[(self.vid(s), tuple(s.prefix) + (lv.var,))] = Rg(self.vid(s), len(s.prefix) + 1,
                                                              f"{tag}c{j}b", f"{tag}c{j}e")
-    code.append(f"{ind}switch ({drv_expr}) {{")
+    code.append(f"{ind}if ({active}) {{")
+    code.append(f"{ind}  switch ({drv_expr}) {{")
     for j, r in enumerate(rngs):
-      code.append(f"{ind}  case {j}: {{")
-      i2 = ind + "    "
+      code.append(f"{ind}    case {j}: {{")
+      i2 = ind + "      "
       if r.full0:
         code.append(f"{i2}const uint32_t pos_ = uint32_t({item});")
         code.append(f"{i2}const uint32_t val_ = V.v[{r.view}].root_vals[pos_];")
@@ -312,25 +319,39 @@ class PipelineGen:
         else:
           code.append(f"{i2}const bool first_ = pos_ == {r.b} || col_[pos_ - 1] != val_;")
           code.append(f"{i2}const srd::Rng cd_ = srdj::run_at(col_, pos_, {r.e}, val_, false);")
-      code.append(f"{i2}if (first_) {{")
-      i3 = i2 + "  "
-      code.append(f"{i3}bool ok_ = true;")
-      for m, rm in enumerate(rngs):
-        if m == j:
-          continue
-        code.append(f"{i3}const srd::Rng cf{m}_ = ok_ ? {self.find_expr(rm, 'val_')} : srd::Rng{{0, 0}};")
-        code.append(f"{i3}ok_ = ok_ && cf{m}_.e > cf{m}_.b;")
-      code.append(f"{i3}if (ok_) {{")
-      code.append(f"{i3}  {hit} = true;")
-      code.append(f"{i3}  {var} = val_;")
-      for m in range(len(rngs)):
-        src = "cd_" if m == j else f"cf{m}_"
-        code.append(f"{i3}  {tag}c{m}b = {src}.b; {tag}c{m}e = {src}.e;")
-      code.append(f"{i3}}}")
-      code.append(f"{i2}}}")
+      code.append(f"{i2}{tag}on = first_;")
+      code.append(f"{i2}{var} = val_;")
+      code.append(f"{i2}{tag}c{j}b = cd_.b; {tag}c{j}e = cd_.e;")
       code.append(f"{i2}break;")
-      code.append(f"{ind}  }}")
+      code.append(f"{ind}    }}")
+    code.append(f"{ind}  }}")
     code.append(f"{ind}}}")
+    if len(rngs) > 1:
+      for m, rm in enumerate(rngs):
+        code.append(f"{ind}{{  // search source {m}")
+        i2 = ind + "  "
+        code.append(f"{i2}const bool on_ = {tag}on && {drv_expr} != {m}u;")
+        if rm.full0:
+          code.append(f"{i2}const uint32_t* col_ = V.v[{rm.view}].root_vals;")
+          code.append(f"{i2}const uint32_t b_ = 0u, e_ = V.v[{rm.view}].r;")
+        else:
+          code.append(f"{i2}const uint32_t* col_ = {self.col_expr(rm.view, rm.depth)};")
+          code.append(f"{i2}const uint32_t b_ = {rm.b}, e_ = {rm.e};")
+        code.append(f"{i2}const uint32_t lb_ = srdj::seg_lower_bound(col_, b_, e_, {var}, on_, {seg});")
+        code.append(f"{i2}if (on_) {{")
+        code.append(f"{i2}  const bool hit_ = lb_ < e_ && col_[lb_] == {var};")
+        code.append(f"{i2}  {tag}on = hit_;")
+        if rm.full0:
+          code.append(f"{i2}  if (hit_) {{ {tag}c{m}b = V.v[{rm.view}].root_offs[lb_]; "
+                      f"{tag}c{m}e = V.v[{rm.view}].root_offs[lb_ + 1]; }}")
+        elif rm.depth == self.view_arity(rm.view) - 1:
+          code.append(f"{i2}  if (hit_) {{ {tag}c{m}b = lb_; {tag}c{m}e = lb_ + 1; }}")
+        else:
+          code.append(f"{i2}  if (hit_) {{ {tag}c{m}b = lb_; "
+                      f"{tag}c{m}e = srd::upper_bound(col_, lb_ + 1, e_, {var}); }}")
+        code.append(f"{i2}}}")
+        code.append(f"{ind}}}")
+    code.append(f"{ind}{hit} = {tag}on;")
     return code, child
 
   def gen_cart(self, lv: M.CartesianJoin, rngs: list[Rg], item: str, guard: str, ind: str,
@@ -420,14 +441,8 @@ class PipelineGen:
         lv = self.levels[0]
         rngs = [self.resolve(reg, self.vid(s), tuple(s.prefix), o, ind, "a") for s in lv.sources]
         if isinstance(lv, M.ColumnJoin):
-          o.append(f"{ind}if (act) {{")
-          code, child = self.gen_cj_candidate(lv, rngs, "l1drv", "li", "h1", ind + "  ", "l1")
-          # declarations must be visible after the if: hoist them
-          decls = [c for c in code if c.strip().startswith("uint32_t ")]
-          rest = [c for c in code if not c.strip().startswith("uint32_t ")]
-          o[-1:-1] = [d.replace(ind + "  ", ind, 1) for d in decls]
-          o += rest
-          o.append(f"{ind}}}")
+          code, child = self.gen_cj_candidate(lv, rngs, "l1drv", "li", "h1", ind, "l1", "act", "k")
+          o += code
           reg.update(child)
           bound.add(lv.var)
         else:
@@ -512,13 +527,9 @@ class PipelineGen:
         o.append(f"{i3}const uint32_t {self.var(v0name)} = {self.var(v0name)}_o;")
       o.append(f"{i3}bool h2 = false;")
       if isinstance(lv2, M.ColumnJoin):
-        o.append(f"{i3}if (act2) {{")
-        code, child = self.gen_cj_candidate(lv2, rngs2o, "d2o", "loc", "h2", i3 + "  ", "l2")
-        decls = [c for c in code if c.strip().startswith("uint32_t ")]
-        rest = [c for c in code if not c.strip().startswith("uint32_t ")]
-        o[-1:-1] = [d.replace(i3 + "  ", i3, 1) for d in decls]
-        o += rest
-        o.append(f"{i3}}}")
+        code, child = self.gen_cj_candidate(lv2, rngs2o, "d2o", "loc", "h2", i3, "l2", "act2",
+                                            "own")
+        o += code
         bound.add(lv2.var)
       else:
         o.append(f"{i3}h2 = act2;")
