@@ -501,6 +501,61 @@ __global__ __launch_bounds__(kBlock) void col_max_kernel(PackSpec spec, uint64_t
   if (fl && lane_id() == 0) atomicOr(&out_max[kMaxCols], fl);
 }
 
+// col_max_kernel for 16-byte aligned columns: each lane reads rows
+// 4g..4g+3 of every column with one 16-byte load; the row before 4g is the
+// previous lane's last row (a shuffle), or, for the first lane of a wave, a
+// direct load.  Rows past the last full group of 4 are done one per thread.
+__global__ __launch_bounds__(kBlock) void col_max_vec_kernel(PackSpec spec, uint64_t n,
+                                                             uint32_t* __restrict__ out_max) {
+  uint32_t m[kMaxCols];
+  for (uint32_t c = 0; c < spec.k; ++c) m[c] = 0;
+  bool unsorted = false, dup = false;
+  const uint32_t lane = lane_id();
+  const uint64_t ngroups = n >> 2;
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  for (uint64_t g = blockIdx.x * uint64_t(kBlock) + threadIdx.x; g < ngroups; g += stride) {
+    int ord[4] = {0, 0, 0, 0};
+    for (uint32_t c = 0; c < spec.k; ++c) {
+      const uint4 v = reinterpret_cast<const uint4*>(spec.in[c])[g];
+      m[c] = max(m[c], max(max(v.x, v.y), max(v.z, v.w)));
+      uint32_t p0 = __shfl_up(v.w, 1, kWave);
+      if (lane == 0) p0 = g > 0 ? spec.in[c][4 * g - 1] : 0u;
+      const uint32_t cur[4] = {v.x, v.y, v.z, v.w};
+      const uint32_t prv[4] = {p0, v.x, v.y, v.z};
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (ord[j] == 0) ord[j] = cur[j] < prv[j] ? -1 : (cur[j] > prv[j] ? 1 : 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool has_prev = g > 0 || j > 0;
+      unsorted |= has_prev && ord[j] < 0;
+      dup |= has_prev && ord[j] == 0;
+    }
+  }
+  const uint64_t t = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  const uint64_t i = 4 * ngroups + t;
+  if (i < n) {
+    int order = 0;
+    for (uint32_t c = 0; c < spec.k; ++c) {
+      const uint32_t v = spec.in[c][i];
+      m[c] = max(m[c], v);
+      if (i > 0 && order == 0) {
+        const uint32_t p = spec.in[c][i - 1];
+        order = v < p ? -1 : (v > p ? 1 : 0);
+      }
+    }
+    unsorted |= order < 0;
+    dup |= i > 0 && order == 0;
+  }
+  for (uint32_t c = 0; c < spec.k; ++c) {
+    const uint32_t v = wave_max(m[c]);
+    if (lane == 0 && v) atomicMax(&out_max[c], v);
+  }
+  const uint32_t fl = (ballot(unsorted) ? 1u : 0u) | (ballot(dup) ? 2u : 0u);
+  if (fl && lane == 0) atomicOr(&out_max[kMaxCols], fl);
+}
+
 template <int W>
 __device__ __forceinline__ uint32_t digit_of(uint64_t lo, uint64_t hi, uint32_t shift) {
   if (W == 1) return uint32_t(lo >> shift) & (kRadix - 1);
@@ -840,7 +895,13 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
     fill_u32(dmax, 0, kMaxCols + 1, s);
     {
       TimeScope t("sort.colmax", s, double(n) * 4 * k);
-      col_max_kernel<<<grid_for(n, kBlock * 16, 2048), kBlock, 0, s>>>(ps, n, dmax);
+      bool aligned = true;
+      for (uint32_t c = 0; c < k; ++c) aligned &= (reinterpret_cast<uintptr_t>(in[c]) & 15u) == 0;
+      if (aligned)
+        col_max_vec_kernel<<<grid_for(std::max<uint64_t>(n / 4, 1), kBlock * 4, 4096), kBlock, 0,
+                             s>>>(ps, n, dmax);
+      else
+        col_max_kernel<<<grid_for(n, kBlock * 16, 2048), kBlock, 0, s>>>(ps, n, dmax);
       SRD_CHECK(hipGetLastError());
     }
     hp = pinned_words(ws, kMaxCols + 1);
