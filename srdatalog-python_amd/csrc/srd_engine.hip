@@ -237,6 +237,7 @@ struct srd_engine {
   bool two_pass = false;           // SRDATALOG_JOIN_MODE=twopass: count + write only
   bool no_direct = false;          // SRDATALOG_JOIN_MODE=fused: never the direct mode
   uint64_t stage_cap_env = 0;      // SRDATALOG_STAGE_CAP: fixed staging rows (tests overflow)
+  uint64_t stage_max_bytes = uint64_t(16) << 30;  // SRDATALOG_STAGE_MAX_BYTES
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
   // instance table of a packed batch: n_inst+1 ascending column-0 bounds
@@ -510,6 +511,39 @@ static uint64_t read_u64(srd_engine& e, const uint64_t* dev) {
   return h[0];
 }
 
+// Staging columns of a fused launch: destination k's columns start at 8*k.
+static uint32_t stage_cols(const SrdPipelineDesc* d) {
+  uint32_t kmax = 0;
+  for (uint32_t k = 0; k < d->n_dest; ++k) kmax = std::max(kmax, 8 * k + d->dest[k].arity);
+  return kmax;
+}
+
+// Size the fused mode's staging buffer for a launch of `slices` slices: the
+// expected output (the rule's largest so far, +25%) plus one segment per
+// slice, which may stay partly filled, clamped to stage_max_bytes.  A launch
+// that still overflows falls back to the write pass over the counts it
+// produced.  Returns false, leaving the two-pass mode to run, when the
+// buffer cannot be allocated.
+static bool stage_ready(srd_engine& e, const SrdPipelineDesc* d, uint64_t slices) {
+  const uint32_t kmax = stage_cols(d);
+  uint64_t cap = std::max<uint64_t>(e.stage_hint[d->rule], uint64_t(1) << 24);
+  cap = std::max<uint64_t>(cap, slices * SRD_SEG + (uint64_t(1) << 20));
+  cap = std::min<uint64_t>(cap, e.stage_max_bytes / (4ull * kmax));
+  if (e.stage_cap_env) cap = std::max<uint64_t>(e.stage_cap_env, 1);
+  if (e.stage.n >= cap * kmax) return true;
+  e.stage.release();
+  try {
+    e.stage.alloc(cap * kmax, e.s);
+  } catch (const std::exception& ex) {
+    if (e.trace) {
+      std::fprintf(stderr, "[srd]   %s: no staging (%s): two-pass\n", d->rule, ex.what());
+      std::fflush(stderr);
+    }
+    return false;
+  }
+  return true;
+}
+
 static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t iteration) {
   SrdLaunchArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -599,7 +633,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         total = a.total_work;
         bind_dests(total);
         launch(d->direct, "direct");
-      } else if (d->count_only || e.two_pass) {
+      } else if (d->count_only || e.two_pass || !stage_ready(e, d, slices)) {
         launch(d->count, "count");
         exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
         total = read_u64(e, a.offsets + slices);
@@ -608,14 +642,14 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
           launch(d->write, "write");
         }
       } else {
-        // fused: one traversal into staging segments, then gather
-        uint32_t kmax = 0;
-        for (uint32_t k = 0; k < d->n_dest; ++k) kmax = std::max(kmax, 8 * k + d->dest[k].arity);
-        uint64_t cap = std::max<uint64_t>(e.stage_hint[d->rule], uint64_t(1) << 24);
-        cap = std::max<uint64_t>(cap, slices * 2 * SRD_SEG);
-        if (e.stage_cap_env) cap = e.stage_cap_env;
-        if (e.stage.n < cap * kmax) e.stage.alloc(cap * kmax, e.s);
-        if (!e.stage_cap_env) cap = e.stage.n / kmax;
+        // fused: one traversal into staging segments, then gather.  Every
+        // slice may hold one partly filled segment, so the capacity wanted is
+        // the expected output plus a segment per slice; it is clamped to
+        // stage_max_bytes, and a launch that overflows falls back to the
+        // write pass over the counts it produced.
+        const uint32_t kmax = stage_cols(d);
+        uint64_t cap = e.stage.n / kmax;
+        if (e.stage_cap_env) cap = std::min<uint64_t>(cap, e.stage_cap_env);
         const uint64_t rec_cap = slices + cap / (SRD_SEG - 64) + 64;
         e.recs.reserve(3 * rec_cap, e.s);
         e.stage_ctr.reserve(2, e.s);
@@ -979,6 +1013,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     e->quiet = q && *q && *q != '0';
     if (const char* g = std::getenv("SRDATALOG_GRAIN")) e->grain_override = uint32_t(std::atoi(g));
     if (const char* c = std::getenv("SRDATALOG_STAGE_CAP")) e->stage_cap_env = std::strtoull(c, nullptr, 10);
+    if (const char* c = std::getenv("SRDATALOG_STAGE_MAX_BYTES"))
+      e->stage_max_bytes = std::max<uint64_t>(std::strtoull(c, nullptr, 10), uint64_t(1) << 20);
     if (const char* m = std::getenv("SRDATALOG_JOIN_MODE")) {
       e->two_pass = std::string(m) == "twopass";
       e->no_direct = std::string(m) == "fused";

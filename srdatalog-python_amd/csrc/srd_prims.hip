@@ -77,6 +77,20 @@ void* pool_alloc(uint64_t bytes, hipStream_t s) {
   void* p = nullptr;
   hipError_t e = hipMallocAsync(&p, bytes, s);
   if (e != hipSuccess) {
+    // The pool keeps freed blocks reserved; a large request may not fit any
+    // of them while their sum would.  Hand the unused reservations back to
+    // the device and try once more.
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s);
+    int dev = 0;
+    hipMemPool_t pool;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+      (void)hipMemPoolTrimTo(pool, 0);
+    p = nullptr;
+    e = hipMallocAsync(&p, bytes, s);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
     std::lock_guard<std::mutex> g(g_pool_mu);
     g_stats.in_use -= bytes;
     fail("srdatalog pool: hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
