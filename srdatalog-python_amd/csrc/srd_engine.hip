@@ -168,6 +168,9 @@ struct Step {
   bool recursive = false;
   std::vector<Op> ops;
   std::vector<int> canonical_rels;  // relations with compute-delta in the plan
+  // relations of a recursive step whose FULL no pipeline of the step reads:
+  // their canonical FULL is kept as sorted segments while the step loops
+  std::vector<int> segmented_rels;
 };
 
 using Order = std::vector<uint32_t>;
@@ -207,6 +210,10 @@ struct Relation {
   int deltas_step = -1;
   // with an instance table: per iteration, |DELTA| rows of every instance
   std::vector<DevBuf<uint64_t>> inst_deltas;
+  // segmented canonical FULL: full[canonical] is the oldest segment, segs
+  // the younger ones (pairwise disjoint, sizes roughly halving)
+  std::vector<Index> segs;
+  bool segmented = false;
   uint64_t count = 0;               // count-only result
   bool counted = false;
 };
@@ -329,6 +336,27 @@ static void collect_canonical(srd_engine& e, const std::vector<Op>& ops, Step& s
   }
 }
 
+// FULL readers of a step: pipeline views at FULL, and merges into a
+// non-canonical order.  A relation with compute-delta and no such reader can
+// keep its canonical FULL segmented while the step loops.
+static void full_readers(srd_engine& e, const std::vector<Op>& ops, std::vector<int>& rd) {
+  for (auto& op : ops) {
+    if (op.kind == OpKind::Pipeline && op.pipe) {
+      for (uint32_t i = 0; i < op.pipe->n_views; ++i)
+        if (op.pipe->views[i].ver == SRD_FULL) rd.push_back(rel_id(e, op.pipe->views[i].rel));
+    }
+    if (op.kind == OpKind::MergeIndex && op.index != e.rels[op.rel].canonical) rd.push_back(op.rel);
+    if (op.kind == OpKind::Parallel) full_readers(e, op.kids, rd);
+  }
+}
+
+static void collect_segmented(srd_engine& e, Step& st) {
+  std::vector<int> rd;
+  full_readers(e, st.ops, rd);
+  for (int ri : st.canonical_rels)
+    if (std::find(rd.begin(), rd.end(), ri) == rd.end()) st.segmented_rels.push_back(ri);
+}
+
 static void parse_program(srd_engine& e, const std::string& text) {
   auto top = SParser(text).parse_all();
   for (auto& n : top) {
@@ -364,6 +392,7 @@ static void parse_program(srd_engine& e, const std::string& text) {
         }
       }
       collect_canonical(e, st.ops, st);
+      if (st.recursive && !std::getenv("SRDATALOG_NO_SEGMENTS")) collect_segmented(e, st);
       e.steps.push_back(std::move(st));
     }
   }
@@ -408,7 +437,25 @@ static void upload_input(srd_engine& e, Relation& r) {
 }
 
 // FULL index in `order`, current with the relation's FULL version.
+// Merge a segmented canonical FULL back into one index (smallest segments
+// first).
+static void materialize_full(srd_engine& e, Relation& r) {
+  if (r.segs.empty()) return;
+  Index acc = r.segs.back();
+  for (size_t i = r.segs.size() - 1; i-- > 0;) {
+    Index m;
+    merge_disjoint(r.segs[i], acc, m, e.ws, e.s);
+    acc = std::move(m);
+  }
+  r.segs.clear();
+  auto& base = r.full[r.canonical];
+  Index m;
+  merge_disjoint(base.idx, acc, m, e.ws, e.s);
+  base.idx = std::move(m);
+}
+
 static const Index& full_index(srd_engine& e, Relation& r, const Order& order) {
+  if (!r.segs.empty()) materialize_full(e, r);
   auto it = r.full.find(order);
   if (it != r.full.end() && it->second.version == r.full_version) return it->second.idx;
   Versioned v;
@@ -773,12 +820,31 @@ static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no
     case OpKind::ComputeDelta: {
       Relation& r = e.rels[op.rel];
       const Index& nw = new_index(e, r, op.index);
-      const Index& fl = full_index(e, r, op.index);
       Index d;
-      if (fl.n == 0)
+      uint64_t full_n = 0;
+      if (r.segmented && op.index == r.canonical && !r.segs.empty()) {
+        // NEW minus every segment of FULL (each pass reads only the windows
+        // of the segment that NEW's tiles cover)
+        auto bit = r.full.find(r.canonical);
         copy_index(nw, d, e.s);
-      else
-        set_difference(nw, fl, d, e.ws, e.s);
+        std::vector<const Index*> parts;
+        if (bit != r.full.end()) parts.push_back(&bit->second.idx);
+        for (auto& sg : r.segs) parts.push_back(&sg);
+        for (const Index* part : parts) {
+          full_n += part->n;
+          if (d.n == 0 || part->n == 0) continue;
+          Index t;
+          set_difference(d, *part, t, e.ws, e.s);
+          d = std::move(t);
+        }
+      } else {
+        const Index& fl = full_index(e, r, op.index);
+        full_n = fl.n;
+        if (fl.n == 0)
+          copy_index(nw, d, e.s);
+        else
+          set_difference(nw, fl, d, e.ws, e.s);
+      }
       r.delta.clear();
       if (r.deltas_step != step_no) {
         r.deltas.clear();
@@ -792,8 +858,9 @@ static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no
         r.inst_deltas.push_back(per);
       }
       if (e.trace) {
-        std::fprintf(stderr, "[srd]   delta %s = %llu (new %llu, full %llu)\n", r.name.c_str(),
-                     (unsigned long long)d.n, (unsigned long long)nw.n, (unsigned long long)fl.n);
+        std::fprintf(stderr, "[srd]   delta %s = %llu (new %llu, full %llu, segments %zu)\n",
+                     r.name.c_str(), (unsigned long long)d.n, (unsigned long long)nw.n,
+                     (unsigned long long)full_n, r.segs.size() + 1);
         std::fflush(stderr);
       }
       r.delta[op.index] = std::move(d);
@@ -806,6 +873,32 @@ static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no
       if (target == 0) break;  // nothing computed yet
       auto it = r.full.find(op.index);
       if (it != r.full.end() && it->second.version == target) break;
+      if (r.segmented && op.index == r.canonical) {
+        // append DELTA as the youngest segment; merge the two youngest while
+        // they are within a factor 2 (the oldest is full[canonical])
+        if (it == r.full.end() || it->second.version != target - 1) {
+          full_index(e, r, op.index);
+          it = r.full.find(op.index);
+        }
+        const Index& dl = delta_index(e, r, op.index);
+        if (dl.n > 0) {
+          Index c;
+          copy_index(dl, c, e.s);
+          r.segs.push_back(std::move(c));
+        }
+        while (!r.segs.empty()) {
+          const size_t m = r.segs.size();
+          Index& older = m >= 2 ? r.segs[m - 2] : it->second.idx;
+          if (older.n > 2 * r.segs.back().n && m < 24) break;
+          Index mg;
+          merge_disjoint(older, r.segs.back(), mg, e.ws, e.s);
+          older = std::move(mg);
+          r.segs.pop_back();
+        }
+        it->second.version = target;
+        r.full_version = target;
+        break;
+      }
       const Index& dl = delta_index(e, r, op.index);
       Versioned v;
       v.version = target;
@@ -861,6 +954,8 @@ static void reset_state(srd_engine& e) {
     r.full_version = r.delta_seq = 0;
     r.deltas.clear();
     r.inst_deltas.clear();
+    r.segs.clear();
+    r.segmented = false;
     r.deltas_step = -1;
     r.count = 0;
     r.counted = false;
@@ -880,6 +975,7 @@ static void run_program(srd_engine& e, uint64_t max_iters) {
     if (!st.recursive) {
       exec_ops(e, st.ops, 0, int(si));
     } else {
+      for (int ri : st.segmented_rels) e.rels[ri].segmented = true;
       uint64_t it = 0;
       for (; it < max_iters; ++it) {
         if (it > 0) {
@@ -894,6 +990,11 @@ static void run_program(srd_engine& e, uint64_t max_iters) {
           if (tot == 0) break;
         }
         exec_ops(e, st.ops, it, int(si));
+      }
+      for (int ri : st.segmented_rels) {
+        Relation& r = e.rels[ri];
+        materialize_full(e, r);
+        r.segmented = false;
       }
       e.iterations = it;
     }

@@ -100,3 +100,24 @@ def test_engine_rejects_bad_plan(eng):
                         b"(fixpoint-plan (bogus-op :schema A))))")
   assert eng.srd_engine_ok(e) == 0
   eng.srd_engine_destroy(e)
+
+
+@pytest.mark.parametrize("segments", [True, False])
+@pytest.mark.parametrize("n,extra,seed", [(300, 0, 0), (400, 40, 1), (150, 300, 2)])
+def test_engine_segmented_full_long_tail(eng, n, extra, seed, segments, monkeypatch):
+  """A chain 0->1->...->n-1 plus random edges: many iterations with small
+  deltas, so FULL accumulates many segments and merge cascades."""
+  monkeypatch.delenv("SRDATALOG_JOIN_MODE", raising=False)
+  if segments:
+    monkeypatch.delenv("SRDATALOG_NO_SEGMENTS", raising=False)
+  else:
+    monkeypatch.setenv("SRDATALOG_NO_SEGMENTS", "1")
+  chain = np.stack([np.arange(n - 1), np.arange(1, n)], 1).astype(np.uint32)
+  rng = np.random.default_rng(seed)
+  rnd = rng.integers(0, n, size=(extra, 2)).astype(np.uint32)
+  e = np.concatenate([chain, rnd])
+  want = oracle.run("tc", {"ArcInput": e})
+  path, deltas, size = run_tc(eng, e)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]
+  assert size == want.relations["Path"].shape[0]
