@@ -82,3 +82,18 @@ def reach(edge, src):
   r0 = {int(s) for s in src}
   r, deltas = _fix(r0, lambda r: {z for y in r for z in succ.get(y, ())})
   return {"Reach": r}, {"Reach": deltas}
+
+
+def neg2hop(knows, interest):
+  """LSQB Q9: Path(p1,p2,p3,t) <= Knows(p1,p2), Knows(p2,p3), HasInterest(p3,t),
+  not Knows(p1,p3), p1 != p3."""
+  kn = {(int(a), int(b)) for a, b in knows}
+  hi = {(int(a), int(b)) for a, b in interest}
+  succ, tags = {}, {}
+  for a, b in kn:
+    succ.setdefault(a, set()).add(b)
+  for p, t in hi:
+    tags.setdefault(p, set()).add(t)
+  path = {(p1, p2, p3, t) for (p1, p2) in kn for p3 in succ.get(p2, ())
+          if p1 != p3 and (p1, p3) not in kn for t in tags.get(p3, ())}
+  return {"Knows": kn, "HasInterest": hi, "Path": path}, {"Path": [len(path)]}

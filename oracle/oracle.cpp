@@ -409,6 +409,64 @@ void run_reach(const u32* edge_rows, u64 m, const u32* src, u64 n_src, u64 max_i
   r.rels["Reach"] = full;
 }
 
+// ------------------------------------------------------------- Neg2hop ---
+// LSQB Q9 (examples/lsqb_q9_neg2hop.py): Path(p1,p2,p3,t) <= Knows(p1,p2),
+// Knows(p2,p3), HasInterest(p3,t), ~Knows(p1,p3), p1 != p3.  Knows and
+// HasInterest are loaded from their inputs (KnowsLoad / InterestLoad).  One
+// non-recursive step; count_only mirrors the rule's .with_count().
+void run_neg2hop(const u32* knows_rows, u64 mk, const u32* hi_rows, u64 mh, bool count_only,
+                 int threads, Result& r) {
+  Keys kn = make_index(knows_rows, mk, false);     // (p1,p2) / (p2,p3) / (p1,p3)
+  Keys hi = make_index(hi_rows, mh, false);        // (p3,t)
+  Groups kg(kn), hg(hi);
+  auto has_knows = [&](u32 a, u32 b) { return std::binary_search(kn.begin(), kn.end(), pack(a, b)); };
+  std::vector<u64> counts(threads, 0);
+  std::vector<std::vector<u128>> outs(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t g1 = 0; g1 < int64_t(kg.vals.size()); ++g1) {
+    const u32 p1 = kg.vals[g1];
+    u64 local = 0;
+    auto& out = outs[omp_get_thread_num()];
+    for (u64 i = kg.offs[g1]; i < kg.offs[g1 + 1]; ++i) {
+      const u32 p2 = lo32(kn[i]);
+      auto [b2, e2] = kg.find(p2);
+      for (u64 j = b2; j < e2; ++j) {
+        const u32 p3 = lo32(kn[j]);
+        if (p1 == p3 || has_knows(p1, p3)) continue;
+        auto [b3, e3] = hg.find(p3);
+        for (u64 q = b3; q < e3; ++q) {
+          ++local;
+          if (!count_only)
+            out.push_back((u128(p1) << 96) | (u128(p2) << 64) | (u128(p3) << 32) | lo32(hi[q]));
+        }
+      }
+    }
+    counts[omp_get_thread_num()] += local;
+  }
+  for (u64 c : counts) r.count += c;
+  store_binary(r, "Knows", kn, false);
+  store_binary(r, "HasInterest", hi, false);
+  auto& dl = r.deltas["Path"];
+  if (!count_only) {
+    std::vector<u128> all;
+    for (auto& v : outs) all.insert(all.end(), v.begin(), v.end());
+    sort_unique(all);
+    auto& out = r.rels["Path"];
+    out.resize(4 * all.size());
+    for (u64 i = 0; i < all.size(); ++i) {
+      out[4 * i] = u32(all[i] >> 96);
+      out[4 * i + 1] = u32(all[i] >> 64);
+      out[4 * i + 2] = u32(all[i] >> 32);
+      out[4 * i + 3] = u32(all[i]);
+    }
+    dl.push_back(all.size());
+  } else {
+    r.rels["Path"];
+    dl.push_back(0);
+  }
+  r.iterations = 1;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- C ABI ------
@@ -427,7 +485,8 @@ static const orc_input* find_in(const orc_input* in, int n, const char* name) {
   return nullptr;
 }
 
-// program: "tc" | "sg" | "andersen" | "triangle" | "triangle_count" | "reach".
+// program: "tc" | "sg" | "andersen" | "triangle" | "triangle_count" | "reach" |
+// "neg2hop" | "neg2hop_count".
 // Returns an opaque result (check orc_error) or null for an unknown program.
 void* orc_run(const char* program, const orc_input* inputs, int n_inputs, uint64_t max_iters,
               int threads, uint32_t key_lo, uint32_t key_hi) {
@@ -461,6 +520,9 @@ void* orc_run(const char* program, const orc_input* inputs, int n_inputs, uint64
   } else if (p == "reach") {
     auto *e = need("Edge", 2), *s = need("Src", 1);
     if (e && s) run_reach(e->rows, e->n, s->rows, s->n, max_iters, threads, *r);
+  } else if (p == "neg2hop" || p == "neg2hop_count") {
+    auto *k = need("KnowsInput", 2), *h = need("HasInterestInput", 2);
+    if (k && h) run_neg2hop(k->rows, k->n, h->rows, h->n, p == "neg2hop_count", threads, *r);
   } else {
     delete r;
     return nullptr;

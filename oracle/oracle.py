@@ -61,6 +61,8 @@ def _load():
 
 
 ARITY = {"Triangle": 3, "Reach": 1}
+# relations whose arity depends on the program
+PROGRAM_ARITY = {("neg2hop", "Path"): 4, ("neg2hop_count", "Path"): 4}
 
 
 class OracleResult:
@@ -80,9 +82,10 @@ class OracleResult:
 def run(program: str, inputs: dict[str, np.ndarray], max_iters: int = 0, threads: int = 0,
         key_range: tuple[int, int] | None = None,
         relations: tuple[str, ...] = ("Edge", "Path", "Sg", "PointsTo", "Knows", "Triangle",
-                                      "Reach")) -> OracleResult:
+                                      "Reach", "HasInterest")) -> OracleResult:
   """Run ``program`` ("tc", "sg", "andersen", "triangle", "triangle_count",
-  "reach") on ``inputs`` (relation name -> (n, arity) uint32 rows)."""
+  "reach", "neg2hop", "neg2hop_count") on ``inputs`` (relation name ->
+  (n, arity) uint32 rows)."""
   lib = _load()
   keep = []
   arr = (_Input * max(len(inputs), 1))()
@@ -106,7 +109,7 @@ def run(program: str, inputs: dict[str, np.ndarray], max_iters: int = 0, threads
     rels, deltas = {}, {}
     for name in relations:
       nw = lib.orc_words(h, name.encode())
-      arity = ARITY.get(name, 2)
+      arity = PROGRAM_ARITY.get((program, name), ARITY.get(name, 2))
       buf = np.empty(nw, dtype=np.uint32)
       if lib.orc_export(h, name.encode(), buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                         nw) == 0:

@@ -165,6 +165,22 @@ __device__ __forceinline__ uint32_t seg_lower_bound(const uint32_t* __restrict__
   return l2;
 }
 
+// Whether index view v holds a row whose first sizeof...(key) columns equal
+// key (negated atoms: the rule's binding survives only when this is false).
+template <class... T>
+__device__ __forceinline__ bool has_prefix(const IndexView& v, T... key) {
+  constexpr int n = int(sizeof...(T));
+  const uint32_t k[n] = {uint32_t(key)...};
+  if (v.n == 0) return false;
+  Rng r = srd::root_find(v, k[0]);
+#pragma unroll
+  for (int d = 1; d < n; ++d) {
+    if (r.e <= r.b) return false;
+    r = srd::child_find(v, d, r, k[d], d == int(v.arity) - 1);
+  }
+  return r.e > r.b;
+}
+
 // Wave-uniform copy of a 64-bit value held by `lane`.
 __device__ __forceinline__ uint64_t uniform_from(uint64_t v, int lane) {
   uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);

@@ -80,9 +80,10 @@ class PipelineGen:
     self.info = PipeInfo(ep)
     self.ops = list(ep.ops)
     for op in self.ops:
-      if isinstance(op, M.Negation):
-        raise CodegenError(f"{ep.rule}: negation is not supported by the HIP emitter yet")
-      if not isinstance(op, (M.Scan, M.ColumnJoin, M.CartesianJoin, M.Filter, M.InsertInto)):
+      if isinstance(op, M.Negation) and not op.prefix:
+        raise CodegenError(f"{ep.rule}: negation without bound variables")
+      if not isinstance(op, (M.Scan, M.ColumnJoin, M.CartesianJoin, M.Filter, M.Negation,
+                             M.InsertInto)):
         raise CodegenError(f"{ep.rule}: unsupported op {type(op).__name__}")
     self.root = self.ops[0]
     if not isinstance(self.root, (M.Scan, M.ColumnJoin)):
@@ -95,7 +96,9 @@ class PipelineGen:
     for i, lv in enumerate(self.levels):
       if isinstance(lv, M.CartesianJoin) and i != len(self.levels) - 1:
         raise CodegenError(f"{ep.rule}: cartesian join must be the last join level")
-    self.filters = [op for op in self.ops if isinstance(op, M.Filter)]
+    # per-lane checks applied as soon as their variables are bound: filters
+    # and negations (no row of the negated index has the bound prefix)
+    self.filters = [op for op in self.ops if isinstance(op, (M.Filter, M.Negation))]
     self.dests = [op for op in self.ops if isinstance(op, M.InsertInto)]
     if not self.dests:
       raise CodegenError(f"{ep.rule}: pipeline without insert-into")
@@ -137,11 +140,15 @@ class PipelineGen:
   def filters_ready(self, bound: set[str], done: set[int]) -> list[tuple[int, M.Filter]]:
     out = []
     for i, f in enumerate(self.filters):
-      if i not in done and all(v in bound for v in f.vars):
+      vs = f.prefix if isinstance(f, M.Negation) else f.vars
+      if i not in done and all(v in bound for v in vs):
         out.append((i, f))
     return out
 
-  def filter_code(self, f: M.Filter) -> str:
+  def filter_code(self, f) -> str:
+    if isinstance(f, M.Negation):
+      args = ", ".join(self.var(v) for v in f.prefix)
+      return f"!srdj::has_prefix(V.v[{self.vid(f)}], {args})"
     params = ", ".join(f"uint32_t {cident(v)}" for v in f.vars)
     args = ", ".join(self.var(v) for v in f.vars)
     return f"([&]({params}) -> bool {{ {f.code} }})({args})"

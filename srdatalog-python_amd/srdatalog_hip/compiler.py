@@ -606,13 +606,46 @@ def relation_decls(program: Program) -> list[RelDecl]:
   return [RelDecl(r.name, r.arity, r.input_file, r.print_size) for r in program.relations]
 
 
+def rename_wildcards(rules: list[Rule]) -> list[Rule]:
+  """Give every `_`-prefixed variable occurrence a fresh name `_gen<N>`, N
+  counting from 1 over the program's rules in order, heads before body
+  (src/srdatalog/rule_rewrite.py:34-67): two wildcards are never the same
+  variable."""
+  import dataclasses
+  counter = 0
+
+  def atom(a: Atom) -> Atom:
+    nonlocal counter
+    args = []
+    for v in a.args:
+      if v.startswith("_"):
+        counter += 1
+        v = f"_gen{counter}"
+      args.append(v)
+    return dataclasses.replace(a, args=tuple(args))
+
+  out = []
+  for r in rules:
+    heads = tuple(atom(h) for h in r.heads)
+    body = []
+    for c in r.body:
+      if isinstance(c, Atom):
+        body.append(atom(c))
+      elif isinstance(c, Negation):
+        body.append(dataclasses.replace(c, atom=atom(c.atom)))
+      else:
+        body.append(c)
+    out.append(dataclasses.replace(r, heads=heads, body=tuple(body)))
+  return out
+
+
 def compile_to_mir(program: Program) -> M.Program:
   arity = {r.name: r.arity for r in program.relations}
   for r in program.rules:
     for c in r.body:
       if not isinstance(c, (Atom, Negation, Filter)):
         raise CompileError(f"rule {r.name}: unsupported body clause {type(c).__name__}")
-  strata = stratify(list(program.rules))
+  strata = stratify(rename_wildcards(list(program.rules)))
   for s in strata:
     if s.recursive:
       for r in s.rules:

@@ -93,6 +93,27 @@ def reach_program() -> Program:
   ])
 
 
+def neg2hop_program(count_only: bool) -> Program:
+  """LSQB Q9, two-hop paths not closed by a direct edge
+  (examples/lsqb_q9_neg2hop.py; the reference rule is count-only, the
+  listing variant materialises Path for tuple-level parity)."""
+  P, P1, P2, P3, T, X, Y = (Var(n) for n in ("p", "p1", "p2", "p3", "t", "x", "y"))
+  k_in = Relation("KnowsInput", 2, input_file="Person_knows_Person.csv")
+  hi_in = Relation("HasInterestInput", 2, input_file="Person_hasInterest_Tag.csv")
+  knows, hi = Relation("Knows", 2), Relation("HasInterest", 2)
+  path = Relation("Path", 4, print_size=not count_only)
+  rule = (path(P1, P2, P3, T) <= knows(P1, P2) & knows(P2, P3) & hi(P3, T) & ~knows(P1, P3)
+          & Filter(("p1", "p3"), "return p1 != p3;")) \
+      .named("NegPath").with_plan(var_order=["p2", "p3", "p1", "t"])
+  if count_only:
+    rule = rule.with_count()
+  return Program(rules=[
+    (knows(X, Y) <= k_in(X, Y)).named("KnowsLoad"),
+    (hi(P, T) <= hi_in(P, T)).named("InterestLoad"),
+    rule,
+  ])
+
+
 def reach_multi_program() -> Program:
   """Many reachability queries in one fixpoint, the query id as column 0:
   ReachQ(q,y) <= SrcQ(q,y); ReachQ(q,z) <= ReachQ(q,y), Edge(y,z).  Queries
@@ -117,6 +138,8 @@ PROGRAMS = {
   "triangle_count": lambda: triangle_program(True),
   "reach": reach_program,
   "reach_multi": reach_multi_program,
+  "neg2hop": lambda: neg2hop_program(False),
+  "neg2hop_count": lambda: neg2hop_program(True),
 }
 
 

@@ -76,7 +76,45 @@ def triangle() -> Program:
   ])
 
 
+def lsqb_q9_neg2hop() -> Program:
+  """examples/lsqb_q9_neg2hop.py: 2-hop paths closed by no direct edge."""
+  P, P1, P2, P3, T, X, Y = (Var(n) for n in ("p", "p1", "p2", "p3", "t", "x", "y"))
+  k_in, hi_in = Relation("KnowsInput", 2), Relation("HasInterestInput", 2)
+  knows, hi, path = Relation("Knows", 2), Relation("HasInterest", 2), Relation("Path", 4)
+  return Program(rules=[
+    (knows(X, Y) <= k_in(X, Y)).named("KnowsLoad"),
+    (hi(P, T) <= hi_in(P, T)).named("InterestLoad"),
+    (path(P1, P2, P3, T) <= knows(P1, P2) & knows(P2, P3) & hi(P3, T) & ~knows(P1, P3)
+     & Filter(("p1", "p3"), "return p1 != p3;"))
+    .named("NegPath").with_plan(var_order=["p2", "p3", "p1", "t"]).with_count(),
+  ])
+
+
+def lsqb_q7_optional() -> Program:
+  """examples/lsqb_q7_optional.py: the four OPTIONAL-MATCH cases, two of
+  them with negated marker relations."""
+  C, CM, L, M_, T, U = (Var(n) for n in ("c", "comment", "liker", "m", "t", "_"))
+  has_tag, has_creator = Relation("HasTag", 2), Relation("HasCreator", 2)
+  reply_of, likes = Relation("ReplyOf", 2), Relation("Likes", 2)
+  has_reply, has_liker = Relation("HasReply", 1), Relation("HasLiker", 1)
+  c1, c2 = Relation("Case1", 5), Relation("Case2", 4)
+  c3, c4 = Relation("Case3", 4), Relation("Case4", 3)
+  return Program(rules=[
+    (has_reply(M_) <= reply_of(U, M_)).named("MarkHasReply"),
+    (has_liker(M_) <= likes(Var("_"), M_)).named("MarkHasLiker"),
+    (c1(M_, T, C, CM, L) <= has_tag(M_, T) & has_creator(M_, C) & reply_of(CM, M_) & likes(L, M_))
+    .named("Case1Both").with_plan(var_order=["m", "comment", "liker", "t", "c"]).with_count(),
+    (c2(M_, T, C, CM) <= has_tag(M_, T) & has_creator(M_, C) & reply_of(CM, M_) & ~has_liker(M_))
+    .named("Case2ReplyOnly").with_plan(var_order=["m", "comment", "t", "c"]).with_count(),
+    (c3(M_, T, C, L) <= has_tag(M_, T) & has_creator(M_, C) & likes(L, M_) & ~has_reply(M_))
+    .named("Case3LikeOnly").with_plan(var_order=["m", "liker", "t", "c"]).with_count(),
+    (c4(M_, T, C) <= has_tag(M_, T) & has_creator(M_, C) & ~has_reply(M_) & ~has_liker(M_))
+    .named("Case4Neither").with_plan(var_order=["m", "t", "c"]).with_count(),
+  ])
+
+
 GOLDEN = {
   "tc": tc, "tc_2level_bg": tc_2level_bg, "sg": sg, "andersen": andersen,
   "lsqb_triangle_count": lsqb_triangle_count, "triangle": triangle,
+  "lsqb_q9_neg2hop": lsqb_q9_neg2hop, "lsqb_q7_optional": lsqb_q7_optional,
 }

@@ -124,3 +124,36 @@ def test_sorted_unique_exports():
   p = r.relations["Path"].astype(np.uint64)
   k = (p[:, 0] << np.uint64(32)) | p[:, 1]
   assert np.all(k[1:] > k[:-1])
+
+
+def _neg2hop_inputs(seed, n=60, m=300, tags=12, mi=150):
+  rng = np.random.default_rng(seed)
+  k = ds.tc_edges(seed, n, m)
+  hi = np.unique(np.stack([rng.integers(0, n, mi), rng.integers(0, tags, mi)], 1).astype(np.uint32), axis=0)
+  return {"KnowsInput": k, "HasInterestInput": hi}
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_neg2hop_matches_brute(seed):
+  inp = _neg2hop_inputs(seed)
+  r = oracle.run("neg2hop", inp)
+  b, _ = brute.neg2hop(inp["KnowsInput"], inp["HasInterestInput"])
+  assert as_set(r.relations["Path"]) == b["Path"]
+  c = oracle.run("neg2hop_count", inp)
+  assert c.count == len(b["Path"]) == r.count
+
+
+def test_neg2hop_closed_form():
+  """count = sum over 2-paths (p1,p2,p3) with p1 != p3 and no edge p1->p3
+  of |tags(p3)|, with dense matrices: A2 = A@A masked by (1-A) and off-diagonal."""
+  inp = _neg2hop_inputs(5, n=80, m=500, tags=9, mi=200)
+  n = 80
+  a = np.zeros((n, n), np.int64)
+  e = inp["KnowsInput"]
+  a[e[:, 0], e[:, 1]] = 1
+  t = np.zeros(n, np.int64)
+  np.add.at(t, inp["HasInterestInput"][:, 0], 1)
+  a2 = a @ a
+  mask = (1 - a) * (1 - np.eye(n, dtype=np.int64))
+  want = int(((a2 * mask) @ t).sum())
+  assert oracle.run("neg2hop_count", inp).count == want

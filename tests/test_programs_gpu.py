@@ -232,3 +232,30 @@ def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
       assert lib.delta_sizes(rel) == orc.deltas[rel], (prog, label)
     finally:
       lib.shutdown()
+
+
+# --------------------------------------------------------------- negation ---
+def _neg2hop_inputs(seed, n, m, tags, mi):
+  rng = np.random.default_rng(seed)
+  k = ds.tc_edges(seed, n, m)
+  hi = np.unique(np.stack([rng.integers(0, n, mi), rng.integers(0, tags, mi)], 1)
+                 .astype(np.uint32), axis=0)
+  return {"KnowsInput": k, "HasInterestInput": hi}
+
+
+@pytest.mark.parametrize("n,m,tags,mi,seed", [(50, 200, 8, 100, 0), (2000, 20000, 50, 4000, 1),
+                                              (20000, 150000, 200, 30000, 2)])
+def test_neg2hop_listing_and_count(progs, n, m, tags, mi, seed):
+  inp = _neg2hop_inputs(seed, n, m, tags, mi)
+  orc = oracle.run("neg2hop", inp)
+  lib = run_prog(progs, "neg2hop", inp)
+  try:
+    check_rel(lib, orc, "Path", 4)
+    check_rel(lib, orc, "Knows", 2)
+  finally:
+    lib.shutdown()
+  lib = run_prog(progs, "neg2hop_count", inp)
+  try:
+    assert lib.size("Path") == orc.count
+  finally:
+    lib.shutdown()
