@@ -97,3 +97,22 @@ def neg2hop(knows, interest):
   path = {(p1, p2, p3, t) for (p1, p2) in kn for p3 in succ.get(p2, ())
           if p1 != p3 and (p1, p3) not in kn for t in tags.get(p3, ())}
   return {"Knows": kn, "HasInterest": hi, "Path": path}, {"Path": [len(path)]}
+
+
+def q7(has_tag, has_creator, reply_of, likes):
+  """LSQB Q7 by enumerating each case's distinct head tuples."""
+  ht = {(int(a), int(b)) for a, b in has_tag}
+  hc = {(int(a), int(b)) for a, b in has_creator}
+  ro = {(int(a), int(b)) for a, b in reply_of}      # (comment, m)
+  lk = {(int(a), int(b)) for a, b in likes}         # (liker, m)
+  has_reply = {m for _, m in ro}
+  has_liker = {m for _, m in lk}
+  c1 = {(m, t, c, cm, l) for (m, t) in ht for (m2, c) in hc if m2 == m
+        for (cm, m3) in ro if m3 == m for (l, m4) in lk if m4 == m}
+  c2 = {(m, t, c, cm) for (m, t) in ht for (m2, c) in hc if m2 == m
+        for (cm, m3) in ro if m3 == m if m not in has_liker}
+  c3 = {(m, t, c, l) for (m, t) in ht for (m2, c) in hc if m2 == m
+        for (l, m4) in lk if m4 == m if m not in has_reply}
+  c4 = {(m, t, c) for (m, t) in ht for (m2, c) in hc if m2 == m
+        if m not in has_reply and m not in has_liker}
+  return {"Case1": len(c1), "Case2": len(c2), "Case3": len(c3), "Case4": len(c4)}

@@ -142,6 +142,7 @@ struct Result {
   std::map<std::string, std::vector<u64>> deltas;  // per-iteration |DELTA| of the recursive step
   u64 iterations = 0;
   u64 count = 0;  // count-only rules
+  std::map<std::string, u64> rule_counts;  // count-only head relation -> count
   double seconds = 0;
   std::string error;
 };
@@ -467,6 +468,41 @@ void run_neg2hop(const u32* knows_rows, u64 mk, const u32* hi_rows, u64 mh, bool
   r.iterations = 1;
 }
 
+// ---------------------------------------------------------------- Q7 ------
+// LSQB Q7 (examples/lsqb_q7_optional.py), four count-only rules over messages
+// m:  HasReply(m) <= ReplyOf(_, m);  HasLiker(m) <= Likes(_, m);
+//   Case1(m,t,c,comment,liker) <= HasTag(m,t), HasCreator(m,c), ReplyOf(comment,m), Likes(liker,m)
+//   Case2(m,t,c,comment) <= HasTag, HasCreator, ReplyOf(comment,m), ~HasLiker(m)
+//   Case3(m,t,c,liker)   <= HasTag, HasCreator, Likes(liker,m), ~HasReply(m)
+//   Case4(m,t,c)         <= HasTag, HasCreator, ~HasReply(m), ~HasLiker(m)
+// Each count is the number of distinct head tuples, enumerated per m.
+void run_q7(const u32* tag, u64 nt, const u32* cre, u64 nc, const u32* rep, u64 nr, const u32* lik,
+            u64 nl, Result& r) {
+  Keys ht = make_index(tag, nt, false);    // (m,t)
+  Keys hc = make_index(cre, nc, false);    // (m,c)
+  Keys ro = make_index(rep, nr, true);     // ReplyOf(comment,m) keyed (m,comment)
+  Keys li = make_index(lik, nl, true);     // Likes(liker,m) keyed (m,liker)
+  Groups gt(ht), gc(hc), gr(ro), gl(li);
+  u64 c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+  for (size_t g = 0; g < gt.vals.size(); ++g) {
+    const u32 m = gt.vals[g];
+    const u64 t = gt.offs[g + 1] - gt.offs[g];
+    auto [cb, ce] = gc.find(m);
+    auto [rb, re] = gr.find(m);
+    auto [lb, le] = gl.find(m);
+    const u64 c = ce - cb, rr = re - rb, ll = le - lb;
+    c1 += t * c * rr * ll;
+    if (ll == 0) c2 += t * c * rr;
+    if (rr == 0) c3 += t * c * ll;
+    if (rr == 0 && ll == 0) c4 += t * c;
+  }
+  r.rule_counts["Case1"] = c1;
+  r.rule_counts["Case2"] = c2;
+  r.rule_counts["Case3"] = c3;
+  r.rule_counts["Case4"] = c4;
+  r.iterations = 1;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- C ABI ------
@@ -486,7 +522,7 @@ static const orc_input* find_in(const orc_input* in, int n, const char* name) {
 }
 
 // program: "tc" | "sg" | "andersen" | "triangle" | "triangle_count" | "reach" |
-// "neg2hop" | "neg2hop_count".
+// "neg2hop" | "neg2hop_count" | "q7".
 // Returns an opaque result (check orc_error) or null for an unknown program.
 void* orc_run(const char* program, const orc_input* inputs, int n_inputs, uint64_t max_iters,
               int threads, uint32_t key_lo, uint32_t key_hi) {
@@ -520,6 +556,10 @@ void* orc_run(const char* program, const orc_input* inputs, int n_inputs, uint64
   } else if (p == "reach") {
     auto *e = need("Edge", 2), *s = need("Src", 1);
     if (e && s) run_reach(e->rows, e->n, s->rows, s->n, max_iters, threads, *r);
+  } else if (p == "q7") {
+    auto *t = need("HasTag", 2), *c = need("HasCreator", 2), *rp = need("ReplyOf", 2),
+         *l = need("Likes", 2);
+    if (t && c && rp && l) run_q7(t->rows, t->n, c->rows, c->n, rp->rows, rp->n, l->rows, l->n, *r);
   } else if (p == "neg2hop" || p == "neg2hop_count") {
     auto *k = need("KnowsInput", 2), *h = need("HasInterestInput", 2);
     if (k && h) run_neg2hop(k->rows, k->n, h->rows, h->n, p == "neg2hop_count", threads, *r);
@@ -569,6 +609,12 @@ int orc_deltas(void* h, const char* rel, uint64_t* out, uint64_t cap) {
 
 uint64_t orc_iterations(void* h) { return static_cast<Result*>(h)->iterations; }
 uint64_t orc_count(void* h) { return static_cast<Result*>(h)->count; }
+// Count of a count-only rule's head relation (0 if absent).
+uint64_t orc_rule_count(void* h, const char* rel) {
+  auto* r = static_cast<Result*>(h);
+  auto it = r->rule_counts.find(rel);
+  return it == r->rule_counts.end() ? 0 : it->second;
+}
 double orc_seconds(void* h) { return static_cast<Result*>(h)->seconds; }
 void orc_free(void* h) { delete static_cast<Result*>(h); }
 

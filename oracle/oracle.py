@@ -49,6 +49,8 @@ def _load():
     lib.orc_deltas.restype = ctypes.c_int
     lib.orc_deltas.argtypes = [ctypes.c_void_p, ctypes.c_char_p,
                                ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
+    lib.orc_rule_count.restype = ctypes.c_uint64
+    lib.orc_rule_count.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     for f in ("orc_iterations", "orc_count"):
       getattr(lib, f).restype = ctypes.c_uint64
       getattr(lib, f).argtypes = [ctypes.c_void_p]
@@ -68,12 +70,13 @@ PROGRAM_ARITY = {("neg2hop", "Path"): 4, ("neg2hop_count", "Path"): 4}
 class OracleResult:
   """Sorted relation contents, per-iteration deltas and timing of one run."""
 
-  def __init__(self, relations, deltas, iterations, count, seconds):
+  def __init__(self, relations, deltas, iterations, count, seconds, rule_counts=None):
     self.relations = relations      # name -> (n, arity) uint32, lexicographically sorted
     self.deltas = deltas            # name -> list[int], |DELTA| per iteration
     self.iterations = iterations
     self.count = count
     self.seconds = seconds
+    self.rule_counts = rule_counts or {}   # count-only head relation -> count
 
   def size(self, rel: str) -> int:
     return int(self.relations[rel].shape[0])
@@ -84,7 +87,7 @@ def run(program: str, inputs: dict[str, np.ndarray], max_iters: int = 0, threads
         relations: tuple[str, ...] = ("Edge", "Path", "Sg", "PointsTo", "Knows", "Triangle",
                                       "Reach", "HasInterest")) -> OracleResult:
   """Run ``program`` ("tc", "sg", "andersen", "triangle", "triangle_count",
-  "reach", "neg2hop", "neg2hop_count") on ``inputs`` (relation name ->
+  "reach", "neg2hop", "neg2hop_count", "q7") on ``inputs`` (relation name ->
   (n, arity) uint32 rows)."""
   lib = _load()
   keep = []
@@ -119,7 +122,9 @@ def run(program: str, inputs: dict[str, np.ndarray], max_iters: int = 0, threads
         d = np.empty(nd, dtype=np.uint64)
         lib.orc_deltas(h, name.encode(), d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), nd)
         deltas[name] = [int(x) for x in d]
+    rule_counts = {name: int(lib.orc_rule_count(h, name.encode()))
+                   for name in ("Case1", "Case2", "Case3", "Case4")}
     return OracleResult(rels, deltas, int(lib.orc_iterations(h)), int(lib.orc_count(h)),
-                        float(lib.orc_seconds(h)))
+                        float(lib.orc_seconds(h)), rule_counts)
   finally:
     lib.orc_free(h)

@@ -114,6 +114,31 @@ def neg2hop_program(count_only: bool) -> Program:
   ])
 
 
+def q7_program() -> Program:
+  """LSQB Q7, the four OPTIONAL-MATCH cases as count-only rules, two with
+  negated marker relations (examples/lsqb_q7_optional.py)."""
+  C, CM, L, M_, T = (Var(n) for n in ("c", "comment", "liker", "m", "t"))
+  has_tag = Relation("HasTag", 2, input_file="Message_hasTag_Tag.csv")
+  has_creator = Relation("HasCreator", 2, input_file="Message_hasCreator_Person.csv")
+  reply_of = Relation("ReplyOf", 2, input_file="Comment_replyOf_Message.csv")
+  likes = Relation("Likes", 2, input_file="Person_likes_Message.csv")
+  has_reply, has_liker = Relation("HasReply", 1), Relation("HasLiker", 1)
+  c1, c2 = Relation("Case1", 5), Relation("Case2", 4)
+  c3, c4 = Relation("Case3", 4), Relation("Case4", 3)
+  return Program(rules=[
+    (has_reply(M_) <= reply_of(Var("_"), M_)).named("MarkHasReply"),
+    (has_liker(M_) <= likes(Var("_"), M_)).named("MarkHasLiker"),
+    (c1(M_, T, C, CM, L) <= has_tag(M_, T) & has_creator(M_, C) & reply_of(CM, M_) & likes(L, M_))
+    .named("Case1Both").with_plan(var_order=["m", "comment", "liker", "t", "c"]).with_count(),
+    (c2(M_, T, C, CM) <= has_tag(M_, T) & has_creator(M_, C) & reply_of(CM, M_) & ~has_liker(M_))
+    .named("Case2ReplyOnly").with_plan(var_order=["m", "comment", "t", "c"]).with_count(),
+    (c3(M_, T, C, L) <= has_tag(M_, T) & has_creator(M_, C) & likes(L, M_) & ~has_reply(M_))
+    .named("Case3LikeOnly").with_plan(var_order=["m", "liker", "t", "c"]).with_count(),
+    (c4(M_, T, C) <= has_tag(M_, T) & has_creator(M_, C) & ~has_reply(M_) & ~has_liker(M_))
+    .named("Case4Neither").with_plan(var_order=["m", "t", "c"]).with_count(),
+  ])
+
+
 def reach_multi_program() -> Program:
   """Many reachability queries in one fixpoint, the query id as column 0:
   ReachQ(q,y) <= SrcQ(q,y); ReachQ(q,z) <= ReachQ(q,y), Edge(y,z).  Queries
@@ -140,6 +165,7 @@ PROGRAMS = {
   "reach_multi": reach_multi_program,
   "neg2hop": lambda: neg2hop_program(False),
   "neg2hop_count": lambda: neg2hop_program(True),
+  "q7": q7_program,
 }
 
 

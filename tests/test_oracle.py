@@ -157,3 +157,20 @@ def test_neg2hop_closed_form():
   mask = (1 - a) * (1 - np.eye(n, dtype=np.int64))
   want = int(((a2 * mask) @ t).sum())
   assert oracle.run("neg2hop_count", inp).count == want
+
+
+def q7_inputs(seed, n_msg=40, n_tag=6, n_person=30, sizes=(80, 50, 60, 70)):
+  rng = np.random.default_rng(seed)
+  def pairs(n, a, b):
+    return np.unique(np.stack([rng.integers(0, a, n), rng.integers(0, b, n)], 1).astype(np.uint32), axis=0)
+  return {"HasTag": pairs(sizes[0], n_msg, n_tag), "HasCreator": pairs(sizes[1], n_msg, n_person),
+          "ReplyOf": pairs(sizes[2], 1000, n_msg), "Likes": pairs(sizes[3], n_person, n_msg)}
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_q7_matches_brute(seed):
+  inp = q7_inputs(seed)
+  r = oracle.run("q7", inp)
+  want = brute.q7(inp["HasTag"], inp["HasCreator"], inp["ReplyOf"], inp["Likes"])
+  assert r.rule_counts == want
+  assert all(v > 0 for v in want.values()) or seed > 0

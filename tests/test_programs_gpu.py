@@ -259,3 +259,27 @@ def test_neg2hop_listing_and_count(progs, n, m, tags, mi, seed):
     assert lib.size("Path") == orc.count
   finally:
     lib.shutdown()
+
+
+def _q7_inputs(seed, n_msg, n_tag, n_person, sizes):
+  rng = np.random.default_rng(seed)
+
+  def pairs(n, a, b):
+    return np.unique(np.stack([rng.integers(0, a, n), rng.integers(0, b, n)], 1)
+                     .astype(np.uint32), axis=0)
+  return {"HasTag": pairs(sizes[0], n_msg, n_tag), "HasCreator": pairs(sizes[1], n_msg, n_person),
+          "ReplyOf": pairs(sizes[2], 10 * n_msg, n_msg), "Likes": pairs(sizes[3], n_person, n_msg)}
+
+
+@pytest.mark.parametrize("n_msg,sizes,seed", [(40, (80, 50, 60, 70), 0),
+                                              (5000, (20000, 6000, 8000, 9000), 1),
+                                              (200000, (600000, 200000, 150000, 150000), 2)])
+def test_q7_counts(progs, n_msg, sizes, seed):
+  inp = _q7_inputs(seed, n_msg, 20, 3000, sizes)
+  orc = oracle.run("q7", inp)
+  lib = run_prog(progs, "q7", inp)
+  try:
+    for case, want in orc.rule_counts.items():
+      assert lib.size(case) == want, case
+  finally:
+    lib.shutdown()
