@@ -113,8 +113,68 @@ def lsqb_q7_optional() -> Program:
   ])
 
 
+def lsqb_q3_triangle() -> Program:
+  """examples/lsqb_q3_triangle.py: triangles whose three persons live in
+  cities of one country."""
+  V = {n: Var(n) for n in ("c", "c1", "c2", "c3", "co", "p", "p1", "p2", "p3", "x", "y")}
+  k_in, loc_in, part_in = (Relation("KnowsInput", 2), Relation("IsLocatedInInput", 2),
+                           Relation("IsPartOfInput", 2))
+  knows, loc, part = Relation("Knows", 2), Relation("IsLocatedIn", 2), Relation("IsPartOf", 2)
+  tri = Relation("Triangle", 3)
+  x, y, p, c, co = V["x"], V["y"], V["p"], V["c"], V["co"]
+  p1, p2, p3, c1, c2, c3 = V["p1"], V["p2"], V["p3"], V["c1"], V["c2"], V["c3"]
+  return Program(rules=[
+    (knows(x, y) <= k_in(x, y)).named("KnowsLoad"),
+    (loc(p, c) <= loc_in(p, c)).named("LocLoad"),
+    (part(c, co) <= part_in(c, co)).named("PartOfLoad"),
+    (tri(p1, p2, p3) <= knows(p1, p2) & knows(p2, p3) & knows(p3, p1) & loc(p1, c1) & loc(p2, c2)
+     & loc(p3, c3) & part(c1, co) & part(c2, co) & part(c3, co))
+    .named("LabeledTriangle").with_plan(var_order=["p1", "p2", "p3", "c1", "c2", "c3", "co"]),
+  ])
+
+
+def _q6(two_sources: bool, rev: bool, head: str, rule: str, order, count=False, bg=False):
+  P, P1, P2, P3, T, X, Y = (Var(n) for n in ("p", "p1", "p2", "p3", "t", "x", "y"))
+  k_in, hi_in = Relation("KnowsInput", 2), Relation("HasInterestInput", 2)
+  knows, hi, path = Relation("Knows", 2), Relation("HasInterest", 2), Relation(head, 4)
+  rules = [(knows(X, Y) <= k_in(X, Y)).named("KnowsLoad")]
+  second = knows
+  if rev:
+    rules.append((knows(Y, X) <= k_in(X, Y)).named("KnowsLoadRev"))
+  if two_sources:
+    k2_in, second = Relation("KnowsInput2", 2), Relation("Knows2", 2)
+    rules.append((second(X, Y) <= k2_in(X, Y)).named("Knows2Load"))
+  rules.append((hi(P, T) <= hi_in(P, T)).named("InterestLoad"))
+  r = (path(P1, P2, P3, T) <= knows(P1, P2) & second(P2, P3) & hi(P3, T)
+       & Filter(("p1", "p3"), "return p1 != p3;")).named(rule).with_plan(var_order=order,
+                                                                         block_group=bg)
+  if count:
+    r = r.with_count()
+  rules.append(r)
+  return Program(rules=rules)
+
+
+def lsqb_q6_2hop() -> Program:
+  return _q6(False, False, "Path", "TwoHopPath", ["p1", "p2", "p3", "t"])
+
+
+def lsqb_q6_count() -> Program:
+  return _q6(False, True, "Path", "TwoHopPath", ["p2", "p3", "p1", "t"], count=True)
+
+
+def lsqb_q6_nosj() -> Program:
+  return _q6(True, False, "Path", "TwoHopPath", ["p1", "p2", "p3", "t"])
+
+
+def lsqb_q6_bg_test() -> Program:
+  return _q6(True, False, "PathBG", "TwoHopBG", ["p2", "p3", "p1", "t"], bg=True)
+
+
 GOLDEN = {
   "tc": tc, "tc_2level_bg": tc_2level_bg, "sg": sg, "andersen": andersen,
   "lsqb_triangle_count": lsqb_triangle_count, "triangle": triangle,
   "lsqb_q9_neg2hop": lsqb_q9_neg2hop, "lsqb_q7_optional": lsqb_q7_optional,
+  "lsqb_q3_triangle": lsqb_q3_triangle, "lsqb_q6_2hop": lsqb_q6_2hop,
+  "lsqb_q6_count": lsqb_q6_count, "lsqb_q6_nosj": lsqb_q6_nosj,
+  "lsqb_q6_bg_test": lsqb_q6_bg_test,
 }
