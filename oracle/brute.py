@@ -116,3 +116,16 @@ def q7(has_tag, has_creator, reply_of, likes):
   c4 = {(m, t, c) for (m, t) in ht for (m2, c) in hc if m2 == m
         if m not in has_reply and m not in has_liker}
   return {"Case1": len(c1), "Case2": len(c2), "Case3": len(c3), "Case4": len(c4)}
+
+
+def q3(knows, located, part_of):
+  """LSQB Q3: directed 3-cycles p1->p2->p3->p1 whose persons' cities lie in
+  one common country."""
+  kn = {(int(a), int(b)) for a, b in knows}
+  loc = {(int(a), int(b)) for a, b in located}
+  part = {(int(a), int(b)) for a, b in part_of}
+  tri = {(p1, p2, p3) for (p1, p2) in kn for (q, p3) in kn if q == p2 and (p3, p1) in kn
+         for (x1, c1) in loc if x1 == p1 for (x2, c2) in loc if x2 == p2
+         for (x3, c3) in loc if x3 == p3 for (d1, co) in part if d1 == c1
+         if (c2, co) in part and (c3, co) in part}
+  return {"Triangle": tri}

@@ -503,6 +503,67 @@ void run_q7(const u32* tag, u64 nt, const u32* cre, u64 nc, const u32* rep, u64 
   r.iterations = 1;
 }
 
+// ---------------------------------------------------------------- Q3 ------
+// LSQB Q3 (examples/lsqb_q3_triangle.py): Triangle(p1,p2,p3) <= Knows(p1,p2),
+// Knows(p2,p3), Knows(p3,p1), IsLocatedIn(p_i,c_i), IsPartOf(c_i,co) for one
+// country co shared by the three cities.  The head drops c1..c3 and co, so
+// the result is the set of directed 3-cycles whose persons share a country.
+void run_q3(const u32* kr, u64 mk, const u32* lr, u64 ml, const u32* pr, u64 mp, int threads,
+            Result& r) {
+  Keys kn = make_index(kr, mk, false), loc = make_index(lr, ml, false),
+       part = make_index(pr, mp, false);
+  Groups kg(kn), lg(loc), pg(part);
+  auto countries = [&](u32 p) {
+    std::vector<u32> out;
+    auto [b, e] = lg.find(p);
+    for (u64 i = b; i < e; ++i) {
+      auto [cb, ce] = pg.find(lo32(loc[i]));
+      for (u64 j = cb; j < ce; ++j) out.push_back(lo32(part[j]));
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+  };
+  std::vector<std::vector<u32>> cty(kg.vals.size());
+  std::vector<std::vector<u128>> outs(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t g1 = 0; g1 < int64_t(kg.vals.size()); ++g1) {
+    const u32 p1 = kg.vals[g1];
+    auto& out = outs[omp_get_thread_num()];
+    const std::vector<u32> c1 = countries(p1);
+    if (c1.empty()) continue;
+    for (u64 i = kg.offs[g1]; i < kg.offs[g1 + 1]; ++i) {
+      const u32 p2 = lo32(kn[i]);
+      auto [b2, e2] = kg.find(p2);
+      if (b2 == e2) continue;
+      std::vector<u32> c12;
+      const std::vector<u32> c2 = countries(p2);
+      std::set_intersection(c1.begin(), c1.end(), c2.begin(), c2.end(), std::back_inserter(c12));
+      if (c12.empty()) continue;
+      for (u64 j = b2; j < e2; ++j) {
+        const u32 p3 = lo32(kn[j]);
+        if (!std::binary_search(kn.begin(), kn.end(), pack(p3, p1))) continue;
+        const std::vector<u32> c3 = countries(p3);
+        std::vector<u32> all;
+        std::set_intersection(c12.begin(), c12.end(), c3.begin(), c3.end(), std::back_inserter(all));
+        if (!all.empty()) out.push_back((u128(p1) << 64) | (u128(p2) << 32) | p3);
+      }
+    }
+  }
+  std::vector<u128> tri;
+  for (auto& v : outs) tri.insert(tri.end(), v.begin(), v.end());
+  sort_unique(tri);
+  auto& o = r.rels["Triangle"];
+  o.resize(3 * tri.size());
+  for (u64 i = 0; i < tri.size(); ++i) {
+    o[3 * i] = u32(tri[i] >> 64);
+    o[3 * i + 1] = u32(tri[i] >> 32);
+    o[3 * i + 2] = u32(tri[i]);
+  }
+  r.deltas["Triangle"].push_back(tri.size());
+  r.iterations = 1;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- C ABI ------
@@ -522,7 +583,7 @@ static const orc_input* find_in(const orc_input* in, int n, const char* name) {
 }
 
 // program: "tc" | "sg" | "andersen" | "triangle" | "triangle_count" | "reach" |
-// "neg2hop" | "neg2hop_count" | "q7".
+// "neg2hop" | "neg2hop_count" | "q3" | "q7".
 // Returns an opaque result (check orc_error) or null for an unknown program.
 void* orc_run(const char* program, const orc_input* inputs, int n_inputs, uint64_t max_iters,
               int threads, uint32_t key_lo, uint32_t key_hi) {
@@ -556,6 +617,9 @@ void* orc_run(const char* program, const orc_input* inputs, int n_inputs, uint64
   } else if (p == "reach") {
     auto *e = need("Edge", 2), *s = need("Src", 1);
     if (e && s) run_reach(e->rows, e->n, s->rows, s->n, max_iters, threads, *r);
+  } else if (p == "q3") {
+    auto *k = need("KnowsInput", 2), *l = need("IsLocatedInInput", 2), *pt = need("IsPartOfInput", 2);
+    if (k && l && pt) run_q3(k->rows, k->n, l->rows, l->n, pt->rows, pt->n, threads, *r);
   } else if (p == "q7") {
     auto *t = need("HasTag", 2), *c = need("HasCreator", 2), *rp = need("ReplyOf", 2),
          *l = need("Likes", 2);

@@ -87,7 +87,7 @@ def run(program: str, inputs: dict[str, np.ndarray], max_iters: int = 0, threads
         relations: tuple[str, ...] = ("Edge", "Path", "Sg", "PointsTo", "Knows", "Triangle",
                                       "Reach", "HasInterest")) -> OracleResult:
   """Run ``program`` ("tc", "sg", "andersen", "triangle", "triangle_count",
-  "reach", "neg2hop", "neg2hop_count", "q7") on ``inputs`` (relation name ->
+  "reach", "neg2hop", "neg2hop_count", "q3", "q7") on ``inputs`` (relation name ->
   (n, arity) uint32 rows)."""
   lib = _load()
   keep = []

@@ -91,8 +91,6 @@ class PipelineGen:
     self.levels = [op for op in self.ops[1:] if isinstance(op, (M.ColumnJoin, M.CartesianJoin))]
     if isinstance(self.root, M.Scan) and self.levels:
       raise CodegenError(f"{ep.rule}: joins after a scan root are not supported")
-    if len(self.levels) > 2:
-      raise CodegenError(f"{ep.rule}: more than two nested join levels")
     for i, lv in enumerate(self.levels):
       if isinstance(lv, M.CartesianJoin) and i != len(self.levels) - 1:
         raise CodegenError(f"{ep.rule}: cartesian join must be the last join level")
@@ -217,7 +215,8 @@ class PipelineGen:
       reg[key] = r
       return r
     parent = self.resolve(reg, v, prefix[:-1], code, ind, tag)
-    nm = f"nr_{tag}_{v}_{len(prefix)}"
+    self._nr = getattr(self, "_nr", 0) + 1
+    nm = f"nr_{tag}_{v}_{len(prefix)}_{self._nr}"
     code.append(f"{ind}const srd::Rng {nm} = {self.find_expr(parent, self.var(prefix[-1]))};")
     r = Rg(v, len(prefix), f"{nm}.b", f"{nm}.e")
     reg[key] = r
@@ -391,11 +390,11 @@ class PipelineGen:
          "__global__ __launch_bounds__(256) void run_k(SrdLaunchArgs A) {",
          "  const SrdViews& V = A.views;",
          "  const uint32_t lane = srd::lane_id();"]
-    two = len(self.levels) == 2
-    nfr = 0
-    if two:
-      o += ["  const uint32_t wv = threadIdx.x / 64u;",
-            "  __shared__ uint64_t pref_s[4][65];"]
+    deep = len(self.levels) >= 2
+    self._lds = []
+    if deep:
+      o += ["  const uint32_t wv = threadIdx.x / 64u;"]
+    lds_at = len(o)
     slots = self.root_slots()
     ns = len(slots)
     o += ["  for (;;) {",
@@ -460,104 +459,132 @@ class PipelineGen:
     for i, f in self.filters_ready(bound, done):
       o.append(f"{ind}h1 = h1 && {self.filter_code(f)};")
       done.add(i)
-    if not two:
+    if not deep:
       if len(done) != len(self.filters):
         raise CodegenError(f"{self.ep.rule}: filter over unbound variables")
       o.append(self.emit_code("h1", ind))
     else:
-      lv2 = self.levels[1]
-      l1var = self.levels[0].var
-      # frame: level-1 var + ranges of level-2 sources
-      rngs2 = [self.resolve(reg, self.vid(s), tuple(s.prefix), o, ind, "b") for s in lv2.sources]
-      frame = [self.var(l1var)]
-      per_lane = []
-      for j, r in enumerate(rngs2):
-        if r.full0:
-          per_lane.append(None)
-        else:
-          per_lane.append((len(frame), len(frame) + 1))
-          frame += [r.b, r.e]
-      # vars bound at level 0 that are per-lane (root var) and needed later
-      v0name = None if isinstance(self.root, M.Scan) else self.root.var
-      if v0name is not None:
-        frame.append(self.var(v0name))
-      nfr = len(frame)
-      o.insert(5, f"  __shared__ uint32_t fr_s[4][{nfr + 1}][64];")
-      # level-2 size per lane
-      o.append(f"{ind}uint64_t s2 = 0;")
-      o.append(f"{ind}uint32_t d2 = 0;")
-      o.append(f"{ind}if (h1) {{")
-      if isinstance(lv2, M.ColumnJoin):
-        o.append(f"{ind}  s2 = {self.size_expr(rngs2[0], False)};")
-        for j in range(1, len(rngs2)):
-          o.append(f"{ind}  {{ const uint64_t z = {self.size_expr(rngs2[j], False)}; "
-                   f"if (z < s2) {{ s2 = z; d2 = {j}; }} }}")
-      else:
-        o.append(f"{ind}  s2 = 1;")
-        for r in rngs2:
-          o.append(f"{ind}  s2 *= uint64_t({self.size_expr(r, True)});")
-      o.append(f"{ind}}}")
-      o.append(f"{ind}const uint64_t inc2 = srd::wave_incl_scan<uint64_t>(s2);")
-      o.append(f"{ind}const uint64_t tot2 = srdj::uniform_from(inc2, 63);")
-      o.append(f"{ind}pref_s[wv][lane + 1] = inc2;")
-      o.append(f"{ind}if (lane == 0) pref_s[wv][0] = 0;")
-      for fi, expr in enumerate(frame):
-        o.append(f"{ind}fr_s[wv][{fi}][lane] = {expr};")
-      o.append(f"{ind}fr_s[wv][{nfr}][lane] = d2;")
-      o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
-      o.append(f"{ind}__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
-      o.append(f"{ind}for (uint64_t j0 = 0; j0 < tot2; j0 += 64) {{")
-      i2 = ind + "  "
-      o.append(f"{i2}const uint64_t jj = j0 + lane;")
-      o.append(f"{i2}const bool act2 = jj < tot2;")
-      o.append(f"{i2}const uint32_t own = act2 ? srdj::owner_of(pref_s[wv], jj) : 0u;")
-      o.append(f"{i2}const uint64_t loc = act2 ? jj - pref_s[wv][own] : 0;")
-      # reload frame values of the owner lane under fresh names
-      o.append(f"{i2}const uint32_t {self.var(l1var)}_o = fr_s[wv][0][own];")
-      rngs2o = []
-      for j, r in enumerate(rngs2):
-        if per_lane[j] is None:
-          rngs2o.append(r)
-        else:
-          bi, ei = per_lane[j]
-          o.append(f"{i2}const uint32_t f{j}b = fr_s[wv][{bi}][own];")
-          o.append(f"{i2}const uint32_t f{j}e = fr_s[wv][{ei}][own];")
-          rngs2o.append(Rg(r.view, r.depth, f"f{j}b", f"f{j}e"))
-      if v0name is not None:
-        o.append(f"{i2}const uint32_t {self.var(v0name)}_o = fr_s[wv][{nfr - 1}][own];")
-      o.append(f"{i2}const uint32_t d2o = fr_s[wv][{nfr}][own];")
-      # shadow the bound vars with the owner's values inside this scope
-      o.append(f"{i2}{{")
-      i3 = i2 + "  "
-      o.append(f"{i3}const uint32_t {self.var(l1var)} = {self.var(l1var)}_o;")
-      if v0name is not None:
-        o.append(f"{i3}const uint32_t {self.var(v0name)} = {self.var(v0name)}_o;")
-      o.append(f"{i3}bool h2 = false;")
-      if isinstance(lv2, M.ColumnJoin):
-        code, child = self.gen_cj_candidate(lv2, rngs2o, "d2o", "loc", "h2", i3, "l2", "act2",
-                                            "own")
-        o += code
-        bound.add(lv2.var)
-      else:
-        o.append(f"{i3}h2 = act2;")
-        o += self.gen_cart(lv2, rngs2o, "loc", "act2", i3, "b")
-        for vs in lv2.var_from_source:
-          bound.update(vs)
-      for i, f in self.filters_ready(bound, done):
-        o.append(f"{i3}h2 = h2 && {self.filter_code(f)};")
-        done.add(i)
-      if len(done) != len(self.filters):
-        raise CodegenError(f"{self.ep.rule}: filter over unbound variables")
-      o.append(self.emit_code("h2", i3))
-      o.append(f"{i2}}}")
-      o.append(f"{ind}}}")
-      o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
+      self.gen_nested(1, reg, bound, done, o, ind, "h1")
     o += ["    }",
           "    if (MODE == 2) srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
           "    if ((MODE == 0 || MODE == 2) && lane == 0) A.counts[slice] = cnt;",
           "  }",
           "}"]
+    o[lds_at:lds_at] = self._lds
     return "\n".join(o)
+
+  def needed_after(self, li: int) -> set[str]:
+    """Variables read from level li on: source prefixes, checks, heads."""
+    need: set[str] = set()
+    for lv in self.levels[li:]:
+      for s_ in lv.sources:
+        need.update(s_.prefix)
+    for f in self.filters:
+      need.update(f.prefix if isinstance(f, M.Negation) else f.vars)
+    for d in self.dests:
+      need.update(d.terms)
+    return need
+
+  def gen_nested(self, li: int, reg: dict, bound: set[str], done: set[int], o: list[str], ind: str,
+                 hvar: str) -> None:
+    """Level li >= 1 of the join, emitted in wave-uniform control flow.
+
+    Every lane holds one binding of the levels above (alive when `hvar`).
+    The lanes publish their bindings and this level's source ranges in LDS
+    frames, and the wave walks the concatenation of their iteration spaces
+    64 items at a time; each item finds its owner lane through the 65-entry
+    prefix array and re-reads the owner's frame.  Deeper levels repeat the
+    pattern inside the batch loop, so every loop stays wave-uniform and the
+    ballot-based emission at the innermost level stays valid."""
+    if li == len(self.levels):
+      if len(done) != len(self.filters):
+        raise CodegenError(f"{self.ep.rule}: filter over unbound variables")
+      o.append(self.emit_code(hvar, ind))
+      return
+    lv = self.levels[li]
+    t = f"{li}"
+    rngs = [self.resolve(reg, self.vid(s_), tuple(s_.prefix), o, ind, f"b{t}") for s_ in lv.sources]
+    keep = sorted(v for v in bound if v in self.needed_after(li))
+    frame = [self.var(v) for v in keep]
+    per_lane = []
+    for r in rngs:
+      if r.full0:
+        per_lane.append(None)
+      else:
+        per_lane.append((len(frame), len(frame) + 1))
+        frame += [r.b, r.e]
+    nfr = len(frame)
+    self._lds += [f"  __shared__ uint64_t pref_s{t}[4][65];",
+                  f"  __shared__ uint32_t fr_s{t}[4][{nfr + 1}][64];"]
+    o.append(f"{ind}uint64_t sz{t} = 0;")
+    o.append(f"{ind}uint32_t dv{t} = 0;")
+    o.append(f"{ind}if ({hvar}) {{")
+    if isinstance(lv, M.ColumnJoin):
+      o.append(f"{ind}  sz{t} = {self.size_expr(rngs[0], False)};")
+      for j in range(1, len(rngs)):
+        o.append(f"{ind}  {{ const uint64_t z = {self.size_expr(rngs[j], False)}; "
+                 f"if (z < sz{t}) {{ sz{t} = z; dv{t} = {j}; }} }}")
+    else:
+      o.append(f"{ind}  sz{t} = 1;")
+      for r in rngs:
+        o.append(f"{ind}  sz{t} *= uint64_t({self.size_expr(r, True)});")
+    o.append(f"{ind}}}")
+    o.append(f"{ind}const uint64_t inc{t} = srd::wave_incl_scan<uint64_t>(sz{t});")
+    o.append(f"{ind}const uint64_t tot{t} = srdj::uniform_from(inc{t}, 63);")
+    o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
+    o.append(f"{ind}pref_s{t}[wv][lane + 1] = inc{t};")
+    o.append(f"{ind}if (lane == 0) pref_s{t}[wv][0] = 0;")
+    for fi, expr in enumerate(frame):
+      o.append(f"{ind}fr_s{t}[wv][{fi}][lane] = {expr};")
+    o.append(f"{ind}fr_s{t}[wv][{nfr}][lane] = dv{t};")
+    o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
+    o.append(f"{ind}__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
+    o.append(f"{ind}for (uint64_t j{t} = 0; j{t} < tot{t}; j{t} += 64) {{")
+    i2 = ind + "  "
+    o.append(f"{i2}const uint64_t jj{t} = j{t} + lane;")
+    o.append(f"{i2}const bool act{t} = jj{t} < tot{t};")
+    o.append(f"{i2}const uint32_t own{t} = act{t} ? srdj::owner_of(pref_s{t}[wv], jj{t}) : 0u;")
+    o.append(f"{i2}const uint64_t loc{t} = act{t} ? jj{t} - pref_s{t}[wv][own{t}] : 0;")
+    o.append(f"{i2}{{")
+    i3 = i2 + "  "
+    for fi, v in enumerate(keep):
+      o.append(f"{i3}const uint32_t {self.var(v)} = fr_s{t}[wv][{fi}][own{t}];")
+    reg2: dict = {k: r for k, r in reg.items() if r.uniform}
+    rngs_o = []
+    for j, r in enumerate(rngs):
+      if per_lane[j] is None:
+        rngs_o.append(r)
+      else:
+        bi, ei = per_lane[j]
+        o.append(f"{i3}const uint32_t f{t}_{j}b = fr_s{t}[wv][{bi}][own{t}];")
+        o.append(f"{i3}const uint32_t f{t}_{j}e = fr_s{t}[wv][{ei}][own{t}];")
+        rr = Rg(r.view, r.depth, f"f{t}_{j}b", f"f{t}_{j}e")
+        rngs_o.append(rr)
+        s_ = lv.sources[j]
+        reg2[(self.vid(s_), tuple(s_.prefix))] = rr
+    o.append(f"{i3}const uint32_t dvo{t} = fr_s{t}[wv][{nfr}][own{t}];")
+    hn = f"h{li + 1}"
+    o.append(f"{i3}bool {hn} = false;")
+    bound2 = set(keep)
+    if isinstance(lv, M.ColumnJoin):
+      code, child = self.gen_cj_candidate(lv, rngs_o, f"dvo{t}", f"loc{t}", hn, i3, f"l{li + 1}",
+                                          f"act{t}", f"own{t}")
+      o += code
+      reg2.update(child)
+      bound2.add(lv.var)
+    else:
+      o.append(f"{i3}{hn} = act{t};")
+      o += self.gen_cart(lv, rngs_o, f"loc{t}", f"act{t}", i3, f"c{t}")
+      for vs in lv.var_from_source:
+        bound2.update(vs)
+    done2 = set(done)
+    for i, f in self.filters_ready(bound2, done2):
+      o.append(f"{i3}{hn} = {hn} && {self.filter_code(f)};")
+      done2.add(i)
+    self.gen_nested(li + 1, reg2, bound2, done2, o, i3, hn)
+    o.append(f"{i2}}}")
+    o.append(f"{i2}__builtin_amdgcn_wave_barrier();")
+    o.append(f"{ind}}}")
 
   # --------------------------------------------------------- assembly ----
   def gen_kernels(self) -> str:

@@ -114,6 +114,27 @@ def neg2hop_program(count_only: bool) -> Program:
   ])
 
 
+def q3_program() -> Program:
+  """LSQB Q3, directed triangles whose persons live in one country
+  (examples/lsqb_q3_triangle.py): a seven-variable, six-level join."""
+  V = {n: Var(n) for n in ("c", "c1", "c2", "c3", "co", "p", "p1", "p2", "p3", "x", "y")}
+  k_in = Relation("KnowsInput", 2, input_file="Person_knows_Person.csv")
+  loc_in = Relation("IsLocatedInInput", 2, input_file="Person_isLocatedIn_City.csv")
+  part_in = Relation("IsPartOfInput", 2, input_file="City_isPartOf_Country.csv")
+  knows, loc, part = Relation("Knows", 2), Relation("IsLocatedIn", 2), Relation("IsPartOf", 2)
+  tri = Relation("Triangle", 3, print_size=True)
+  x, y, p, c, co = V["x"], V["y"], V["p"], V["c"], V["co"]
+  p1, p2, p3, c1, c2, c3 = V["p1"], V["p2"], V["p3"], V["c1"], V["c2"], V["c3"]
+  return Program(rules=[
+    (knows(x, y) <= k_in(x, y)).named("KnowsLoad"),
+    (loc(p, c) <= loc_in(p, c)).named("LocLoad"),
+    (part(c, co) <= part_in(c, co)).named("PartOfLoad"),
+    (tri(p1, p2, p3) <= knows(p1, p2) & knows(p2, p3) & knows(p3, p1) & loc(p1, c1) & loc(p2, c2)
+     & loc(p3, c3) & part(c1, co) & part(c2, co) & part(c3, co))
+    .named("LabeledTriangle").with_plan(var_order=["p1", "p2", "p3", "c1", "c2", "c3", "co"]),
+  ])
+
+
 def q7_program() -> Program:
   """LSQB Q7, the four OPTIONAL-MATCH cases as count-only rules, two with
   negated marker relations (examples/lsqb_q7_optional.py)."""
@@ -166,6 +187,7 @@ PROGRAMS = {
   "neg2hop": lambda: neg2hop_program(False),
   "neg2hop_count": lambda: neg2hop_program(True),
   "q7": q7_program,
+  "q3": q3_program,
 }
 
 

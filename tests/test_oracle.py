@@ -174,3 +174,23 @@ def test_q7_matches_brute(seed):
   want = brute.q7(inp["HasTag"], inp["HasCreator"], inp["ReplyOf"], inp["Likes"])
   assert r.rule_counts == want
   assert all(v > 0 for v in want.values()) or seed > 0
+
+
+def q3_inputs(seed, n=40, m=260, cities=10, countries=3, extra_loc=6):
+  rng = np.random.default_rng(seed)
+  k = ds.tc_edges(seed, n, m)
+  loc = np.stack([np.arange(n), rng.integers(0, cities, n)], 1)
+  more = np.stack([rng.integers(0, n, extra_loc), rng.integers(0, cities, extra_loc)], 1)
+  loc = np.unique(np.concatenate([loc, more]).astype(np.uint32), axis=0)
+  part = np.stack([np.arange(cities), rng.integers(0, countries, cities)], 1)
+  part = np.unique(np.concatenate([part, [[0, countries - 1]]]).astype(np.uint32), axis=0)
+  return {"KnowsInput": k, "IsLocatedInInput": loc, "IsPartOfInput": part}
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_q3_matches_brute(seed):
+  inp = q3_inputs(seed)
+  r = oracle.run("q3", inp)
+  want = brute.q3(inp["KnowsInput"], inp["IsLocatedInInput"], inp["IsPartOfInput"])["Triangle"]
+  assert as_set(r.relations["Triangle"]) == want
+  assert len(want) > 0

@@ -283,3 +283,27 @@ def test_q7_counts(progs, n_msg, sizes, seed):
       assert lib.size(case) == want, case
   finally:
     lib.shutdown()
+
+
+def _q3_inputs(seed, n, m, cities, countries, extra_loc):
+  rng = np.random.default_rng(seed)
+  k = ds.tc_edges(seed, n, m)
+  loc = np.stack([np.arange(n), rng.integers(0, cities, n)], 1)
+  more = np.stack([rng.integers(0, n, extra_loc), rng.integers(0, cities, extra_loc)], 1)
+  loc = np.unique(np.concatenate([loc, more]).astype(np.uint32), axis=0)
+  part = np.stack([np.arange(cities), rng.integers(0, countries, cities)], 1)
+  part = np.unique(part.astype(np.uint32), axis=0)
+  return {"KnowsInput": k, "IsLocatedInInput": loc, "IsPartOfInput": part}
+
+
+@pytest.mark.parametrize("n,m,cities,countries,extra,seed", [(40, 260, 10, 3, 6, 0),
+                                                             (3000, 60000, 200, 8, 300, 1),
+                                                             (50000, 800000, 2000, 20, 5000, 2)])
+def test_q3_six_level_join(progs, n, m, cities, countries, extra, seed):
+  inp = _q3_inputs(seed, n, m, cities, countries, extra)
+  orc = oracle.run("q3", inp)
+  lib = run_prog(progs, "q3", inp)
+  try:
+    check_rel(lib, orc, "Triangle", 3)
+  finally:
+    lib.shutdown()
