@@ -470,6 +470,17 @@ class PipelineGen:
           "    if ((MODE == 0 || MODE == 2) && lane == 0) A.counts[slice] = cnt;",
           "  }",
           "}"]
+    lds_bytes = 0
+    for d in self._lds:
+      dims = [int(x) for x in re.findall(r"\[(\d+)\]", d)]
+      elem = 8 if "uint64_t" in d else 4
+      n = elem
+      for x in dims:
+        n *= x
+      lds_bytes += n
+    if lds_bytes > 64 * 1024:
+      raise CodegenError(f"{self.ep.rule}: the join's level frames need {lds_bytes} bytes of LDS "
+                         "per block (limit 65536); split the rule")
     o[lds_at:lds_at] = self._lds
     return "\n".join(o)
 
