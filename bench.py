@@ -156,6 +156,30 @@ def bench_batch(args) -> dict:
   t_p = time.perf_counter() - t
   log(f"batch: plain {len(ok)} instances in {t_p:.2f}s")
 
+  # where the time goes: kernel time (HIP events) against wall time, for one
+  # more batched run and for plain runs of a 16-instance sample
+  from srdatalog_hip import native
+  diag = {}
+  for label, runner, sets in (("batched", batch.run_batched, kinds),
+                              ("plain_sample16", batch.run_plain,
+                               {k: v[:: max(1, len(v) // 8)][:8] for k, v in kinds.items()})):
+    native.timing_reset()
+    native.timing_enable(True)
+    t = time.perf_counter()
+    for k, lst in sets.items():
+      runner(k, lst, grain=grains[k])
+    wall = time.perf_counter() - t
+    native.timing_enable(False)
+    ks = native.timing_report()
+    native.timing_reset()
+    diag[label] = {"wall_s": round(wall, 4), "instances": sum(len(v) for v in sets.values()),
+                   "kernel_ms": round(sum(k_["ms"] for k_ in ks), 2),
+                   "launches": int(sum(k_["launches"] for k_ in ks)),
+                   "top": [{"name": k_["name"], "ms": round(k_["ms"], 2), "launches": k_["launches"]}
+                           for k_ in sorted(ks, key=lambda z: -z["ms"])[:6]]}
+    log(f"batch: {label}: wall {wall:.2f}s, kernels {diag[label]['kernel_ms']:.0f} ms in "
+        f"{diag[label]['launches']} launches")
+
   mismatch = 0
   tuples = 0
   for k in kinds:
@@ -182,6 +206,7 @@ def bench_batch(args) -> dict:
     "mismatched_instances": mismatch,
     "grain": grains,
     "admission_s": round(admit_s, 2),
+    "time_split": diag,
   }
 
 

@@ -83,10 +83,18 @@ def estimate_output(inst: Instance, rng: np.random.Generator, samples: int = 24)
              for s in src]
     return float(n) * float(np.mean(reach))
   if inst.kind == "sg":
-    # pairs of distinct vertices at equal depth below the forest roots
-    indeg = np.asarray((a != 0).sum(0)).ravel()
+    # Pairs at equal depth below the source components of the condensation
+    # (a forest's roots; a root that received a cross edge sits on a cycle
+    # and stays a source component).  A cycle makes generations repeat
+    # around it, so everything reachable from a cycle is counted as one
+    # generation (pessimistic: admission rather overestimates).
+    ncomp, comp = csg.connected_components(a, directed=True, connection="strong")
+    csize = np.bincount(comp, minlength=ncomp)
+    cross = comp[e[:, 0]] != comp[e[:, 1]]
+    has_in = np.zeros(ncomp, bool)
+    has_in[comp[e[cross, 1]]] = True
     depth = np.full(n, -1, np.int64)
-    frontier = np.flatnonzero(indeg == 0)
+    frontier = np.flatnonzero(~has_in[comp])
     depth[frontier] = 0
     d = 0
     while frontier.size:
@@ -96,7 +104,20 @@ def estimate_output(inst: Instance, rng: np.random.Generator, samples: int = 24)
       depth[nxt] = d
       frontier = nxt
     cnt = np.bincount(depth[depth >= 0]).astype(np.float64)
-    return float((cnt * cnt).sum())
+    est = float((cnt * cnt).sum())
+    cyclic = np.flatnonzero((csize[comp] > 1) | (a.diagonal() != 0))
+    if cyclic.size:
+      seen = np.zeros(n, bool)
+      seen[cyclic] = True
+      frontier = cyclic
+      while frontier.size:
+        nxt = np.unique(a[frontier].indices)
+        nxt = nxt[~seen[nxt]]
+        seen[nxt] = True
+        frontier = nxt
+      r = float(seen.sum())
+      est += r * r
+    return est
   raise ValueError(f"unknown instance kind {inst.kind!r}")
 
 

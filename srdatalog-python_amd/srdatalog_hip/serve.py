@@ -251,11 +251,15 @@ class Server:
   the call of run())."""
 
   def __init__(self, graphs: list[Graph], budget_bytes: float = 200e9, cap_tuples: float = 5e8,
-               max_batch: int = 256):
+               max_batch: int = 256, batch_tuples: float = 5e8):
     self.graphs = graphs
     self.budget = budget_bytes
     self.cap = cap_tuples
     self.max_batch = max_batch
+    # summed estimate of one batch: a batch's join output (NEW, before
+    # deduplication, several times the estimate) must stay far below the
+    # 2^32 rows an index holds
+    self.batch_tuples = batch_tuples
     self.runners = {"reach": ReachRunner(graphs), "tri": PackedRunner("tri"),
                     "tc": PackedRunner("tc")}
     self.lock = threading.Lock()
@@ -271,6 +275,7 @@ class Server:
     """Admit arrived, prepared requests in arrival order while the in-flight
     estimate fits the budget."""
     take = []
+    batch_est = 0.0
     with self.lock:
       while pos < len(queue) and queue[pos].t <= now and len(take) < self.max_batch:
         r = queue[pos]
@@ -283,7 +288,10 @@ class Server:
         need = r.est * BYTES_PER_TUPLE
         if self.in_flight > 0 and self.in_flight + need > self.budget:
           break                      # wait for memory to free up
+        if take and batch_est + r.est > self.batch_tuples:
+          break                      # the rest goes into the next batch
         self.in_flight += need
+        batch_est += r.est
         take.append(r)
         pos += 1
     return take, pos
