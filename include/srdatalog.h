@@ -91,6 +91,14 @@ int srdatalog_set_instances(const uint32_t* bounds, uint64_t n_bounds);
 long long srdatalog_instance_delta_sizes(const char* rel_name, unsigned long long* out,
                                          unsigned long long cap_iters);
 
+/* Export of a packed batch: like srdatalog_export, but every value of a row
+ * is rebased to its instance (minus the instance's first bound), and cuts
+ * (n_inst+1 entries, may be NULL) receives each instance's first row, so
+ * instance g owns rows [cuts[g], cuts[g+1]).  Needs an instance table.
+ * 0 ok, 1 null, 2 unknown relation, 3 error. */
+int srdatalog_export_instances(const char* rel_name, uint32_t* out, uint64_t cap_rows,
+                               uint64_t* n_rows, uint64_t* cuts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,8 @@ unsigned long long srd_engine_join_tuples(struct srd_engine* e);
 void srd_engine_set_quiet(struct srd_engine* e, int quiet);
 void srd_engine_set_grain(struct srd_engine* e, unsigned grain);
 int srd_engine_set_instances(struct srd_engine* e, const uint32_t* bounds, uint64_t n_bounds);
+int srd_engine_export_instances(struct srd_engine* e, const char* rel, uint32_t* out,
+                                uint64_t cap_rows, uint64_t* n_rows, uint64_t* cuts);
 long long srd_engine_instance_delta_sizes(struct srd_engine* e, const char* rel,
                                           unsigned long long* out, unsigned long long cap_iters);
 

@@ -214,6 +214,10 @@ struct Relation {
   // the younger ones (pairwise disjoint, sizes roughly halving)
   std::vector<Index> segs;
   bool segmented = false;
+  // FULL in relation column order for export, valid while export_version
+  // equals full_version (the size query and the copy share one re-sort)
+  Index export_idx;
+  uint64_t export_version = ~uint64_t(0);
   uint64_t count = 0;               // count-only result
   bool counted = false;
 };
@@ -956,6 +960,8 @@ static void reset_state(srd_engine& e) {
     r.inst_deltas.clear();
     r.segs.clear();
     r.segmented = false;
+    r.export_idx = Index();
+    r.export_version = ~uint64_t(0);
     r.deltas_step = -1;
     r.count = 0;
     r.counted = false;
@@ -1213,6 +1219,20 @@ unsigned long long srd_engine_size(srd_engine* e, const char* rel) {
   return 0;
 }
 
+// FULL of r in relation column order (0..k-1), cached per FULL version.
+static const Index* export_index(srd_engine& e, Relation& r) {
+  auto f = r.full.find(r.canonical);
+  if (f == r.full.end() || f->second.idx.n == 0) return nullptr;
+  Order ident(r.arity);
+  for (uint32_t c = 0; c < r.arity; ++c) ident[c] = c;
+  if (r.canonical == ident) return &f->second.idx;
+  if (r.export_version != r.full_version) {
+    build_index_from(f->second.idx, perm_between(r.canonical, ident), r.export_idx, e.ws, e.s);
+    r.export_version = r.full_version;
+  }
+  return &r.export_idx;
+}
+
 int srd_engine_export(srd_engine* e, const char* rel, uint32_t* out, uint64_t cap_rows,
                       uint64_t* n_rows) {
   if (!e || !rel || !n_rows) return 1;
@@ -1221,24 +1241,46 @@ int srd_engine_export(srd_engine* e, const char* rel, uint32_t* out, uint64_t ca
   Relation& r = e->rels[it->second];
   return eguard(e, 3, [&] {
     *n_rows = 0;
-    auto f = r.full.find(r.canonical);
-    if (f == r.full.end() || f->second.idx.n == 0) return;
-    const Index* x = &f->second.idx;
-    Index tmp;
-    Order ident(r.arity);
-    for (uint32_t c = 0; c < r.arity; ++c) ident[c] = c;
-    if (r.canonical != ident) {
-      build_index_from(*x, perm_between(r.canonical, ident), tmp, e->ws, e->s);
-      x = &tmp;
-    }
+    const Index* x = export_index(*e, r);
+    if (!x) return;
     *n_rows = x->n;
     if (!out) return;
     if (cap_rows < x->n) fail("srdatalog_export: buffer too small");
-    std::vector<uint32_t> t(x->stride * x->arity);
-    SRD_CHECK(hipMemcpyAsync(t.data(), x->cols.p, t.size() * 4, hipMemcpyDeviceToHost, e->s));
-    SRD_CHECK(hipStreamSynchronize(e->s));
-    for (uint64_t i = 0; i < x->n; ++i)
-      for (uint32_t c = 0; c < x->arity; ++c) out[i * x->arity + c] = t[c * x->stride + i];
+    export_rows(*x, out, nullptr, 0, e->ws, e->s);
+  });
+}
+
+int srd_engine_export_instances(srd_engine* e, const char* rel, uint32_t* out, uint64_t cap_rows,
+                                uint64_t* n_rows, uint64_t* cuts) {
+  if (!e || !rel || !n_rows) return 1;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return 2;
+  Relation& r = e->rels[it->second];
+  return eguard(e, 3, [&] {
+    if (!e->n_inst) fail("srdatalog_export_instances: no instance table");
+    *n_rows = 0;
+    const Index* x = export_index(*e, r);
+    if (cuts) std::fill(cuts, cuts + e->n_inst + 1, 0);
+    if (!x) return;
+    *n_rows = x->n;
+    if (!out) return;
+    if (cap_rows < x->n) fail("srdatalog_export_instances: buffer too small");
+    if (cuts) {
+      DevBuf<uint64_t> per(e->n_inst, e->s);
+      instance_counts(x->cols.p, x->n, e->inst_bounds.p, e->n_inst, per.p, e->s);
+      std::vector<uint64_t> h(e->n_inst);
+      SRD_CHECK(hipMemcpyAsync(h.data(), per.p, e->n_inst * 8, hipMemcpyDeviceToHost, e->s));
+      SRD_CHECK(hipStreamSynchronize(e->s));
+      // rows below the first bound belong to no instance; they precede cut 0
+      uint64_t acc = x->n;
+      for (uint32_t g = 0; g < e->n_inst; ++g) acc -= h[g];
+      for (uint32_t g = 0; g < e->n_inst; ++g) {
+        cuts[g] = acc;
+        acc += h[g];
+      }
+      cuts[e->n_inst] = acc;
+    }
+    export_rows(*x, out, e->inst_bounds.p, e->n_inst + 1, e->ws, e->s);
   });
 }
 

@@ -379,6 +379,38 @@ void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, u
   SRD_CHECK(hipGetLastError());
 }
 
+// Row-major copy of an index for the host: out[i*k + c] = column c of row i,
+// minus the base of the row's instance (the largest bounds[g] <= column 0)
+// when an instance table is given.
+__global__ __launch_bounds__(kBlock) void interleave_kernel(const uint32_t* __restrict__ cols,
+                                                            uint64_t stride, uint64_t n,
+                                                            uint32_t k,
+                                                            const uint32_t* __restrict__ bounds,
+                                                            uint32_t n_bounds,
+                                                            uint32_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    uint32_t base = 0;
+    if (n_bounds) {
+      const uint32_t g = upper_bound(bounds, 0, n_bounds, cols[i]);
+      base = g ? bounds[g - 1] : 0u;
+    }
+    for (uint32_t c = 0; c < k; ++c) out[i * k + c] = cols[uint64_t(c) * stride + i] - base;
+  }
+}
+
+void export_rows(const Index& x, uint32_t* host_out, const uint32_t* dev_bounds,
+                 uint32_t n_bounds, Workspace& ws, hipStream_t s) {
+  (void)ws;
+  if (x.n == 0) return;
+  DevBuf<uint32_t> buf(x.n * x.arity, s);
+  interleave_kernel<<<grid_for(x.n, kBlock * 4, 8192), kBlock, 0, s>>>(
+      x.cols.p, x.stride, x.n, x.arity, dev_bounds, n_bounds, buf.p);
+  SRD_CHECK(hipGetLastError());
+  SRD_CHECK(hipMemcpyAsync(host_out, buf.p, x.n * x.arity * 4, hipMemcpyDeviceToHost, s));
+  SRD_CHECK(hipStreamSynchronize(s));
+}
+
 // ================================================================ scan ====
 constexpr int kItems = 16;
 constexpr uint32_t kTile = kBlock * kItems;  // 4096

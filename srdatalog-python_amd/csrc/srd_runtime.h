@@ -181,6 +181,12 @@ void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep);
 // Synthetic columns (k fields of one 64-bit key per row, low k-1 fields
 // fbits wide): key = mul*i + off when range == 0 (ascending, distinct for
 // mul > 0), else a hash of i+off reduced mod range.
+// Copy index x to host memory as row-major rows (n x arity u32).  With an
+// instance table (dev_bounds, n_bounds ascending column-0 range starts) every
+// value of a row is rebased by its instance's start.
+void export_rows(const Index& x, uint32_t* host_out, const uint32_t* dev_bounds,
+                 uint32_t n_bounds, Workspace& ws, hipStream_t s);
+
 // out[g] = rows of the column-0-sorted column col0 (n rows) with value in
 // [bounds[g], bounds[g+1]), g < n_inst (bounds has n_inst+1 entries).
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,

@@ -218,11 +218,14 @@ def run_batched(kind: str, instances: list[Instance], grain: int = 0) -> list[Re
     lib.set_instances(bounds)
     if lib.run(0) != 0:
       raise RuntimeError(f"srdatalog_run failed for the packed {kind} batch")
-    blocks = split(lib.export(rout, arity), bounds)
+    # rows come back rebased to instance-local ids with per-instance offsets,
+    # so each instance's result is a view of one buffer
+    rows_out, cuts = lib.export_instances(rout, arity, len(instances))
     per_iter = lib.instance_delta_sizes(rout, len(instances))
   finally:
     lib.shutdown()
-  return [Result(blocks[i], trim_deltas(per_iter[:, i])) for i in range(len(instances))]
+  return [Result(rows_out[cuts[i]:cuts[i + 1]], trim_deltas(per_iter[:, i]))
+          for i in range(len(instances))]
 
 
 def autotune_grain(kind: str, instances: list[Instance],

@@ -756,6 +756,12 @@ long long srdatalog_instance_delta_sizes(const char* rel_name, unsigned long lon
   return srd_engine_instance_delta_sizes(g_eng, rel_name, out, cap_iters);
 }
 
+int srdatalog_export_instances(const char* rel_name, uint32_t* out, uint64_t cap_rows,
+                               uint64_t* n_rows, uint64_t* cuts) {
+  if (!g_eng) return 3;
+  return srd_engine_export_instances(g_eng, rel_name, out, cap_rows, n_rows, cuts);
+}
+
 }  // extern "C"
 """
 

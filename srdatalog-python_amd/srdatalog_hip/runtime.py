@@ -56,6 +56,9 @@ class ProgramLib:
                                                    ctypes.POINTER(ctypes.c_ulonglong),
                                                    ctypes.c_ulonglong]
     lib.srdatalog_instance_delta_sizes.restype = ctypes.c_longlong
+    lib.srdatalog_export_instances.argtypes = [ctypes.c_char_p, _u32p, ctypes.c_uint64, _u64p,
+                                               _u64p]
+    lib.srdatalog_export_instances.restype = ctypes.c_int
     self.lib = lib
     self.path = so_path
 
@@ -140,3 +143,20 @@ class ProgramLib:
       if rc != it:
         raise SrdatalogError(f"srdatalog_instance_delta_sizes({rel}) returned {rc}")
     return out[:it, :n_inst].astype(np.int64)
+
+  def export_instances(self, rel: str, arity: int, n_inst: int) -> tuple[np.ndarray, np.ndarray]:
+    """Rows of rel with instance-local ids, and the n_inst+1 row offsets of
+    the instances (instance g owns rows cuts[g]:cuts[g+1])."""
+    n = ctypes.c_uint64()
+    cuts = np.zeros(n_inst + 1, dtype=np.uint64)
+    rc = self.lib.srdatalog_export_instances(rel.encode(), None, 0, ctypes.byref(n), None)
+    if rc != 0:
+      raise SrdatalogError(f"srdatalog_export_instances({rel}) returned {rc}")
+    out = np.empty((max(n.value, 1), arity), dtype=np.uint32)
+    rc = self.lib.srdatalog_export_instances(rel.encode(), out.ctypes.data_as(_u32p), n.value,
+                                             ctypes.byref(n), cuts.ctypes.data_as(_u64p))
+    if rc != 0:
+      raise SrdatalogError(f"srdatalog_export_instances({rel}) returned {rc}")
+    if n.value == 0:
+      cuts[:] = 0
+    return out[:n.value], cuts.astype(np.int64)

@@ -235,10 +235,12 @@ class PackedRunner:
              for r in reqs]
     rows, bounds = batch.pack(insts)
     self.lib.load_rows(self.rin, rows)
+    self.lib.set_instances(bounds)
     if self.lib.run(0) != 0:
       raise RuntimeError(f"srdatalog_run failed ({self.kind} batch)")
-    blocks = batch.split(self.lib.export(self.rout, self.arity), bounds)
-    for r, blk in zip(reqs, blocks):
+    out, cuts = self.lib.export_instances(self.rout, self.arity, len(reqs))
+    for i, r in enumerate(reqs):
+      blk = out[cuts[i]:cuts[i + 1]]
       r.result = int(blk.shape[0]) if self.kind == "tri" else blk
 
   def close(self) -> None:

@@ -129,6 +129,19 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
 void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
 
+void export_rows(const Index& x, uint32_t* host_out, const uint32_t* bounds, uint32_t n_bounds,
+                 Workspace&, hipStream_t) {
+  for (uint64_t i = 0; i < x.n; ++i) {
+    uint32_t base = 0;
+    if (n_bounds) {
+      const uint32_t v = x.cols.p[i];
+      const uint32_t* g = std::upper_bound(bounds, bounds + n_bounds, v);
+      base = g == bounds ? 0u : *(g - 1);
+    }
+    for (uint32_t c = 0; c < x.arity; ++c) host_out[i * x.arity + c] = x.cols.p[c * x.stride + i] - base;
+  }
+}
+
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t) {
   for (uint32_t g = 0; g < n_inst; ++g) {
