@@ -48,7 +48,7 @@ def parse():
                       "Poisson request stream, continuous batching vs plain serial")
   p.add_argument("--serve-rate", type=float, default=200.0)
   p.add_argument("--serve-seconds", type=float, default=60.0)
-  p.add_argument("--serve-cap", type=float, default=2e8,
+  p.add_argument("--serve-cap", type=float, default=5e7,
                  help="serve: per-request cap on the estimated output tuples")
   p.add_argument("--serve-budget", type=float, default=200e9)
   p.add_argument("--serve-plain-cap", type=float, default=60.0,

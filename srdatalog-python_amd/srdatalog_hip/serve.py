@@ -252,8 +252,8 @@ class Server:
   """Continuous batching over one request stream (arrival times relative to
   the call of run())."""
 
-  def __init__(self, graphs: list[Graph], budget_bytes: float = 200e9, cap_tuples: float = 5e8,
-               max_batch: int = 256, batch_tuples: float = 5e8):
+  def __init__(self, graphs: list[Graph], budget_bytes: float = 200e9, cap_tuples: float = 5e7,
+               max_batch: int = 256, batch_tuples: float = 2e8):
     self.graphs = graphs
     self.budget = budget_bytes
     self.cap = cap_tuples
