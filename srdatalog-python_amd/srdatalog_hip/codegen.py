@@ -73,8 +73,13 @@ class PipeInfo:
 class PipelineGen:
   """Emits the kernels of one pipeline."""
 
-  def __init__(self, ep: M.ExecutePipeline, arity: dict[str, int], uid: int):
+  def __init__(self, ep: M.ExecutePipeline, arity: dict[str, int], uid: int,
+               coop_search: bool = False):
     self.ep = ep
+    # run-narrowed cooperative searches (srdj::seg_lower_bound) instead of one
+    # independent binary search per lane; measured slower on the triangle
+    # join, whose searches are latency-bound, so off by default
+    self.coop_search = coop_search
     self.arity = arity
     self.ns = f"srdp_{uid}_{cident(ep.rule)}"
     self.info = PipeInfo(ep)
@@ -173,11 +178,12 @@ class PipelineGen:
                      f" = {self.var(d.terms[relcol])};")
     lines += [f"{ind}  }}",
               f"{ind}  if (MODE == 2 && c_) {{",
-              f"{ind}    if (seg_start == ~uint64_t(0) || seg_used + c_ > SRD_SEG) {{",
+              f"{ind}    if (!seg_dead && (seg_start == ~uint64_t(0) || seg_used + c_ > SRD_SEG)) {{",
               f"{ind}      srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
               f"{ind}      seg_start = srdj::seg_open(A);",
               f"{ind}      seg_used = 0;",
               f"{ind}      seg_local = cnt;",
+              f"{ind}      seg_dead = seg_start == ~uint64_t(0);",
               f"{ind}    }}",
               f"{ind}    if (seg_start != ~uint64_t(0) && ({cond})) {{",
               f"{ind}      const uint64_t o_ = seg_start + seg_used + srd::popc_below(m_);"]
@@ -343,7 +349,10 @@ class PipelineGen:
         else:
           code.append(f"{i2}const uint32_t* col_ = {self.col_expr(rm.view, rm.depth)};")
           code.append(f"{i2}const uint32_t b_ = {rm.b}, e_ = {rm.e};")
-        code.append(f"{i2}const uint32_t lb_ = srdj::seg_lower_bound(col_, b_, e_, {var}, on_, {seg});")
+        if self.coop_search:
+          code.append(f"{i2}const uint32_t lb_ = srdj::seg_lower_bound(col_, b_, e_, {var}, on_, {seg});")
+        else:
+          code.append(f"{i2}const uint32_t lb_ = on_ ? srd::lower_bound(col_, b_, e_, {var}) : 0u;")
         code.append(f"{i2}if (on_) {{")
         code.append(f"{i2}  const bool hit_ = lb_ < e_ && col_[lb_] == {var};")
         code.append(f"{i2}  {tag}on = hit_;")
@@ -405,6 +414,7 @@ class PipelineGen:
           "    uint64_t opos = MODE == 1 ? A.offsets[slice] : 0;",
           "    uint32_t cnt = 0;",
           "    uint64_t seg_start = ~uint64_t(0);",
+          "    bool seg_dead = false;  // staging exhausted: count only (host re-runs the write pass)",
           "    uint32_t seg_used = 0, seg_local = 0;",
           "    for (uint64_t base = it0; base < it1; base += 64) {",
           "      const uint64_t item = base + lane;",
