@@ -1344,6 +1344,217 @@ __global__ __launch_bounds__(kBlock) void window_kernel(
   }
 }
 
+// ====================================================== merge path ====
+// Merge and (dense) set difference of two sorted, duplicate-free k-column
+// row sets A and B by merge-path partitioning (Odeh, Green et al., "Merge
+// Path", 2012): the merged order is cut into tiles of kMpTile positions at
+// diagonal split points, each tile stages its piece of A and of B in LDS
+// with coalesced loads and merges them there, and the output leaves through
+// LDS with coalesced stores.  Ties are broken B-first (a row of A comes
+// before a row of B only when strictly less), so a row of A that is also in
+// B directly follows its equal B row; for the first rows of a tile that B row
+// may be the last one before the tile, B[j0-1], which the tile stages too.
+constexpr int kMpItems = 8;
+constexpr uint32_t kMpTile = kBlock * kMpItems;
+
+// Rows of A among the first d rows of the B-first merged order.
+template <int K>
+__device__ __forceinline__ uint64_t mp_split(const uint32_t* a, uint64_t sa, uint64_t na,
+                                             const uint32_t* b, uint64_t sb, uint64_t nb,
+                                             uint64_t d) {
+  uint64_t lo = d > nb ? d - nb : 0, hi = min<uint64_t>(d, na);
+  while (lo < hi) {
+    const uint64_t i = (lo + hi + 1) >> 1;  // A[i-1] and B[d-i] both exist
+    if (row_less<K>(load_row<K>(a, sa, i - 1), load_row<K>(b, sb, d - i))) lo = i;
+    else hi = i - 1;
+  }
+  return lo;
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void mp_partition_kernel(
+    const uint32_t* __restrict__ a, uint64_t sa, uint64_t na, const uint32_t* __restrict__ b,
+    uint64_t sb, uint64_t nb, uint64_t tiles, uint64_t* __restrict__ split) {
+  const uint64_t t = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  if (t > tiles) return;
+  const uint64_t d = min<uint64_t>(t * kMpTile, na + nb);
+  split[t] = mp_split<K>(a, sa, na, b, sb, nb, d);
+}
+
+// LDS row r of the staged tile: A rows at [0, an), then B[j0-1] at an (when
+// j0 > 0), then B rows.
+template <int K>
+__device__ __forceinline__ RowK<K> lds_row(const uint32_t (*st)[kMpTile + 2], uint32_t r) {
+  RowK<K> x;
+#pragma unroll
+  for (int c = 0; c < K; ++c) x.v[c] = st[c][r];
+  return x;
+}
+
+// MODE 1: out gets the merged rows at d0 + position.  MODE 0: out gets the
+// rows of A not in B, compacted; tile offsets come from a decoupled
+// look-back in ticket order, and the last tile stores the total.
+template <int K, int MODE>
+__global__ __launch_bounds__(kBlock) void mp_tile_kernel(
+    const uint32_t* __restrict__ a, uint64_t sa, uint64_t na, const uint32_t* __restrict__ b,
+    uint64_t sb, uint64_t nb, const uint64_t* __restrict__ split, uint32_t* __restrict__ out,
+    uint64_t so, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctr, uint32_t epoch,
+    uint64_t* __restrict__ total_out) {
+  __shared__ uint32_t st[K][kMpTile + 2];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wsum[kWavesPerBlock];
+  __shared__ uint64_t s_prefix;
+  const uint32_t tid = threadIdx.x;
+  uint32_t tile;
+  if (MODE == 0) {
+    if (tid == 0) s_tile = atomicAdd(&ctr[0], 1u);
+    __syncthreads();
+    tile = s_tile;
+  } else {
+    tile = blockIdx.x;
+  }
+  const uint64_t d0 = uint64_t(tile) * kMpTile;
+  const uint64_t d1 = min<uint64_t>(d0 + kMpTile, na + nb);
+  const uint64_t i0 = split[tile], i1 = split[tile + 1];
+  const uint64_t j0 = d0 - i0, j1 = d1 - i1;
+  const uint32_t an = uint32_t(i1 - i0), bn = uint32_t(j1 - j0);
+  const uint32_t pre = (MODE == 0 && j0 > 0) ? 1u : 0u;  // B[j0-1] staged at an
+  const uint32_t bb = an + pre;                          // first staged B row of the tile
+  for (uint32_t r = tid; r < an + pre + bn; r += kBlock) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      uint32_t v;
+      if (r < an) v = a[uint64_t(c) * sa + i0 + r];
+      else v = b[uint64_t(c) * sb + j0 - pre + (r - an)];
+      st[c][r] = v;
+    }
+  }
+  __syncthreads();
+
+  // this thread's merged positions [p0, p0 + kMpItems) of the tile
+  const uint32_t tn = an + bn;
+  const uint32_t p0 = min<uint32_t>(tid * kMpItems, tn);
+  uint32_t lo = p0 > bn ? p0 - bn : 0, hi = min<uint32_t>(p0, an);
+  while (lo < hi) {
+    const uint32_t i = (lo + hi + 1) >> 1;
+    if (row_less<K>(lds_row<K>(st, i - 1), lds_row<K>(st, bb + (p0 - i)))) lo = i;
+    else hi = i - 1;
+  }
+  uint32_t ai = lo, bi = p0 - lo;
+  RowK<K> rows[kMpItems];
+  bool keep[kMpItems];
+  uint32_t kept = 0;
+#pragma unroll
+  for (int it = 0; it < kMpItems; ++it) {
+    keep[it] = false;
+    if (p0 + uint32_t(it) >= tn) continue;
+    const bool take_a =
+        bi >= bn || (ai < an && row_less<K>(lds_row<K>(st, ai), lds_row<K>(st, bb + bi)));
+    if (take_a) {
+      rows[it] = lds_row<K>(st, ai);
+      if (MODE == 0) {
+        // in B iff the last B row at or before this position equals it
+        const bool has_prev = bi > 0 || pre;
+        keep[it] = !(has_prev && row_eq<K>(lds_row<K>(st, bb + bi - 1), rows[it]));
+        kept += keep[it] ? 1u : 0u;
+      }
+      ++ai;
+    } else {
+      if (MODE == 1) rows[it] = lds_row<K>(st, bb + bi);
+      ++bi;
+    }
+  }
+  __syncthreads();  // every thread is done reading the staged input
+
+  if (MODE == 1) {
+#pragma unroll
+    for (int it = 0; it < kMpItems; ++it)
+      if (p0 + uint32_t(it) < tn)
+#pragma unroll
+        for (int c = 0; c < K; ++c) st[c][p0 + it] = rows[it].v[c];
+    __syncthreads();
+    for (uint32_t r = tid; r < tn; r += kBlock)
+#pragma unroll
+      for (int c = 0; c < K; ++c) out[uint64_t(c) * so + d0 + r] = st[c][r];
+    return;
+  }
+
+  uint32_t tile_tot;
+  uint32_t off = block_excl_scan<uint32_t>(kept, s_wsum, &tile_tot);
+#pragma unroll
+  for (int it = 0; it < kMpItems; ++it)
+    if (keep[it]) {
+#pragma unroll
+      for (int c = 0; c < K; ++c) st[c][off] = rows[it].v[c];
+      ++off;
+    }
+  if (tid == 0) {
+    uint64_t excl = 0;
+    uint64_t* mine = lookback + tile;
+    if (tile == 0) {
+      lb_store(mine, lb_pack(epoch, kLbPrefix, tile_tot));
+    } else {
+      lb_store(mine, lb_pack(epoch, kLbAgg, tile_tot));
+      int64_t t = int64_t(tile) - 1;
+      uint32_t spins = 0;
+      while (t >= 0) {
+        const uint64_t v = lb_load(lookback + t);
+        const uint64_t flag = (v >> 46) & 3;
+        if (uint32_t(v >> 48) != epoch || flag == 0) {
+          if (++spins > kSpinLimit) {
+            atomicOr(&ctr[1], 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += v & kLbMask;
+        if (flag == kLbPrefix) break;
+        --t;
+      }
+      lb_store(mine, lb_pack(epoch, kLbPrefix, excl + tile_tot));
+    }
+    s_prefix = excl;
+    if (d1 == na + nb) *total_out = excl + tile_tot;
+  }
+  __syncthreads();
+  const uint64_t base = s_prefix;
+  for (uint32_t r = tid; r < tile_tot; r += kBlock)
+#pragma unroll
+    for (int c = 0; c < K; ++c) out[uint64_t(c) * so + base + r] = st[c][r];
+}
+
+// Launch the partition and the tiles; returns the number of tiles.
+template <int K, int MODE>
+static uint64_t merge_path(const Index& a, const Index& b, uint32_t* out, uint64_t so,
+                           Workspace& ws, hipStream_t s) {
+  const uint64_t tiles = (a.n + b.n + kMpTile - 1) / kMpTile;
+  ws.mp_split.reserve(tiles + 1, s);
+  mp_partition_kernel<K><<<uint32_t((tiles + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, tiles, ws.mp_split.p);
+  SRD_CHECK(hipGetLastError());
+  if (MODE == 0) {
+    if (tiles > ws.mp_lb.n) {
+      ws.mp_lb.alloc(tiles + tiles / 4 + 64, s);
+      fill_u64(ws.mp_lb.p, 0, ws.mp_lb.n, s);
+      ws.mp_epoch = 0;
+    }
+    if (++ws.mp_epoch >= 0xFFFF) {
+      fill_u64(ws.mp_lb.p, 0, ws.mp_lb.n, s);
+      ws.mp_epoch = 1;
+    }
+    ws.mp_ctr.reserve(2, s);
+    ws.mp_total.reserve(1, s);
+    fill_u32(ws.mp_ctr.p, 0, 2, s);
+  }
+  mp_tile_kernel<K, MODE><<<uint32_t(tiles), kBlock, 0, s>>>(
+      a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, out, so,
+      MODE == 0 ? ws.mp_lb.p : nullptr, MODE == 0 ? ws.mp_ctr.p : nullptr, ws.mp_epoch,
+      MODE == 0 ? ws.mp_total.p : nullptr);
+  SRD_CHECK(hipGetLastError());
+  return tiles;
+}
+
 template <int K>
 __global__ __launch_bounds__(kBlock) void compact_kernel(const uint32_t* __restrict__ a,
                                                          uint64_t sa, uint64_t na,
@@ -1381,6 +1592,34 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(const uint32_t* __restr
 template <int K>
 static void set_difference_k(const Index& a, const Index& b, Index& out, Workspace& ws,
                              hipStream_t s) {
+  if (b.n <= 8 * a.n) {
+    // dense: one merge-path pass over NEW and FULL writes the kept rows
+    out = Index();
+    out.arity = K;
+    out.col_max = a.col_max;
+    out.stride = round_stride(a.n);
+    out.cols.alloc(out.stride * K, s);
+    {
+      TimeScope t("setdiff.merge_path", s, double(a.n + b.n) * 4 * K);
+      merge_path<K, 0>(a, b, out.cols.p, out.stride, ws, s);
+    }
+    uint64_t* h64 = reinterpret_cast<uint64_t*>(pinned_words(ws, 4));
+    SRD_CHECK(hipMemcpyAsync(h64, ws.mp_total.p, 8, hipMemcpyDeviceToHost, s));
+    SRD_CHECK(hipMemcpyAsync(h64 + 1, ws.mp_ctr.p, 8, hipMemcpyDeviceToHost, s));
+    SRD_CHECK(hipStreamSynchronize(s));
+    if (uint32_t(h64[1] >> 32) != 0) fail("set_difference: tile look-back did not resolve");
+    out.n = h64[0];
+    if (out.n == 0) {
+      out = Index();
+      out.arity = K;
+      out.col_max = a.col_max;
+      return;
+    }
+    build_roots(out, ws, s);
+    return;
+  }
+  // sparse (FULL much larger than NEW): each tile of NEW probes only the
+  // window of FULL its key range covers
   const uint64_t tiles = (a.n + kTile - 1) / kTile;
   ws.flags.reserve(a.n, s);
   DevBuf<uint32_t> tc(tiles, s);
@@ -1458,21 +1697,8 @@ static void merge_k(const Index& a, const Index& b, Index& out, Workspace& ws, h
     out.col_max[c] = std::max(a.col_max.size() == K ? a.col_max[c] : 0u,
                               b.col_max.size() == K ? b.col_max[c] : 0u);
   {
-    TimeScope t("merge.scatter", s, double(out.n) * 8 * K);
-    if (a.n) {
-      uint64_t tiles = (a.n + kTile - 1) / kTile;
-      window_kernel<K, 1><<<uint32_t(tiles), kBlock, 0, s>>>(a.cols.p, a.stride, a.n, b.cols.p,
-                                                             b.stride, b.n, nullptr, nullptr,
-                                                             out.cols.p, out.stride);
-      SRD_CHECK(hipGetLastError());
-    }
-    if (b.n) {
-      uint64_t tiles = (b.n + kTile - 1) / kTile;
-      window_kernel<K, 1><<<uint32_t(tiles), kBlock, 0, s>>>(b.cols.p, b.stride, b.n, a.cols.p,
-                                                             a.stride, a.n, nullptr, nullptr,
-                                                             out.cols.p, out.stride);
-      SRD_CHECK(hipGetLastError());
-    }
+    TimeScope t("merge.merge_path", s, double(out.n) * 8 * K);
+    merge_path<K, 1>(a, b, out.cols.p, out.stride, ws, s);
   }
   build_roots(out, ws, s);
 }

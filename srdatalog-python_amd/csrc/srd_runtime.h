@@ -118,6 +118,11 @@ struct Workspace {
   DevBuf<uint8_t> flags;
   DevBuf<uint32_t> hostbuf_dev;  // staging for values read back
   uint32_t epoch = 0;            // lookback epoch (avoids per-pass clears)
+  DevBuf<uint64_t> mp_lb;        // merge-path set-difference inter-tile state
+  DevBuf<uint64_t> mp_split;     // merge-path tile split points
+  DevBuf<uint32_t> mp_ctr;       // [tile ticket, error flag]
+  DevBuf<uint64_t> mp_total;     // rows the set difference kept
+  uint32_t mp_epoch = 0;
   uint32_t* pinned = nullptr;    // pinned host words for read-backs
   uint64_t pinned_words = 0;
   ~Workspace();
