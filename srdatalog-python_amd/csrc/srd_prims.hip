@@ -675,6 +675,44 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
   return m;
 }
 
+// Decoupled look-back by one whole wave: the 64 lanes read the status words
+// of the 64 tiles before `tile` at once (lane 0 the nearest), wait until every
+// word up to the nearest inclusive prefix is published, and add the
+// aggregates up to and including that prefix; with no prefix in the window
+// they add all 64 aggregates and move 64 tiles further back.  Status of tile
+// t is at base[t * stride].  Returns the exclusive prefix of `tile`; raises
+// *err after kSpinLimit unsuccessful polls.  All 64 lanes must call it.
+__device__ __forceinline__ uint64_t wave_lookback(uint64_t* base, uint64_t stride, uint32_t tile,
+                                                  uint32_t epoch, uint32_t* err) {
+  const uint32_t l = lane_id();
+  uint64_t excl = 0;
+  int64_t hi = int64_t(tile) - 1;
+  uint32_t spins = 0;
+  while (hi >= 0) {
+    const int64_t idx = hi - int64_t(l);
+    const uint64_t v = idx >= 0 ? lb_load(base + uint64_t(idx) * stride)
+                                : lb_pack(epoch, kLbPrefix, 0);
+    const uint64_t flag = (v >> 46) & 3;
+    const bool ready = uint32_t(v >> 48) == epoch && flag != 0;
+    const uint64_t pm = ballot(ready && flag == kLbPrefix);
+    const uint64_t need = pm ? (((pm & (~pm + 1)) << 1) - 1) : ~0ull;
+    const uint64_t rm = ballot(ready);
+    if ((rm & need) != need) {
+      if (++spins > kSpinLimit) {
+        if (l == 0) atomicOr(err, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    const uint64_t mine = ((need >> l) & 1ull) ? (v & kLbMask) : 0;
+    excl += wave_sum<uint64_t>(mine);
+    if (pm) break;
+    hi -= 64;
+  }
+  return excl;
+}
+
 template <int W>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const uint64_t* __restrict__ in_lo, const uint64_t* __restrict__ in_hi,
@@ -1488,34 +1526,15 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
       for (int c = 0; c < K; ++c) st[c][off] = rows[it].v[c];
       ++off;
     }
-  if (tid == 0) {
-    uint64_t excl = 0;
+  if (tid < kWave) {  // wave 0 resolves the tile's output offset
     uint64_t* mine = lookback + tile;
-    if (tile == 0) {
-      lb_store(mine, lb_pack(epoch, kLbPrefix, tile_tot));
-    } else {
-      lb_store(mine, lb_pack(epoch, kLbAgg, tile_tot));
-      int64_t t = int64_t(tile) - 1;
-      uint32_t spins = 0;
-      while (t >= 0) {
-        const uint64_t v = lb_load(lookback + t);
-        const uint64_t flag = (v >> 46) & 3;
-        if (uint32_t(v >> 48) != epoch || flag == 0) {
-          if (++spins > kSpinLimit) {
-            atomicOr(&ctr[1], 1u);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        excl += v & kLbMask;
-        if (flag == kLbPrefix) break;
-        --t;
-      }
-      lb_store(mine, lb_pack(epoch, kLbPrefix, excl + tile_tot));
+    if (tid == 0) lb_store(mine, lb_pack(epoch, tile == 0 ? kLbPrefix : kLbAgg, tile_tot));
+    const uint64_t excl = tile == 0 ? 0 : wave_lookback(lookback, 1, tile, epoch, &ctr[1]);
+    if (tid == 0) {
+      if (tile != 0) lb_store(mine, lb_pack(epoch, kLbPrefix, excl + tile_tot));
+      s_prefix = excl;
+      if (d1 == na + nb) *total_out = excl + tile_tot;
     }
-    s_prefix = excl;
-    if (d1 == na + nb) *total_out = excl + tile_tot;
   }
   __syncthreads();
   const uint64_t base = s_prefix;
