@@ -517,6 +517,31 @@ struct PackSpec {
   uint32_t passes;
 };
 
+// Block-level reduction of per-thread column maxima and order flags, then one
+// atomic per column per block (every wave hitting the same words with
+// atomics would serialise them in L2).
+__device__ __forceinline__ void colmax_flush(const uint32_t* m, uint32_t k, bool unsorted,
+                                             bool dup, uint32_t* __restrict__ out_max) {
+  __shared__ uint32_t s_max[kWavesPerBlock][kMaxCols + 1];
+  const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+  for (uint32_t c = 0; c < k; ++c) {
+    const uint32_t v = wave_max(m[c]);
+    if (lane == 0) s_max[w][c] = v;
+  }
+  const uint32_t fl = (ballot(unsorted) ? 1u : 0u) | (ballot(dup) ? 2u : 0u);
+  if (lane == 0) s_max[w][kMaxCols] = fl;
+  __syncthreads();
+  if (threadIdx.x <= k) {
+    const uint32_t c = threadIdx.x == k ? uint32_t(kMaxCols) : threadIdx.x;
+    uint32_t v = 0;
+    for (int j = 0; j < kWavesPerBlock; ++j) v = c == kMaxCols ? (v | s_max[j][c]) : max(v, s_max[j][c]);
+    if (v) {
+      if (c == kMaxCols) atomicOr(&out_max[c], v);
+      else atomicMax(&out_max[c], v);
+    }
+  }
+}
+
 // Per-column maxima plus order flags in out_max[kMaxCols]: bit 0 = some row
 // is lexicographically smaller than its predecessor, bit 1 = some row equals
 // its predecessor.  0 means strictly increasing: already a valid index.
@@ -539,12 +564,7 @@ __global__ __launch_bounds__(kBlock) void col_max_kernel(PackSpec spec, uint64_t
     unsorted |= order < 0;
     dup |= (i > 0 && order == 0);
   }
-  for (uint32_t c = 0; c < spec.k; ++c) {
-    uint32_t v = wave_max(m[c]);
-    if (lane_id() == 0 && v) atomicMax(&out_max[c], v);
-  }
-  const uint32_t fl = (ballot(unsorted) ? 1u : 0u) | (ballot(dup) ? 2u : 0u);
-  if (fl && lane_id() == 0) atomicOr(&out_max[kMaxCols], fl);
+  colmax_flush(m, spec.k, unsorted, dup, out_max);
 }
 
 // col_max_kernel for 16-byte aligned columns: each lane reads rows
@@ -594,12 +614,7 @@ __global__ __launch_bounds__(kBlock) void col_max_vec_kernel(PackSpec spec, uint
     unsorted |= order < 0;
     dup |= i > 0 && order == 0;
   }
-  for (uint32_t c = 0; c < spec.k; ++c) {
-    const uint32_t v = wave_max(m[c]);
-    if (lane == 0 && v) atomicMax(&out_max[c], v);
-  }
-  const uint32_t fl = (ballot(unsorted) ? 1u : 0u) | (ballot(dup) ? 2u : 0u);
-  if (fl && lane == 0) atomicOr(&out_max[kMaxCols], fl);
+  colmax_flush(m, spec.k, unsorted, dup, out_max);
 }
 
 template <int W>
@@ -982,10 +997,10 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
       bool aligned = true;
       for (uint32_t c = 0; c < k; ++c) aligned &= (reinterpret_cast<uintptr_t>(in[c]) & 15u) == 0;
       if (aligned)
-        col_max_vec_kernel<<<grid_for(std::max<uint64_t>(n / 4, 1), kBlock * 4, 4096), kBlock, 0,
+        col_max_vec_kernel<<<grid_for(std::max<uint64_t>(n / 4, 1), kBlock * 4, 1024), kBlock, 0,
                              s>>>(ps, n, dmax);
       else
-        col_max_kernel<<<grid_for(n, kBlock * 16, 2048), kBlock, 0, s>>>(ps, n, dmax);
+        col_max_kernel<<<grid_for(n, kBlock * 16, 1024), kBlock, 0, s>>>(ps, n, dmax);
       SRD_CHECK(hipGetLastError());
     }
     hp = pinned_words(ws, kMaxCols + 1);
@@ -1431,13 +1446,17 @@ __device__ __forceinline__ RowK<K> lds_row(const uint32_t (*st)[kMpTile + 2], ui
 
 // MODE 1: out gets the merged rows at d0 + position.  MODE 0: out gets the
 // rows of A not in B, compacted; tile offsets come from a decoupled
-// look-back in ticket order, and the last tile stores the total.
+// look-back in ticket order, and the last tile stores the total.  MODE 2:
+// only count the kept rows of each tile (tile_cnt[tile]); MODE 3: write
+// them at the scanned offsets tile_off[tile] (the two-pass variant).
 template <int K, int MODE>
 __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
     const uint32_t* __restrict__ a, uint64_t sa, uint64_t na, const uint32_t* __restrict__ b,
     uint64_t sb, uint64_t nb, const uint64_t* __restrict__ split, uint32_t* __restrict__ out,
     uint64_t so, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctr, uint32_t epoch,
-    uint64_t* __restrict__ total_out) {
+    uint64_t* __restrict__ total_out, uint32_t* __restrict__ tile_cnt,
+    const uint64_t* __restrict__ tile_off) {
+  constexpr bool DIFF = MODE != 1;
   __shared__ uint32_t st[K][kMpTile + 2];
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wsum[kWavesPerBlock];
@@ -1456,7 +1475,7 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
   const uint64_t i0 = split[tile], i1 = split[tile + 1];
   const uint64_t j0 = d0 - i0, j1 = d1 - i1;
   const uint32_t an = uint32_t(i1 - i0), bn = uint32_t(j1 - j0);
-  const uint32_t pre = (MODE == 0 && j0 > 0) ? 1u : 0u;  // B[j0-1] staged at an
+  const uint32_t pre = (DIFF && j0 > 0) ? 1u : 0u;  // B[j0-1] staged at an
   const uint32_t bb = an + pre;                          // first staged B row of the tile
   for (uint32_t r = tid; r < an + pre + bn; r += kBlock) {
 #pragma unroll
@@ -1490,7 +1509,7 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
         bi >= bn || (ai < an && row_less<K>(lds_row<K>(st, ai), lds_row<K>(st, bb + bi)));
     if (take_a) {
       rows[it] = lds_row<K>(st, ai);
-      if (MODE == 0) {
+      if (DIFF) {
         // in B iff the last B row at or before this position equals it
         const bool has_prev = bi > 0 || pre;
         keep[it] = !(has_prev && row_eq<K>(lds_row<K>(st, bb + bi - 1), rows[it]));
@@ -1519,6 +1538,10 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
 
   uint32_t tile_tot;
   uint32_t off = block_excl_scan<uint32_t>(kept, s_wsum, &tile_tot);
+  if (MODE == 2) {
+    if (tid == 0) tile_cnt[tile] = tile_tot;
+    return;
+  }
 #pragma unroll
   for (int it = 0; it < kMpItems; ++it)
     if (keep[it]) {
@@ -1526,7 +1549,9 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
       for (int c = 0; c < K; ++c) st[c][off] = rows[it].v[c];
       ++off;
     }
-  if (tid < kWave) {  // wave 0 resolves the tile's output offset
+  if (MODE == 3) {
+    if (tid == 0) s_prefix = tile_off[tile];
+  } else if (tid < kWave) {  // wave 0 resolves the tile's output offset
     uint64_t* mine = lookback + tile;
     if (tid == 0) lb_store(mine, lb_pack(epoch, tile == 0 ? kLbPrefix : kLbAgg, tile_tot));
     const uint64_t excl = tile == 0 ? 0 : wave_lookback(lookback, 1, tile, epoch, &ctr[1]);
@@ -1569,9 +1594,36 @@ static uint64_t merge_path(const Index& a, const Index& b, uint32_t* out, uint64
   mp_tile_kernel<K, MODE><<<uint32_t(tiles), kBlock, 0, s>>>(
       a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, out, so,
       MODE == 0 ? ws.mp_lb.p : nullptr, MODE == 0 ? ws.mp_ctr.p : nullptr, ws.mp_epoch,
-      MODE == 0 ? ws.mp_total.p : nullptr);
+      MODE == 0 ? ws.mp_total.p : nullptr, nullptr, nullptr);
   SRD_CHECK(hipGetLastError());
   return tiles;
+}
+
+// Two-pass set difference: count the kept rows per tile, scan, write.  The
+// total goes to ws.mp_total like the single-pass variant's.
+template <int K>
+static void set_difference_two_pass(const Index& a, const Index& b, uint32_t* out, uint64_t so,
+                                    Workspace& ws, hipStream_t s) {
+  const uint64_t tiles = (a.n + b.n + kMpTile - 1) / kMpTile;
+  ws.mp_split.reserve(tiles + 1, s);
+  mp_partition_kernel<K><<<uint32_t((tiles + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, tiles, ws.mp_split.p);
+  SRD_CHECK(hipGetLastError());
+  DevBuf<uint32_t> cnt(tiles, s);
+  DevBuf<uint64_t> off(tiles + 1, s);
+  mp_tile_kernel<K, 2><<<uint32_t(tiles), kBlock, 0, s>>>(
+      a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, out, so, nullptr, nullptr,
+      0, nullptr, cnt.p, nullptr);
+  SRD_CHECK(hipGetLastError());
+  scan_impl<uint32_t>(cnt.p, off.p, tiles, s);
+  mp_tile_kernel<K, 3><<<uint32_t(tiles), kBlock, 0, s>>>(
+      a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, out, so, nullptr, nullptr,
+      0, nullptr, nullptr, off.p);
+  SRD_CHECK(hipGetLastError());
+  ws.mp_total.reserve(1, s);
+  ws.mp_ctr.reserve(2, s);
+  fill_u32(ws.mp_ctr.p, 0, 2, s);
+  SRD_CHECK(hipMemcpyAsync(ws.mp_total.p, off.p + tiles, 8, hipMemcpyDeviceToDevice, s));
 }
 
 template <int K>
@@ -1620,7 +1672,11 @@ static void set_difference_k(const Index& a, const Index& b, Index& out, Workspa
     out.cols.alloc(out.stride * K, s);
     {
       TimeScope t("setdiff.merge_path", s, double(a.n + b.n) * 4 * K);
-      merge_path<K, 0>(a, b, out.cols.p, out.stride, ws, s);
+      static const bool two_pass = std::getenv("SRDATALOG_SETDIFF_TWO_PASS") != nullptr;
+      if (two_pass)
+        set_difference_two_pass<K>(a, b, out.cols.p, out.stride, ws, s);
+      else
+        merge_path<K, 0>(a, b, out.cols.p, out.stride, ws, s);
     }
     uint64_t* h64 = reinterpret_cast<uint64_t*>(pinned_words(ws, 4));
     SRD_CHECK(hipMemcpyAsync(h64, ws.mp_total.p, 8, hipMemcpyDeviceToHost, s));
