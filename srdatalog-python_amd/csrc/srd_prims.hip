@@ -665,7 +665,10 @@ __global__ __launch_bounds__(kBlock) void pack_hist_kernel(PackSpec spec, uint64
 // 2 = inclusive prefix), [45:0] count.
 constexpr uint64_t kLbAgg = 1, kLbPrefix = 2;
 constexpr uint64_t kLbMask = (uint64_t(1) << 46) - 1;
-constexpr uint32_t kSpinLimit = 1u << 26;
+// Polls before a look-back gives up and raises the error flag.  Tiles wait
+// only on tiles with smaller block ids, which every XCD dispatches first, so
+// a wait is short; the limit bounds a broken invariant to seconds.
+constexpr uint32_t kSpinLimit = 1u << 22;
 
 __device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint64_t flag, uint64_t v) {
   return (uint64_t(epoch) << 48) | (flag << 46) | v;
@@ -740,14 +743,16 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   __shared__ uint32_t s_wcnt[kWavesPerBlock][kRadix];
   __shared__ uint32_t s_local[kRadix];
   __shared__ uint64_t s_global[kRadix];
-  __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wsum[kWavesPerBlock];
 
   const uint32_t tid = threadIdx.x, w = tid / kWave, l = lane_id();
-  if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  // tile order = block order: blocks are dispatched in id order, so a
+  // tile's predecessors are running or done (a ticket atomic per tile costs
+  // ~10 ns of same-address serialisation)
+  (void)tile_ctr;
   for (uint32_t i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&s_wcnt[0][0])[i] = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = blockIdx.x;
   const uint64_t base = uint64_t(tile) * TILE;
   const uint32_t count = uint32_t(min<uint64_t>(TILE, n - base));
 
@@ -1458,18 +1463,10 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
     const uint64_t* __restrict__ tile_off) {
   constexpr bool DIFF = MODE != 1;
   __shared__ uint32_t st[K][kMpTile + 2];
-  __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wsum[kWavesPerBlock];
   __shared__ uint64_t s_prefix;
   const uint32_t tid = threadIdx.x;
-  uint32_t tile;
-  if (MODE == 0) {
-    if (tid == 0) s_tile = atomicAdd(&ctr[0], 1u);
-    __syncthreads();
-    tile = s_tile;
-  } else {
-    tile = blockIdx.x;
-  }
+  const uint32_t tile = blockIdx.x;  // look-back order = block dispatch order
   const uint64_t d0 = uint64_t(tile) * kMpTile;
   const uint64_t d1 = min<uint64_t>(d0 + kMpTile, na + nb);
   const uint64_t i0 = split[tile], i1 = split[tile + 1];
