@@ -141,13 +141,68 @@ __device__ __forceinline__ uint32_t upper_bound(const uint32_t* __restrict__ a, 
   return lo;
 }
 
+// lower_bound / upper_bound that take two halving steps per memory round
+// trip: with the midpoint it also loads the midpoints of both halves, so the
+// next step's value is already there whichever way the comparison goes.
+// Latency-bound searches (the join's) then wait for half as many dependent
+// loads, at 1.5x the loads.
+template <bool UPPER>
+__device__ __forceinline__ uint32_t bound_la(const uint32_t* __restrict__ a, uint32_t lo,
+                                             uint32_t hi, uint32_t v) {
+  uint32_t n = hi - lo;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    const uint32_t mid = lo + half;
+    const uint32_t nl = half, nr = n - half - 1;
+    const uint32_t ml = lo + (nl >> 1), mr = mid + 1 + (nr >> 1);
+    const uint32_t x = a[mid];
+    const uint32_t xl = a[ml];                    // ml < mid: in range
+    const uint32_t xr = nr ? a[mr] : 0u;          // mr < hi when nr > 0
+    const bool right = UPPER ? x <= v : x < v;
+    if (right) {
+      lo = mid + 1;
+      n = nr;
+      if (n > 0) {
+        const uint32_t h2 = n >> 1;
+        if (UPPER ? xr <= v : xr < v) {
+          lo = mr + 1;
+          n -= h2 + 1;
+        } else {
+          n = h2;
+        }
+      }
+    } else {
+      n = nl;
+      const uint32_t h2 = n >> 1;  // n >= 1 here
+      if (UPPER ? xl <= v : xl < v) {
+        lo = ml + 1;
+        n -= h2 + 1;
+      } else {
+        n = h2;
+      }
+    }
+  }
+  if (n == 1 && (UPPER ? a[lo] <= v : a[lo] < v)) ++lo;
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t lower_bound_la(const uint32_t* __restrict__ a, uint32_t lo,
+                                                   uint32_t hi, uint32_t v) {
+  return bound_la<false>(a, lo, hi, v);
+}
+
+__device__ __forceinline__ uint32_t upper_bound_la(const uint32_t* __restrict__ a, uint32_t lo,
+                                                   uint32_t hi, uint32_t v) {
+  return bound_la<true>(a, lo, hi, v);
+}
+
 __device__ __forceinline__ const uint32_t* col(const IndexView& v, int c) {
   return v.cols + uint64_t(c) * v.stride;
 }
 
 // Rows whose column 0 equals key (empty range if absent), via root arrays.
 __device__ __forceinline__ Rng root_find(const IndexView& v, uint32_t key) {
-  uint32_t g = lower_bound(v.root_vals, 0, v.r, key);
+  uint32_t g = lower_bound_la(v.root_vals, 0, v.r, key);
   if (g < v.r && v.root_vals[g] == key) return Rng{v.root_offs[g], v.root_offs[g + 1]};
   return Rng{0, 0};
 }
@@ -157,10 +212,10 @@ __device__ __forceinline__ Rng root_find(const IndexView& v, uint32_t key) {
 __device__ __forceinline__ Rng child_find(const IndexView& v, int depth, Rng r, uint32_t key,
                                           bool last) {
   const uint32_t* c = col(v, depth);
-  uint32_t lo = lower_bound(c, r.b, r.e, key);
+  uint32_t lo = lower_bound_la(c, r.b, r.e, key);
   if (lo >= r.e || c[lo] != key) return Rng{0, 0};
   if (last) return Rng{lo, lo + 1};
-  return Rng{lo, upper_bound(c, lo + 1, r.e, key)};
+  return Rng{lo, upper_bound_la(c, lo + 1, r.e, key)};
 }
 
 // Lexicographic comparison of row i of a against row j of b over k columns.

@@ -48,7 +48,7 @@ __device__ __forceinline__ uint32_t find_slot(const uint64_t* __restrict__ start
 __device__ __forceinline__ Rng run_at(const uint32_t* __restrict__ col, uint32_t pos, uint32_t e,
                                       uint32_t v, bool last) {
   if (last) return Rng{pos, pos + 1};
-  return Rng{pos, srd::upper_bound(col, pos + 1, e, v)};
+  return Rng{pos, srd::upper_bound_la(col, pos + 1, e, v)};
 }
 
 // Owner lane of flattened item j given the wave's 65-entry exclusive

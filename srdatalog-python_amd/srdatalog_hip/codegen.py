@@ -352,7 +352,7 @@ class PipelineGen:
         if self.coop_search:
           code.append(f"{i2}const uint32_t lb_ = srdj::seg_lower_bound(col_, b_, e_, {var}, on_, {seg});")
         else:
-          code.append(f"{i2}const uint32_t lb_ = on_ ? srd::lower_bound(col_, b_, e_, {var}) : 0u;")
+          code.append(f"{i2}const uint32_t lb_ = on_ ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;")
         code.append(f"{i2}if (on_) {{")
         code.append(f"{i2}  const bool hit_ = lb_ < e_ && col_[lb_] == {var};")
         code.append(f"{i2}  {tag}on = hit_;")
@@ -363,7 +363,7 @@ class PipelineGen:
           code.append(f"{i2}  if (hit_) {{ {tag}c{m}b = lb_; {tag}c{m}e = lb_ + 1; }}")
         else:
           code.append(f"{i2}  if (hit_) {{ {tag}c{m}b = lb_; "
-                      f"{tag}c{m}e = srd::upper_bound(col_, lb_ + 1, e_, {var}); }}")
+                      f"{tag}c{m}e = srd::upper_bound_la(col_, lb_ + 1, e_, {var}); }}")
         code.append(f"{i2}}}")
         code.append(f"{ind}}}")
     code.append(f"{ind}{hit} = {tag}on;")
