@@ -141,59 +141,16 @@ __device__ __forceinline__ uint32_t upper_bound(const uint32_t* __restrict__ a, 
   return lo;
 }
 
-// lower_bound / upper_bound that take several halving steps per memory round
-// trip: with the midpoint they also load the midpoints of the halves (and of
-// the quarters), so the next steps' values are already there whichever way
-// the comparisons go.  Latency-bound searches (the join's) then wait for a
-// third as many dependent loads, at 7/3 the loads.
+// lower_bound / upper_bound that take two halving steps per memory round
+// trip: with the midpoint it also loads the midpoints of both halves, so the
+// next step's value is already there whichever way the comparison goes.
+// Latency-bound searches (the join's) then wait for half as many dependent
+// loads, at 1.5x the loads.  (Three steps per trip, 7 loads, measured slower
+// on the triangle join: the extra loads cost more than the saved trips.)
 template <bool UPPER>
 __device__ __forceinline__ uint32_t bound_la(const uint32_t* __restrict__ a, uint32_t lo,
                                              uint32_t hi, uint32_t v) {
   uint32_t n = hi - lo;
-  // three halving steps per round trip while every grandchild range is
-  // non-empty (n >= 7): the midpoint, both children's and all four
-  // grandchildren's midpoints are loaded together
-  while (n >= 7) {
-    const uint32_t h = n >> 1, m = lo + h;
-    const uint32_t nl = h, nr = n - h - 1;
-    const uint32_t ml = lo + (nl >> 1), mr = m + 1 + (nr >> 1);
-    const uint32_t nll = nl >> 1, nlr = nl - nll - 1, nrl = nr >> 1, nrr = nr - nrl - 1;
-    const uint32_t mll = lo + (nll >> 1), mlr = ml + 1 + (nlr >> 1);
-    const uint32_t mrl = m + 1 + (nrl >> 1), mrr = mr + 1 + (nrr >> 1);
-    const uint32_t x = a[m], xl = a[ml], xr = a[mr];
-    const uint32_t xll = a[mll], xlr = a[mlr], xrl = a[mrl], xrr = a[mrr];
-    uint32_t mn, xn;  // midpoint (and its value) of the range after two steps
-    if (UPPER ? x <= v : x < v) {
-      if (UPPER ? xr <= v : xr < v) {
-        lo = mr + 1;
-        n = nrr;
-        mn = mrr;
-        xn = xrr;
-      } else {
-        lo = m + 1;
-        n = nrl;
-        mn = mrl;
-        xn = xrl;
-      }
-    } else {
-      if (UPPER ? xl <= v : xl < v) {
-        lo = ml + 1;
-        n = nlr;
-        mn = mlr;
-        xn = xlr;
-      } else {
-        n = nll;
-        mn = mll;
-        xn = xll;
-      }
-    }
-    if (UPPER ? xn <= v : xn < v) {  // n >= 1 and mn == lo + n / 2
-      lo = mn + 1;
-      n -= (n >> 1) + 1;
-    } else {
-      n >>= 1;
-    }
-  }
   while (n > 1) {
     const uint32_t half = n >> 1;
     const uint32_t mid = lo + half;
