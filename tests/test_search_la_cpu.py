@@ -11,32 +11,6 @@ import numpy as np
 def bound_la(a, lo, hi, v, upper):
   less = (lambda x: x <= v) if upper else (lambda x: x < v)
   n = hi - lo
-  while n >= 7:
-    h = n >> 1
-    m = lo + h
-    nl, nr = h, n - h - 1
-    ml, mr = lo + (nl >> 1), m + 1 + (nr >> 1)
-    nll = nl >> 1
-    nlr = nl - nll - 1
-    nrl = nr >> 1
-    nrr = nr - nrl - 1
-    mll, mlr = lo + (nll >> 1), ml + 1 + (nlr >> 1)
-    mrl, mrr = m + 1 + (nrl >> 1), mr + 1 + (nrr >> 1)
-    if less(a[m]):
-      if less(a[mr]):
-        lo, n, mn = mr + 1, nrr, mrr
-      else:
-        lo, n, mn = m + 1, nrl, mrl
-    else:
-      if less(a[ml]):
-        lo, n, mn = ml + 1, nlr, mlr
-      else:
-        n, mn = nll, mll
-    assert n >= 1 and mn == lo + (n >> 1)
-    if less(a[mn]):
-      lo, n = mn + 1, n - (n >> 1) - 1
-    else:
-      n >>= 1
   while n > 1:
     half = n >> 1
     mid = lo + half
