@@ -682,6 +682,17 @@ __device__ __forceinline__ uint64_t lb_load(uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Lanes (among `valid`) whose digit equals this lane's digit.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+  uint64_t m = ballot(valid);
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    uint64_t bb = ballot((d >> b) & 1u);
+    m &= ((d >> b) & 1u) ? bb : ~bb;
+  }
+  return m;
+}
+
 // Decoupled look-back by one whole wave: the 64 lanes read the status words
 // of the 64 tiles before `tile` at once (lane 0 the nearest), wait until every
 // word up to the nearest inclusive prefix is published, and add the
@@ -730,8 +741,6 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   constexpr uint32_t TILE = kBlock * ITEMS;
   __shared__ uint64_t s_keys[W][TILE];
   __shared__ uint32_t s_wcnt[kWavesPerBlock][kRadix];
-  // per wave and digit, the lanes of the current item holding that digit
-  __shared__ unsigned long long s_peer[kWavesPerBlock][kRadix];
   __shared__ uint32_t s_local[kRadix];
   __shared__ uint64_t s_global[kRadix];
   __shared__ uint32_t s_wsum[kWavesPerBlock];
@@ -741,10 +750,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   // tile's predecessors are running or done (a ticket atomic per tile costs
   // ~10 ns of same-address serialisation)
   (void)tile_ctr;
-  for (uint32_t i = tid; i < kWavesPerBlock * kRadix; i += kBlock) {
-    (&s_wcnt[0][0])[i] = 0;
-    (&s_peer[0][0])[i] = 0;
-  }
+  for (uint32_t i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&s_wcnt[0][0])[i] = 0;
   __syncthreads();
   const uint32_t tile = blockIdx.x;
   const uint64_t base = uint64_t(tile) * TILE;
@@ -760,22 +766,10 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     khi[i] = (W == 2 && valid) ? in_hi[base + p] : 0;
     uint32_t d = valid ? digit_of<W>(klo[i], khi[i], shift) : 0;
     dig[i] = d;
-    // lanes with this lane's digit: every valid lane ORs its bit into the
-    // digit's mask (order-independent), reads the mask back, and the lowest
-    // lane clears it for the next item; LDS instructions of one wave execute
-    // in order, so the read sees all the ORs and precedes the clear
-    if (valid) atomicOr(&s_peer[w][d], 1ull << l);
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t peers = valid ? s_peer[w][d] : 0ull;
-    __builtin_amdgcn_wave_barrier();
-    const uint32_t below = popc_below(peers);
+    uint64_t peers = match_digit(d, valid);
     uint32_t before = valid ? s_wcnt[w][d] : 0;
-    rank[i] = before + below;
-    if (valid && below == 0) {
-      s_wcnt[w][d] = before + __popcll(peers);
-      s_peer[w][d] = 0ull;
-    }
-    __builtin_amdgcn_wave_barrier();
+    rank[i] = before + popc_below(peers);
+    if (valid && popc_below(peers) == 0) s_wcnt[w][d] = before + __popcll(peers);
   }
   __syncthreads();
 
