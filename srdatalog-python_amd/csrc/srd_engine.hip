@@ -214,6 +214,10 @@ struct Relation {
   // the younger ones (pairwise disjoint, sizes roughly halving)
   std::vector<Index> segs;
   bool segmented = false;
+  // NEW rows of pipelines too large to materialise at once, written in
+  // slice chunks, each sorted and deduplicated (all in new_parts_order)
+  std::vector<Index> new_parts;
+  Order new_parts_order;
   // FULL in relation column order for export, valid while export_version
   // equals full_version (the size query and the copy share one re-sort)
   Index export_idx;
@@ -249,6 +253,7 @@ struct srd_engine {
   bool no_direct = false;          // SRDATALOG_JOIN_MODE=fused: never the direct mode
   uint64_t stage_cap_env = 0;      // SRDATALOG_STAGE_CAP: fixed staging rows (tests overflow)
   uint64_t stage_max_bytes = uint64_t(16) << 30;  // SRDATALOG_STAGE_MAX_BYTES
+  uint64_t new_chunk_rows = 0;     // SRDATALOG_NEW_CHUNK_ROWS: fixed NEW chunk (tests)
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
   // instance table of a packed batch: n_inst+1 ascending column-0 bounds
@@ -506,11 +511,35 @@ static const Index& delta_index(srd_engine& e, Relation& r, const Order& order) 
   return slot;
 }
 
+static void union_into(srd_engine& e, Index& a, const Index& b);
+
 static const Index& new_index(srd_engine& e, Relation& r, const Order& order) {
   auto it = r.newv.find(order);
   if (it != r.newv.end()) return it->second;
   Index x;
-  if (r.raw_n > 0) {
+  if (!r.new_parts.empty()) {
+    // chunked pipelines' parts, united with the rows other pipelines wrote
+    const Order po = r.new_parts_order;
+    Index acc;
+    if (r.raw_n > 0) {
+      Order p = perm_between(r.raw_order, po);
+      std::vector<const uint32_t*> in(r.arity);
+      for (uint32_t c = 0; c < r.arity; ++c) in[c] = r.raw.p + uint64_t(p[c]) * r.raw_cap;
+      build_index(in.data(), r.arity, r.raw_n, acc, e.ws, e.s);
+    } else {
+      acc = empty_index(r.arity);
+    }
+    for (auto& part : r.new_parts) union_into(e, acc, part);
+    r.new_parts.clear();
+    r.raw.release();
+    r.raw_n = r.raw_cap = 0;
+    if (order == po) {
+      x = std::move(acc);
+    } else {
+      r.newv[po] = acc;
+      build_index_from(acc, perm_between(po, order), x, e.ws, e.s);
+    }
+  } else if (r.raw_n > 0) {
     Order p = perm_between(r.raw_order, order);
     std::vector<const uint32_t*> in(r.arity);
     for (uint32_t c = 0; c < r.arity; ++c) in[c] = r.raw.p + uint64_t(p[c]) * r.raw_cap;
@@ -552,6 +581,35 @@ static void clear_new(Relation& r) {
   r.newv.clear();
   r.raw.release();
   r.raw_n = r.raw_cap = 0;
+  r.new_parts.clear();
+}
+
+// a := a u b for sorted duplicate-free indexes of one order.
+static void union_into(srd_engine& e, Index& a, const Index& b) {
+  if (b.n == 0) return;
+  if (a.n == 0) {
+    copy_index(b, a, e.s);
+    return;
+  }
+  Index d, m;
+  set_difference(b, a, d, e.ws, e.s);
+  if (d.n == 0) return;
+  merge_disjoint(a, d, m, e.ws, e.s);
+  a = std::move(m);
+}
+
+// Add a deduplicated chunk of NEW rows; the two youngest parts are united
+// while within a factor 2 of each other, so few parts are kept.
+static void add_new_part(srd_engine& e, Relation& r, const Order& order, Index&& part) {
+  if (r.new_parts.empty()) r.new_parts_order = order;
+  if (order != r.new_parts_order) fail("relation " + r.name + ": NEW chunks in different orders");
+  r.new_parts.push_back(std::move(part));
+  while (r.new_parts.size() >= 2) {
+    Index& older = r.new_parts[r.new_parts.size() - 2];
+    if (older.n > 2 * r.new_parts.back().n) break;
+    union_into(e, older, r.new_parts.back());
+    r.new_parts.pop_back();
+  }
 }
 
 // ============================================================ pipelines ====
@@ -679,19 +737,80 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         SRD_CHECK(hipGetLastError());
         timing_end(e.s);
       };
+      // Rows of the write (direct) pass, materialised at once when they fit
+      // one chunk, else in slice ranges whose rows are each sorted and
+      // deduplicated into a NEW part (a join's output before deduplication
+      // can be far larger than HBM).
+      uint32_t amax = 1;
+      for (uint32_t k = 0; k < d->n_dest; ++k) amax = std::max(amax, d->dest[k].arity);
+      const uint64_t chunk_rows =
+          e.new_chunk_rows ? e.new_chunk_rows
+                           : std::min<uint64_t>(e.stage_max_bytes / (4ull * amax), uint64_t(1) << 31);
+      auto write_rows = [&](SrdLaunchFn fn, const char* phase, bool by_items) {
+        if (total <= chunk_rows) {
+          bind_dests(total);
+          launch(fn, phase);
+          return;
+        }
+        std::vector<uint64_t> off;
+        if (!by_items) {
+          off.resize(slices + 1);
+          SRD_CHECK(hipMemcpyAsync(off.data(), a.offsets, (slices + 1) * 8, hipMemcpyDeviceToHost, e.s));
+          SRD_CHECK(hipStreamSynchronize(e.s));
+        }
+        auto start_of = [&](uint64_t sl) {
+          return by_items ? std::min<uint64_t>(sl * a.grain, a.total_work) : off[sl];
+        };
+        std::vector<DevBuf<uint32_t>> bufs(d->n_dest);
+        uint64_t s0 = 0;
+        while (s0 < slices) {
+          uint64_t s1 = s0 + 1;  // at least one slice, even when it alone is larger
+          while (s1 < slices && start_of(s1 + 1) - start_of(s0) <= chunk_rows) ++s1;
+          const uint64_t rows = start_of(s1) - start_of(s0);
+          if (rows > 0) {
+            const uint64_t stride = round_stride(rows);
+            for (uint32_t k = 0; k < d->n_dest; ++k) {
+              bufs[k].reserve(stride * d->dest[k].arity, e.s);
+              a.out[k].cols = bufs[k].p;
+              a.out[k].stride = stride;
+              a.out[k].base = uint64_t(0) - start_of(s0);  // row = position - chunk start
+            }
+            fill_u32(a.work_ctr, uint32_t(s0), 1, e.s);
+            fill_u32(a.work_ctr + 1, 0, 1, e.s);
+            a.n_slices = s1;
+            std::string nm = std::string("join.") + phase + ":" + d->rule;
+            timing_begin(nm.c_str(), e.s);
+            fn(&a, e.s);
+            SRD_CHECK(hipGetLastError());
+            timing_end(e.s);
+            for (uint32_t k = 0; k < d->n_dest; ++k) {
+              const SrdDestSpec& ds = d->dest[k];
+              Relation& r = e.rels[rel_id(e, ds.rel)];
+              std::vector<const uint32_t*> in(ds.arity);
+              for (uint32_t c = 0; c < ds.arity; ++c) in[c] = bufs[k].p + uint64_t(c) * stride;
+              Index part;
+              build_index(in.data(), ds.arity, rows, part, e.ws, e.s);
+              add_new_part(e, r, Order(ds.cols, ds.cols + ds.arity), std::move(part));
+            }
+          }
+          s0 = s1;
+        }
+        a.n_slices = slices;
+        if (e.trace) {
+          std::fprintf(stderr, "[srd]   %s: %llu rows written in chunks of <= %llu\n", d->rule,
+                       (unsigned long long)total, (unsigned long long)chunk_rows);
+          std::fflush(stderr);
+        }
+      };
       if (d->direct_ok && !d->count_only && !e.two_pass && !e.no_direct) {
         // one tuple per item: rows land at base + item, nothing to count
         total = a.total_work;
-        bind_dests(total);
-        launch(d->direct, "direct");
+        write_rows(d->direct, "direct", true);
       } else if (d->count_only || e.two_pass || !stage_ready(e, d, slices)) {
         launch(d->count, "count");
         exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
         total = read_u64(e, a.offsets + slices);
-        if (total > 0 && !d->count_only) {
-          bind_dests(total);
-          launch(d->write, "write");
-        }
+        if (total > 0 && !d->count_only) write_rows(d->write, "write", false);
       } else {
         // fused: one traversal into staging segments, then gather.  Every
         // slice may hold one partly filled segment, so the capacity wanted is
@@ -725,15 +844,15 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         e.stage_hint[d->rule] = std::max<uint64_t>(e.stage_hint[d->rule],
                                                    total + total / 4 + slices * SRD_SEG);
         if (total > 0) {
-          bind_dests(total);
           if (!ovf) {
+            bind_dests(total);
             GatherDest gd[SRD_MAX_DESTS];
             for (uint32_t k = 0; k < d->n_dest; ++k)
               gd[k] = GatherDest{a.out[k].cols, a.out[k].stride, a.out[k].base, d->dest[k].arity};
             gather_segments(a.recs, a.rec_ctr, rec_cap, a.stage, cap, a.offsets, gd, d->n_dest,
                             total, e.s);
           } else {
-            launch(d->write, "write");
+            write_rows(d->write, "write", false);
           }
         }
         if (e.trace) {
@@ -1120,6 +1239,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     e->quiet = q && *q && *q != '0';
     if (const char* g = std::getenv("SRDATALOG_GRAIN")) e->grain_override = uint32_t(std::atoi(g));
     if (const char* c = std::getenv("SRDATALOG_STAGE_CAP")) e->stage_cap_env = std::strtoull(c, nullptr, 10);
+    if (const char* c = std::getenv("SRDATALOG_NEW_CHUNK_ROWS"))
+      e->new_chunk_rows = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_STAGE_MAX_BYTES"))
       e->stage_max_bytes = std::max<uint64_t>(std::strtoull(c, nullptr, 10), uint64_t(1) << 20);
     if (const char* m = std::getenv("SRDATALOG_JOIN_MODE")) {

@@ -74,7 +74,8 @@ void scan_root(const SrdLaunchArgs* a, hipStream_t) {
 template <bool SWAP, int MODE>
 void scan_run(const SrdLaunchArgs* a, hipStream_t) {
   const srd::IndexView& v = a->views.v[0];
-  for (uint64_t s = 0; s < a->n_slices; ++s) {
+  // slices [*work_ctr, n_slices), as the device kernels take them
+  for (uint64_t s = a->work_ctr[0]; s < a->n_slices; ++s) {
     uint64_t lo = s * a->grain, hi = std::min<uint64_t>(lo + a->grain, a->total_work);
     Emit em{a, MODE};
     em.slice = s;
@@ -106,7 +107,8 @@ void rec_root(const SrdLaunchArgs* a, hipStream_t) {
 
 template <int MODE>
 void rec_run(const SrdLaunchArgs* a, hipStream_t) {
-  for (uint64_t s = 0; s < a->n_slices; ++s) {
+  // slices [*work_ctr, n_slices), as the device kernels take them
+  for (uint64_t s = a->work_ctr[0]; s < a->n_slices; ++s) {
     uint64_t lo = s * a->grain, hi = std::min<uint64_t>(lo + a->grain, a->total_work);
     Emit em{a, MODE};
     em.slice = s;

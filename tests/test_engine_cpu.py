@@ -121,3 +121,21 @@ def test_engine_segmented_full_long_tail(eng, n, extra, seed, segments, monkeypa
   np.testing.assert_array_equal(path, want.relations["Path"])
   assert deltas == want.deltas["Path"]
   assert size == want.relations["Path"].shape[0]
+
+
+@pytest.mark.parametrize("chunk", [1, 37, 500])
+@pytest.mark.parametrize("mode", ["direct", "twopass"])
+def test_engine_chunked_new(eng, chunk, mode, monkeypatch):
+  """Join outputs written in slice chunks, each deduplicated into a NEW
+  part, give the same fixpoint and deltas as materialising them at once."""
+  monkeypatch.setenv("SRDATALOG_NEW_CHUNK_ROWS", str(chunk))
+  monkeypatch.setenv("SRDATALOG_GRAIN", "16")
+  if mode == "twopass":
+    monkeypatch.setenv("SRDATALOG_JOIN_MODE", "twopass")
+  else:
+    monkeypatch.delenv("SRDATALOG_JOIN_MODE", raising=False)
+  e = ds.tc_edges(4, 120, 400)
+  want = oracle.run("tc", {"ArcInput": e})
+  path, deltas, size = run_tc(eng, e)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]

@@ -203,17 +203,22 @@ def test_csv_load_all(progs, tmp_path):
 # forced here (SRDATALOG_JOIN_MODE is read when the engine is created) and
 # must give the same relation and deltas as the oracle.  SRDATALOG_STAGE_CAP
 # shrinks the fused staging buffer so the overflow -> write-pass fallback runs;
-# SRDATALOG_NO_SEGMENTS keeps every FULL as one index (no segmented FULL).
+# SRDATALOG_NO_SEGMENTS keeps every FULL as one index (no segmented FULL);
+# SRDATALOG_NEW_CHUNK_ROWS writes join outputs in small deduplicated chunks.
 MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_JOIN_MODE": "fused"}, "fused"),
          ({"SRDATALOG_JOIN_MODE": "fused", "SRDATALOG_STAGE_CAP": "3000"}, "fused-overflow"),
          ({"SRDATALOG_NO_SEGMENTS": "1"}, "one-segment-full"),
+         ({"SRDATALOG_NEW_CHUNK_ROWS": "20000", "SRDATALOG_GRAIN": "256"}, "chunked-new"),
+         ({"SRDATALOG_NEW_CHUNK_ROWS": "20000", "SRDATALOG_JOIN_MODE": "twopass"},
+          "chunked-new-twopass"),
          ({}, "default")]
 
 
 @pytest.mark.parametrize("env,label", MODES, ids=[m[1] for m in MODES])
 def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
-  for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NO_SEGMENTS"):
+  for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NO_SEGMENTS",
+            "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN"):
     monkeypatch.delenv(k, raising=False)
   for k, v in env.items():
     monkeypatch.setenv(k, v)
