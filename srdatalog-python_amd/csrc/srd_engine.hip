@@ -254,6 +254,11 @@ struct srd_engine {
   uint64_t stage_cap_env = 0;      // SRDATALOG_STAGE_CAP: fixed staging rows (tests overflow)
   uint64_t stage_max_bytes = uint64_t(16) << 30;  // SRDATALOG_STAGE_MAX_BYTES
   uint64_t new_chunk_rows = 0;     // SRDATALOG_NEW_CHUNK_ROWS: fixed NEW chunk (tests)
+  // cost-balanced slicing of rules with >= 2 nested levels
+  bool no_weigh = false;           // SRDATALOG_NO_WEIGH=1: uniform slices only
+  uint64_t weigh_min = 4096;       // SRDATALOG_WEIGH_MIN: fewer items: uniform slices
+  uint64_t weigh_target = 16384;   // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
+  srd::DevBuf<uint64_t> item_cost, item_pref, slice_start;
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
   // instance table of a packed batch: n_inst+1 ascending column-0 bounds
@@ -612,6 +617,20 @@ static void add_new_part(srd_engine& e, Relation& r, const Order& order, Index&&
   }
 }
 
+// Turn the relation's raw NEW rows into a deduplicated NEW part, so that
+// raw rows never exceed one chunk (several pipelines may write one NEW).
+static void flush_raw(srd_engine& e, Relation& r) {
+  if (r.raw_n == 0) return;
+  std::vector<const uint32_t*> in(r.arity);
+  for (uint32_t c = 0; c < r.arity; ++c) in[c] = r.raw.p + uint64_t(c) * r.raw_cap;
+  Index part;
+  build_index(in.data(), r.arity, r.raw_n, part, e.ws, e.s);
+  const Order order = r.raw_order;
+  r.raw.release();
+  r.raw_n = r.raw_cap = 0;
+  add_new_part(e, r, order, std::move(part));
+}
+
 // ============================================================ pipelines ====
 static uint64_t read_u64(srd_engine& e, const uint64_t* dev) {
   uint64_t* h = reinterpret_cast<uint64_t*>(pinned_words(e.ws, 2));
@@ -651,6 +670,42 @@ static bool stage_ready(srd_engine& e, const SrdPipelineDesc* d, uint64_t slices
     return false;
   }
   return true;
+}
+
+// Replace the uniform slices (grain items each) of a rule with >= 2 nested
+// levels by slices of about equal cost: the weigh launch writes each item's
+// level-2 candidate count + SRD_ITEM_COST, a scan prefixes the costs, and
+// slice s is cut at the item where the prefix crosses s/S of the total.  An
+// item is never split, so one item's candidates bound a slice's cost.
+static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs& a) {
+  const uint64_t n = a.total_work;
+  e.item_cost.reserve(n, e.s);
+  e.item_pref.reserve(n + 1, e.s);
+  a.item_cost = e.item_cost.p;
+  a.grid_blocks = uint32_t(std::min<uint64_t>((a.n_slices + 3) / 4, 2048));
+  fill_u32(a.work_ctr, 0, 2, e.s);
+  std::string nm = std::string("join.weigh:") + d->rule;
+  timing_begin(nm.c_str(), e.s);
+  d->weigh(&a, e.s);
+  SRD_CHECK(hipGetLastError());
+  timing_end(e.s);
+  exclusive_scan_u64(a.item_cost, e.item_pref.p, n, e.ws, e.s);
+  const uint64_t w = read_u64(e, e.item_pref.p + n);
+  // about 24 slices per persistent wave of a full grid (4 x 2048), never
+  // fewer than the uniform slicing had, never more than one per item
+  const uint64_t cap = std::max<uint64_t>(a.n_slices, 24ull * 4 * 2048);
+  uint64_t S = std::min<uint64_t>(std::max<uint64_t>(a.n_slices, w / e.weigh_target), cap);
+  S = std::max<uint64_t>(std::min<uint64_t>(S, n), 1);
+  e.slice_start.reserve(S + 1, e.s);
+  slice_bounds(e.item_pref.p, n, w, S, e.slice_start.p, e.s);
+  a.slice_start = e.slice_start.p;
+  a.n_slices = S;
+  a.item_cost = nullptr;
+  if (e.trace) {
+    std::fprintf(stderr, "[srd]   %s: weighed %llu items, cost %llu, %llu slices\n", d->rule,
+                 (unsigned long long)n, (unsigned long long)w, (unsigned long long)S);
+    std::fflush(stderr);
+  }
 }
 
 static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t iteration) {
@@ -706,6 +761,10 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
     exclusive_scan_u64(a.key_work, a.work_start, nk, e.ws, e.s);
     a.total_work = read_u64(e, a.work_start + nk);
     a.n_slices = (a.total_work + a.grain - 1) / a.grain;
+    const bool direct = d->direct_ok && !d->count_only && !e.two_pass && !e.no_direct;
+    if (!direct && d->weigh && !e.no_weigh && a.total_work >= e.weigh_min &&
+        a.total_work <= (uint64_t(1) << 28))
+      weigh_slices(e, d, a);
     if (e.trace) {
       std::fprintf(stderr, "[srd]   %s: keys %u drv %u work %llu slices %llu\n", d->rule, a.n_root,
                    a.drv_view, (unsigned long long)a.total_work, (unsigned long long)a.n_slices);
@@ -718,10 +777,20 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
       a.counts = e.counts_buf.p;
       a.offsets = e.offsets.p;
       a.grid_blocks = uint32_t(std::min<uint64_t>((slices + 3) / 4, 2048));
+      // Rows of the write (direct) pass, materialised at once when they fit
+      // one chunk, else in slice ranges whose rows are each sorted and
+      // deduplicated into a NEW part (a join's output before deduplication
+      // can be far larger than HBM).
+      uint32_t amax = 1;
+      for (uint32_t k = 0; k < d->n_dest; ++k) amax = std::max(amax, d->dest[k].arity);
+      const uint64_t chunk_rows =
+          e.new_chunk_rows ? e.new_chunk_rows
+                           : std::min<uint64_t>(e.stage_max_bytes / (4ull * amax), uint64_t(1) << 31);
       auto bind_dests = [&](uint64_t n_rows) {
         for (uint32_t k = 0; k < d->n_dest; ++k) {
           const SrdDestSpec& ds = d->dest[k];
           Relation& r = e.rels[rel_id(e, ds.rel)];
+          if (r.raw_n > 0 && r.raw_n + n_rows > chunk_rows) flush_raw(e, r);
           Order order(ds.cols, ds.cols + ds.arity);
           uint64_t base = reserve_new(e, r, order, n_rows);
           a.out[k].cols = r.raw.p;
@@ -737,15 +806,6 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         SRD_CHECK(hipGetLastError());
         timing_end(e.s);
       };
-      // Rows of the write (direct) pass, materialised at once when they fit
-      // one chunk, else in slice ranges whose rows are each sorted and
-      // deduplicated into a NEW part (a join's output before deduplication
-      // can be far larger than HBM).
-      uint32_t amax = 1;
-      for (uint32_t k = 0; k < d->n_dest; ++k) amax = std::max(amax, d->dest[k].arity);
-      const uint64_t chunk_rows =
-          e.new_chunk_rows ? e.new_chunk_rows
-                           : std::min<uint64_t>(e.stage_max_bytes / (4ull * amax), uint64_t(1) << 31);
       auto write_rows = [&](SrdLaunchFn fn, const char* phase, bool by_items) {
         if (total <= chunk_rows) {
           bind_dests(total);
@@ -792,6 +852,11 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
               build_index(in.data(), ds.arity, rows, part, e.ws, e.s);
               add_new_part(e, r, Order(ds.cols, ds.cols + ds.arity), std::move(part));
             }
+            if (e.trace) {
+              std::fprintf(stderr, "[srd]   %s: chunk of slices [%llu, %llu): %llu rows\n", d->rule,
+                           (unsigned long long)s0, (unsigned long long)s1, (unsigned long long)rows);
+              std::fflush(stderr);
+            }
           }
           s0 = s1;
         }
@@ -802,7 +867,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
           std::fflush(stderr);
         }
       };
-      if (d->direct_ok && !d->count_only && !e.two_pass && !e.no_direct) {
+      if (direct) {
         // one tuple per item: rows land at base + item, nothing to count
         total = a.total_work;
         write_rows(d->direct, "direct", true);
@@ -1241,6 +1306,10 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     if (const char* c = std::getenv("SRDATALOG_STAGE_CAP")) e->stage_cap_env = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_NEW_CHUNK_ROWS"))
       e->new_chunk_rows = std::strtoull(c, nullptr, 10);
+    if (const char* c = std::getenv("SRDATALOG_NO_WEIGH")) e->no_weigh = *c && *c != '0';
+    if (const char* c = std::getenv("SRDATALOG_WEIGH_MIN")) e->weigh_min = std::strtoull(c, nullptr, 10);
+    if (const char* c = std::getenv("SRDATALOG_WEIGH_TARGET"))
+      e->weigh_target = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);
     if (const char* c = std::getenv("SRDATALOG_STAGE_MAX_BYTES"))
       e->stage_max_bytes = std::max<uint64_t>(std::strtoull(c, nullptr, 10), uint64_t(1) << 20);
     if (const char* m = std::getenv("SRDATALOG_JOIN_MODE")) {

@@ -19,7 +19,12 @@
 //               segment to its slice's offset in NEW;
 //   count  (0)  counts only (count-only rules; fallback when staging
 //               overflows);
-//   write  (1)  re-runs the traversal writing at the scanned offsets.
+//   write  (1)  re-runs the traversal writing at the scanned offsets;
+//   weigh  (4)  rules with two or more nested levels: writes each level-1
+//               item's cost (its level-2 candidates + SRD_ITEM_COST) so the
+//               runtime can cut slices of equal cost instead of equal item
+//               counts (slice_start); a few items of two hub keys can carry
+//               more level-2 work than whole slices of ordinary ones.
 // Output order depends only on the item order, never on which wave ran a
 // slice, so every mode yields the same NEW rows in the same order.
 //
@@ -88,7 +93,13 @@ struct SrdLaunchArgs {
   uint64_t rec_cap;
   uint32_t* rec_ctr;
   uint32_t* overflow;          // set when staging or the record log ran out
+  // cost-balanced slicing: slice s covers items [slice_start[s], slice_start[s+1])
+  // (nullptr: [s*grain, min((s+1)*grain, total_work)))
+  const uint64_t* slice_start;
+  uint64_t* item_cost;         // weigh mode output, one entry per item
 };
+
+#define SRD_ITEM_COST 4u       // level-1 cost of an item, in level-2 candidates
 
 #define SRD_SEG 1024u          // staging segment size (tuples)
 
@@ -111,6 +122,7 @@ struct SrdPipelineDesc {
   SrdLaunchFn write;
   SrdLaunchFn fused;
   SrdLaunchFn direct;
+  SrdLaunchFn weigh;                 // nullptr unless the rule has >= 2 nested levels
 };
 
 #ifdef __cplusplus

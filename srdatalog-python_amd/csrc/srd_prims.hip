@@ -370,6 +370,33 @@ __global__ __launch_bounds__(kBlock) void instance_count_kernel(const uint32_t* 
   out[g] = hi - lo;
 }
 
+__global__ __launch_bounds__(kBlock) void slice_bounds_kernel(const uint64_t* __restrict__ pref,
+                                                              uint64_t n, uint64_t q, uint64_t rem,
+                                                              uint64_t S, uint64_t* __restrict__ out) {
+  const uint64_t s = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (s > S) return;
+  const uint64_t target = s * q + (s < rem ? s : rem);  // s*w/S split without overflow
+  uint64_t lo = 0, len = n + 1;                          // first i in [0, n] with pref[i] >= target
+  while (len > 0) {
+    const uint64_t half = len >> 1;
+    if (pref[lo + half] < target) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  out[s] = lo;
+}
+
+void slice_bounds(const uint64_t* pref, uint64_t n, uint64_t w, uint64_t S, uint64_t* out,
+                  hipStream_t s) {
+  if (S == 0) fail("slice_bounds: no slices");
+  slice_bounds_kernel<<<uint32_t((S + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(pref, n, w / S,
+                                                                               w % S, S, out);
+  SRD_CHECK(hipGetLastError());
+}
+
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t s) {
   if (n_inst == 0) return;

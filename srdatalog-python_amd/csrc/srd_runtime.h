@@ -183,9 +183,6 @@ void gather_segments(const uint64_t* recs, const uint32_t* rec_ctr, uint64_t rec
 // Streaming copy of `bytes` (reps launches) for bandwidth / counter calibration.
 void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep);
 
-// Synthetic columns (k fields of one 64-bit key per row, low k-1 fields
-// fbits wide): key = mul*i + off when range == 0 (ascending, distinct for
-// mul > 0), else a hash of i+off reduced mod range.
 // Copy index x to host memory as row-major rows (n x arity u32).  With an
 // instance table (dev_bounds, n_bounds ascending column-0 range starts) every
 // value of a row is rebased by its instance's start.
@@ -197,6 +194,16 @@ void export_rows(const Index& x, uint32_t* host_out, const uint32_t* dev_bounds,
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t s);
 
+// Cost-balanced slice bounds: pref holds n+1 ascending item-cost prefix sums
+// (pref[n] = w, every item costing at least 1); out[s] (s = 0..S) = first
+// item i with pref[i] >= s*w/S, so out[0] = 0, out[S] = n and slice s =
+// [out[s], out[s+1]) carries about w/S of the cost.
+void slice_bounds(const uint64_t* pref, uint64_t n, uint64_t w, uint64_t S, uint64_t* out,
+                  hipStream_t s);
+
+// Synthetic columns (k fields of one 64-bit key per row, low k-1 fields
+// fbits wide): key = mul*i + off when range == 0 (ascending, distinct for
+// mul > 0), else a hash of i+off reduced mod range.
 void fill_pattern(uint32_t* cols, uint64_t stride, uint64_t n, uint32_t k, uint32_t fbits,
                   uint64_t mul, uint64_t off, uint64_t range, hipStream_t s);
 

@@ -409,8 +409,9 @@ class PipelineGen:
     o += ["  for (;;) {",
           "    const uint64_t slice = srdj::next_slice(A.work_ctr);  // wave-uniform",
           "    if (slice >= A.n_slices) break;",
-          "    const uint64_t it0 = slice * A.grain;",
-          "    const uint64_t it1 = min(it0 + A.grain, A.total_work);",
+          "    const uint64_t it0 = A.slice_start ? A.slice_start[slice] : slice * A.grain;",
+          "    const uint64_t it1 = A.slice_start ? A.slice_start[slice + 1]"
+          " : min(it0 + A.grain, A.total_work);",
           "    uint64_t opos = MODE == 1 ? A.offsets[slice] : 0;",
           "    uint32_t cnt = 0;",
           "    uint64_t seg_start = ~uint64_t(0);",
@@ -550,6 +551,11 @@ class PipelineGen:
       for r in rngs:
         o.append(f"{ind}  sz{t} *= uint64_t({self.size_expr(r, True)});")
     o.append(f"{ind}}}")
+    if li == 1:
+      o.append(f"{ind}if constexpr (MODE == 4) {{  // weigh: the item's level-2 candidates")
+      o.append(f"{ind}  if (act) A.item_cost[item] = sz{t} + SRD_ITEM_COST;")
+      o.append(f"{ind}  continue;")
+      o.append(f"{ind}}}")
     o.append(f"{ind}const uint64_t inc{t} = srd::wave_incl_scan<uint64_t>(sz{t});")
     o.append(f"{ind}const uint64_t tot{t} = srdj::uniform_from(inc{t}, 63);")
     o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
@@ -624,7 +630,10 @@ class PipelineGen:
            "}",
            "void launch_direct(const SrdLaunchArgs* a, hipStream_t s) {",
            "  run_k<3><<<a->grid_blocks, 256, 0, s>>>(*a);",
-           "}",
+           "}"] + ([
+           "void launch_weigh(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<4><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}"] if len(self.levels) >= 2 else []) + [
            f"}}  // namespace {self.ns}", ""]
     return "\n".join(out)
 
@@ -647,7 +656,8 @@ class PipelineGen:
             f"{{{', '.join(dests)}}}, {1 if self.ep.count else 0}, {grain}, "
             f"{1 if self.direct_ok() else 0}, "
             f"&{self.ns}::launch_root, &{self.ns}::launch_count, &{self.ns}::launch_write, "
-            f"&{self.ns}::launch_fused, &{self.ns}::launch_direct" + "}")
+            f"&{self.ns}::launch_fused, &{self.ns}::launch_direct, "
+            f"{'&' + self.ns + '::launch_weigh' if len(self.levels) >= 2 else 'nullptr'}" + "}")
 
 
 @dataclass

@@ -9,6 +9,7 @@ examples/lsqb_triangle_count.py), plus single-source reachability.
 from __future__ import annotations
 
 import os
+import shutil
 from concurrent.futures import ThreadPoolExecutor
 
 from . import build as B
@@ -206,8 +207,26 @@ def build_one(name: str) -> str:
 
 def build_all(jobs: int = 6) -> list[str]:
   os.makedirs(PROG_DIR, exist_ok=True)
+  src_dir = os.path.join(PROG_DIR, "src")
+  before = set(os.listdir(src_dir)) if os.path.isdir(src_dir) else set()
   with ThreadPoolExecutor(max_workers=jobs) as ex:
-    return list(ex.map(build_one, PROGRAMS))
+    out = list(ex.map(build_one, PROGRAMS))
+  # generated sources of earlier builds (named by content hash) are stale
+  for d in before - set(os.path.basename(os.path.dirname(p)) for p in _latest_sources(src_dir)):
+    shutil.rmtree(os.path.join(src_dir, d), ignore_errors=True)
+  return out
+
+
+def _latest_sources(src_dir: str) -> list[str]:
+  """The newest generated source file of each bundled program."""
+  out = []
+  for name in PROGRAMS:
+    cands = [os.path.join(src_dir, d) for d in os.listdir(src_dir)
+             if d.rsplit("_", 1)[0] == name]
+    if cands:
+      newest = max(cands, key=os.path.getmtime)
+      out += [os.path.join(newest, f) for f in os.listdir(newest)][:1]
+  return out
 
 
 _loaded: dict[str, ProgramLib] = {}

@@ -142,6 +142,15 @@ void export_rows(const Index& x, uint32_t* host_out, const uint32_t* bounds, uin
   }
 }
 
+void slice_bounds(const uint64_t* pref, uint64_t n, uint64_t w, uint64_t S, uint64_t* out,
+                  hipStream_t) {
+  const uint64_t q = w / S, rem = w % S;
+  for (uint64_t s = 0; s <= S; ++s) {
+    const uint64_t target = s * q + std::min(s, rem);
+    out[s] = uint64_t(std::lower_bound(pref, pref + n + 1, target) - pref);
+  }
+}
+
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t) {
   for (uint32_t g = 0; g < n_inst; ++g) {

@@ -76,7 +76,8 @@ void scan_run(const SrdLaunchArgs* a, hipStream_t) {
   const srd::IndexView& v = a->views.v[0];
   // slices [*work_ctr, n_slices), as the device kernels take them
   for (uint64_t s = a->work_ctr[0]; s < a->n_slices; ++s) {
-    uint64_t lo = s * a->grain, hi = std::min<uint64_t>(lo + a->grain, a->total_work);
+    uint64_t lo = a->slice_start ? a->slice_start[s] : s * a->grain;
+    uint64_t hi = a->slice_start ? a->slice_start[s + 1] : std::min<uint64_t>(lo + a->grain, a->total_work);
     Emit em{a, MODE};
     em.slice = s;
     em.opos = MODE == 1 ? a->offsets[s] : 0;
@@ -109,7 +110,8 @@ template <int MODE>
 void rec_run(const SrdLaunchArgs* a, hipStream_t) {
   // slices [*work_ctr, n_slices), as the device kernels take them
   for (uint64_t s = a->work_ctr[0]; s < a->n_slices; ++s) {
-    uint64_t lo = s * a->grain, hi = std::min<uint64_t>(lo + a->grain, a->total_work);
+    uint64_t lo = a->slice_start ? a->slice_start[s] : s * a->grain;
+    uint64_t hi = a->slice_start ? a->slice_start[s + 1] : std::min<uint64_t>(lo + a->grain, a->total_work);
     Emit em{a, MODE};
     em.slice = s;
     em.opos = MODE == 1 ? a->offsets[s] : 0;
@@ -129,6 +131,16 @@ void rec_run(const SrdLaunchArgs* a, hipStream_t) {
   }
 }
 
+// weigh mode stand-in: any positive cost per item is a valid slicing, so a
+// hash of the item exercises uneven slices (empty ones included)
+void rec_weigh(const SrdLaunchArgs* a, hipStream_t) {
+  for (uint64_t it = 0; it < a->total_work; ++it) {
+    uint64_t h = (it + 1) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    a->item_cost[it] = 1 + ((h % 7 == 0) ? h % 5000 : h % 3);
+  }
+}
+
 const SrdPipelineDesc k_pipes[] = {
   {"EdgeLoad", 1, {{"ArcInput", 2, {0, 1}, SRD_FULL}}, SRD_ROOT_SCAN, 1, {0}, 1,
    {{"Edge", 2, {0, 1}}}, 0, 4096, 1, &scan_root<false>, &scan_run<false, 0>, &scan_run<false, 1>,
@@ -138,7 +150,7 @@ const SrdPipelineDesc k_pipes[] = {
    &scan_run<true, 2>, &scan_run<true, 3>},
   {"TCRec_D0", 2, {{"Path", 2, {1, 0}, SRD_DELTA}, {"Edge", 2, {0, 1}, SRD_FULL}}, SRD_ROOT_JOIN, 2,
    {0, 1}, 1, {{"Path", 2, {1, 0}}}, 0, 7, 1, &rec_root, &rec_run<0>, &rec_run<1>, &rec_run<2>,
-   &rec_run<3>},
+   &rec_run<3>, &rec_weigh},
 };
 
 }  // namespace

@@ -22,8 +22,7 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 @pytest.fixture(scope="module")
 def eng():
-  if not os.path.exists(LIB):
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+  subprocess.run(["make", "-s", "-C", HERE], check=True)   # rebuilds when a source changed
   lib = ctypes.CDLL(LIB)
   lib.cpu_tc_engine.restype = ctypes.c_void_p
   lib.cpu_tc_engine.argtypes = [ctypes.c_char_p]
@@ -135,6 +134,26 @@ def test_engine_chunked_new(eng, chunk, mode, monkeypatch):
   else:
     monkeypatch.delenv("SRDATALOG_JOIN_MODE", raising=False)
   e = ds.tc_edges(4, 120, 400)
+  want = oracle.run("tc", {"ArcInput": e})
+  path, deltas, size = run_tc(eng, e)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]
+
+
+@pytest.mark.parametrize("target", [1, 50, 100000])
+@pytest.mark.parametrize("mode", ["fused", "twopass", "chunked"])
+def test_engine_weighted_slices(eng, target, mode, monkeypatch):
+  """Cost-balanced slices (weigh launch, prefix of item costs, slice_bounds)
+  give the same fixpoint and deltas as uniform slices; the CPU pipeline's
+  costs are an uneven hash of the item, so slices of one item, empty slices
+  and long runs of cheap items all occur."""
+  monkeypatch.setenv("SRDATALOG_WEIGH_MIN", "1")
+  monkeypatch.setenv("SRDATALOG_WEIGH_TARGET", str(target))
+  monkeypatch.setenv("SRDATALOG_GRAIN", "16")
+  monkeypatch.setenv("SRDATALOG_JOIN_MODE", "fused" if mode == "fused" else "twopass")
+  if mode == "chunked":
+    monkeypatch.setenv("SRDATALOG_NEW_CHUNK_ROWS", "300")
+  e = ds.tc_edges(5, 150, 500)
   want = oracle.run("tc", {"ArcInput": e})
   path, deltas, size = run_tc(eng, e)
   np.testing.assert_array_equal(path, want.relations["Path"])

@@ -204,7 +204,10 @@ def test_csv_load_all(progs, tmp_path):
 # must give the same relation and deltas as the oracle.  SRDATALOG_STAGE_CAP
 # shrinks the fused staging buffer so the overflow -> write-pass fallback runs;
 # SRDATALOG_NO_SEGMENTS keeps every FULL as one index (no segmented FULL);
-# SRDATALOG_NEW_CHUNK_ROWS writes join outputs in small deduplicated chunks.
+# SRDATALOG_NEW_CHUNK_ROWS writes join outputs in small deduplicated chunks;
+# SRDATALOG_WEIGH_MIN=1 cuts cost-balanced slices for every rule with two or
+# more nested levels (SRDATALOG_WEIGH_TARGET: cost per slice), and
+# SRDATALOG_NO_WEIGH=1 keeps uniform slices everywhere.
 MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_JOIN_MODE": "fused"}, "fused"),
          ({"SRDATALOG_JOIN_MODE": "fused", "SRDATALOG_STAGE_CAP": "3000"}, "fused-overflow"),
@@ -212,13 +215,19 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_NEW_CHUNK_ROWS": "20000", "SRDATALOG_GRAIN": "256"}, "chunked-new"),
          ({"SRDATALOG_NEW_CHUNK_ROWS": "20000", "SRDATALOG_JOIN_MODE": "twopass"},
           "chunked-new-twopass"),
+         ({"SRDATALOG_WEIGH_MIN": "1", "SRDATALOG_WEIGH_TARGET": "1"}, "weighted-1"),
+         ({"SRDATALOG_WEIGH_MIN": "1", "SRDATALOG_WEIGH_TARGET": "500",
+           "SRDATALOG_JOIN_MODE": "twopass"}, "weighted-twopass"),
+         ({"SRDATALOG_WEIGH_MIN": "1", "SRDATALOG_NEW_CHUNK_ROWS": "20000"}, "weighted-chunked"),
+         ({"SRDATALOG_NO_WEIGH": "1"}, "uniform-slices"),
          ({}, "default")]
 
 
 @pytest.mark.parametrize("env,label", MODES, ids=[m[1] for m in MODES])
 def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
   for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NO_SEGMENTS",
-            "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN"):
+            "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN", "SRDATALOG_WEIGH_MIN",
+            "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH"):
     monkeypatch.delenv(k, raising=False)
   for k, v in env.items():
     monkeypatch.setenv(k, v)
