@@ -145,8 +145,10 @@ __device__ __forceinline__ uint32_t upper_bound(const uint32_t* __restrict__ a, 
 // trip: with the midpoint it also loads the midpoints of both halves, so the
 // next step's value is already there whichever way the comparison goes.
 // Latency-bound searches (the join's) then wait for half as many dependent
-// loads, at 1.5x the loads.  (Three steps per trip, 7 loads, measured slower
-// on the triangle join: the extra loads cost more than the saved trips.)
+// loads, at 1.5x the loads.  Both steps are written as selects, so lanes
+// that go different ways do not split the wave into two branch paths.
+// (Three steps per trip, 7 loads, measured slower on the triangle join: the
+// extra loads cost more than the saved trips.)
 template <bool UPPER>
 __device__ __forceinline__ uint32_t bound_la(const uint32_t* __restrict__ a, uint32_t lo,
                                              uint32_t hi, uint32_t v) {
@@ -154,34 +156,21 @@ __device__ __forceinline__ uint32_t bound_la(const uint32_t* __restrict__ a, uin
   while (n > 1) {
     const uint32_t half = n >> 1;
     const uint32_t mid = lo + half;
-    const uint32_t nl = half, nr = n - half - 1;
-    const uint32_t ml = lo + (nl >> 1), mr = mid + 1 + (nr >> 1);
+    const uint32_t nl = half, nr = n - half - 1;  // nl >= 1
+    const uint32_t ml = lo + (nl >> 1);           // < mid
+    const uint32_t mr = mid + 1 + (nr >> 1);      // < hi when nr > 0
     const uint32_t x = a[mid];
-    const uint32_t xl = a[ml];                    // ml < mid: in range
-    const uint32_t xr = nr ? a[mr] : 0u;          // mr < hi when nr > 0
+    const uint32_t xl = a[ml];
+    const uint32_t xr = a[nr ? mr : mid];
     const bool right = UPPER ? x <= v : x < v;
-    if (right) {
-      lo = mid + 1;
-      n = nr;
-      if (n > 0) {
-        const uint32_t h2 = n >> 1;
-        if (UPPER ? xr <= v : xr < v) {
-          lo = mr + 1;
-          n -= h2 + 1;
-        } else {
-          n = h2;
-        }
-      }
-    } else {
-      n = nl;
-      const uint32_t h2 = n >> 1;  // n >= 1 here
-      if (UPPER ? xl <= v : xl < v) {
-        lo = ml + 1;
-        n -= h2 + 1;
-      } else {
-        n = h2;
-      }
-    }
+    const uint32_t lo1 = right ? mid + 1 : lo;    // after the first step
+    const uint32_t n1 = right ? nr : nl;
+    const uint32_t m2 = right ? mr : ml;          // = lo1 + (n1 >> 1)
+    const uint32_t y = right ? xr : xl;
+    const bool right2 = n1 > 0 && (UPPER ? y <= v : y < v);
+    const uint32_t h2 = n1 >> 1;
+    lo = right2 ? m2 + 1 : lo1;
+    n = n1 == 0 ? 0u : (right2 ? n1 - h2 - 1 : h2);
   }
   if (n == 1 && (UPPER ? a[lo] <= v : a[lo] < v)) ++lo;
   return lo;

@@ -298,9 +298,11 @@ class PipelineGen:
     registered for (view, prefix + var).
 
     Each active lane takes its item's value from the driving source
-    (`drv_expr`, uniform within a run of lanes with equal `seg`); the other
-    sources are then searched with srdj::seg_lower_bound, which needs every
-    lane of the wave to take part."""
+    (`drv_expr`, per lane); the other sources are then searched one slot at
+    a time, slot t being source t or t+1 depending on the lane's driver.
+    With cooperative search on, each source is searched instead with
+    srdj::seg_lower_bound, which needs every lane of the wave to take part
+    (`seg` groups the lanes of one owner)."""
     var = self.var(lv.var)
     code = [f"{ind}uint32_t {var} = 0;", f"{ind}bool {tag}on = false;"]
     child = {}
@@ -308,37 +310,107 @@ class PipelineGen:
       code.append(f"{ind}uint32_t {tag}c{j}b = 0, {tag}c{j}e = 0;")
       child[(self.vid(s), tuple(s.prefix) + (lv.var,))] = Rg(self.vid(s), len(s.prefix) + 1,
                                                              f"{tag}c{j}b", f"{tag}c{j}e")
-    code.append(f"{ind}if ({active}) {{")
-    code.append(f"{ind}  switch ({drv_expr}) {{")
-    for j, r in enumerate(rngs):
-      code.append(f"{ind}    case {j}: {{")
-      i2 = ind + "      "
-      if r.full0:
-        code.append(f"{i2}const uint32_t pos_ = uint32_t({item});")
-        code.append(f"{i2}const uint32_t val_ = V.v[{r.view}].root_vals[pos_];")
-        code.append(f"{i2}const srd::Rng cd_ = srd::Rng{{V.v[{r.view}].root_offs[pos_], "
-                    f"V.v[{r.view}].root_offs[pos_ + 1]}};")
-        code.append(f"{i2}const bool first_ = true;")
+    lasts = {r.depth == self.view_arity(r.view) - 1 for r in rngs if not r.full0}
+    if len(rngs) > 1 and not any(r.full0 for r in rngs) and len(lasts) == 1:
+      # every source is a column range of one kind: read the driver's value
+      # through per-lane selects instead of a switch (no divergent cases)
+      last = lasts.pop()
+      i2 = ind + "  "
+
+      def pick(vals):
+        e = vals[-1]
+        for j in range(len(vals) - 2, -1, -1):
+          e = f"({drv_expr} == {j}u ? {vals[j]} : {e})"
+        return e
+      code.append(f"{ind}if ({active}) {{")
+      code.append(f"{i2}const uint32_t* col_ = {pick([self.col_expr(r.view, r.depth) for r in rngs])};")
+      code.append(f"{i2}const uint32_t b_ = {pick([f'uint32_t({r.b})' for r in rngs])};")
+      code.append(f"{i2}const uint32_t pos_ = b_ + uint32_t({item});")
+      code.append(f"{i2}const uint32_t val_ = col_[pos_];")
+      if last:
+        code.append(f"{i2}{tag}on = true;")
+        code.append(f"{i2}const srd::Rng cd_ = srd::Rng{{pos_, pos_ + 1}};")
       else:
-        col = self.col_expr(r.view, r.depth)
-        last = r.depth == self.view_arity(r.view) - 1
-        code.append(f"{i2}const uint32_t* col_ = {col};")
-        code.append(f"{i2}const uint32_t pos_ = {r.b} + uint32_t({item});")
-        code.append(f"{i2}const uint32_t val_ = col_[pos_];")
-        if last:
-          code.append(f"{i2}const bool first_ = true;")
-          code.append(f"{i2}const srd::Rng cd_ = srd::Rng{{pos_, pos_ + 1}};")
-        else:
-          code.append(f"{i2}const bool first_ = pos_ == {r.b} || col_[pos_ - 1] != val_;")
-          code.append(f"{i2}const srd::Rng cd_ = srdj::run_at(col_, pos_, {r.e}, val_, false);")
-      code.append(f"{i2}{tag}on = first_;")
+        code.append(f"{i2}const uint32_t e_ = {pick([f'uint32_t({r.e})' for r in rngs])};")
+        code.append(f"{i2}{tag}on = pos_ == b_ || col_[pos_ - 1] != val_;")
+        code.append(f"{i2}const srd::Rng cd_ = srdj::run_at(col_, pos_, e_, val_, false);")
       code.append(f"{i2}{var} = val_;")
-      code.append(f"{i2}{tag}c{j}b = cd_.b; {tag}c{j}e = cd_.e;")
-      code.append(f"{i2}break;")
-      code.append(f"{ind}    }}")
-    code.append(f"{ind}  }}")
-    code.append(f"{ind}}}")
-    if len(rngs) > 1:
+      for j in range(len(rngs)):
+        code.append(f"{i2}if ({drv_expr} == {j}u) {{ {tag}c{j}b = cd_.b; {tag}c{j}e = cd_.e; }}")
+      code.append(f"{ind}}}")
+    else:
+      code.append(f"{ind}if ({active}) {{")
+      code.append(f"{ind}  switch ({drv_expr}) {{")
+      for j, r in enumerate(rngs):
+        code.append(f"{ind}    case {j}: {{")
+        i2 = ind + "      "
+        if r.full0:
+          code.append(f"{i2}const uint32_t pos_ = uint32_t({item});")
+          code.append(f"{i2}const uint32_t val_ = V.v[{r.view}].root_vals[pos_];")
+          code.append(f"{i2}const srd::Rng cd_ = srd::Rng{{V.v[{r.view}].root_offs[pos_], "
+                      f"V.v[{r.view}].root_offs[pos_ + 1]}};")
+          code.append(f"{i2}const bool first_ = true;")
+        else:
+          col = self.col_expr(r.view, r.depth)
+          last = r.depth == self.view_arity(r.view) - 1
+          code.append(f"{i2}const uint32_t* col_ = {col};")
+          code.append(f"{i2}const uint32_t pos_ = {r.b} + uint32_t({item});")
+          code.append(f"{i2}const uint32_t val_ = col_[pos_];")
+          if last:
+            code.append(f"{i2}const bool first_ = true;")
+            code.append(f"{i2}const srd::Rng cd_ = srd::Rng{{pos_, pos_ + 1}};")
+          else:
+            code.append(f"{i2}const bool first_ = pos_ == {r.b} || col_[pos_ - 1] != val_;")
+            code.append(f"{i2}const srd::Rng cd_ = srdj::run_at(col_, pos_, {r.e}, val_, false);")
+        code.append(f"{i2}{tag}on = first_;")
+        code.append(f"{i2}{var} = val_;")
+        code.append(f"{i2}{tag}c{j}b = cd_.b; {tag}c{j}e = cd_.e;")
+        code.append(f"{i2}break;")
+        code.append(f"{ind}    }}")
+      code.append(f"{ind}  }}")
+      code.append(f"{ind}}}")
+    if len(rngs) > 1 and not self.coop_search:
+      # One search per slot t: source t when the driver lies above t, else
+      # source t+1, so a wave whose lanes drive from different sources runs
+      # each slot's search once (with per-lane column and range) instead of
+      # one divergent search per source.
+      def src(m):
+        rm = rngs[m]
+        if rm.full0:
+          return f"V.v[{rm.view}].root_vals", "0u", f"V.v[{rm.view}].r"
+        return self.col_expr(rm.view, rm.depth), rm.b, rm.e
+
+      def child_code(m, i3):
+        rm = rngs[m]
+        if rm.full0:
+          return [f"{i3}{tag}c{m}b = V.v[{rm.view}].root_offs[lb_]; "
+                  f"{tag}c{m}e = V.v[{rm.view}].root_offs[lb_ + 1];"]
+        if rm.depth == self.view_arity(rm.view) - 1:
+          return [f"{i3}{tag}c{m}b = lb_; {tag}c{m}e = lb_ + 1;"]
+        return [f"{i3}{tag}c{m}b = lb_; {tag}c{m}e = srd::upper_bound_la(col_, lb_ + 1, e_, {var});"]
+
+      for t in range(len(rngs) - 1):
+        (ca, ba, ea), (cb, bb, eb) = src(t), src(t + 1)
+        i2 = ind + "  "
+        code.append(f"{ind}{{  // search slot {t}: source {t} if the driver is above {t}, else {t + 1}")
+        code.append(f"{i2}const bool lo_ = {drv_expr} > {t}u;")
+        code.append(f"{i2}const uint32_t* col_ = lo_ ? {ca} : {cb};")
+        code.append(f"{i2}const uint32_t b_ = lo_ ? uint32_t({ba}) : uint32_t({bb});")
+        code.append(f"{i2}const uint32_t e_ = lo_ ? uint32_t({ea}) : uint32_t({eb});")
+        code.append(f"{i2}const uint32_t lb_ = {tag}on ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;")
+        code.append(f"{i2}if ({tag}on) {{")
+        code.append(f"{i2}  const bool hit_ = lb_ < e_ && col_[lb_] == {var};")
+        code.append(f"{i2}  {tag}on = hit_;")
+        code.append(f"{i2}  if (hit_) {{")
+        code.append(f"{i2}    if (lo_) {{")
+        code += child_code(t, i2 + "      ")
+        code.append(f"{i2}    }} else {{")
+        code += child_code(t + 1, i2 + "      ")
+        code.append(f"{i2}    }}")
+        code.append(f"{i2}  }}")
+        code.append(f"{i2}}}")
+        code.append(f"{ind}}}")
+    elif len(rngs) > 1:
       for m, rm in enumerate(rngs):
         code.append(f"{ind}{{  // search source {m}")
         i2 = ind + "  "
@@ -484,7 +556,7 @@ class PipelineGen:
     lds_bytes = 0
     for d in self._lds:
       dims = [int(x) for x in re.findall(r"\[(\d+)\]", d)]
-      elem = 8 if "uint64_t" in d else 4
+      elem = 8 if "uint64_t" in d else (1 if "uint8_t" in d else 4)
       n = elem
       for x in dims:
         n *= x
@@ -537,7 +609,9 @@ class PipelineGen:
         frame += [r.b, r.e]
     nfr = len(frame)
     self._lds += [f"  __shared__ uint64_t pref_s{t}[4][65];",
-                  f"  __shared__ uint32_t fr_s{t}[4][{nfr + 1}][64];"]
+                  f"  __shared__ uint32_t fr_s{t}[4][{nfr + 1}][64];",
+                  f"  __shared__ uint32_t own_s{t}[4][64];",
+                  f"  __shared__ uint8_t flg_s{t}[4][64];"]
     o.append(f"{ind}uint64_t sz{t} = 0;")
     o.append(f"{ind}uint32_t dv{t} = 0;")
     o.append(f"{ind}if ({hvar}) {{")
@@ -564,13 +638,29 @@ class PipelineGen:
     for fi, expr in enumerate(frame):
       o.append(f"{ind}fr_s{t}[wv][{fi}][lane] = {expr};")
     o.append(f"{ind}fr_s{t}[wv][{nfr}][lane] = dv{t};")
+    # Owner lookup: the lanes with a non-empty space, in order, are listed in
+    # own_s; per batch of 64 positions each such lane that starts inside the
+    # batch flags its start, and a position's owner is the one of rank
+    # (non-empty lanes starting before the batch) + (flags at or below it).
+    o.append(f"{ind}const uint64_t ne{t} = srd::ballot(sz{t} > 0);")
+    o.append(f"{ind}if (sz{t} > 0) own_s{t}[wv][srd::popc_below(ne{t})] = lane;")
+    o.append(f"{ind}flg_s{t}[wv][lane] = 0;")
+    o.append(f"{ind}const uint64_t st{t} = inc{t} - sz{t};")
+    o.append(f"{ind}uint32_t base{t} = 0;")
     o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
     o.append(f"{ind}__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
     o.append(f"{ind}for (uint64_t j{t} = 0; j{t} < tot{t}; j{t} += 64) {{")
     i2 = ind + "  "
+    o.append(f"{i2}if (sz{t} > 0 && st{t} >= j{t} && st{t} < j{t} + 64) flg_s{t}[wv][uint32_t(st{t} - j{t})] = 1;")
+    o.append(f"{i2}__builtin_amdgcn_wave_barrier();")
+    o.append(f"{i2}__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
+    o.append(f"{i2}const uint64_t sm{t} = srd::ballot(flg_s{t}[wv][lane] != 0);")
+    o.append(f"{i2}flg_s{t}[wv][lane] = 0;")
+    o.append(f"{i2}const uint32_t rk{t} = base{t} + srd::popc_below(sm{t}) + uint32_t((sm{t} >> lane) & 1ull);")
+    o.append(f"{i2}base{t} += uint32_t(__popcll(sm{t}));")
     o.append(f"{i2}const uint64_t jj{t} = j{t} + lane;")
     o.append(f"{i2}const bool act{t} = jj{t} < tot{t};")
-    o.append(f"{i2}const uint32_t own{t} = act{t} ? srdj::owner_of(pref_s{t}[wv], jj{t}) : 0u;")
+    o.append(f"{i2}const uint32_t own{t} = act{t} ? own_s{t}[wv][rk{t} - 1] : 0u;")
     o.append(f"{i2}const uint64_t loc{t} = act{t} ? jj{t} - pref_s{t}[wv][own{t}] : 0;")
     o.append(f"{i2}{{")
     i3 = i2 + "  "

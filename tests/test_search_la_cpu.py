@@ -1,49 +1,56 @@
-"""The two-steps-per-round-trip binary search (srd::bound_la in
-csrc/srd_device.h), modelled line by line and checked against bisect on
-random sorted arrays, ranges and probe values (present, absent, out of
-range, duplicates)."""
+"""The device binary searches of csrc/srd_device.h -- srd::lower_bound /
+upper_bound and the two-steps-per-round-trip srd::lower_bound_la /
+upper_bound_la -- compiled for the host from the same source (g++ through
+tests/cpu_sim's stand-in HIP header) and checked against bisect on random
+sorted arrays, sub-ranges and probe values (present, absent, below and above
+the range, long runs of duplicates)."""
 
 import bisect
+import ctypes
+import os
+import subprocess
 
 import numpy as np
+import pytest
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_sim")
+_u32p = ctypes.POINTER(ctypes.c_uint32)
 
 
-def bound_la(a, lo, hi, v, upper):
-  less = (lambda x: x <= v) if upper else (lambda x: x < v)
-  n = hi - lo
-  while n > 1:
-    half = n >> 1
-    mid = lo + half
-    nl, nr = half, n - half - 1
-    ml, mr = lo + (nl >> 1), mid + 1 + (nr >> 1)
-    x, xl = a[mid], a[ml]
-    xr = a[mr] if nr else 0
-    if less(x):
-      lo, n = mid + 1, nr
-      if n > 0:
-        h2 = n >> 1
-        if less(xr):
-          lo, n = mr + 1, n - h2 - 1
-        else:
-          n = h2
-    else:
-      n = nl
-      h2 = n >> 1
-      if less(xl):
-        lo, n = ml + 1, n - h2 - 1
-      else:
-        n = h2
-  if n == 1 and less(a[lo]):
-    lo += 1
-  return lo
+@pytest.fixture(scope="module")
+def lib():
+  subprocess.run(["make", "-s", "-C", HERE, "libsearch_host.so"], check=True)
+  lib = ctypes.CDLL(os.path.join(HERE, "libsearch_host.so"))
+  for f in (lib.host_bound_la, lib.host_bound):
+    f.restype = ctypes.c_uint32
+    f.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+  return lib
 
 
-def test_bound_la_matches_bisect():
+@pytest.mark.parametrize("fn", ["host_bound_la", "host_bound"])
+def test_bounds_match_bisect(lib, fn):
+  f = getattr(lib, fn)
   rng = np.random.default_rng(0)
-  for _ in range(6000):
-    a = sorted(rng.integers(0, 300, int(rng.integers(1, 400))).tolist())
-    lo = int(rng.integers(0, len(a) + 1))
-    hi = int(rng.integers(lo, len(a) + 1))
-    v = int(rng.integers(-3, 55))
-    assert bound_la(a, lo, hi, v, False) == bisect.bisect_left(a, v, lo, hi)
-    assert bound_la(a, lo, hi, v, True) == bisect.bisect_right(a, v, lo, hi)
+  for _ in range(20000):
+    n = int(rng.integers(1, 400))
+    top = int(rng.choice([3, 50, 300, 1 << 30]))
+    a = np.sort(rng.integers(0, top, n)).astype(np.uint32)
+    lo = int(rng.integers(0, n + 1))
+    hi = int(rng.integers(lo, n + 1))
+    v = int(rng.integers(0, top + 3))
+    al = a.tolist()
+    p = a.ctypes.data_as(_u32p)
+    assert f(p, lo, hi, v, 0) == bisect.bisect_left(al, v, lo, hi), (fn, al, lo, hi, v)
+    assert f(p, lo, hi, v, 1) == bisect.bisect_right(al, v, lo, hi), (fn, al, lo, hi, v)
+
+
+def test_bounds_every_length_and_probe(lib):
+  """Every range length up to 70 and every probe around every element."""
+  a = (np.arange(70, dtype=np.uint32) // 2) * 3 + 1   # pairs of duplicates
+  al = a.tolist()
+  p = a.ctypes.data_as(_u32p)
+  for lo in range(0, 71, 7):
+    for hi in range(lo, 71):
+      for v in range(0, 110):
+        assert lib.host_bound_la(p, lo, hi, v, 0) == bisect.bisect_left(al, v, lo, hi)
+        assert lib.host_bound_la(p, lo, hi, v, 1) == bisect.bisect_right(al, v, lo, hi)
