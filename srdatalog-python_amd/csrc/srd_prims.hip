@@ -696,6 +696,7 @@ constexpr uint64_t kLbMask = (uint64_t(1) << 46) - 1;
 // only on tiles with smaller block ids, which every XCD dispatches first, so
 // a wait is short; the limit bounds a broken invariant to seconds.
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kLbWin = 8;  // onesweep look-back words loaded per round trip
 
 __device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint64_t flag, uint64_t v) {
   return (uint64_t(epoch) << 48) | (flag << 46) | v;
@@ -815,22 +816,38 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     lb_store(my_lb, lb_pack(epoch, kLbPrefix, tot));
   } else {
     lb_store(my_lb, lb_pack(epoch, kLbAgg, tot));
+    // Walk back kLbWin tiles per round trip: the words of the kLbWin nearest
+    // unresolved predecessors are loaded together, then summed nearest first
+    // up to the first inclusive prefix; a window with a word not yet
+    // published is reloaded from that word on.
     int64_t t = int64_t(tile) - 1;
     uint32_t spins = 0;
     while (t >= 0) {
-      uint64_t v = lb_load(lookback + uint64_t(t) * kRadix + d);
-      uint64_t flag = (v >> 46) & 3;
-      if (uint32_t(v >> 48) != epoch || flag == 0) {
+      uint64_t v[kLbWin];
+#pragma unroll
+      for (int k = 0; k < kLbWin; ++k)
+        v[k] = t - k >= 0 ? lb_load(lookback + uint64_t(t - k) * kRadix + d)
+                          : lb_pack(epoch, kLbPrefix, 0);
+      bool done = false;
+      int used = 0;
+#pragma unroll
+      for (int k = 0; k < kLbWin; ++k) {
+        if (done || used != k) continue;
+        const uint64_t flag = (v[k] >> 46) & 3;
+        if (uint32_t(v[k] >> 48) != epoch || flag == 0) continue;  // not yet published
+        excl += v[k] & kLbMask;
+        used = k + 1;
+        if (flag == kLbPrefix) done = true;
+      }
+      if (done) break;
+      t -= used;
+      if (used == 0) {
         if (++spins > kSpinLimit) {
           atomicOr(err, 1u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      excl += v & kLbMask;
-      if (flag == kLbPrefix) break;
-      --t;
     }
     lb_store(my_lb, lb_pack(epoch, kLbPrefix, excl + tot));
   }
@@ -1439,8 +1456,16 @@ __global__ __launch_bounds__(kBlock) void window_kernel(
 // before a row of B only when strictly less), so a row of A that is also in
 // B directly follows its equal B row; for the first rows of a tile that B row
 // may be the last one before the tile, B[j0-1], which the tile stages too.
-constexpr int kMpItems = 8;
+#ifndef SRD_MP_ITEMS
+#define SRD_MP_ITEMS 8
+#endif
+constexpr int kMpItems = SRD_MP_ITEMS;
 constexpr uint32_t kMpTile = kBlock * kMpItems;
+// Staged rows are stored one padding word per 32: thread t works on rows
+// near t*kMpItems, a stride that would put a wave's accesses in a few LDS
+// banks; the padding spreads them over all banks.
+constexpr uint32_t kMpLds = kMpTile + 2 + (kMpTile + 2) / 32 + 1;
+__device__ __forceinline__ uint32_t mp_pad(uint32_t r) { return r + (r >> 5); }
 
 // Rows of A among the first d rows of the B-first merged order.
 template <int K>
@@ -1469,10 +1494,11 @@ __global__ __launch_bounds__(kBlock) void mp_partition_kernel(
 // LDS row r of the staged tile: A rows at [0, an), then B[j0-1] at an (when
 // j0 > 0), then B rows.
 template <int K>
-__device__ __forceinline__ RowK<K> lds_row(const uint32_t (*st)[kMpTile + 2], uint32_t r) {
+__device__ __forceinline__ RowK<K> lds_row(const uint32_t (*st)[kMpLds], uint32_t r) {
   RowK<K> x;
+  const uint32_t q = mp_pad(r);
 #pragma unroll
-  for (int c = 0; c < K; ++c) x.v[c] = st[c][r];
+  for (int c = 0; c < K; ++c) x.v[c] = st[c][q];
   return x;
 }
 
@@ -1489,7 +1515,7 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
     uint64_t* __restrict__ total_out, uint32_t* __restrict__ tile_cnt,
     const uint64_t* __restrict__ tile_off) {
   constexpr bool DIFF = MODE != 1;
-  __shared__ uint32_t st[K][kMpTile + 2];
+  __shared__ uint32_t st[K][kMpLds];
   __shared__ uint32_t s_wsum[kWavesPerBlock];
   __shared__ uint64_t s_prefix;
   const uint32_t tid = threadIdx.x;
@@ -1507,7 +1533,7 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
       uint32_t v;
       if (r < an) v = a[uint64_t(c) * sa + i0 + r];
       else v = b[uint64_t(c) * sb + j0 - pre + (r - an)];
-      st[c][r] = v;
+      st[c][mp_pad(r)] = v;
     }
   }
   __syncthreads();
@@ -1552,11 +1578,11 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
     for (int it = 0; it < kMpItems; ++it)
       if (p0 + uint32_t(it) < tn)
 #pragma unroll
-        for (int c = 0; c < K; ++c) st[c][p0 + it] = rows[it].v[c];
+        for (int c = 0; c < K; ++c) st[c][mp_pad(p0 + it)] = rows[it].v[c];
     __syncthreads();
     for (uint32_t r = tid; r < tn; r += kBlock)
 #pragma unroll
-      for (int c = 0; c < K; ++c) out[uint64_t(c) * so + d0 + r] = st[c][r];
+      for (int c = 0; c < K; ++c) out[uint64_t(c) * so + d0 + r] = st[c][mp_pad(r)];
     return;
   }
 
@@ -1570,7 +1596,7 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
   for (int it = 0; it < kMpItems; ++it)
     if (keep[it]) {
 #pragma unroll
-      for (int c = 0; c < K; ++c) st[c][off] = rows[it].v[c];
+      for (int c = 0; c < K; ++c) st[c][mp_pad(off)] = rows[it].v[c];
       ++off;
     }
   if (MODE == 3) {
@@ -1589,7 +1615,7 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
   const uint64_t base = s_prefix;
   for (uint32_t r = tid; r < tile_tot; r += kBlock)
 #pragma unroll
-    for (int c = 0; c < K; ++c) out[uint64_t(c) * so + base + r] = st[c][r];
+    for (int c = 0; c < K; ++c) out[uint64_t(c) * so + base + r] = st[c][mp_pad(r)];
 }
 
 // Launch the partition and the tiles; returns the number of tiles.
