@@ -1507,6 +1507,10 @@ __device__ __forceinline__ RowK<K> lds_row(const uint32_t (*st)[kMpLds], uint32_
 // look-back in ticket order, and the last tile stores the total.  MODE 2:
 // only count the kept rows of each tile (tile_cnt[tile]); MODE 3: write
 // them at the scanned offsets tile_off[tile] (the two-pass variant).
+// MODE 4: write the kept rows at the tile's first A row i0 (tiles own
+// disjoint A ranges, and a tile keeps at most its A rows, so no offset is
+// needed) and their count to tile_cnt[tile]; mp_gather_kernel then packs
+// them (the gathered variant: no look-back, no second merge).
 template <int K, int MODE>
 __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
     const uint32_t* __restrict__ a, uint64_t sa, uint64_t na, const uint32_t* __restrict__ b,
@@ -1599,8 +1603,11 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
       for (int c = 0; c < K; ++c) st[c][mp_pad(off)] = rows[it].v[c];
       ++off;
     }
-  if (MODE == 3) {
-    if (tid == 0) s_prefix = tile_off[tile];
+  if (MODE == 3 || MODE == 4) {
+    if (tid == 0) {
+      s_prefix = MODE == 3 ? tile_off[tile] : i0;
+      if (MODE == 4) tile_cnt[tile] = tile_tot;
+    }
   } else if (tid < kWave) {  // wave 0 resolves the tile's output offset
     uint64_t* mine = lookback + tile;
     if (tid == 0) lb_store(mine, lb_pack(epoch, tile == 0 ? kLbPrefix : kLbAgg, tile_tot));
@@ -1647,6 +1654,55 @@ static uint64_t merge_path(const Index& a, const Index& b, uint32_t* out, uint64
       MODE == 0 ? ws.mp_total.p : nullptr, nullptr, nullptr);
   SRD_CHECK(hipGetLastError());
   return tiles;
+}
+
+// Pack the rows each MODE 4 tile left at its first A row into consecutive
+// output rows: block t copies tile_cnt[t] rows from tmp row split[t] to out
+// row tile_off[t].
+template <int K>
+__global__ __launch_bounds__(kBlock) void mp_gather_kernel(const uint32_t* __restrict__ tmp,
+                                                           uint64_t st,
+                                                           const uint64_t* __restrict__ split,
+                                                           const uint32_t* __restrict__ tile_cnt,
+                                                           const uint64_t* __restrict__ tile_off,
+                                                           uint32_t* __restrict__ out, uint64_t so) {
+  const uint32_t t = blockIdx.x;
+  const uint64_t src = split[t], dst = tile_off[t];
+  const uint32_t cnt = tile_cnt[t];
+  for (uint32_t r = threadIdx.x; r < cnt; r += kBlock)
+#pragma unroll
+    for (int c = 0; c < K; ++c) out[uint64_t(c) * so + dst + r] = tmp[uint64_t(c) * st + src + r];
+}
+
+// Gathered set difference: one merge pass leaves each tile's kept rows at
+// its first A row of a scratch copy (MODE 4), a scan of the per-tile counts
+// gives the output offsets, and mp_gather_kernel packs the rows.  Costs
+// 2*|D|*w more traffic than the look-back variant but no tile waits on
+// another.  The total goes to ws.mp_total like the other variants'.
+template <int K>
+static void set_difference_gathered(const Index& a, const Index& b, uint32_t* out, uint64_t so,
+                                    Workspace& ws, hipStream_t s) {
+  const uint64_t tiles = (a.n + b.n + kMpTile - 1) / kMpTile;
+  ws.mp_split.reserve(tiles + 1, s);
+  mp_partition_kernel<K><<<uint32_t((tiles + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, tiles, ws.mp_split.p);
+  SRD_CHECK(hipGetLastError());
+  const uint64_t ts = round_stride(a.n);
+  DevBuf<uint32_t> tmp(ts * K, s);
+  DevBuf<uint32_t> cnt(tiles, s);
+  DevBuf<uint64_t> off(tiles + 1, s);
+  mp_tile_kernel<K, 4><<<uint32_t(tiles), kBlock, 0, s>>>(
+      a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, tmp.p, ts, nullptr,
+      nullptr, 0, nullptr, cnt.p, nullptr);
+  SRD_CHECK(hipGetLastError());
+  scan_impl<uint32_t>(cnt.p, off.p, tiles, s);
+  mp_gather_kernel<K><<<uint32_t(tiles), kBlock, 0, s>>>(tmp.p, ts, ws.mp_split.p, cnt.p, off.p,
+                                                         out, so);
+  SRD_CHECK(hipGetLastError());
+  ws.mp_total.reserve(1, s);
+  ws.mp_ctr.reserve(2, s);
+  fill_u32(ws.mp_ctr.p, 0, 2, s);
+  SRD_CHECK(hipMemcpyAsync(ws.mp_total.p, off.p + tiles, 8, hipMemcpyDeviceToDevice, s));
 }
 
 // Two-pass set difference: count the kept rows per tile, scan, write.  The
@@ -1722,11 +1778,16 @@ static void set_difference_k(const Index& a, const Index& b, Index& out, Workspa
     out.cols.alloc(out.stride * K, s);
     {
       TimeScope t("setdiff.merge_path", s, double(a.n + b.n) * 4 * K);
-      static const bool two_pass = std::getenv("SRDATALOG_SETDIFF_TWO_PASS") != nullptr;
-      if (two_pass)
+      // SRDATALOG_SETDIFF = gather (default) | lookback | twopass; read per
+      // call so tests can switch variants within one process
+      const char* v = std::getenv("SRDATALOG_SETDIFF");
+      const std::string variant = v ? v : (std::getenv("SRDATALOG_SETDIFF_TWO_PASS") ? "twopass" : "gather");
+      if (variant == "twopass")
         set_difference_two_pass<K>(a, b, out.cols.p, out.stride, ws, s);
-      else
+      else if (variant == "lookback")
         merge_path<K, 0>(a, b, out.cols.p, out.stride, ws, s);
+      else
+        set_difference_gathered<K>(a, b, out.cols.p, out.stride, ws, s);
     }
     uint64_t* h64 = reinterpret_cast<uint64_t*>(pinned_words(ws, 4));
     SRD_CHECK(hipMemcpyAsync(h64, ws.mp_total.p, 8, hipMemcpyDeviceToHost, s));

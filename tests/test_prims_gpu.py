@@ -64,10 +64,13 @@ def test_sort_dedup_large(nat, k, n):
   np.testing.assert_array_equal(ro, wo)
 
 
+@pytest.mark.parametrize("variant", ["gather", "lookback", "twopass"])
 @pytest.mark.parametrize("k", [1, 2, 3])
 @pytest.mark.parametrize("na,nb", [(0, 10), (10, 0), (1000, 1000), (50, 200_000),
-                                   (200_000, 50), (300_000, 300_000)])
-def test_set_difference(nat, k, na, nb):
+                                   (200_000, 50), (300_000, 300_000), (3_000_000, 2_000_000)])
+def test_set_difference(nat, k, na, nb, variant, monkeypatch):
+  # SRDATALOG_SETDIFF picks the dense variant; it is read on every call
+  monkeypatch.setenv("SRDATALOG_SETDIFF", variant)
   rng = np.random.default_rng(na + nb + k)
   hi = 300 if k > 1 else 400_000
   a = npref.sort_unique(rng.integers(0, hi, size=(na, k), dtype=np.uint32))
