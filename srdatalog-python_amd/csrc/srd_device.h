@@ -32,6 +32,13 @@ struct IndexView {
   uint32_t r;      // distinct column-0 values
   uint32_t arity;
   uint32_t pad_;
+  // Membership bitmaps of the column-1 values of heavy keys (two-column
+  // indexes a join searches at depth 1): bitmap i covers the rows starting
+  // at hv_start[i] (ascending), bm_words u32 each; n_hv = 0 when absent.
+  const uint32_t* bm;
+  const uint32_t* hv_start;
+  uint32_t n_hv;
+  uint32_t bm_words;
 };
 
 struct Rng {

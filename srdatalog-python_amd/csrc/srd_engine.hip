@@ -259,6 +259,9 @@ struct srd_engine {
   uint64_t weigh_min = 4096;       // SRDATALOG_WEIGH_MIN: fewer items: uniform slices
   uint64_t weigh_target = 16384;   // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
   srd::DevBuf<uint64_t> item_cost, item_pref, slice_start;
+  // column-1 bitmaps of heavy keys for views nested levels search
+  uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
+  uint64_t bitmap_bytes = uint64_t(1) << 30;  // SRDATALOG_BITMAP_BYTES (0: none)
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
   // instance table of a packed batch: n_inst+1 ascending column-0 bounds
@@ -723,6 +726,19 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
       x = &delta_index(e, r, order);
     else
       x = &new_index(e, r, order);
+    if (vs.want_bitmaps && x->arity == 2 && e.bitmap_bytes) {
+      const HeavyBits* h = x->heavy.get();
+      if (!h || h->built_for != x->cols.p || h->built_n != x->n) {
+        timing_begin("join.bitmaps", e.s);
+        build_heavy_bitmaps(*x, e.bitmap_min, e.bitmap_bytes, e.ws, e.s);
+        timing_end(e.s);
+        if (e.trace) {
+          std::fprintf(stderr, "[srd]   %s view %u: %u heavy-key bitmaps of %u words\n", d->rule, v,
+                       x->heavy->n, x->heavy->words);
+          std::fflush(stderr);
+        }
+      }
+    }
     a.views.v[v] = x->view();
   }
   a.grain = e.grain_override ? e.grain_override : d->default_grain;
@@ -1307,6 +1323,9 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     if (const char* c = std::getenv("SRDATALOG_NEW_CHUNK_ROWS"))
       e->new_chunk_rows = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_NO_WEIGH")) e->no_weigh = *c && *c != '0';
+    if (const char* c = std::getenv("SRDATALOG_BITMAP_MIN"))
+      e->bitmap_min = std::max<uint32_t>(uint32_t(std::strtoul(c, nullptr, 10)), SRD_BM_MIN);
+    if (const char* c = std::getenv("SRDATALOG_BITMAP_BYTES")) e->bitmap_bytes = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_WEIGH_MIN")) e->weigh_min = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_WEIGH_TARGET"))
       e->weigh_target = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);

@@ -181,6 +181,20 @@ __device__ __forceinline__ bool has_prefix(const IndexView& v, T... key) {
   return r.e > r.b;
 }
 
+// Bitmap slot of the heavy key whose depth-1 range of view v starts at row
+// b, or ~0u when that key has no bitmap.
+__device__ __forceinline__ uint32_t heavy_slot(const IndexView& v, uint32_t b) {
+  if (v.n_hv == 0) return ~0u;
+  const uint32_t p = srd::lower_bound(v.hv_start, 0, v.n_hv, b);
+  return (p < v.n_hv && v.hv_start[p] == b) ? p : ~0u;
+}
+
+// Whether value x is in the range whose bitmap is `slot` of view v.
+__device__ __forceinline__ bool heavy_has(const IndexView& v, uint32_t slot, uint32_t x) {
+  const uint32_t w = x >> 5;
+  return w < v.bm_words && ((v.bm[uint64_t(slot) * v.bm_words + w] >> (x & 31u)) & 1u);
+}
+
 // Wave-uniform copy of a 64-bit value held by `lane`.
 __device__ __forceinline__ uint64_t uniform_from(uint64_t v, int lane) {
   uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);

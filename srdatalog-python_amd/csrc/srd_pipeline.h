@@ -51,6 +51,7 @@ struct SrdViewSpec {
   uint32_t ncols;
   uint32_t cols[SRD_MAX_COLS];  // index column order
   uint32_t ver;                 // SrdVersion
+  uint32_t want_bitmaps;        // a nested level searches this two-column view at depth 1
 };
 
 struct SrdDestSpec {
@@ -100,6 +101,7 @@ struct SrdLaunchArgs {
 };
 
 #define SRD_ITEM_COST 4u       // level-1 cost of an item, in level-2 candidates
+#define SRD_BM_MIN 512u        // ranges at least this long are looked up among the heavy keys
 
 #define SRD_SEG 1024u          // staging segment size (tuples)
 

@@ -397,6 +397,78 @@ void slice_bounds(const uint64_t* pref, uint64_t n, uint64_t w, uint64_t S, uint
   SRD_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------------- heavy-key bitmaps ----
+__global__ __launch_bounds__(kBlock) void heavy_flag_kernel(const uint32_t* __restrict__ offs,
+                                                            uint32_t r, uint32_t min_len,
+                                                            uint32_t* __restrict__ flag) {
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g < r) flag[g] = offs[g + 1] - offs[g] >= min_len ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void heavy_scatter_kernel(
+    const uint32_t* __restrict__ offs, uint32_t r, const uint32_t* __restrict__ flag,
+    const uint64_t* __restrict__ pos, uint32_t* __restrict__ starts, uint32_t* __restrict__ ends) {
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g < r && flag[g]) {
+    starts[pos[g]] = offs[g];
+    ends[pos[g]] = offs[g + 1];
+  }
+}
+
+// Block i sets the bits of heavy key i's column-1 values.
+__global__ __launch_bounds__(kBlock) void heavy_set_kernel(const uint32_t* __restrict__ col1,
+                                                           const uint32_t* __restrict__ starts,
+                                                           const uint32_t* __restrict__ ends,
+                                                           uint32_t words,
+                                                           uint32_t* __restrict__ bits) {
+  const uint32_t i = blockIdx.x;
+  uint32_t* b = bits + uint64_t(i) * words;
+  for (uint32_t row = starts[i] + threadIdx.x; row < ends[i]; row += kBlock) {
+    const uint32_t v = col1[row];
+    atomicOr(&b[v >> 5], 1u << (v & 31u));
+  }
+}
+
+void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, Workspace& ws,
+                         hipStream_t s) {
+  if (!x.heavy) x.heavy = std::make_shared<HeavyBits>();
+  HeavyBits& h = *x.heavy;
+  h.n = 0;
+  h.built_for = x.cols.p;
+  h.built_n = x.n;
+  if (x.arity != 2 || x.r == 0 || x.col_max.size() < 2 || max_bytes == 0) return;
+  const uint32_t words = (x.col_max[1] >> 5) + 1;
+  DevBuf<uint32_t> flag(x.r, s), ends;
+  DevBuf<uint64_t> pos(uint64_t(x.r) + 1, s);
+  for (;;) {
+    heavy_flag_kernel<<<(x.r + kBlock - 1) / kBlock, kBlock, 0, s>>>(x.root_offs.p, x.r, min_len,
+                                                                     flag.p);
+    SRD_CHECK(hipGetLastError());
+    exclusive_scan_u32(flag.p, pos.p, x.r, ws, s);
+    uint32_t* hw = pinned_words(ws, 2);
+    SRD_CHECK(hipMemcpyAsync(hw, pos.p + x.r, 8, hipMemcpyDeviceToHost, s));
+    SRD_CHECK(hipStreamSynchronize(s));
+    const uint32_t n = hw[0];  // at most r
+    if (n == 0) return;
+    if (uint64_t(n) * words * 4 <= max_bytes) {
+      h.starts.alloc(n, s);
+      ends.alloc(n, s);
+      heavy_scatter_kernel<<<(x.r + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+          x.root_offs.p, x.r, flag.p, pos.p, h.starts.p, ends.p);
+      SRD_CHECK(hipGetLastError());
+      h.bits.alloc(uint64_t(n) * words, s);
+      fill_u32(h.bits.p, 0, uint64_t(n) * words, s);
+      heavy_set_kernel<<<n, kBlock, 0, s>>>(x.col(1), h.starts.p, ends.p, words, h.bits.p);
+      SRD_CHECK(hipGetLastError());
+      h.n = n;
+      h.words = words;
+      return;
+    }
+    if (min_len >= (1u << 30)) return;
+    min_len *= 2;  // too many heavy keys for the budget: keep the longer ones
+  }
+}
+
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t s) {
   if (n_inst == 0) return;

@@ -81,6 +81,17 @@ struct DevBuf {
 // ----------------------------------------------------------------- index --
 // Sorted, duplicate-free k-column index with root (distinct column-0)
 // arrays; see srd_device.h for the layout.
+// Column-1 membership bitmaps of the keys of a two-column index whose
+// ranges are long (built on demand for joins that search them; see
+// build_heavy_bitmaps).
+struct HeavyBits {
+  DevBuf<uint32_t> bits;     // n x words
+  DevBuf<uint32_t> starts;   // first row of each heavy key's range, ascending
+  uint32_t n = 0, words = 0;
+  const uint32_t* built_for = nullptr;  // cols buffer and row count they describe
+  uint64_t built_n = 0;
+};
+
 struct Index {
   uint32_t arity = 0;
   uint64_t n = 0;
@@ -90,6 +101,7 @@ struct Index {
   DevBuf<uint32_t> root_vals;  // r
   DevBuf<uint32_t> root_offs;  // r + 1
   std::vector<uint32_t> col_max;  // per-column maximum (0 when empty)
+  mutable std::shared_ptr<HeavyBits> heavy;  // derived data, shared by copies
 
   IndexView view() const {
     IndexView v;
@@ -101,6 +113,11 @@ struct Index {
     v.r = r;
     v.arity = arity;
     v.pad_ = 0;
+    const bool hb = heavy && heavy->n && heavy->built_for == cols.p && heavy->built_n == n;
+    v.bm = hb ? heavy->bits.p : nullptr;
+    v.hv_start = hb ? heavy->starts.p : nullptr;
+    v.n_hv = hb ? heavy->n : 0;
+    v.bm_words = hb ? heavy->words : 0;
     return v;
   }
   const uint32_t* col(uint32_t c) const { return cols.p + uint64_t(c) * stride; }
@@ -193,6 +210,12 @@ void export_rows(const Index& x, uint32_t* host_out, const uint32_t* dev_bounds,
 // [bounds[g], bounds[g+1]), g < n_inst (bounds has n_inst+1 entries).
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t s);
+
+// Membership bitmaps for the keys of two-column index x whose ranges hold at
+// least min_len rows (min_len doubled until they fit max_bytes): bitmap i
+// has bit v set when row (key_i, v) exists.  Fills x.heavy.
+void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, Workspace& ws,
+                         hipStream_t s);
 
 // Cost-balanced slice bounds: pref holds n+1 ascending item-cost prefix sums
 // (pref[n] = w, every item costing at least 1); out[s] (s = 0..S) = first

@@ -54,6 +54,7 @@ class Rg:
   e: str
   full0: bool = False     # the whole index at depth 0 (distinct values = root_vals)
   uniform: bool = False   # wave-uniform (not per lane)
+  bm: str = ""            # heavy-key bitmap slot of the range (~0u: none), when eligible
 
 
 @dataclass
@@ -211,6 +212,20 @@ class PipelineGen:
     return len(self.levels) == 1 and isinstance(self.levels[0], M.CartesianJoin)
 
   # ------------------------------------------------------------ ranges ---
+  def bm_eligible(self, r: Rg) -> bool:
+    """A per-lane depth-1 range of a two-column view: its values are the
+    last column, so a membership bitmap can replace the search."""
+    return (not r.full0 and not r.uniform and r.depth == 1 and self.view_arity(r.view) == 2)
+
+  def bm_views(self) -> set:
+    """Views that the second level of the join searches at depth 1."""
+    out = set()
+    for lv in self.levels[1:2]:
+      for s_ in lv.sources:
+        if len(s_.prefix) == 1 and self.view_arity(self.vid(s_)) == 2:
+          out.add(self.vid(s_))
+    return out
+
   def resolve(self, reg: dict, v: int, prefix: tuple, code: list[str], ind: str, tag: str) -> Rg:
     """Range of view v with the given bound prefix, narrowing if needed."""
     key = (v, prefix)
@@ -397,9 +412,24 @@ class PipelineGen:
         code.append(f"{i2}const uint32_t* col_ = lo_ ? {ca} : {cb};")
         code.append(f"{i2}const uint32_t b_ = lo_ ? uint32_t({ba}) : uint32_t({bb});")
         code.append(f"{i2}const uint32_t e_ = lo_ ? uint32_t({ea}) : uint32_t({eb});")
-        code.append(f"{i2}const uint32_t lb_ = {tag}on ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;")
+        ha, hb_ = rngs[t].bm or "~0u", rngs[t + 1].bm or "~0u"
+        if rngs[t].bm or rngs[t + 1].bm:
+          # a heavy key's range: one bitmap probe instead of the search (the
+          # range is the view's last column, so its child range is unused)
+          va, vb = rngs[t].view, rngs[t + 1].view
+          code.append(f"{i2}const uint32_t hs_ = lo_ ? {ha} : {hb_};")
+          code.append(f"{i2}bool hit_ = false;")
+          code.append(f"{i2}uint32_t lb_ = 0u;")
+          code.append(f"{i2}if (hs_ != ~0u) {{")
+          code.append(f"{i2}  hit_ = {tag}on && srdj::heavy_has(lo_ ? V.v[{va}] : V.v[{vb}], hs_, {var});")
+          code.append(f"{i2}}} else {{")
+          code.append(f"{i2}  lb_ = {tag}on ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;")
+          code.append(f"{i2}  hit_ = {tag}on && lb_ < e_ && col_[lb_] == {var};")
+          code.append(f"{i2}}}")
+        else:
+          code.append(f"{i2}const uint32_t lb_ = {tag}on ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;")
+          code.append(f"{i2}const bool hit_ = {tag}on && lb_ < e_ && col_[lb_] == {var};")
         code.append(f"{i2}if ({tag}on) {{")
-        code.append(f"{i2}  const bool hit_ = lb_ < e_ && col_[lb_] == {var};")
         code.append(f"{i2}  {tag}on = hit_;")
         code.append(f"{i2}  if (hit_) {{")
         code.append(f"{i2}    if (lo_) {{")
@@ -607,6 +637,13 @@ class PipelineGen:
       else:
         per_lane.append((len(frame), len(frame) + 1))
         frame += [r.b, r.e]
+    bm_at = {}
+    for j, r in enumerate(rngs):
+      if li == 1 and self.bm_eligible(r):  # level 2 only: deeper frames would outgrow LDS
+        o.append(f"{ind}const uint32_t hb{t}_{j} = ({hvar} && V.v[{r.view}].n_hv && "
+                 f"{r.e} - {r.b} >= SRD_BM_MIN) ? srdj::heavy_slot(V.v[{r.view}], {r.b}) : ~0u;")
+        bm_at[j] = len(frame)
+        frame.append(f"hb{t}_{j}")
     nfr = len(frame)
     self._lds += [f"  __shared__ uint64_t pref_s{t}[4][65];",
                   f"  __shared__ uint32_t fr_s{t}[4][{nfr + 1}][64];",
@@ -675,7 +712,11 @@ class PipelineGen:
         bi, ei = per_lane[j]
         o.append(f"{i3}const uint32_t f{t}_{j}b = fr_s{t}[wv][{bi}][own{t}];")
         o.append(f"{i3}const uint32_t f{t}_{j}e = fr_s{t}[wv][{ei}][own{t}];")
-        rr = Rg(r.view, r.depth, f"f{t}_{j}b", f"f{t}_{j}e")
+        bm = ""
+        if j in bm_at:
+          o.append(f"{i3}const uint32_t f{t}_{j}h = fr_s{t}[wv][{bm_at[j]}][own{t}];")
+          bm = f"f{t}_{j}h"
+        rr = Rg(r.view, r.depth, f"f{t}_{j}b", f"f{t}_{j}e", bm=bm)
         rngs_o.append(rr)
         s_ = lv.sources[j]
         reg2[(self.vid(s_), tuple(s_.prefix))] = rr
@@ -729,9 +770,11 @@ class PipelineGen:
 
   def gen_desc(self) -> str:
     views = []
-    for rel, index, ver in self.info.views:
+    bmv = self.bm_views()
+    for vi, (rel, index, ver) in enumerate(self.info.views):
       cols = ", ".join(str(c) for c in index)
-      views.append(f"{{\"{rel}\", {len(index)}, {{{cols}}}, {VER_CODE[ver]}}}")
+      views.append(f"{{\"{rel}\", {len(index)}, {{{cols}}}, {VER_CODE[ver]}, "
+                   f"{1 if vi in bmv else 0}}}")
     if isinstance(self.root, M.Scan):
       kind, roots = "SRD_ROOT_SCAN", [self.vid(self.root)]
     else:

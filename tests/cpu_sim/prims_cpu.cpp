@@ -151,6 +151,42 @@ void slice_bounds(const uint64_t* pref, uint64_t n, uint64_t w, uint64_t S, uint
   }
 }
 
+void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, Workspace&,
+                         hipStream_t s) {
+  if (!x.heavy) x.heavy = std::make_shared<HeavyBits>();
+  HeavyBits& h = *x.heavy;
+  h.n = 0;
+  h.built_for = x.cols.p;
+  h.built_n = x.n;
+  if (x.arity != 2 || x.r == 0 || x.col_max.size() < 2 || max_bytes == 0) return;
+  const uint32_t words = (x.col_max[1] >> 5) + 1;
+  for (;;) {
+    std::vector<uint32_t> st, en;
+    for (uint32_t g = 0; g < x.r; ++g)
+      if (x.root_offs.p[g + 1] - x.root_offs.p[g] >= min_len)
+        st.push_back(x.root_offs.p[g]), en.push_back(x.root_offs.p[g + 1]);
+    if (st.empty()) return;
+    if (uint64_t(st.size()) * words * 4 <= max_bytes) {
+      const uint32_t n = uint32_t(st.size());
+      h.starts.alloc(n, s);
+      h.bits.alloc(uint64_t(n) * words, s);
+      std::fill(h.bits.p, h.bits.p + uint64_t(n) * words, 0u);
+      for (uint32_t i = 0; i < n; ++i) {
+        h.starts.p[i] = st[i];
+        for (uint32_t row = st[i]; row < en[i]; ++row) {
+          const uint32_t v = x.col(1)[row];
+          h.bits.p[uint64_t(i) * words + (v >> 5)] |= 1u << (v & 31u);
+        }
+      }
+      h.n = n;
+      h.words = words;
+      return;
+    }
+    if (min_len >= (1u << 30)) return;
+    min_len *= 2;
+  }
+}
+
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t) {
   for (uint32_t g = 0; g < n_inst; ++g) {

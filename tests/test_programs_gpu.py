@@ -207,7 +207,9 @@ def test_csv_load_all(progs, tmp_path):
 # SRDATALOG_NEW_CHUNK_ROWS writes join outputs in small deduplicated chunks;
 # SRDATALOG_WEIGH_MIN=1 cuts cost-balanced slices for every rule with two or
 # more nested levels (SRDATALOG_WEIGH_TARGET: cost per slice), and
-# SRDATALOG_NO_WEIGH=1 keeps uniform slices everywhere.
+# SRDATALOG_NO_WEIGH=1 keeps uniform slices everywhere; SRDATALOG_BITMAP_BYTES
+# bounds the heavy-key bitmaps (0: none; a small budget keeps only the
+# longest keys' bitmaps, so bitmap probes and searches mix in one wave).
 MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_JOIN_MODE": "fused"}, "fused"),
          ({"SRDATALOG_JOIN_MODE": "fused", "SRDATALOG_STAGE_CAP": "3000"}, "fused-overflow"),
@@ -220,6 +222,8 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
            "SRDATALOG_JOIN_MODE": "twopass"}, "weighted-twopass"),
          ({"SRDATALOG_WEIGH_MIN": "1", "SRDATALOG_NEW_CHUNK_ROWS": "20000"}, "weighted-chunked"),
          ({"SRDATALOG_NO_WEIGH": "1"}, "uniform-slices"),
+         ({"SRDATALOG_BITMAP_BYTES": "0"}, "no-bitmaps"),
+         ({"SRDATALOG_BITMAP_BYTES": "20000"}, "few-bitmaps"),
          ({}, "default")]
 
 
@@ -227,7 +231,7 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
 def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
   for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NO_SEGMENTS",
             "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN", "SRDATALOG_WEIGH_MIN",
-            "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH"):
+            "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH", "SRDATALOG_BITMAP_BYTES"):
     monkeypatch.delenv(k, raising=False)
   for k, v in env.items():
     monkeypatch.setenv(k, v)
