@@ -31,7 +31,7 @@ struct IndexView {
   uint32_t n;      // rows
   uint32_t r;      // distinct column-0 values
   uint32_t arity;
-  uint32_t pad_;
+  uint32_t bm_min;  // ranges shorter than this have no bitmap
   // Membership bitmaps of the column-1 values of heavy keys (two-column
   // indexes a join searches at depth 1): bitmap i covers the rows starting
   // at hv_start[i] (ascending), bm_words u32 each; n_hv = 0 when absent.

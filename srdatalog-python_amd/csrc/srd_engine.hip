@@ -1324,7 +1324,7 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
       e->new_chunk_rows = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_NO_WEIGH")) e->no_weigh = *c && *c != '0';
     if (const char* c = std::getenv("SRDATALOG_BITMAP_MIN"))
-      e->bitmap_min = std::max<uint32_t>(uint32_t(std::strtoul(c, nullptr, 10)), SRD_BM_MIN);
+      e->bitmap_min = std::max<uint32_t>(uint32_t(std::strtoul(c, nullptr, 10)), 16);
     if (const char* c = std::getenv("SRDATALOG_BITMAP_BYTES")) e->bitmap_bytes = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_WEIGH_MIN")) e->weigh_min = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_WEIGH_TARGET"))

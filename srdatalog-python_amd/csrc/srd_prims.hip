@@ -462,6 +462,7 @@ void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, W
       SRD_CHECK(hipGetLastError());
       h.n = n;
       h.words = words;
+      h.min_len = min_len;
       return;
     }
     if (min_len >= (1u << 30)) return;

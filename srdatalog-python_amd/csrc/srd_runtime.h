@@ -88,6 +88,7 @@ struct HeavyBits {
   DevBuf<uint32_t> bits;     // n x words
   DevBuf<uint32_t> starts;   // first row of each heavy key's range, ascending
   uint32_t n = 0, words = 0;
+  uint32_t min_len = 0;                 // shortest range that has a bitmap
   const uint32_t* built_for = nullptr;  // cols buffer and row count they describe
   uint64_t built_n = 0;
 };
@@ -112,12 +113,12 @@ struct Index {
     v.n = uint32_t(n);
     v.r = r;
     v.arity = arity;
-    v.pad_ = 0;
     const bool hb = heavy && heavy->n && heavy->built_for == cols.p && heavy->built_n == n;
     v.bm = hb ? heavy->bits.p : nullptr;
     v.hv_start = hb ? heavy->starts.p : nullptr;
     v.n_hv = hb ? heavy->n : 0;
     v.bm_words = hb ? heavy->words : 0;
+    v.bm_min = hb ? heavy->min_len : ~0u;
     return v;
   }
   const uint32_t* col(uint32_t c) const { return cols.p + uint64_t(c) * stride; }

@@ -641,7 +641,7 @@ class PipelineGen:
     for j, r in enumerate(rngs):
       if li == 1 and self.bm_eligible(r):  # level 2 only: deeper frames would outgrow LDS
         o.append(f"{ind}const uint32_t hb{t}_{j} = ({hvar} && V.v[{r.view}].n_hv && "
-                 f"{r.e} - {r.b} >= SRD_BM_MIN) ? srdj::heavy_slot(V.v[{r.view}], {r.b}) : ~0u;")
+                 f"{r.e} - {r.b} >= V.v[{r.view}].bm_min) ? srdj::heavy_slot(V.v[{r.view}], {r.b}) : ~0u;")
         bm_at[j] = len(frame)
         frame.append(f"hb{t}_{j}")
     nfr = len(frame)
