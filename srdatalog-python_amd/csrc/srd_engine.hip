@@ -264,6 +264,8 @@ struct srd_engine {
   uint64_t bitmap_bytes = uint64_t(1) << 30;  // SRDATALOG_BITMAP_BYTES (0: none)
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
+  bool op_profile = false;         // SRDATALOG_OP_PROFILE: host time per op, printed per run
+  std::map<std::string, std::pair<double, uint64_t>> op_times;
   // instance table of a packed batch: n_inst+1 ascending column-0 bounds
   srd::DevBuf<uint32_t> inst_bounds;
   uint32_t n_inst = 0;
@@ -979,7 +981,25 @@ static const char* op_name(OpKind k) {
   return "?";
 }
 
+static void exec_op_body(srd_engine& e, const Op& op, uint64_t iteration, int step_no);
+
+// With SRDATALOG_OP_PROFILE set, the host wall time of every op (launches,
+// and the waits of the ops that read sizes back) is summed per op and rule
+// and printed at the end of srdatalog_run.
 static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no) {
+  if (!e.op_profile) return exec_op_body(e, op, iteration, step_no);
+  const auto t0 = std::chrono::steady_clock::now();
+  exec_op_body(e, op, iteration, step_no);
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (op.kind == OpKind::Parallel) return;  // its children are counted
+  std::string key = std::string(op_name(op.kind)) + " " +
+                    (op.kind == OpKind::Pipeline ? op.rule : (op.rel >= 0 ? e.rels[op.rel].name : ""));
+  auto& slot = e.op_times[key];
+  slot.first += dt;
+  slot.second += 1;
+}
+
+static void exec_op_body(srd_engine& e, const Op& op, uint64_t iteration, int step_no) {
   if (e.trace) {
     std::fprintf(stderr, "[srd] step %d iter %llu %s %s\n", step_no, (unsigned long long)iteration,
                  op_name(op.kind), op.kind == OpKind::Pipeline ? op.rule.c_str()
@@ -1213,6 +1233,17 @@ static void run_program(srd_engine& e, uint64_t max_iters) {
   SRD_CHECK(hipStreamSynchronize(e.s));
   e.run_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   e.ran = true;
+  if (e.op_profile) {
+    std::vector<std::pair<std::string, std::pair<double, uint64_t>>> v(e.op_times.begin(),
+                                                                       e.op_times.end());
+    std::sort(v.begin(), v.end(), [](auto& a, auto& b) { return a.second.first > b.second.first; });
+    std::fprintf(stderr, "[srd] op profile of a %.3f s run (host ms, count):\n", e.run_seconds);
+    for (auto& x : v)
+      std::fprintf(stderr, "[srd]   %9.3f %7llu  %s\n", x.second.first * 1e3,
+                   (unsigned long long)x.second.second, x.first.c_str());
+    std::fflush(stderr);
+    e.op_times.clear();
+  }
   if (!e.quiet)
     for (auto& r : e.rels)
       if (r.print_size) {
@@ -1316,6 +1347,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     SRD_CHECK(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
     const char* tr = std::getenv("SRDATALOG_TRACE");
     e->trace = tr && *tr && *tr != '0';
+    const char* op = std::getenv("SRDATALOG_OP_PROFILE");
+    e->op_profile = op && *op && *op != '0';
     const char* q = std::getenv("SRDATALOG_QUIET");
     e->quiet = q && *q && *q != '0';
     if (const char* g = std::getenv("SRDATALOG_GRAIN")) e->grain_override = uint32_t(std::atoi(g));

@@ -143,6 +143,7 @@ struct Workspace {
   uint32_t mp_epoch = 0;
   uint32_t* pinned = nullptr;    // pinned host words for read-backs
   uint64_t pinned_words = 0;
+  uint8_t* stage_host = nullptr; // two pinned chunks for exports to pageable memory
   ~Workspace();
 };
 
