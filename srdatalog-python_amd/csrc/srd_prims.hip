@@ -150,6 +150,7 @@ std::vector<hipEvent_t> g_free_events;
 std::map<std::string, KernelTime> g_times;
 std::string g_cur_name;
 hipEvent_t g_cur_start;
+int g_depth = 0;  // nested scopes: only the outermost one is recorded
 
 hipEvent_t get_event() {
   if (!g_free_events.empty()) {
@@ -163,18 +164,23 @@ hipEvent_t get_event() {
 }
 }  // namespace
 
-void timing_enable(bool on) { g_timing = on; }
+void timing_enable(bool on) {
+  g_timing = on;
+  g_depth = 0;
+}
 bool timing_enabled() { return g_timing; }
 
 void timing_begin(const char* name, hipStream_t s) {
   if (!g_timing) return;
+  if (g_depth++ > 0) return;  // inside another timed scope: that one holds the time
   g_cur_name = name;
   g_cur_start = get_event();
   SRD_CHECK(hipEventRecord(g_cur_start, s));
 }
 
 void timing_end(hipStream_t s, double alg_bytes) {
-  if (!g_timing) return;
+  if (!g_timing || g_depth == 0) return;
+  if (--g_depth > 0) return;
   hipEvent_t e = get_event();
   SRD_CHECK(hipEventRecord(e, s));
   g_pending.push_back({g_cur_name, g_cur_start, e, alg_bytes});
