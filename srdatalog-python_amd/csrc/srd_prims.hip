@@ -885,13 +885,19 @@ __device__ __forceinline__ uint64_t wave_lookback(uint64_t* base, uint64_t strid
   return excl;
 }
 
+// Keys per thread of a onesweep tile (W = 1: 64-bit keys, W = 2: 128-bit).
+#ifndef SRD_OS_ITEMS1
+#define SRD_OS_ITEMS1 16
+#endif
+constexpr int os_items(int W) { return W == 1 ? SRD_OS_ITEMS1 : 8; }
+
 template <int W>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const uint64_t* __restrict__ in_lo, const uint64_t* __restrict__ in_hi,
     uint64_t* __restrict__ out_lo, uint64_t* __restrict__ out_hi, uint32_t n, uint32_t shift,
     const uint32_t* __restrict__ bin_start, uint64_t* __restrict__ lookback,
     uint32_t* __restrict__ tile_ctr, uint32_t epoch, uint32_t* __restrict__ err) {
-  constexpr int ITEMS = (W == 1) ? 16 : 8;
+  constexpr int ITEMS = os_items(W);
   constexpr uint32_t TILE = kBlock * ITEMS;
   __shared__ uint64_t s_keys[W][TILE];
   __shared__ uint32_t s_wcnt[kWavesPerBlock][kRadix];
@@ -1244,8 +1250,7 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
       for (int d = 0; d < kRadix; ++d)
         if (hp[p * kRadix + d] == n || already_sorted) trivial[p] = true;
 
-    const uint32_t items = (W == 1) ? 16 : 8;
-    const uint64_t tile = uint64_t(kBlock) * items;
+    const uint64_t tile = uint64_t(kBlock) * os_items(W);
     const uint64_t tiles = (n + tile - 1) / tile;
     uint64_t lb_words = tiles * kRadix;
     if (lb_words > ws.lookback.n) {
