@@ -688,7 +688,7 @@ static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs&
   e.item_pref.reserve(n + 1, e.s);
   a.item_cost = e.item_cost.p;
   a.grid_blocks = uint32_t(std::min<uint64_t>((a.n_slices + 3) / 4, 2048));
-  fill_u32(a.work_ctr, 0, 2, e.s);
+  fill_u32(a.work_ctr, 0, SRD_XCDS, e.s);
   std::string nm = std::string("join.weigh:") + d->rule;
   timing_begin(nm.c_str(), e.s);
   d->weigh(&a, e.s);
@@ -763,7 +763,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
     e.key_aux.reserve(nk, e.s);
     e.key_work.reserve(nk, e.s);
     e.work_start.reserve(nk + 1, e.s);
-    e.work_ctr.reserve(2, e.s);
+    e.work_ctr.reserve(SRD_XCDS, e.s);
     a.key_rng = e.key_rng.p;
     a.key_aux = e.key_aux.p;
     a.key_work = e.key_work.p;
@@ -817,7 +817,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         }
       };
       auto launch = [&](SrdLaunchFn fn, const char* phase) {
-        fill_u32(a.work_ctr, 0, 2, e.s);
+        fill_u32(a.work_ctr, 0, SRD_XCDS, e.s);
         std::string nm = std::string("join.") + phase + ":" + d->rule;
         timing_begin(nm.c_str(), e.s);
         fn(&a, e.s);
@@ -853,8 +853,8 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
               a.out[k].stride = stride;
               a.out[k].base = uint64_t(0) - start_of(s0);  // row = position - chunk start
             }
-            fill_u32(a.work_ctr, uint32_t(s0), 1, e.s);
-            fill_u32(a.work_ctr + 1, 0, 1, e.s);
+            fill_u32(a.work_ctr, 0, SRD_XCDS, e.s);
+            a.slice_lo = s0;
             a.n_slices = s1;
             std::string nm = std::string("join.") + phase + ":" + d->rule;
             timing_begin(nm.c_str(), e.s);
@@ -878,6 +878,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
           }
           s0 = s1;
         }
+        a.slice_lo = 0;
         a.n_slices = slices;
         if (e.trace) {
           std::fprintf(stderr, "[srd]   %s: %llu rows written in chunks of <= %llu\n", d->rule,

@@ -70,9 +70,23 @@ __device__ __forceinline__ uint32_t owner_of(const uint64_t* pref, uint64_t j) {
 // skip the branch (whose s is the constant 0) straight back to the
 // broadcast, forming a second loop in which lanes 1..63 re-read lane 0's
 // stale register and never fetch a new slice.
-__device__ __forceinline__ uint32_t next_slice(uint32_t* ctr) {
-  const uint32_t s = atomicAdd(ctr, srd::lane_id() == 0 ? 1u : 0u);
-  return __builtin_amdgcn_readlane(s, 0);
+// Slices [slice_lo, n_slices) are split into SRD_XCDS ranges; a wave takes
+// the next slice of its own XCD's range, then of the following ranges in
+// turn; it returns n_slices when every range is exhausted.
+__device__ __forceinline__ uint64_t next_slice(const SrdLaunchArgs& A) {
+  const uint64_t lo = A.slice_lo, n = A.n_slices;
+  if (n <= lo) return n;
+  const uint64_t span = n - lo;
+  const uint32_t home = blockIdx.x % SRD_XCDS;
+  for (uint32_t k = 0; k < SRD_XCDS; ++k) {
+    const uint32_t x = (home + k) % SRD_XCDS;
+    const uint64_t b = lo + span * x / SRD_XCDS, e = lo + span * (x + 1) / SRD_XCDS;
+    if (b >= e) continue;
+    const uint32_t t = atomicAdd(A.work_ctr + x, srd::lane_id() == 0 ? 1u : 0u);
+    const uint64_t s = b + __builtin_amdgcn_readlane(t, 0);
+    if (s < e) return s;
+  }
+  return n;
 }
 
 // Fused mode: open a staging segment of SRD_SEG rows for the wave (all lanes

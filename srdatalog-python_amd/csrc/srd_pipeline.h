@@ -84,7 +84,7 @@ struct SrdLaunchArgs {
   uint64_t* work_start;        // exclusive scan of key_work, n_root+1 entries
   uint32_t* counts;            // per slice tuple count
   uint64_t* offsets;           // exclusive scan of counts
-  uint32_t* work_ctr;          // atomic slice counter (zeroed before each launch)
+  uint32_t* work_ctr;          // SRD_XCDS slice counters (zeroed before each launch)
   SrdOut out[SRD_MAX_DESTS];
   // fused mode staging (column c of dest d at stage + (d*SRD_MAX_COLS + c)*stage_cap)
   uint32_t* stage;
@@ -98,7 +98,15 @@ struct SrdLaunchArgs {
   // (nullptr: [s*grain, min((s+1)*grain, total_work)))
   const uint64_t* slice_start;
   uint64_t* item_cost;         // weigh mode output, one entry per item
+  uint64_t slice_lo;           // the launch takes slices [slice_lo, n_slices)
 };
+
+// Slices are split into SRD_XCDS contiguous ranges, one per XCD (workgroups
+// are dispatched to the XCDs round-robin, so block b runs on XCD b % 8):
+// the waves of one XCD work through neighbouring slices, whose keys (and
+// their index ranges and bitmaps) then stay in that XCD's L2, and take
+// slices of the other ranges once their own is done.
+#define SRD_XCDS 8u
 
 #define SRD_ITEM_COST 4u       // level-1 cost of an item, in level-2 candidates
 #define SRD_BM_MIN 512u        // default shortest range given a heavy-key bitmap

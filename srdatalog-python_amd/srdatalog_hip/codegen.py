@@ -509,7 +509,7 @@ class PipelineGen:
     slots = self.root_slots()
     ns = len(slots)
     o += ["  for (;;) {",
-          "    const uint64_t slice = srdj::next_slice(A.work_ctr);  // wave-uniform",
+          "    const uint64_t slice = srdj::next_slice(A);  // wave-uniform",
           "    if (slice >= A.n_slices) break;",
           "    const uint64_t it0 = A.slice_start ? A.slice_start[slice] : slice * A.grain;",
           "    const uint64_t it1 = A.slice_start ? A.slice_start[slice + 1]"

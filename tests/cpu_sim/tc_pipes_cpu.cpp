@@ -74,8 +74,8 @@ void scan_root(const SrdLaunchArgs* a, hipStream_t) {
 template <bool SWAP, int MODE>
 void scan_run(const SrdLaunchArgs* a, hipStream_t) {
   const srd::IndexView& v = a->views.v[0];
-  // slices [*work_ctr, n_slices), as the device kernels take them
-  for (uint64_t s = a->work_ctr[0]; s < a->n_slices; ++s) {
+  // slices [slice_lo, n_slices), as the device kernels take them
+  for (uint64_t s = a->slice_lo; s < a->n_slices; ++s) {
     uint64_t lo = a->slice_start ? a->slice_start[s] : s * a->grain;
     uint64_t hi = a->slice_start ? a->slice_start[s + 1] : std::min<uint64_t>(lo + a->grain, a->total_work);
     Emit em{a, MODE};
@@ -108,8 +108,8 @@ void rec_root(const SrdLaunchArgs* a, hipStream_t) {
 
 template <int MODE>
 void rec_run(const SrdLaunchArgs* a, hipStream_t) {
-  // slices [*work_ctr, n_slices), as the device kernels take them
-  for (uint64_t s = a->work_ctr[0]; s < a->n_slices; ++s) {
+  // slices [slice_lo, n_slices), as the device kernels take them
+  for (uint64_t s = a->slice_lo; s < a->n_slices; ++s) {
     uint64_t lo = a->slice_start ? a->slice_start[s] : s * a->grain;
     uint64_t hi = a->slice_start ? a->slice_start[s + 1] : std::min<uint64_t>(lo + a->grain, a->total_work);
     Emit em{a, MODE};
