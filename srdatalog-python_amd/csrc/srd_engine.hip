@@ -197,6 +197,10 @@ struct Relation {
   DevBuf<uint32_t> raw;
   uint64_t raw_n = 0, raw_cap = 0;
   Order raw_order;
+  // raw rows written by one sorted-output pipeline (strictly increasing in
+  // raw_order), with per-column upper bounds: the index adopts them as is
+  bool raw_sorted = false;
+  std::vector<uint32_t> raw_bound;
   // input data (EDB)
   std::vector<uint32_t> host_rows;  // row-major
   uint64_t host_n = 0;
@@ -543,6 +547,7 @@ static const Index& new_index(srd_engine& e, Relation& r, const Order& order) {
     r.new_parts.clear();
     r.raw.release();
     r.raw_n = r.raw_cap = 0;
+    r.raw_sorted = false;
     if (order == po) {
       x = std::move(acc);
     } else {
@@ -554,8 +559,9 @@ static const Index& new_index(srd_engine& e, Relation& r, const Order& order) {
     std::vector<const uint32_t*> in(r.arity);
     for (uint32_t c = 0; c < r.arity; ++c) in[c] = r.raw.p + uint64_t(p[c]) * r.raw_cap;
     // NEW rows that a join emitted already in index order are adopted
-    build_index(in.data(), r.arity, r.raw_n, x, e.ws, e.s, nullptr,
-                r.raw_order == order ? &r.raw : nullptr, r.raw_cap);
+    const bool pre = r.raw_sorted && r.raw_order == order && r.raw_bound.size() == r.arity;
+    build_index(in.data(), r.arity, r.raw_n, x, e.ws, e.s, pre ? &r.raw_bound : nullptr,
+                r.raw_order == order ? &r.raw : nullptr, r.raw_cap, pre);
   } else if (!r.newv.empty()) {
     auto& any = *r.newv.begin();
     build_index_from(any.second, perm_between(any.first, order), x, e.ws, e.s);
@@ -591,6 +597,7 @@ static void clear_new(Relation& r) {
   r.newv.clear();
   r.raw.release();
   r.raw_n = r.raw_cap = 0;
+  r.raw_sorted = false;
   r.new_parts.clear();
 }
 
@@ -633,6 +640,7 @@ static void flush_raw(srd_engine& e, Relation& r) {
   const Order order = r.raw_order;
   r.raw.release();
   r.raw_n = r.raw_cap = 0;
+  r.raw_sorted = false;
   add_new_part(e, r, order, std::move(part));
 }
 
@@ -716,6 +724,7 @@ static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs&
 static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t iteration) {
   SrdLaunchArgs a;
   std::memset(&a, 0, sizeof(a));
+  std::vector<uint32_t> view_max[SRD_MAX_VIEWS];  // column maxima of the views read
   // Resolve views (iteration 0 reads FULL wherever the rule reads DELTA).
   for (uint32_t v = 0; v < d->n_views; ++v) {
     const SrdViewSpec& vs = d->views[v];
@@ -728,6 +737,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
       x = &delta_index(e, r, order);
     else
       x = &new_index(e, r, order);
+    view_max[v] = x->col_max;
     if (vs.want_bitmaps && x->arity == 2 && e.bitmap_bytes) {
       const HeavyBits* h = x->heavy.get();
       if (!h || h->built_for != x->cols.p || h->built_n != x->n) {
@@ -809,6 +819,15 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
           const SrdDestSpec& ds = d->dest[k];
           Relation& r = e.rels[rel_id(e, ds.rel)];
           if (r.raw_n > 0 && r.raw_n + n_rows > chunk_rows) flush_raw(e, r);
+          // rows of one sorted-output pipeline into an empty NEW stay an index;
+          // bounds: the largest value of any column the join reads
+          r.raw_sorted = r.raw_n == 0 && d->sorted_output && d->n_dest == 1;
+          if (r.raw_sorted) {
+            uint32_t bound = 0;
+            for (uint32_t v = 0; v < d->n_views; ++v)
+              for (uint32_t m : view_max[v]) bound = std::max(bound, m);
+            r.raw_bound.assign(ds.arity, bound);
+          }
           Order order(ds.cols, ds.cols + ds.arity);
           uint64_t base = reserve_new(e, r, order, n_rows);
           a.out[k].cols = r.raw.p;

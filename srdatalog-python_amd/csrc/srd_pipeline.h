@@ -133,6 +133,11 @@ struct SrdPipelineDesc {
   SrdLaunchFn fused;
   SrdLaunchFn direct;
   SrdLaunchFn weigh;                 // nullptr unless the rule has >= 2 nested levels
+  // The rule writes its single destination's rows strictly increasing in
+  // the destination's index order: every level is a column join binding the
+  // next index column, so tuples come out in (root key, level 1, level 2 ...)
+  // order with no repeats, in every mode.
+  uint32_t sorted_output;
 };
 
 #ifdef __cplusplus

@@ -1141,7 +1141,7 @@ static uint32_t bit_width(uint32_t v) { return v ? 32 - __builtin_clz(v) : 0; }
 
 void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace& ws,
                  hipStream_t s, const std::vector<uint32_t>* col_max, const DevBuf<uint32_t>* adopt,
-                 uint64_t adopt_stride) {
+                 uint64_t adopt_stride, bool presorted) {
   if (k == 0 || k > uint32_t(kMaxCols)) fail("build_index: arity must be in [1, 8]");
   if (n >= (uint64_t(1) << 32)) fail("build_index: more than 2^32-1 rows");
   out = Index();
@@ -1171,6 +1171,22 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
   uint32_t* hp = nullptr;
   if (col_max && col_max->size() == k) {
     mx = *col_max;
+    if (presorted) {
+      out.n = n;
+      out.col_max = mx;
+      if (adopt && adopt->p) {
+        out.cols = *adopt;
+        out.stride = adopt_stride;
+      } else {
+        out.stride = round_stride(n);
+        out.cols.alloc(out.stride * k, s);
+        for (uint32_t c = 0; c < k; ++c)
+          SRD_CHECK(hipMemcpyAsync(out.cols.p + uint64_t(c) * out.stride, in[c], n * 4,
+                                   hipMemcpyDeviceToDevice, s));
+      }
+      build_roots(out, ws, s);
+      return;
+    }
   } else {
     fill_u32(dmax, 0, kMaxCols + 1, s);
     {

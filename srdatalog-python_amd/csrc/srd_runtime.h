@@ -155,9 +155,12 @@ uint32_t* pinned_words(Workspace& ws, uint64_t words);
 // order check).  Rows already strictly increasing skip the sort entirely;
 // if `adopt` holds those same columns (column c at adopt->p + c*stride) the
 // index takes the buffer over without copying.
+// presorted: the caller guarantees the rows are strictly increasing and
+// col_max holds per-column upper bounds; only the root arrays are built.
 void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace& ws,
                  hipStream_t s, const std::vector<uint32_t>* col_max = nullptr,
-                 const DevBuf<uint32_t>* adopt = nullptr, uint64_t adopt_stride = 0);
+                 const DevBuf<uint32_t>* adopt = nullptr, uint64_t adopt_stride = 0,
+                 bool presorted = false);
 
 // Same, with the source columns permuted: out column c = src column perm[c].
 void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index& out,

@@ -212,6 +212,22 @@ class PipelineGen:
     return len(self.levels) == 1 and isinstance(self.levels[0], M.CartesianJoin)
 
   # ------------------------------------------------------------ ranges ---
+  def sorted_output(self) -> bool:
+    """Whether the rule writes its one destination strictly increasing in
+    the destination's index order: the root and every level are column
+    joins and the written row is exactly (root var, level-1 var, ...).  Keys
+    come in ascending order, each level's candidates are the distinct values
+    of a sorted range in ascending order, and emission keeps candidate order,
+    so rows come out sorted and without repeats in every mode."""
+    if isinstance(self.root, M.Scan) or len(self.dests) != 1:
+      return False
+    if not all(isinstance(lv, M.ColumnJoin) for lv in self.levels):
+      return False
+    d = self.dests[0]
+    row = [d.terms[relcol] for relcol in d.index]
+    want = [self.root.var] + [lv.var for lv in self.levels]
+    return all(isinstance(t, str) for t in row) and row == want
+
   def bm_eligible(self, r: Rg) -> bool:
     """A per-lane depth-1 range of a two-column view: its values are the
     last column, so a membership bitmap can replace the search."""
@@ -790,7 +806,8 @@ class PipelineGen:
             f"{1 if self.direct_ok() else 0}, "
             f"&{self.ns}::launch_root, &{self.ns}::launch_count, &{self.ns}::launch_write, "
             f"&{self.ns}::launch_fused, &{self.ns}::launch_direct, "
-            f"{'&' + self.ns + '::launch_weigh' if len(self.levels) >= 2 else 'nullptr'}" + "}")
+            f"{'&' + self.ns + '::launch_weigh' if len(self.levels) >= 2 else 'nullptr'}, "
+            f"{1 if self.sorted_output() else 0}" + "}")
 
 
 @dataclass

@@ -84,10 +84,14 @@ void build_roots(Index& x, Workspace&, hipStream_t s) {
 }
 
 void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, Workspace&,
-                 hipStream_t s, const std::vector<uint32_t>*, const DevBuf<uint32_t>*, uint64_t) {
+                 hipStream_t s, const std::vector<uint32_t>*, const DevBuf<uint32_t>*, uint64_t,
+                 bool presorted) {
   std::vector<Row> rows(n, Row(k));
   for (uint64_t i = 0; i < n; ++i)
     for (uint32_t c = 0; c < k; ++c) rows[i][c] = in[c][i];
+  if (presorted)  // the caller's promise: strictly increasing rows
+    for (uint64_t i = 1; i < n; ++i)
+      if (!(rows[i - 1] < rows[i])) throw std::runtime_error("build_index: presorted rows out of order");
   std::sort(rows.begin(), rows.end());
   rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
   fill_index(out, k, rows, s);
@@ -97,7 +101,7 @@ void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index
                       Workspace& ws, hipStream_t s) {
   std::vector<const uint32_t*> in(src.arity);
   for (uint32_t c = 0; c < src.arity; ++c) in[c] = src.n ? src.col(perm[c]) : nullptr;
-  build_index(in.data(), src.arity, src.n, out, ws, s, nullptr, nullptr, 0);
+  build_index(in.data(), src.arity, src.n, out, ws, s, nullptr, nullptr, 0, false);
 }
 
 void set_difference(const Index& a, const Index& b, Index& out, Workspace&, hipStream_t s) {

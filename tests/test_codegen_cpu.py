@@ -1,0 +1,38 @@
+"""Static properties the join code generator derives from a rule's plan:
+which rules write their rows already sorted and duplicate-free (the engine
+then adopts NEW as an index without checking it) and which views get
+heavy-key bitmaps.  A wrong "sorted" claim would corrupt NEW, so the claim
+is checked against plans where it must and must not hold."""
+
+from srdatalog_hip import codegen, compiler, programs
+
+
+def gens(prog):
+  decls = compiler.relation_decls(prog)
+  arity = {d.name: d.arity for d in decls}
+  mir = compiler.compile_to_mir(prog)
+  return {ep.rule: codegen.PipelineGen(ep, arity, i) for i, ep in enumerate(mir.pipelines())}
+
+
+def test_triangle_join_is_sorted_output():
+  g = gens(programs.PROGRAMS["triangle"]())
+  assert g["TriangleJoin"].sorted_output()
+  assert not g["KnowsLoad"].sorted_output()          # a scan root
+  assert g["TriangleJoin"].bm_views() == {0}          # Knows, searched at depth 1 by level 2
+
+
+def test_rules_with_permuted_or_projected_heads_are_not_sorted_output():
+  # none of these writes (root var, level-1 var, ...) in its destination's
+  # index order: TC's recursive rule is rooted at the join variable y, which
+  # its head drops; the others project join variables away or reorder them
+  for name in ("tc", "sg", "andersen", "q3", "q7", "neg2hop", "reach"):
+    for rule, g in gens(programs.PROGRAMS[name]()).items():
+      assert not g.sorted_output(), (name, rule)
+
+
+def test_generated_desc_carries_the_flags():
+  src = codegen.generate_program_source("triangle", compiler.relation_decls(programs.PROGRAMS["triangle"]()),
+                                        compiler.compile_to_mir(programs.PROGRAMS["triangle"]()))
+  line = next(l for l in src.splitlines() if l.lstrip().startswith('{"TriangleJoin"'))
+  assert line.rstrip().endswith("launch_weigh, 1}")
+  assert '{"Knows", 2, {0, 1}, SRD_FULL, 1}' in line
