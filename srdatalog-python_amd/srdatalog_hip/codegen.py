@@ -654,10 +654,12 @@ class PipelineGen:
         per_lane.append((len(frame), len(frame) + 1))
         frame += [r.b, r.e]
     bm_at = {}
+    bm_code = []  # emitted after the weigh mode's exit: weighing needs no bitmap slots
     for j, r in enumerate(rngs):
       if li == 1 and self.bm_eligible(r):  # level 2 only: deeper frames would outgrow LDS
-        o.append(f"{ind}const uint32_t hb{t}_{j} = ({hvar} && V.v[{r.view}].n_hv && "
-                 f"{r.e} - {r.b} >= V.v[{r.view}].bm_min) ? srdj::heavy_slot(V.v[{r.view}], {r.b}) : ~0u;")
+        bm_code.append(f"{ind}const uint32_t hb{t}_{j} = ({hvar} && V.v[{r.view}].n_hv && "
+                       f"{r.e} - {r.b} >= V.v[{r.view}].bm_min) ? "
+                       f"srdj::heavy_slot(V.v[{r.view}], {r.b}) : ~0u;")
         bm_at[j] = len(frame)
         frame.append(f"hb{t}_{j}")
     nfr = len(frame)
@@ -683,6 +685,7 @@ class PipelineGen:
       o.append(f"{ind}  if (act) A.item_cost[item] = sz{t} + SRD_ITEM_COST;")
       o.append(f"{ind}  continue;")
       o.append(f"{ind}}}")
+    o += bm_code
     o.append(f"{ind}const uint64_t inc{t} = srd::wave_incl_scan<uint64_t>(sz{t});")
     o.append(f"{ind}const uint64_t tot{t} = srdj::uniform_from(inc{t}, 63);")
     o.append(f"{ind}__builtin_amdgcn_wave_barrier();")
