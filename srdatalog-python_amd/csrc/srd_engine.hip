@@ -262,6 +262,8 @@ struct srd_engine {
   bool no_weigh = false;           // SRDATALOG_NO_WEIGH=1: uniform slices only
   uint64_t weigh_min = 4096;       // SRDATALOG_WEIGH_MIN: fewer items: uniform slices
   uint64_t weigh_target = 16384;   // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
+  uint32_t xcd_ranges = SRD_XCDS;  // SRDATALOG_XCD_RANGES: slice ranges of the join launches
+  uint32_t weigh_ranges = 1;       // SRDATALOG_WEIGH_RANGES: of the weigh launch
   srd::DevBuf<uint64_t> item_cost, item_pref, slice_start;
   // column-1 bitmaps of heavy keys for views nested levels search
   uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
@@ -695,6 +697,7 @@ static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs&
   e.item_cost.reserve(n, e.s);
   e.item_pref.reserve(n + 1, e.s);
   a.item_cost = e.item_cost.p;
+  a.n_ranges = e.weigh_ranges;
   a.grid_blocks = uint32_t(std::min<uint64_t>((a.n_slices + 3) / 4, 2048));
   fill_u32(a.work_ctr, 0, SRD_XCDS, e.s);
   std::string nm = std::string("join.weigh:") + d->rule;
@@ -714,6 +717,7 @@ static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs&
   a.slice_start = e.slice_start.p;
   a.n_slices = S;
   a.item_cost = nullptr;
+  a.n_ranges = e.xcd_ranges;
   if (e.trace) {
     std::fprintf(stderr, "[srd]   %s: weighed %llu items, cost %llu, %llu slices\n", d->rule,
                  (unsigned long long)n, (unsigned long long)w, (unsigned long long)S);
@@ -753,6 +757,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
     }
     a.views.v[v] = x->view();
   }
+  a.n_ranges = e.xcd_ranges;
   a.grain = e.grain_override ? e.grain_override : d->default_grain;
   if (d->root_kind == SRD_ROOT_JOIN) {
     uint32_t drv = d->root_view[0];
@@ -1376,6 +1381,10 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     if (const char* c = std::getenv("SRDATALOG_NEW_CHUNK_ROWS"))
       e->new_chunk_rows = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_NO_WEIGH")) e->no_weigh = *c && *c != '0';
+    if (const char* c = std::getenv("SRDATALOG_XCD_RANGES"))
+      e->xcd_ranges = std::min<uint32_t>(std::max<uint32_t>(uint32_t(std::atoi(c)), 1), SRD_XCDS);
+    if (const char* c = std::getenv("SRDATALOG_WEIGH_RANGES"))
+      e->weigh_ranges = std::min<uint32_t>(std::max<uint32_t>(uint32_t(std::atoi(c)), 1), SRD_XCDS);
     if (const char* c = std::getenv("SRDATALOG_BITMAP_MIN"))
       e->bitmap_min = std::max<uint32_t>(uint32_t(std::strtoul(c, nullptr, 10)), 16);
     if (const char* c = std::getenv("SRDATALOG_BITMAP_BYTES")) e->bitmap_bytes = std::strtoull(c, nullptr, 10);

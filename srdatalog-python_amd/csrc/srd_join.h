@@ -77,10 +77,11 @@ __device__ __forceinline__ uint64_t next_slice(const SrdLaunchArgs& A) {
   const uint64_t lo = A.slice_lo, n = A.n_slices;
   if (n <= lo) return n;
   const uint64_t span = n - lo;
-  const uint32_t home = blockIdx.x % SRD_XCDS;
-  for (uint32_t k = 0; k < SRD_XCDS; ++k) {
-    const uint32_t x = (home + k) % SRD_XCDS;
-    const uint64_t b = lo + span * x / SRD_XCDS, e = lo + span * (x + 1) / SRD_XCDS;
+  const uint32_t nr = A.n_ranges ? A.n_ranges : 1u;
+  const uint32_t home = blockIdx.x % nr;
+  for (uint32_t k = 0; k < nr; ++k) {
+    const uint32_t x = (home + k) % nr;
+    const uint64_t b = lo + span * x / nr, e = lo + span * (x + 1) / nr;
     if (b >= e) continue;
     const uint32_t t = atomicAdd(A.work_ctr + x, srd::lane_id() == 0 ? 1u : 0u);
     const uint64_t s = b + __builtin_amdgcn_readlane(t, 0);

@@ -99,6 +99,8 @@ struct SrdLaunchArgs {
   const uint64_t* slice_start;
   uint64_t* item_cost;         // weigh mode output, one entry per item
   uint64_t slice_lo;           // the launch takes slices [slice_lo, n_slices)
+  uint32_t n_ranges;           // slice ranges, 1..SRD_XCDS (0 is taken as 1)
+  uint32_t pad2_;
 };
 
 // Slices are split into SRD_XCDS contiguous ranges, one per XCD (workgroups
