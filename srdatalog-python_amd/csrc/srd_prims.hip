@@ -1364,28 +1364,13 @@ void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index
 }
 
 // ============================================================== roots ====
-__global__ __launch_bounds__(kBlock) void root_count_kernel(const uint32_t* __restrict__ c0,
-                                                            uint64_t n,
-                                                            uint32_t* __restrict__ tile_r) {
-  __shared__ uint32_t sr[kWavesPerBlock];
-  const uint32_t w = threadIdx.x / kWave, l = lane_id();
-  const uint64_t base = uint64_t(blockIdx.x) * kTile;
-  uint32_t cr = 0;
-  for (int i = 0; i < kItems; ++i) {
-    uint64_t idx = base + tile_pos(w, i, l);
-    bool f = idx < n && (idx == 0 || c0[idx] != c0[idx - 1]);
-    cr += __popcll(ballot(f));
-  }
-  if (l == 0) sr[w] = cr;
-  __syncthreads();
-  if (threadIdx.x == 0) tile_r[blockIdx.x] = sr[0] + sr[1] + sr[2] + sr[3];
-}
-
-__global__ __launch_bounds__(kBlock) void root_write_kernel(const uint32_t* __restrict__ c0,
-                                                            uint64_t n,
-                                                            const uint64_t* __restrict__ roff,
-                                                            uint32_t* __restrict__ root_vals,
-                                                            uint32_t* __restrict__ root_offs) {
+// One read of column 0: each tile writes the positions of its key starts
+// at its own first row of `pos` (a tile has at most kTile of them) and their
+// count; root_gather_kernel packs them after a scan of the counts.
+__global__ __launch_bounds__(kBlock) void root_mark_kernel(const uint32_t* __restrict__ c0,
+                                                           uint64_t n,
+                                                           uint32_t* __restrict__ pos,
+                                                           uint32_t* __restrict__ tile_r) {
   __shared__ uint32_t sr[kWavesPerBlock];
   const uint32_t w = threadIdx.x / kWave, l = lane_id();
   const uint64_t base = uint64_t(blockIdx.x) * kTile;
@@ -1393,24 +1378,39 @@ __global__ __launch_bounds__(kBlock) void root_write_kernel(const uint32_t* __re
   uint32_t cr = 0;
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    uint64_t idx = base + tile_pos(w, i, l);
-    bool f = idx < n && (idx == 0 || c0[idx] != c0[idx - 1]);
+    const uint64_t idx = base + tile_pos(w, i, l);
+    const bool f = idx < n && (idx == 0 || c0[idx] != c0[idx - 1]);
     m[i] = ballot(f);
     cr += __popcll(m[i]);
   }
   if (l == 0) sr[w] = cr;
   __syncthreads();
-  uint64_t pr = roff[blockIdx.x];
-  for (uint32_t j = 0; j < w; ++j) pr += sr[j];
+  uint32_t pr = 0, tot = 0;
+  for (uint32_t j = 0; j < kWavesPerBlock; ++j) {
+    if (j < w) pr += sr[j];
+    tot += sr[j];
+  }
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    if ((m[i] >> l) & 1) {
-      uint64_t idx = base + tile_pos(w, i, l);
-      uint64_t g = pr + popc_below(m[i]);
-      root_vals[g] = c0[idx];
-      root_offs[g] = uint32_t(idx);
-    }
+    if ((m[i] >> l) & 1) pos[base + pr + popc_below(m[i])] = uint32_t(base + tile_pos(w, i, l));
     pr += __popcll(m[i]);
+  }
+  if (threadIdx.x == 0) tile_r[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void root_gather_kernel(const uint32_t* __restrict__ c0,
+                                                             const uint32_t* __restrict__ pos,
+                                                             const uint32_t* __restrict__ tile_r,
+                                                             const uint64_t* __restrict__ roff,
+                                                             uint32_t* __restrict__ root_vals,
+                                                             uint32_t* __restrict__ root_offs) {
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  const uint64_t dst = roff[blockIdx.x];
+  const uint32_t cnt = tile_r[blockIdx.x];
+  for (uint32_t j = threadIdx.x; j < cnt; j += kBlock) {
+    const uint32_t p = pos[base + j];
+    root_offs[dst + j] = p;
+    root_vals[dst + j] = c0[p];
   }
 }
 
@@ -1423,9 +1423,10 @@ void build_roots(Index& x, Workspace& ws, hipStream_t s) {
   const uint64_t tiles = (x.n + kTile - 1) / kTile;
   DevBuf<uint32_t> tr(tiles, s);
   DevBuf<uint64_t> sr(tiles + 1, s);
+  DevBuf<uint32_t> pos(x.n, s);
   {
-    TimeScope t("roots", s, double(x.n) * 8);
-    root_count_kernel<<<uint32_t(tiles), kBlock, 0, s>>>(x.col(0), x.n, tr.p);
+    TimeScope t("roots", s, double(x.n) * 4);
+    root_mark_kernel<<<uint32_t(tiles), kBlock, 0, s>>>(x.col(0), x.n, pos.p, tr.p);
     SRD_CHECK(hipGetLastError());
   }
   scan_impl<uint32_t>(tr.p, sr.p, tiles, s);
@@ -1436,9 +1437,9 @@ void build_roots(Index& x, Workspace& ws, hipStream_t s) {
   x.root_vals.alloc(x.r, s);
   x.root_offs.alloc(x.r + 1, s);
   {
-    TimeScope t("roots", s, double(x.r) * 8);
-    root_write_kernel<<<uint32_t(tiles), kBlock, 0, s>>>(x.col(0), x.n, sr.p, x.root_vals.p,
-                                                         x.root_offs.p);
+    TimeScope t("roots", s, double(x.r) * 12);
+    root_gather_kernel<<<uint32_t(tiles), kBlock, 0, s>>>(x.col(0), pos.p, tr.p, sr.p,
+                                                          x.root_vals.p, x.root_offs.p);
     SRD_CHECK(hipGetLastError());
   }
   fill_u32(x.root_offs.p + x.r, uint32_t(x.n), 1, s);
