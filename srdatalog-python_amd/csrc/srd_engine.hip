@@ -263,7 +263,7 @@ struct srd_engine {
   uint64_t weigh_min = 4096;       // SRDATALOG_WEIGH_MIN: fewer items: uniform slices
   uint64_t weigh_target = 16384;   // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
   uint32_t xcd_ranges = SRD_XCDS;  // SRDATALOG_XCD_RANGES: slice ranges of the join launches
-  uint32_t weigh_ranges = 1;       // SRDATALOG_WEIGH_RANGES: of the weigh launch
+  uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
   srd::DevBuf<uint64_t> item_cost, item_pref, slice_start;
   // column-1 bitmaps of heavy keys for views nested levels search
   uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
@@ -699,7 +699,7 @@ static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs&
   a.item_cost = e.item_cost.p;
   a.n_ranges = e.weigh_ranges;
   a.grid_blocks = uint32_t(std::min<uint64_t>((a.n_slices + 3) / 4, 2048));
-  fill_u32(a.work_ctr, 0, SRD_XCDS, e.s);
+  fill_u32(a.work_ctr, 0, SRD_XCDS * SRD_CTR_STRIDE, e.s);
   std::string nm = std::string("join.weigh:") + d->rule;
   timing_begin(nm.c_str(), e.s);
   d->weigh(&a, e.s);
@@ -778,7 +778,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
     e.key_aux.reserve(nk, e.s);
     e.key_work.reserve(nk, e.s);
     e.work_start.reserve(nk + 1, e.s);
-    e.work_ctr.reserve(SRD_XCDS, e.s);
+    e.work_ctr.reserve(SRD_XCDS * SRD_CTR_STRIDE, e.s);
     a.key_rng = e.key_rng.p;
     a.key_aux = e.key_aux.p;
     a.key_work = e.key_work.p;
@@ -841,7 +841,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         }
       };
       auto launch = [&](SrdLaunchFn fn, const char* phase) {
-        fill_u32(a.work_ctr, 0, SRD_XCDS, e.s);
+        fill_u32(a.work_ctr, 0, SRD_XCDS * SRD_CTR_STRIDE, e.s);
         std::string nm = std::string("join.") + phase + ":" + d->rule;
         timing_begin(nm.c_str(), e.s);
         fn(&a, e.s);
@@ -877,7 +877,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
               a.out[k].stride = stride;
               a.out[k].base = uint64_t(0) - start_of(s0);  // row = position - chunk start
             }
-            fill_u32(a.work_ctr, 0, SRD_XCDS, e.s);
+            fill_u32(a.work_ctr, 0, SRD_XCDS * SRD_CTR_STRIDE, e.s);
             a.slice_lo = s0;
             a.n_slices = s1;
             std::string nm = std::string("join.") + phase + ":" + d->rule;

@@ -83,7 +83,7 @@ __device__ __forceinline__ uint64_t next_slice(const SrdLaunchArgs& A) {
     const uint32_t x = (home + k) % nr;
     const uint64_t b = lo + span * x / nr, e = lo + span * (x + 1) / nr;
     if (b >= e) continue;
-    const uint32_t t = atomicAdd(A.work_ctr + x, srd::lane_id() == 0 ? 1u : 0u);
+    const uint32_t t = atomicAdd(A.work_ctr + x * SRD_CTR_STRIDE, srd::lane_id() == 0 ? 1u : 0u);
     const uint64_t s = b + __builtin_amdgcn_readlane(t, 0);
     if (s < e) return s;
   }

@@ -84,7 +84,7 @@ struct SrdLaunchArgs {
   uint64_t* work_start;        // exclusive scan of key_work, n_root+1 entries
   uint32_t* counts;            // per slice tuple count
   uint64_t* offsets;           // exclusive scan of counts
-  uint32_t* work_ctr;          // SRD_XCDS slice counters (zeroed before each launch)
+  uint32_t* work_ctr;          // SRD_XCDS slice counters, SRD_CTR_STRIDE words apart (zeroed per launch)
   SrdOut out[SRD_MAX_DESTS];
   // fused mode staging (column c of dest d at stage + (d*SRD_MAX_COLS + c)*stage_cap)
   uint32_t* stage;
@@ -109,6 +109,7 @@ struct SrdLaunchArgs {
 // their index ranges and bitmaps) then stay in that XCD's L2, and take
 // slices of the other ranges once their own is done.
 #define SRD_XCDS 8u
+#define SRD_CTR_STRIDE 32u     // one 128-byte line per counter: atomics on one line serialise
 
 #define SRD_ITEM_COST 4u       // level-1 cost of an item, in level-2 candidates
 #define SRD_BM_MIN 512u        // default shortest range given a heavy-key bitmap
