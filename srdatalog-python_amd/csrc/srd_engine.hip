@@ -931,15 +931,15 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         const uint64_t rec_cap = slices + cap / (SRD_SEG - 64) + 64;
         e.recs.reserve(3 * rec_cap, e.s);
         e.stage_ctr.reserve(2, e.s);
-        e.overflow.reserve(2, e.s);
+        e.overflow.reserve(2 * SRD_CTR_STRIDE, e.s);  // flag and record counter on own lines
         fill_u64(e.stage_ctr.p, 0, 2, e.s);
-        fill_u32(e.overflow.p, 0, 2, e.s);
+        fill_u32(e.overflow.p, 0, 2 * SRD_CTR_STRIDE, e.s);
         a.stage = e.stage.p;
         a.stage_cap = cap;
         a.stage_ctr = reinterpret_cast<unsigned long long*>(e.stage_ctr.p);
         a.recs = e.recs.p;
         a.rec_cap = rec_cap;
-        a.rec_ctr = e.overflow.p + 1;
+        a.rec_ctr = e.overflow.p + SRD_CTR_STRIDE;
         a.overflow = e.overflow.p;
         launch(d->fused, "fused");
         exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);

@@ -98,8 +98,8 @@ __device__ __forceinline__ uint64_t seg_open(const SrdLaunchArgs& A) {
   const unsigned long long r =
       atomicAdd(A.stage_ctr, srd::lane_id() == 0 ? (unsigned long long)SRD_SEG : 0ull);
   const uint64_t s = srd::read_lane_u64(r, 0);
-  const bool ovf = s + SRD_SEG > A.stage_cap;
-  atomicOr(A.overflow, (ovf && srd::lane_id() == 0) ? 1u : 0u);
+  const bool ovf = s + SRD_SEG > A.stage_cap;  // wave-uniform
+  if (ovf) atomicOr(A.overflow, srd::lane_id() == 0 ? 1u : 0u);
   return ovf ? ~uint64_t(0) : s;
 }
 
@@ -110,8 +110,8 @@ __device__ __forceinline__ void seg_close(const SrdLaunchArgs& A, uint64_t slice
   if (start == ~uint64_t(0) || used == 0) return;
   const uint32_t r = atomicAdd(A.rec_ctr, srd::lane_id() == 0 ? 1u : 0u);
   const uint32_t idx = __builtin_amdgcn_readlane(r, 0);
-  const bool ovf = idx >= A.rec_cap;
-  atomicOr(A.overflow, (ovf && srd::lane_id() == 0) ? 1u : 0u);
+  const bool ovf = idx >= A.rec_cap;  // wave-uniform
+  if (ovf) atomicOr(A.overflow, srd::lane_id() == 0 ? 1u : 0u);
   if (!ovf && srd::lane_id() == 0) {
     A.recs[3ull * idx] = slice;
     A.recs[3ull * idx + 1] = start;
