@@ -114,7 +114,7 @@ struct SrdLaunchArgs {
 #define SRD_ITEM_COST 4u       // level-1 cost of an item, in level-2 candidates
 #define SRD_BM_MIN 512u        // default shortest range given a heavy-key bitmap
 
-#define SRD_SEG 1024u          // staging segment size (tuples)
+#define SRD_SEG 2048u          // staging segment size (tuples)
 
 typedef void (*SrdLaunchFn)(const SrdLaunchArgs* args, hipStream_t stream);
 
