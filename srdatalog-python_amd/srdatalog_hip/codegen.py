@@ -514,7 +514,11 @@ class PipelineGen:
   # ------------------------------------------------------- slice kernel ---
   def gen_run(self) -> str:
     o = ["template <int MODE>",
-         "__global__ __launch_bounds__(256) void run_k(SrdLaunchArgs A) {",
+         # eight waves per SIMD: the persistent grid launches 8 blocks per CU,
+         # and with more than ~100 SGPRs a wave the eighth block could only
+         # start when another finished (the compiler spills SGPRs to VGPR lanes)
+         "__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) "
+         "void run_k(SrdLaunchArgs A) {",
          "  const SrdViews& V = A.views;",
          "  const uint32_t lane = srd::lane_id();"]
     deep = len(self.levels) >= 2
