@@ -270,7 +270,21 @@ __global__ __launch_bounds__(kBlock) void gather_segments_kernel(GatherArgs g) {
       for (uint32_t c = 0; c < D.k; ++c) {
         const uint32_t* sc = g.stage + (uint64_t(d) * 8 + c) * g.stage_stride + src;
         uint32_t* dc = D.cols + uint64_t(c) * D.stride + D.base + dst0;
-        for (uint64_t j = l; j < cnt; j += kWave) dc[j] = sc[j];
+        // eight loads in flight per lane before the stores that use them
+        constexpr int U = 8;
+        for (uint64_t j0 = 0; j0 < cnt; j0 += uint64_t(kWave) * U) {
+          uint32_t v[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint64_t j = j0 + uint64_t(u) * kWave + l;
+            v[u] = j < cnt ? sc[j] : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint64_t j = j0 + uint64_t(u) * kWave + l;
+            if (j < cnt) dc[j] = v[u];
+          }
+        }
       }
     }
   }
