@@ -438,15 +438,21 @@ __global__ __launch_bounds__(kBlock) void heavy_scatter_kernel(
   }
 }
 
-// Block i sets the bits of heavy key i's column-1 values.
+// Blocks kHeavyParts*i .. kHeavyParts*i+kHeavyParts-1 set the bits of heavy
+// key i's column-1 values, each over one part of its rows (the longest keys
+// hold tens of thousands of rows).
+constexpr uint32_t kHeavyParts = 8;
 __global__ __launch_bounds__(kBlock) void heavy_set_kernel(const uint32_t* __restrict__ col1,
                                                            const uint32_t* __restrict__ starts,
                                                            const uint32_t* __restrict__ ends,
                                                            uint32_t words,
                                                            uint32_t* __restrict__ bits) {
-  const uint32_t i = blockIdx.x;
+  const uint32_t i = blockIdx.x / kHeavyParts, part = blockIdx.x % kHeavyParts;
   uint32_t* b = bits + uint64_t(i) * words;
-  for (uint32_t row = starts[i] + threadIdx.x; row < ends[i]; row += kBlock) {
+  const uint32_t s = starts[i], len = ends[i] - s;
+  const uint32_t lo = s + uint32_t(uint64_t(len) * part / kHeavyParts);
+  const uint32_t hi = s + uint32_t(uint64_t(len) * (part + 1) / kHeavyParts);
+  for (uint32_t row = lo + threadIdx.x; row < hi; row += kBlock) {
     const uint32_t v = col1[row];
     atomicOr(&b[v >> 5], 1u << (v & 31u));
   }
@@ -481,7 +487,8 @@ void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, W
       SRD_CHECK(hipGetLastError());
       h.bits.alloc(uint64_t(n) * words, s);
       fill_u32(h.bits.p, 0, uint64_t(n) * words, s);
-      heavy_set_kernel<<<n, kBlock, 0, s>>>(x.col(1), h.starts.p, ends.p, words, h.bits.p);
+      heavy_set_kernel<<<n * kHeavyParts, kBlock, 0, s>>>(x.col(1), h.starts.p, ends.p, words,
+                                                           h.bits.p);
       SRD_CHECK(hipGetLastError());
       h.n = n;
       h.words = words;
