@@ -42,7 +42,10 @@ def parse():
                  help="CPU baseline: root keys x in [0, K) of the same graph")
   p.add_argument("--no-cpu-baseline", action="store_true")
   p.add_argument("--no-parity", action="store_true")
-  p.add_argument("--workload", choices=("triangle", "batch", "serve"), default="triangle",
+  p.add_argument("--andersen-div", type=int, default=10,
+                 help="andersen workload: run 1/div of the full Andersen input sizes")
+  p.add_argument("--workload", choices=("triangle", "batch", "serve", "tc", "andersen"),
+                 default="triangle",
                  help="triangle: the headline triangle-listing fixpoint; batch: the uneven "
                       "512-instance TC/SG batch, batched vs plain per-instance; serve: the "
                       "Poisson request stream, continuous batching vs plain serial")
@@ -270,6 +273,61 @@ def bench_serve(args) -> dict:
   }
 
 
+def bench_fixpoint(args) -> dict:
+  """Fixpoint wall time of a recursive program with resident inputs:
+  tc = transitive closure of 5,000 distinct uniform pairs over 1,000
+  vertices (self-loops dropped); andersen = the Andersen points-to program
+  on 1/div of its full input sizes (200,000 variables and 50,000 objects at
+  div 1; AddressOf 100k, Assign 400k, Load 100k, Store 100k pairs, Zipf 1.1
+  endpoints).  Parity: every per-iteration delta and the final relation
+  against the C++ oracle when it finishes in seconds (tc; andersen at
+  div >= 30)."""
+  import numpy as np
+
+  from srdatalog_hip import datasets, programs
+  if args.workload == "tc":
+    inputs = {"ArcInput": datasets.tc_edges(args.seed, 1000, 5000)}
+    prog, rel, arity, label = "tc", "Path", 2, "transitive closure, 5000 edges over 1000 vertices"
+  else:
+    d = args.andersen_div
+    inputs = datasets.andersen_inputs(args.seed, n_vars=200_000 // d, n_objs=50_000 // d,
+                                      sizes=(100_000 // d, 400_000 // d, 100_000 // d,
+                                             100_000 // d))
+    prog, rel, arity = "andersen", "PointsTo", 2
+    label = f"Andersen points-to at 1/{d} of the full input sizes"
+  lib = programs.load(prog)
+  times = []
+  for i in range(args.warmup + args.steps):
+    if lib.init() != 0:
+      raise RuntimeError("srdatalog_init failed")
+    for r, rows in inputs.items():
+      lib.load_rows(r, rows)
+    lib.prepare()
+    t = time.perf_counter()
+    if lib.run(0) != 0:
+      raise RuntimeError("srdatalog_run failed")
+    dt = time.perf_counter() - t
+    if i >= args.warmup:
+      times.append(dt)
+    if i == args.warmup + args.steps - 1:
+      got = lib.export(rel, arity)
+      deltas = lib.delta_sizes(rel)
+    lib.shutdown()
+  sec = float(np.median(times))
+  parity = None
+  if not args.no_parity and (args.workload == "tc" or args.andersen_div >= 30):
+    import oracle
+    want = oracle.run(prog, inputs)
+    parity = bool(np.array_equal(got, want.relations[rel]) and deltas == want.deltas[rel])
+  return {"metric": "fixpoint s + derived tuples/s", "value": round(got.shape[0] / sec, 1),
+          "unit": "derived tuples/s", "fixpoint_s": round(sec, 4), "steps": args.steps,
+          "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3), "n_gpus": 1,
+          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+          "data": "synthetic (seeded PCG64), generated in-process",
+          "config": {"workload": label, "tuples": int(got.shape[0]), "iterations": len(deltas),
+                     "deltas": deltas, "parity_vs_oracle": parity}}
+
+
 def main() -> int:
   args = parse()
   if args.gpus != 1:
@@ -286,6 +344,10 @@ def main() -> int:
                            "cap_tuples": args.batch_cap, "budget_bytes": args.batch_budget}})
     print(json.dumps(res))
     return 0 if res["outputs_identical"] else 1
+  if args.workload in ("tc", "andersen"):
+    res = bench_fixpoint(args)
+    print(json.dumps(res))
+    return 0 if res["config"]["parity_vs_oracle"] is not False else 1
   if args.workload == "serve":
     res = bench_serve(args)
     res.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
