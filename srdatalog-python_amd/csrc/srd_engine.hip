@@ -260,7 +260,10 @@ struct srd_engine {
   uint64_t new_chunk_rows = 0;     // SRDATALOG_NEW_CHUNK_ROWS: fixed NEW chunk (tests)
   // cost-balanced slicing of rules with >= 2 nested levels
   bool no_weigh = false;           // SRDATALOG_NO_WEIGH=1: uniform slices only
-  uint64_t weigh_min = 4096;       // SRDATALOG_WEIGH_MIN: fewer items: uniform slices
+  // fewer items: uniform slices.  Low on purpose: a rule with few level-1
+  // items can still carry most of an iteration's work (Andersen's Store rule
+  // at small scales: ~13 uniform slices for seconds of join)
+  uint64_t weigh_min = 64;         // SRDATALOG_WEIGH_MIN
   uint64_t weigh_target = 16384;   // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
   uint32_t xcd_ranges = SRD_XCDS;  // SRDATALOG_XCD_RANGES: slice ranges of the join launches
   uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
