@@ -1,14 +1,15 @@
 // Device helpers used by the generated per-rule join kernels.
 //
 // Execution model of a generated kernel (see srd_pipeline.h for the launch
-// protocol): a wave64 owns one slice = (root key, [lo,hi) of the first
-// nested level's items).  Items are processed 64 at a time, one per lane.
-// If the rule has a second nested level, the lanes that produced a binding
-// publish it in LDS and the wave walks the concatenation of their level-2
-// iteration spaces 64 items at a time (load-balanced search over a 64-entry
-// prefix array), so a lane with a huge level-2 range does not serialize the
-// wave.  Every emitted tuple is one lane's result for one item; ballot +
-// mbcnt give its position, so emission never diverges.
+// protocol): a wave64 owns one slice, a contiguous range of the flattened
+// level-1 items (possibly spanning keys).  Items are processed 64 at a time,
+// one per lane.  If the rule has a second nested level, the lanes that
+// produced a binding publish it in LDS and the wave walks the concatenation
+// of their level-2 iteration spaces 64 items at a time (each position's
+// owner found from per-batch start flags, a ballot and a popcount), so a
+// lane with a huge level-2 range does not serialize the wave.  Every emitted
+// tuple is one lane's result for one item; ballot + mbcnt give its position,
+// so emission never diverges.
 //
 // Counterparts in the reference: the warp-cooperative WCOJ body emitted by
 // src/srdatalog/codegen/jit/pipeline.py:360 / root.py / instructions.py over
@@ -49,18 +50,6 @@ __device__ __forceinline__ Rng run_at(const uint32_t* __restrict__ col, uint32_t
                                       uint32_t v, bool last) {
   if (last) return Rng{pos, pos + 1};
   return Rng{pos, srd::upper_bound_la(col, pos + 1, e, v)};
-}
-
-// Owner lane of flattened item j given the wave's 65-entry exclusive
-// prefix array (pref[64] = total).
-__device__ __forceinline__ uint32_t owner_of(const uint64_t* pref, uint64_t j) {
-  uint32_t lo = 0, hi = 64;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (pref[mid] <= j) lo = mid;
-    else hi = mid;
-  }
-  return lo;
 }
 
 // Persistent-wave slice dispenser.  Every lane executes the atomic (lane 0

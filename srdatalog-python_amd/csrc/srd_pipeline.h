@@ -6,9 +6,10 @@
 //          key up in every root source, record the ranges, and size the
 //          first nested level's iteration space (key_work);
 //   then the per-key work is flattened into one global item space (an
-//   exclusive scan of key_work) cut into fixed-size slices that persistent
-//   waves pull from an atomic counter, so a slice may span many small keys
-//   or a fraction of one huge key.  The traversal runs in one of four modes:
+//   exclusive scan of key_work) cut into slices -- `grain` items each, or,
+//   after a weigh launch, of about equal level-2 cost -- that persistent
+//   waves pull from per-XCD counters, so a slice may span many small keys
+//   or a fraction of one huge key.  The traversal runs in one of these modes:
 //   direct (3)  the rule emits exactly one tuple per item (last level is a
 //               filter-free cartesian product or scan): tuple i goes to row
 //               base+i of NEW, one pass, no counting;
