@@ -1947,10 +1947,17 @@ static void set_difference_k(const Index& a, const Index& b, Index& out, Workspa
     out.cols.alloc(out.stride * K, s);
     {
       TimeScope t("setdiff.merge_path", s, double(a.n + b.n) * 4 * K);
-      // SRDATALOG_SETDIFF = gather (default) | lookback | twopass; read per
-      // call so tests can switch variants within one process
+      // SRDATALOG_SETDIFF = gather | lookback | twopass (read per call so
+      // tests can switch variants within one process).  Default: the
+      // gathered form for large inputs; below 2^22 rows the single-pass
+      // look-back form, whose two launches cost less than the gathered
+      // form's scan, gather and scratch allocation
       const char* v = std::getenv("SRDATALOG_SETDIFF");
-      const std::string variant = v ? v : (std::getenv("SRDATALOG_SETDIFF_TWO_PASS") ? "twopass" : "gather");
+      const std::string variant =
+          v ? v
+            : (std::getenv("SRDATALOG_SETDIFF_TWO_PASS")
+                   ? "twopass"
+                   : (a.n + b.n < (uint64_t(1) << 22) ? "lookback" : "gather"));
       if (variant == "twopass")
         set_difference_two_pass<K>(a, b, out.cols.p, out.stride, ws, s);
       else if (variant == "lookback")
