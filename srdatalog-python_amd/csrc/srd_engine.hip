@@ -264,7 +264,11 @@ struct srd_engine {
   // items can still carry most of an iteration's work (Andersen's Store rule
   // at small scales: ~13 uniform slices for seconds of join)
   uint64_t weigh_min = 64;         // SRDATALOG_WEIGH_MIN
-  uint64_t weigh_target = 16384;   // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
+  // smaller slices keep fewer keys (and their bitmaps) active per XCD at a
+  // time; 8192 candidates and up to 48 slices per wave measured best on the
+  // triangle join (16384/24: +0.8 ms, 2048/192: +4.5 ms)
+  uint64_t weigh_target = 8192;    // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
+  uint64_t slices_per_wave = 48;   // SRDATALOG_SLICES_PER_WAVE: cap of the weighed slices
   uint32_t xcd_ranges = SRD_XCDS;  // SRDATALOG_XCD_RANGES: slice ranges of the join launches
   uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
   srd::DevBuf<uint64_t> item_cost, item_pref, slice_start;
@@ -666,14 +670,16 @@ static uint32_t stage_cols(const SrdPipelineDesc* d) {
 
 // Size the fused mode's staging buffer for a launch of `slices` slices: the
 // expected output (the rule's largest so far, +25%) plus one segment per
-// slice, which may stay partly filled, clamped to stage_max_bytes.  A launch
+// wave, which may stay partly filled, clamped to stage_max_bytes.  A launch
 // that still overflows falls back to the write pass over the counts it
 // produced.  Returns false, leaving the two-pass mode to run, when the
 // buffer cannot be allocated.
 static bool stage_ready(srd_engine& e, const SrdPipelineDesc* d, uint64_t slices) {
   const uint32_t kmax = stage_cols(d);
   uint64_t cap = std::max<uint64_t>(e.stage_hint[d->rule], uint64_t(1) << 24);
-  cap = std::max<uint64_t>(cap, slices * SRD_SEG + (uint64_t(1) << 20));
+  // a wave keeps one segment open across its slices: at most one partly
+  // filled segment per wave of the persistent grid
+  cap = std::max<uint64_t>(cap, std::min<uint64_t>(slices, 4 * 2048) * SRD_SEG + (uint64_t(1) << 20));
   cap = std::min<uint64_t>(cap, e.stage_max_bytes / (4ull * kmax));
   if (e.stage_cap_env) cap = std::max<uint64_t>(e.stage_cap_env, 1);
   if (e.stage.n >= cap * kmax) return true;
@@ -710,9 +716,9 @@ static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs&
   timing_end(e.s);
   exclusive_scan_u64(a.item_cost, e.item_pref.p, n, e.ws, e.s);
   const uint64_t w = read_u64(e, e.item_pref.p + n);
-  // about 24 slices per persistent wave of a full grid (4 x 2048), never
+  // up to e.slices_per_wave slices per persistent wave of a full grid (4 x 2048), never
   // fewer than the uniform slicing had, never more than one per item
-  const uint64_t cap = std::max<uint64_t>(a.n_slices, 24ull * 4 * 2048);
+  const uint64_t cap = std::max<uint64_t>(a.n_slices, e.slices_per_wave * 4 * 2048);
   uint64_t S = std::min<uint64_t>(std::max<uint64_t>(a.n_slices, w / e.weigh_target), cap);
   S = std::max<uint64_t>(std::min<uint64_t>(S, n), 1);
   e.slice_start.reserve(S + 1, e.s);
@@ -953,7 +959,8 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         total = uint64_t(h[0]) | (uint64_t(h[1]) << 32);
         const bool ovf = h[2] != 0;
         e.stage_hint[d->rule] = std::max<uint64_t>(e.stage_hint[d->rule],
-                                                   total + total / 4 + slices * SRD_SEG);
+                                                   total + total / 4 +
+                                                       std::min<uint64_t>(slices, 4 * 2048) * SRD_SEG);
         if (total > 0) {
           if (!ovf) {
             bind_dests(total);
@@ -1392,6 +1399,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
       e->bitmap_min = std::max<uint32_t>(uint32_t(std::strtoul(c, nullptr, 10)), 16);
     if (const char* c = std::getenv("SRDATALOG_BITMAP_BYTES")) e->bitmap_bytes = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_WEIGH_MIN")) e->weigh_min = std::strtoull(c, nullptr, 10);
+    if (const char* c = std::getenv("SRDATALOG_SLICES_PER_WAVE"))
+      e->slices_per_wave = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);
     if (const char* c = std::getenv("SRDATALOG_WEIGH_TARGET"))
       e->weigh_target = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);
     if (const char* c = std::getenv("SRDATALOG_STAGE_MAX_BYTES"))

@@ -179,9 +179,10 @@ class PipelineGen:
                      f" = {self.var(d.terms[relcol])};")
     lines += [f"{ind}  }}",
               f"{ind}  if (MODE == 2 && c_) {{",
-              f"{ind}    if (!seg_dead && (seg_start == ~uint64_t(0) || seg_used + c_ > SRD_SEG)) {{",
+              f"{ind}    if (!seg_dead && (seg_start == ~uint64_t(0) || seg_used + c_ > seg_room)) {{",
               f"{ind}      srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
               f"{ind}      seg_start = srdj::seg_open(A);",
+              f"{ind}      seg_room = SRD_SEG;",
               f"{ind}      seg_used = 0;",
               f"{ind}      seg_local = cnt;",
               f"{ind}      seg_dead = seg_start == ~uint64_t(0);",
@@ -528,7 +529,12 @@ class PipelineGen:
     lds_at = len(o)
     slots = self.root_slots()
     ns = len(slots)
-    o += ["  for (;;) {",
+    o += ["  // fused mode: the wave's open staging segment, kept across slices (a",
+          "  // slice's rows in it are one record; the next slice goes on behind them)",
+          "  uint64_t seg_start = ~uint64_t(0);",
+          "  uint32_t seg_room = 0;",
+          "  bool seg_dead = false;  // staging exhausted: count only (host re-runs the write pass)",
+          "  for (;;) {",
           "    const uint64_t slice = srdj::next_slice(A);  // wave-uniform",
           "    if (slice >= A.n_slices) break;",
           "    const uint64_t it0 = A.slice_start ? A.slice_start[slice] : slice * A.grain;",
@@ -536,8 +542,6 @@ class PipelineGen:
           " : min(it0 + A.grain, A.total_work);",
           "    uint64_t opos = MODE == 1 ? A.offsets[slice] : 0;",
           "    uint32_t cnt = 0;",
-          "    uint64_t seg_start = ~uint64_t(0);",
-          "    bool seg_dead = false;  // staging exhausted: count only (host re-runs the write pass)",
           "    uint32_t seg_used = 0, seg_local = 0;",
           "    for (uint64_t base = it0; base < it1; base += 64) {",
           "      const uint64_t item = base + lane;",
@@ -599,7 +603,10 @@ class PipelineGen:
     else:
       self.gen_nested(1, reg, bound, done, o, ind, "h1")
     o += ["    }",
-          "    if (MODE == 2) srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
+          "    if (MODE == 2) {",
+          "      srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
+          "      if (seg_start != ~uint64_t(0)) { seg_start += seg_used; seg_room -= seg_used; }",
+          "    }",
           "    if ((MODE == 0 || MODE == 2) && lane == 0) A.counts[slice] = cnt;",
           "  }",
           "}"]
