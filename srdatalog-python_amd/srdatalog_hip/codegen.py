@@ -75,7 +75,7 @@ class PipelineGen:
   """Emits the kernels of one pipeline."""
 
   def __init__(self, ep: M.ExecutePipeline, arity: dict[str, int], uid: int,
-               coop_search: bool = False):
+               coop_search: bool = False, ring: bool = True):
     self.ep = ep
     # run-narrowed cooperative searches (srdj::seg_lower_bound) instead of one
     # independent binary search per lane; measured slower on the triangle
@@ -111,6 +111,10 @@ class PipelineGen:
     # view ids in source order
     for s in ep.sources():
       self.info.view_id(s.rel, s.index, s.ver)
+    # fused mode stages its rows through a per-wave LDS ring (decided in
+    # gen_run from the LDS the level frames leave)
+    self.ring = False
+    self.ring_ok = ring
 
   # ------------------------------------------------------------ helpers --
   def vid(self, s) -> int:
@@ -179,21 +183,38 @@ class PipelineGen:
                      f" = {self.var(d.terms[relcol])};")
     lines += [f"{ind}  }}",
               f"{ind}  if (MODE == 2 && c_) {{",
-              f"{ind}    if (!seg_dead && (seg_start == ~uint64_t(0) || seg_used + c_ > seg_room)) {{",
-              f"{ind}      srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
+              f"{ind}    if (!seg_dead && (seg_start == ~uint64_t(0) || seg_used + c_ > seg_room)) {{"]
+    if self.ring:
+      lines.append(f"{ind}      if (seg_start != ~uint64_t(0)) ring_out(seg_start + seg_used, false);")
+    lines += [f"{ind}      srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
               f"{ind}      seg_start = srdj::seg_open(A);",
               f"{ind}      seg_room = SRD_SEG;",
               f"{ind}      seg_used = 0;",
               f"{ind}      seg_local = cnt;",
               f"{ind}      seg_dead = seg_start == ~uint64_t(0);",
               f"{ind}    }}",
-              f"{ind}    if (seg_start != ~uint64_t(0) && ({cond})) {{",
-              f"{ind}      const uint64_t o_ = seg_start + seg_used + srd::popc_below(m_);"]
-    for di, d in enumerate(self.dests):
-      for c, relcol in enumerate(d.index):
-        lines.append(f"{ind}      A.stage[{di * 8 + c}ull * A.stage_cap + o_] = {self.var(d.terms[relcol])};")
-    lines += [f"{ind}    }}",
-              f"{ind}    seg_used += c_;",
+              ]
+    if self.ring:
+      lines += [f"{ind}    if (seg_start != ~uint64_t(0)) {{",
+                f"{ind}      const uint64_t p0_ = seg_start + seg_used;",
+                f"{ind}      if ({cond}) {{",
+                f"{ind}        const uint32_t r_ = uint32_t(p0_ + srd::popc_below(m_)) & 127u;"]
+      j = 0
+      for di, d in enumerate(self.dests):
+        for c, relcol in enumerate(d.index):
+          lines.append(f"{ind}        ring_s[wv][{j}][r_] = {self.var(d.terms[relcol])};")
+          j += 1
+      lines += [f"{ind}      }}",
+                f"{ind}      if (((p0_ + c_) >> 6) != (p0_ >> 6)) ring_out(p0_, true);",
+                f"{ind}    }}"]
+    else:
+      lines += [f"{ind}    if (seg_start != ~uint64_t(0) && ({cond})) {{",
+                f"{ind}      const uint64_t o_ = seg_start + seg_used + srd::popc_below(m_);"]
+      for di, d in enumerate(self.dests):
+        for c, relcol in enumerate(d.index):
+          lines.append(f"{ind}      A.stage[{di * 8 + c}ull * A.stage_cap + o_] = {self.var(d.terms[relcol])};")
+      lines += [f"{ind}    }}"]
+    lines += [f"{ind}    seg_used += c_;",
               f"{ind}  }}",
               f"{ind}  opos += c_;",
               f"{ind}  cnt += c_;",
@@ -524,9 +545,35 @@ class PipelineGen:
          "  const uint32_t lane = srd::lane_id();"]
     deep = len(self.levels) >= 2
     self._lds = []
-    if deep:
+    if deep or self.ring:
       o += ["  const uint32_t wv = threadIdx.x / 64u;"]
     lds_at = len(o)
+    ring_cols = sum(len(d.index) for d in self.dests)
+    if self.ring:
+      # Fused mode: the row for staging position p waits in ring_s[wv][.][p & 127]
+      # until its 64-row block is complete, and the block is then written
+      # with one full-width store per column.  Rows scattered straight to
+      # staging a few at a time left partly written cache lines, which the
+      # L2 evicted and wrote back more than once.  Segments start at
+      # multiples of SRD_SEG, so blocks never straddle two segments.
+      self._lds.append(f"  __shared__ uint32_t ring_s[4][{ring_cols}][128];")
+      o += ["  // write out the block holding staging position p: all of it (full) or",
+            "  // its rows below p (a segment's last, partly filled block)",
+            "  auto ring_out = [&](uint64_t p, bool full) {",
+            "    const uint64_t b_ = p & ~uint64_t(63);",
+            "    const uint32_t n_ = full ? 64u : uint32_t(p & 63u);",
+            "    __builtin_amdgcn_wave_barrier();",
+            "    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");",
+            "    if (lane < n_) {",
+            "      const uint32_t r_ = uint32_t(b_ + lane) & 127u;"]
+      j = 0
+      for di, d in enumerate(self.dests):
+        for c in range(len(d.index)):
+          o.append(f"      A.stage[{di * 8 + c}ull * A.stage_cap + b_ + lane] = ring_s[wv][{j}][r_];")
+          j += 1
+      o += ["    }",
+            "    __builtin_amdgcn_wave_barrier();",
+            "  };"]
     slots = self.root_slots()
     ns = len(slots)
     o += ["  // fused mode: the wave's open staging segment, kept across slices (a",
@@ -608,8 +655,10 @@ class PipelineGen:
           "      if (seg_start != ~uint64_t(0)) { seg_start += seg_used; seg_room -= seg_used; }",
           "    }",
           "    if ((MODE == 0 || MODE == 2) && lane == 0) A.counts[slice] = cnt;",
-          "  }",
-          "}"]
+          "  }"]
+    if self.ring:
+      o += ["  if (MODE == 2 && seg_start != ~uint64_t(0)) ring_out(seg_start, false);"]
+    o += ["}"]
     lds_bytes = 0
     for d in self._lds:
       dims = [int(x) for x in re.findall(r"\[(\d+)\]", d)]
@@ -621,6 +670,15 @@ class PipelineGen:
     if lds_bytes > 64 * 1024:
       raise CodegenError(f"{self.ep.rule}: the join's level frames need {lds_bytes} bytes of LDS "
                          "per block (limit 65536); split the rule")
+    # the ring only where eight blocks per CU still fit the CU's 160 KiB of LDS
+    if (not self.ring and self.ring_ok and not self.direct_ok()
+        and 8 * (lds_bytes + 4 * ring_cols * 128 * 4) <= 160 * 1024):
+      self.ring = True
+      try:
+        return self.gen_run()
+      except Exception:
+        self.ring = False
+        raise
     o[lds_at:lds_at] = self._lds
     return "\n".join(o)
 
