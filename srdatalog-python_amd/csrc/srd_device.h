@@ -193,6 +193,40 @@ __device__ __forceinline__ uint32_t upper_bound_la(const uint32_t* __restrict__ 
   return bound_la<true>(a, lo, hi, v);
 }
 
+// Largest k in [0, n) with start[k] <= x (start has n+1 ascending entries).
+__device__ __forceinline__ uint32_t find_slot(const uint64_t* __restrict__ start, uint32_t n,
+                                              uint64_t x) {
+  uint32_t lo = 0, hi = n;  // answer in [lo, hi)
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (start[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// find_slot for a search that starts from a known slot: the largest k in
+// [lo, n) with start[k] <= x, given start[lo] <= x.  It gallops from lo
+// (lo+1, lo+2, lo+4, ...) before bisecting, so an answer d slots away costs
+// about 2*log2(d) loads; consecutive batches of a slice move a few slots.
+__device__ __forceinline__ uint32_t find_slot_from(const uint64_t* __restrict__ start, uint32_t lo,
+                                                   uint32_t n, uint64_t x) {
+  uint32_t step = 1;
+  uint32_t hi = lo + 1;
+  while (hi < n && start[hi] <= x) {
+    lo = hi;
+    step <<= 1;
+    hi = (n - lo > step) ? lo + step : n;
+  }
+  if (hi > n) hi = n;
+  while (hi - lo > 1) {  // answer in [lo, hi)
+    const uint32_t mid = (lo + hi) >> 1;
+    if (start[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ const uint32_t* col(const IndexView& v, int c) {
   return v.cols + uint64_t(c) * v.stride;
 }

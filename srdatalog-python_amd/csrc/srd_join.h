@@ -32,17 +32,9 @@ __device__ __forceinline__ const uint32_t* colp(const IndexView& v, int c) {
   return v.cols + uint64_t(c) * v.stride;
 }
 
-// Largest k in [0, n) with start[k] <= x (start has n+1 ascending entries).
-__device__ __forceinline__ uint32_t find_slot(const uint64_t* __restrict__ start, uint32_t n,
-                                              uint64_t x) {
-  uint32_t lo = 0, hi = n;  // answer in [lo, hi)
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (start[mid] <= x) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
+// find_slot, find_slot_from: srd_device.h
+using srd::find_slot;
+using srd::find_slot_from;
 
 // Equal range of the value at driver position pos inside [b,e) of column
 // col (values within [b,e) sorted); `last` = column is the index's last.

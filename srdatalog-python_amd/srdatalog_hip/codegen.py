@@ -590,12 +590,15 @@ class PipelineGen:
           "    uint64_t opos = MODE == 1 ? A.offsets[slice] : 0;",
           "    uint32_t cnt = 0;",
           "    uint32_t seg_used = 0, seg_local = 0;",
+          "    uint32_t kc = ~0u;  // key of the previous batch's last item",
           "    for (uint64_t base = it0; base < it1; base += 64) {",
           "      const uint64_t item = base + lane;",
           "      const bool act = item < it1;",
           "      const uint64_t last_item = min(base + 63, it1 - 1);",
-          "      const uint32_t klo = srdj::find_slot(A.work_start, A.n_root, base);",
-          "      const uint32_t khi = srdj::find_slot(A.work_start, A.n_root, last_item) + 1;",
+          "      const uint32_t klo = kc == ~0u ? srdj::find_slot(A.work_start, A.n_root, base)",
+          "                                     : srdj::find_slot_from(A.work_start, kc, A.n_root, base);",
+          "      const uint32_t khi = srdj::find_slot_from(A.work_start, klo, A.n_root, last_item) + 1;",
+          "      kc = khi - 1;",
           "      uint32_t k = klo;",
           "      if (act && khi - klo > 1) {",
           "        uint32_t lo = klo, hi = khi;",

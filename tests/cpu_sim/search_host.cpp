@@ -12,3 +12,8 @@ extern "C" uint32_t host_bound_la(const uint32_t* a, uint32_t lo, uint32_t hi, u
 extern "C" uint32_t host_bound(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t v, int upper) {
   return upper ? srd::upper_bound(a, lo, hi, v) : srd::lower_bound(a, lo, hi, v);
 }
+
+extern "C" uint32_t host_find_slot(const uint64_t* start, uint32_t lo, uint32_t n, uint64_t x,
+                                   int from) {
+  return from ? srd::find_slot_from(start, lo, n, x) : srd::find_slot(start, n, x);
+}

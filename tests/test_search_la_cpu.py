@@ -54,3 +54,29 @@ def test_bounds_every_length_and_probe(lib):
       for v in range(0, 110):
         assert lib.host_bound_la(p, lo, hi, v, 0) == bisect.bisect_left(al, v, lo, hi)
         assert lib.host_bound_la(p, lo, hi, v, 1) == bisect.bisect_right(al, v, lo, hi)
+
+
+def test_find_slot_and_gallop_match_bisect(lib):
+  """srd::find_slot (bisection) and srd::find_slot_from (gallop from a slot
+  known to be at or below the answer) -- the join kernels' item -> key
+  lookup over the scanned per-key work -- against bisect: the largest k in
+  [0, n) with start[k] <= x, keys with no work (equal starts) included."""
+  f = lib.host_find_slot
+  f.restype = ctypes.c_uint32
+  f.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32, ctypes.c_uint32,
+                ctypes.c_uint64, ctypes.c_int]
+  rng = np.random.default_rng(1)
+  for _ in range(4000):
+    n = int(rng.integers(1, 300))
+    work = rng.integers(0, int(rng.choice([2, 5, 100])), n).astype(np.uint64)
+    work[0] = max(int(work[0]), 1)
+    start = np.zeros(n + 1, dtype=np.uint64)
+    start[1:] = np.cumsum(work)
+    p = start.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    total = int(start[n])
+    for _ in range(8):
+      x = int(rng.integers(0, total))
+      want = bisect.bisect_right(start[:n].tolist(), x) - 1
+      assert f(p, 0, n, x, 0) == want
+      lo = int(rng.integers(0, want + 1))   # any slot whose start is <= x
+      assert f(p, lo, n, x, 1) == want, (start.tolist(), lo, x)
