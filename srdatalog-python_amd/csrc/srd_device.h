@@ -39,6 +39,11 @@ struct IndexView {
   const uint32_t* hv_start;
   uint32_t n_hv;
   uint32_t bm_words;
+  // Dense map of the root values: root_pos[x] = g when root_vals[g] == x,
+  // ~0u when x is not a root value, for x <= root_max; null when absent.
+  const uint32_t* root_pos;
+  uint32_t root_max;
+  uint32_t pad_;
 };
 
 struct Rng {
@@ -229,6 +234,12 @@ __device__ __forceinline__ uint32_t find_slot_from(const uint64_t* __restrict__ 
 
 __device__ __forceinline__ const uint32_t* col(const IndexView& v, int c) {
   return v.cols + uint64_t(c) * v.stride;
+}
+
+// Position of key among the root values through the dense map (~0u when
+// absent); v.root_pos must be set.
+__device__ __forceinline__ uint32_t root_pos_of(const IndexView& v, uint32_t key) {
+  return key <= v.root_max ? v.root_pos[key] : ~0u;
 }
 
 // Rows whose column 0 equals key (empty range if absent), via root arrays.

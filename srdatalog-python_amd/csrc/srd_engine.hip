@@ -274,6 +274,7 @@ struct srd_engine {
   srd::DevBuf<uint64_t> item_cost, item_pref, slice_start;
   // column-1 bitmaps of heavy keys for views nested levels search
   uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
+  bool root_pos = true;                    // SRDATALOG_ROOT_POS=0: search root values instead
   // SRDATALOG_BITMAP_BYTES (0: none).  The bitmaps are probed at random, so
   // their total is held to a few hundred MB, the size of the last-level
   // cache: on the R-MAT scale-20 triangle join 308 MB of bitmaps (threshold
@@ -766,6 +767,14 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
                        x->heavy->n, x->heavy->words);
           std::fflush(stderr);
         }
+      }
+    }
+    if (vs.want_root_pos && e.root_pos) {
+      const RootPos* m = x->rootpos.get();
+      if (!m || m->built_for != x->root_vals.p || m->built_r != x->r) {
+        timing_begin("join.root_pos", e.s);
+        build_root_pos(*x, std::max<uint64_t>(8ull * x->r, uint64_t(1) << 20), e.s);
+        timing_end(e.s);
       }
     }
     a.views.v[v] = x->view();
@@ -1402,6 +1411,7 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     if (const char* c = std::getenv("SRDATALOG_BITMAP_MIN"))
       e->bitmap_min = std::max<uint32_t>(uint32_t(std::strtoul(c, nullptr, 10)), 16);
     if (const char* c = std::getenv("SRDATALOG_BITMAP_BYTES")) e->bitmap_bytes = std::strtoull(c, nullptr, 10);
+    if (const char* c = std::getenv("SRDATALOG_ROOT_POS")) e->root_pos = !(*c == '0' && c[1] == 0);
     if (const char* c = std::getenv("SRDATALOG_WEIGH_MIN")) e->weigh_min = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_SLICES_PER_WAVE"))
       e->slices_per_wave = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);

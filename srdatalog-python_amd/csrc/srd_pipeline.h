@@ -53,6 +53,7 @@ struct SrdViewSpec {
   uint32_t cols[SRD_MAX_COLS];  // index column order
   uint32_t ver;                 // SrdVersion
   uint32_t want_bitmaps;        // a nested level searches this two-column view at depth 1
+  uint32_t want_root_pos;       // a nested level looks keys up among this view's root values
 };
 
 struct SrdDestSpec {

@@ -500,6 +500,35 @@ void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, W
   }
 }
 
+// pos[vals[g]] = g; values beyond the map (a column-0 bound that does not
+// hold) are skipped rather than written out of bounds.
+__global__ __launch_bounds__(kBlock) void root_pos_kernel(const uint32_t* __restrict__ vals,
+                                                          uint32_t r, uint64_t entries,
+                                                          uint32_t* __restrict__ pos) {
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= r) return;
+  const uint32_t v = vals[g];
+  if (v < entries) pos[v] = g;
+}
+
+void build_root_pos(const Index& x, uint64_t max_entries, hipStream_t s) {
+  if (!x.rootpos) x.rootpos = std::make_shared<RootPos>();
+  RootPos& m = *x.rootpos;
+  m.ok = false;
+  m.built_for = x.root_vals.p;
+  m.built_r = x.r;
+  if (x.r == 0 || x.col_max.empty()) return;
+  const uint64_t entries = uint64_t(x.col_max[0]) + 1;  // bound on every root value
+  if (entries > max_entries) return;
+  m.pos.reserve(entries, s);
+  fill_u32(m.pos.p, ~0u, entries, s);
+  root_pos_kernel<<<(x.r + kBlock - 1) / kBlock, kBlock, 0, s>>>(x.root_vals.p, x.r, entries,
+                                                                 m.pos.p);
+  SRD_CHECK(hipGetLastError());
+  m.max = uint32_t(entries - 1);
+  m.ok = true;
+}
+
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t s) {
   if (n_inst == 0) return;

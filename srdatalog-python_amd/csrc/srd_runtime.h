@@ -93,6 +93,15 @@ struct HeavyBits {
   uint64_t built_n = 0;
 };
 
+// Dense map value -> position among an index's root values.
+struct RootPos {
+  DevBuf<uint32_t> pos;  // max + 1 entries
+  uint32_t max = 0;
+  bool ok = false;
+  const uint32_t* built_for = nullptr;  // root_vals buffer and count it describes
+  uint32_t built_r = 0;
+};
+
 struct Index {
   uint32_t arity = 0;
   uint64_t n = 0;
@@ -103,6 +112,7 @@ struct Index {
   DevBuf<uint32_t> root_offs;  // r + 1
   std::vector<uint32_t> col_max;  // per-column maximum (0 when empty)
   mutable std::shared_ptr<HeavyBits> heavy;  // derived data, shared by copies
+  mutable std::shared_ptr<RootPos> rootpos;
 
   IndexView view() const {
     IndexView v;
@@ -119,6 +129,11 @@ struct Index {
     v.n_hv = hb ? heavy->n : 0;
     v.bm_words = hb ? heavy->words : 0;
     v.bm_min = hb ? heavy->min_len : ~0u;
+    const bool rp = rootpos && rootpos->ok && rootpos->built_for == root_vals.p &&
+                    rootpos->built_r == r;
+    v.root_pos = rp ? rootpos->pos.p : nullptr;
+    v.root_max = rp ? rootpos->max : 0;
+    v.pad_ = 0;
     return v;
   }
   const uint32_t* col(uint32_t c) const { return cols.p + uint64_t(c) * stride; }
@@ -221,6 +236,10 @@ void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, u
 // has bit v set when row (key_i, v) exists.  Fills x.heavy.
 void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, Workspace& ws,
                          hipStream_t s);
+
+// Dense root-value map of x (x.rootpos) when its column-0 bound (col_max[0])
+// gives at most max_entries entries; otherwise the map is marked absent.
+void build_root_pos(const Index& x, uint64_t max_entries, hipStream_t s);
 
 // Cost-balanced slice bounds: pref holds n+1 ascending item-cost prefix sums
 // (pref[n] = w, every item costing at least 1); out[s] (s = 0..S) = first

@@ -264,6 +264,16 @@ class PipelineGen:
           out.add(self.vid(s_))
     return out
 
+  def root_pos_views(self) -> set:
+    """Views whose root values a nested join level searches for a key."""
+    out = set()
+    for lv in self.levels:
+      if isinstance(lv, M.ColumnJoin) and len(lv.sources) > 1:
+        for s_ in lv.sources:
+          if not s_.prefix:
+            out.add(self.vid(s_))
+    return out
+
   def resolve(self, reg: dict, v: int, prefix: tuple, code: list[str], ind: str, tag: str) -> Rg:
     """Range of view v with the given bound prefix, narrowing if needed."""
     key = (v, prefix)
@@ -451,22 +461,36 @@ class PipelineGen:
         code.append(f"{i2}const uint32_t b_ = lo_ ? uint32_t({ba}) : uint32_t({bb});")
         code.append(f"{i2}const uint32_t e_ = lo_ ? uint32_t({ea}) : uint32_t({eb});")
         ha, hb_ = rngs[t].bm or "~0u", rngs[t + 1].bm or "~0u"
+        va, vb = rngs[t].view, rngs[t + 1].view
+        code.append(f"{i2}bool hit_ = false;")
+        code.append(f"{i2}uint32_t lb_ = 0u;")
+        search = [f"lb_ = {tag}on ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;",
+                  f"hit_ = {tag}on && lb_ < e_ && col_[lb_] == {var};"]
+        if rngs[t].full0 or rngs[t + 1].full0:
+          # a key looked up among a view's root values: one load from the
+          # view's dense root map when it has one
+          pa = f"V.v[{va}].root_pos" if rngs[t].full0 else "nullptr"
+          pb = f"V.v[{vb}].root_pos" if rngs[t + 1].full0 else "nullptr"
+          ma = f"V.v[{va}].root_max" if rngs[t].full0 else "0u"
+          mb = f"V.v[{vb}].root_max" if rngs[t + 1].full0 else "0u"
+          search = [f"const uint32_t* rp_ = lo_ ? {pa} : {pb};",
+                    f"const uint32_t rm_ = lo_ ? {ma} : {mb};",
+                    "if (rp_ != nullptr) {",
+                    f"  const uint32_t q_ = ({tag}on && {var} <= rm_) ? rp_[{var}] : ~0u;",
+                    "  hit_ = q_ != ~0u;",
+                    "  lb_ = hit_ ? q_ : 0u;",
+                    "} else {"] + ["  " + x for x in search] + ["}"]
         if rngs[t].bm or rngs[t + 1].bm:
           # a heavy key's range: one bitmap probe instead of the search (the
           # range is the view's last column, so its child range is unused)
-          va, vb = rngs[t].view, rngs[t + 1].view
           code.append(f"{i2}const uint32_t hs_ = lo_ ? {ha} : {hb_};")
-          code.append(f"{i2}bool hit_ = false;")
-          code.append(f"{i2}uint32_t lb_ = 0u;")
           code.append(f"{i2}if (hs_ != ~0u) {{")
           code.append(f"{i2}  hit_ = {tag}on && srdj::heavy_has(lo_ ? V.v[{va}] : V.v[{vb}], hs_, {var});")
           code.append(f"{i2}}} else {{")
-          code.append(f"{i2}  lb_ = {tag}on ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;")
-          code.append(f"{i2}  hit_ = {tag}on && lb_ < e_ && col_[lb_] == {var};")
+          code += [f"{i2}  {x}" for x in search]
           code.append(f"{i2}}}")
         else:
-          code.append(f"{i2}const uint32_t lb_ = {tag}on ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;")
-          code.append(f"{i2}const bool hit_ = {tag}on && lb_ < e_ && col_[lb_] == {var};")
+          code += [f"{i2}{x}" for x in search]
         code.append(f"{i2}if ({tag}on) {{")
         code.append(f"{i2}  {tag}on = hit_;")
         code.append(f"{i2}  if (hit_) {{")
@@ -862,10 +886,11 @@ class PipelineGen:
   def gen_desc(self) -> str:
     views = []
     bmv = self.bm_views()
+    rpv = self.root_pos_views() if not self.coop_search else set()
     for vi, (rel, index, ver) in enumerate(self.info.views):
       cols = ", ".join(str(c) for c in index)
       views.append(f"{{\"{rel}\", {len(index)}, {{{cols}}}, {VER_CODE[ver]}, "
-                   f"{1 if vi in bmv else 0}}}")
+                   f"{1 if vi in bmv else 0}, {1 if vi in rpv else 0}}}")
     if isinstance(self.root, M.Scan):
       kind, roots = "SRD_ROOT_SCAN", [self.vid(self.root)]
     else:

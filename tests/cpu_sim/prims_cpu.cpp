@@ -192,6 +192,24 @@ void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, W
   }
 }
 
+void build_root_pos(const Index& x, uint64_t max_entries, hipStream_t s) {
+  if (!x.rootpos) x.rootpos = std::make_shared<RootPos>();
+  RootPos& m = *x.rootpos;
+  m.ok = false;
+  m.built_for = x.root_vals.p;
+  m.built_r = x.r;
+  if (x.r == 0 || x.col_max.empty()) return;
+  const uint64_t entries = uint64_t(x.col_max[0]) + 1;
+  if (entries > max_entries) return;
+  for (uint32_t g = 0; g < x.r; ++g)
+    if (x.root_vals.p[g] > x.col_max[0]) throw std::runtime_error("build_root_pos: bound below a root value");
+  m.pos.reserve(entries, s);
+  std::fill(m.pos.p, m.pos.p + entries, ~0u);
+  for (uint32_t g = 0; g < x.r; ++g) m.pos.p[x.root_vals.p[g]] = g;
+  m.max = uint32_t(entries - 1);
+  m.ok = true;
+}
+
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t) {
   for (uint32_t g = 0; g < n_inst; ++g) {

@@ -19,6 +19,8 @@ def test_triangle_join_is_sorted_output():
   assert g["TriangleJoin"].sorted_output()
   assert not g["KnowsLoad"].sorted_output()          # a scan root
   assert g["TriangleJoin"].bm_views() == {0}          # Knows, searched at depth 1 by level 2
+  assert g["TriangleJoin"].root_pos_views() == {0}    # Knows(y, _): level 1 looks y up
+  assert g["KnowsLoad"].root_pos_views() == set()
 
 
 def test_rules_with_permuted_or_projected_heads_are_not_sorted_output():
@@ -35,4 +37,5 @@ def test_generated_desc_carries_the_flags():
                                         compiler.compile_to_mir(programs.PROGRAMS["triangle"]()))
   line = next(l for l in src.splitlines() if l.lstrip().startswith('{"TriangleJoin"'))
   assert line.rstrip().endswith("launch_weigh, 1}")
-  assert '{"Knows", 2, {0, 1}, SRD_FULL, 1}' in line
+  # want_bitmaps, want_root_pos
+  assert '{"Knows", 2, {0, 1}, SRD_FULL, 1, 1}' in line

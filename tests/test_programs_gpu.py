@@ -209,7 +209,9 @@ def test_csv_load_all(progs, tmp_path):
 # more nested levels (SRDATALOG_WEIGH_TARGET: cost per slice), and
 # SRDATALOG_NO_WEIGH=1 keeps uniform slices everywhere; SRDATALOG_BITMAP_BYTES
 # bounds the heavy-key bitmaps (0: none; a small budget keeps only the
-# longest keys' bitmaps, so bitmap probes and searches mix in one wave).
+# longest keys' bitmaps, so bitmap probes and searches mix in one wave);
+# SRDATALOG_ROOT_POS=0 looks nested-level keys up by searching the root
+# values instead of through the dense root map.
 MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_JOIN_MODE": "fused"}, "fused"),
          ({"SRDATALOG_JOIN_MODE": "fused", "SRDATALOG_STAGE_CAP": "3000"}, "fused-overflow"),
@@ -224,6 +226,7 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_NO_WEIGH": "1"}, "uniform-slices"),
          ({"SRDATALOG_BITMAP_BYTES": "0"}, "no-bitmaps"),
          ({"SRDATALOG_BITMAP_BYTES": "20000"}, "few-bitmaps"),
+         ({"SRDATALOG_ROOT_POS": "0"}, "root-search"),
          ({}, "default")]
 
 
@@ -231,7 +234,8 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
 def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
   for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NO_SEGMENTS",
             "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN", "SRDATALOG_WEIGH_MIN",
-            "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH", "SRDATALOG_BITMAP_BYTES"):
+            "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH", "SRDATALOG_BITMAP_BYTES",
+            "SRDATALOG_ROOT_POS"):
     monkeypatch.delenv(k, raising=False)
   for k, v in env.items():
     monkeypatch.setenv(k, v)
