@@ -579,7 +579,10 @@ class PipelineGen:
       # with one full-width store per column.  Rows scattered straight to
       # staging a few at a time left partly written cache lines, which the
       # L2 evicted and wrote back more than once.  Segments start at
-      # multiples of SRD_SEG, so blocks never straddle two segments.
+      # multiples of SRD_SEG, so blocks never straddle two segments.  The
+      # blocks are written non-temporal: staging is read only by the gather
+      # after the launch, and cached it would evict the lines the join's
+      # searches and bitmap probes reuse.
       self._lds.append(f"  __shared__ uint32_t ring_s[4][{ring_cols}][128];")
       o += ["  // write out the block holding staging position p: all of it (full) or",
             "  // its rows below p (a segment's last, partly filled block)",
@@ -593,7 +596,8 @@ class PipelineGen:
       j = 0
       for di, d in enumerate(self.dests):
         for c in range(len(d.index)):
-          o.append(f"      A.stage[{di * 8 + c}ull * A.stage_cap + b_ + lane] = ring_s[wv][{j}][r_];")
+          o.append(f"      __builtin_nontemporal_store(ring_s[wv][{j}][r_], "
+                   f"&A.stage[{di * 8 + c}ull * A.stage_cap + b_ + lane]);")
           j += 1
       o += ["    }",
             "    __builtin_amdgcn_wave_barrier();",
