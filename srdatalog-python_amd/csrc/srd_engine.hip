@@ -267,8 +267,9 @@ struct srd_engine {
   // smaller slices keep fewer keys (and their bitmaps) active per XCD at a
   // time; 8192 candidates and up to 48 slices per wave measured best on the
   // triangle join (16384/24: +0.8 ms, 2048/192: +4.5 ms)
-  uint64_t weigh_target = 8192;    // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
+  uint64_t weigh_target = 16384;   // SRDATALOG_WEIGH_TARGET: cost (candidates) per slice
   uint64_t slices_per_wave = 48;   // SRDATALOG_SLICES_PER_WAVE: cap of the weighed slices
+  uint64_t min_slices_per_wave = 4;  // SRDATALOG_MIN_SLICES_PER_WAVE: floor of the weighed slices
   uint32_t xcd_ranges = SRD_XCDS;  // SRDATALOG_XCD_RANGES: slice ranges of the join launches
   uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
   srd::DevBuf<uint64_t> item_cost, item_pref, slice_start;
@@ -723,8 +724,12 @@ static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs&
   const uint64_t w = read_u64(e, e.item_pref.p + n);
   // up to e.slices_per_wave slices per persistent wave of a full grid (4 x 2048), never
   // fewer than the uniform slicing had, never more than one per item
+  // and at least e.min_slices_per_wave per wave, so that a mid-sized join
+  // whose cost / weigh_target is below the wave count still spreads
   const uint64_t cap = std::max<uint64_t>(a.n_slices, e.slices_per_wave * 4 * 2048);
-  uint64_t S = std::min<uint64_t>(std::max<uint64_t>(a.n_slices, w / e.weigh_target), cap);
+  const uint64_t floor_s = e.min_slices_per_wave * 4 * 2048;
+  uint64_t S = std::min<uint64_t>(
+      std::max<uint64_t>(std::max<uint64_t>(a.n_slices, w / e.weigh_target), floor_s), cap);
   S = std::max<uint64_t>(std::min<uint64_t>(S, n), 1);
   e.slice_start.reserve(S + 1, e.s);
   slice_bounds(e.item_pref.p, n, w, S, e.slice_start.p, e.s);
@@ -1417,6 +1422,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
       e->slices_per_wave = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);
     if (const char* c = std::getenv("SRDATALOG_WEIGH_TARGET"))
       e->weigh_target = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);
+    if (const char* c = std::getenv("SRDATALOG_MIN_SLICES_PER_WAVE"))
+      e->min_slices_per_wave = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_STAGE_MAX_BYTES"))
       e->stage_max_bytes = std::max<uint64_t>(std::strtoull(c, nullptr, 10), uint64_t(1) << 20);
     if (const char* m = std::getenv("SRDATALOG_JOIN_MODE")) {
