@@ -38,9 +38,17 @@ def parse():
   p.add_argument("--scale", type=int, default=20)
   p.add_argument("--edges", type=int, default=1 << 24)
   p.add_argument("--seed", type=int, default=0)
-  p.add_argument("--cpu-sample-keys", type=int, default=400,
-                 help="CPU baseline: root keys x in [0, K) of the same graph")
+  p.add_argument("--cpu-stride-all", type=int, default=16,
+                 help="CPU baseline on all threads: root keys a with a %% S == 0")
+  p.add_argument("--cpu-stride-one", type=int, default=256,
+                 help="CPU baseline on one thread: root keys a with a %% S == 0")
+  p.add_argument("--cpu-reps", type=int, default=3)
   p.add_argument("--no-cpu-baseline", action="store_true")
+  p.add_argument("--no-count-only", action="store_true")
+  p.add_argument("--no-batch", action="store_true",
+                 help="triangle workload: skip the uneven-batch sub-object")
+  p.add_argument("--no-serve", action="store_true",
+                 help="triangle workload: skip the serving sub-object")
   p.add_argument("--no-parity", action="store_true")
   p.add_argument("--andersen-div", type=int, default=10,
                  help="andersen workload: run 1/div of the full Andersen input sizes")
@@ -51,10 +59,12 @@ def parse():
                       "Poisson request stream, continuous batching vs plain serial")
   p.add_argument("--serve-rate", type=float, default=200.0)
   p.add_argument("--serve-seconds", type=float, default=60.0)
-  p.add_argument("--serve-cap", type=float, default=5e7,
-                 help="serve: per-request cap on the estimated output tuples")
+  p.add_argument("--serve-cap", type=float, default=float(1 << 31),
+                 help="serve: per-request cap on the estimated output tuples (default 2^31: "
+                      "an index holds at most 2^32-1 rows, so a request estimated beyond half "
+                      "of that is refused; memory admission is the HBM budget)")
   p.add_argument("--serve-budget", type=float, default=200e9)
-  p.add_argument("--serve-plain-cap", type=float, default=60.0,
+  p.add_argument("--serve-plain-cap", type=float, default=30.0,
                  help="serve: wall seconds after which the plain serial baseline starts "
                       "no new request")
   p.add_argument("--serve-scales", type=str, default="16,18,20,22")
@@ -359,8 +369,78 @@ def main() -> int:
     print(json.dumps(res))
     return 0 if res["results_identical"] else 1
 
+  out = bench_triangle(args)
+  ok = out["config"]["parity_vs_oracle"] is not False
+  if not args.no_batch:
+    b = bench_batch(args)
+    out["batch"] = b
+    ok = ok and b["outputs_identical"]
+  if not args.no_serve:
+    sv = bench_serve(args)
+    out["serve"] = sv
+    ok = ok and sv["results_identical"]
+  print(json.dumps(out))
+  return 0 if ok else 1
+
+
+def cpu_model() -> dict:
+  model = ""
+  try:
+    for line in open("/proc/cpuinfo"):
+      if line.startswith("model name"):
+        model = line.split(":", 1)[1].strip()
+        break
+  except OSError:
+    pass
+  try:
+    nproc = len(os.sched_getaffinity(0))
+  except AttributeError:
+    nproc = os.cpu_count() or 1
+  return {"model": model, "nproc": nproc}
+
+
+def cpu_baseline_triangle(edges, args) -> dict:
+  """The oracle's triangle listing (join + sort/dedup of the listed rows) on
+  root-key samples of the same graph: keys a with a % stride == 0 (an
+  unbiased sample of the R-MAT key distribution), 1 thread and all threads,
+  median of --cpu-reps runs each."""
   import numpy as np
 
+  import oracle
+  cm = cpu_model()
+  res = {}
+  for label, threads, stride in (("all_threads", cm["nproc"], args.cpu_stride_all),
+                                 ("one_thread", 1, args.cpu_stride_one)):
+    oracle.set_key_stride(stride)
+    try:
+      secs, n = [], 0
+      for _ in range(args.cpu_reps):
+        o = oracle.run("triangle", {"KnowsInput": edges}, threads=threads, relations=("Triangle",))
+        secs.append(o.seconds)
+        n = o.relations["Triangle"].shape[0]
+        del o
+    finally:
+      oracle.set_key_stride(1)
+    med = float(np.median(secs))
+    res[label] = {"threads": threads, "key_stride": stride, "triangles": int(n),
+                  "seconds_median": round(med, 4), "seconds": [round(x, 4) for x in secs],
+                  "tuples_per_s": round(n / med, 1)}
+    log(f"cpu baseline {label}: {n} triangles (keys a % {stride} == 0) in {med:.3f}s median "
+        f"on {threads} threads")
+  a = res["all_threads"]
+  return {"value": a["tuples_per_s"], "unit": "derived tuples/s", "cores": a["threads"],
+          "kind": "port",
+          "sample": f"triangle listing (join + sort/dedup) of the same graph for root keys a with "
+                    f"a % {a['key_stride']} == 0 ({a['triangles']} triangles), median of "
+                    f"{args.cpu_reps}; one-thread figure on a % {args.cpu_stride_one} == 0",
+          "one_thread": res["one_thread"], "all_threads": a,
+          "cpu": cm}
+
+
+def bench_triangle(args) -> dict:
+  import numpy as np
+
+  import oracle
   from srdatalog_hip import datasets, native, programs
 
   t0 = time.time()
@@ -403,6 +483,7 @@ def main() -> int:
   n_knows = lib.size("Knows")
   mean_s = float(np.mean(step_s))
   value = n_tri / mean_s
+  pool = native.pool_stats()
 
   # Dominant kernel and its algorithmic bytes per launch.
   dom = max(kernels, key=lambda k: k["ms"]) if kernels else None
@@ -410,9 +491,11 @@ def main() -> int:
   if dom is not None:
     avg_ms = dom["ms"] / max(dom["launches"], 1)
     if dom["name"].startswith("join."):
-      # streaming floor: every source handle's rows once + the output once
-      # (3 Knows handles x 8 B/row + |Triangle| x 12 B)
-      alg = 3 * n_knows * 8 + (n_tri * 12 if dom["name"].startswith("join.write") else 0)
+      # streaming floor: every source handle's rows once (3 Knows handles x
+      # 8 B/row) + the output rows once (|Triangle| x 12 B) for every mode
+      # that materialises them (fused, write, direct; not count)
+      writes = not dom["name"].startswith(("join.count", "join.root", "join.weigh"))
+      alg = 3 * n_knows * 8 + (n_tri * 12 if writes else 0)
     else:
       alg = dom["bytes"] / max(dom["launches"], 1)
     achieved = alg / (avg_ms * 1e-3) / 1e9
@@ -424,6 +507,24 @@ def main() -> int:
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": traffic, "avg_ms": round(avg_ms, 4), "alg_bytes": alg}
 
+  # join probes: one more, untimed run counting the searches, root-map
+  # lookups and bitmap tests (an extra probe-mode launch per pipeline)
+  lib.count_probes(True)
+  if lib.run(0) != 0:
+    raise RuntimeError("srdatalog_run failed (probe run)")
+  probes = lib.join_probes()
+  lib.count_probes(False)
+  join_ms = sum(k["ms"] for k in kernels if k["name"].startswith("join.fused")
+                or k["name"].startswith("join.write") or k["name"].startswith("join.count")
+                or k["name"].startswith("join.direct")) / max(args.steps, 1)
+  jp = {"per_run": int(probes),
+        "per_s_join_kernel": round(probes / (join_ms * 1e-3), 1) if join_ms > 0 else None,
+        "per_s_fixpoint": round(probes / mean_s, 1),
+        "join_kernel_ms": round(join_ms, 4),
+        "definition": "lower/upper-bound searches, root-map lookups and heavy-key bitmap tests "
+                      "of the join's root and nested levels, counted by the probe mode"}
+  log(f"join probes: {probes} per run ({jp['per_s_join_kernel']} /s of join kernel)")
+
   prims = None
   if args.prim_rows:
     prims = primitives(native, args.prim_rows, reps=max(args.steps, 1))
@@ -431,32 +532,49 @@ def main() -> int:
                                     if k != "kernels"))
 
   parity = None
+  digest = None
   if not args.no_parity:
-    import oracle
-    o = oracle.run("triangle_count", {"KnowsInput": edges})
-    parity = bool(o.count == n_tri)
-    log(f"oracle count {o.count} vs device {n_tri}")
+    want, orc_s = oracle.triangle_digest(edges)
+    rows = lib.export("Triangle", 3)
+    got = oracle.digest_rows(rows)
+    sorted_ok = bool(rows.shape[0] < 2 or np.all(np.any(np.diff(rows[:1 << 20].astype(np.int64),
+                                                                 axis=0) != 0, axis=1)))
+    del rows
+    parity = bool(got == want and sorted_ok and n_tri == want[0])
+    digest = {"device": [str(x) for x in got], "oracle": [str(x) for x in want],
+              "oracle_enumeration_s": round(orc_s, 3)}
+    log(f"parity: device digest {got} vs oracle {want}: {parity}")
+  lib.shutdown()
+
+  count_only = None
+  if not args.no_count_only:
+    clib = programs.load("triangle_count")
+    if clib.init() != 0:
+      raise RuntimeError("srdatalog_init failed (triangle_count)")
+    clib.load_rows("KnowsInput", edges)
+    clib.prepare()
+    for _ in range(args.warmup):
+      clib.run(0)
+    cs = []
+    for _ in range(args.steps):
+      t = time.perf_counter()
+      if clib.run(0) != 0:
+        raise RuntimeError("srdatalog_run failed (triangle_count)")
+      cs.append(time.perf_counter() - t)
+    cnt = clib.size("Triangle")
+    clib.shutdown()
+    cm = float(np.mean(cs))
+    count_only = {"ms_per_step": round(cm * 1e3, 3), "ms_median": round(float(np.median(cs)) * 1e3, 3),
+                  "triangles": int(cnt), "triangles_per_s": round(cnt / cm, 1),
+                  "matches_listing": cnt == n_tri}
+    log(f"count-only: {cnt} triangles in {cm * 1e3:.2f} ms mean")
 
   cpu = None
   if not args.no_cpu_baseline:
-    import oracle
-    threads = os.cpu_count() or 1
-    try:
-      threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-      pass
-    o = oracle.run("triangle", {"KnowsInput": edges}, threads=threads,
-                   key_range=(0, args.cpu_sample_keys), relations=("Triangle",))
-    n_sample = o.relations["Triangle"].shape[0]
-    log(f"cpu baseline: {n_sample} tuples in {o.seconds:.2f}s on {threads} threads")
-    cpu = {"value": round(n_sample / o.seconds, 1), "unit": "derived tuples/s", "cores": threads,
-           "kind": "port", "seconds": round(o.seconds, 3),
-           "sample": f"triangle listing (join + sort/dedup) for root keys x in "
-                     f"[0,{args.cpu_sample_keys}) of the same graph: {n_sample} triangles"}
-    del o
-  lib.shutdown()
+    cpu = cpu_baseline_triangle(edges, args)
 
-  out = {
+  med = float(np.median(step_s))
+  return {
     "metric": "derived tuples/s (triangle listing fixpoint)",
     "value": round(value, 1),
     "unit": "derived tuples/s",
@@ -473,15 +591,18 @@ def main() -> int:
                            f"{args.edges} generated edges, (0.57,0.19,0.19,0.05), "
                            "dedup + no self-loops + oriented low->high",
                "edges": int(edges.shape[0]), "triangles": int(n_tri),
-               "fixpoint_s": round(mean_s, 5), "parity_vs_oracle": parity,
+               "fixpoint_s": round(mean_s, 5), "fixpoint_s_median": round(med, 5),
+               "step_ms_max": round(max(step_s) * 1e3, 3),
+               "parity_vs_oracle": parity, "parity_digest": digest,
                "gen_s": round(gen_s, 2)},
     "roofline": roofline,
+    "join_probes": jp,
+    "count_only": count_only,
     "cpu_baseline": cpu,
+    "pool": pool,
     "primitives": prims,
     "kernels": sorted(kernels, key=lambda k: -k["ms"])[:12],
   }
-  print(json.dumps(out))
-  return 0
 
 
 if __name__ == "__main__":

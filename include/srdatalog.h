@@ -77,6 +77,19 @@ int srdatalog_prepare(void);
 /* Wall seconds of the last srdatalog_run's fixpoint (inputs resident). */
 double srdatalog_run_seconds(void);
 
+/* Tuples the joins of the last srdatalog_run emitted before deduplication
+ * (the reference's per-rule count phase totals, complete_runner.py:1072-1112,
+ * summed over rules and iterations). */
+unsigned long long srdatalog_join_tuples(void);
+
+/* Probe runs: while on, every srdatalog_run also counts the joins' probes --
+ * lower/upper-bound searches, root-map lookups and heavy-key bitmap tests
+ * (the reference's search.h group searches, one per source per candidate) --
+ * with one extra launch per pipeline, so a timed run leaves it off.
+ * srdatalog_join_probes returns the count of the last run (0 when off). */
+void srdatalog_count_probes(int on);
+unsigned long long srdatalog_join_probes(void);
+
 /* Instance table of a packed batch (no counterpart in the reference, which
  * runs one instance per process): bounds[0..n_bounds) ascending, instance g
  * owns column-0 values [bounds[g], bounds[g+1]).  While set, every DELTA

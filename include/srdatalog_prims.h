@@ -68,6 +68,13 @@ int srd_prim_bench(int op, uint64_t na, uint64_t nb, uint32_t k, uint32_t reps,
  * Its profiler HBM counters give the scale for the other kernels' counters. */
 int srd_calib_copy(uint64_t bytes, int reps, double* ms_per_rep);
 
+/* Device memory pool counters, out[0..cap) of: bytes in use, peak bytes in
+ * use, budget (0 = none), allocations served, allocations that reached
+ * hipMalloc, bytes obtained from hipMalloc, bytes cached on the free list,
+ * times the free list was handed back.  The reference's RMM pool
+ * (gpu/device_array_rmm_impl.h:102-160) has no counterpart query. */
+int srd_pool_stats(uint64_t* out, uint32_t cap);
+
 /* Kernel timing: while enabled, every primitive and join launch on the
  * runtime's streams is bracketed by HIP events.  srd_timing_report writes a
  * JSON array [{"name", "launches", "ms", "bytes"}] (ms = summed duration,

@@ -8,6 +8,10 @@ baseline; nothing in the ``srdatalog_hip`` package may import it.
                       restatement of the reference semi-naive loop.
 * ``oracle.brute`` -- pure-Python naive Jacobi evaluation for tiny inputs,
                       used to pin the C++ restatement.
+* ``oracle.digest_rows`` / ``oracle.triangle_digest`` -- order-independent
+                      tuple-set digests, for parity at sizes whose tuple sets
+                      are too large to compare row by row.
 """
 
-from .oracle import OracleResult, build, run  # noqa: F401
+from .oracle import (OracleResult, build, digest_rows, run, set_key_stride,  # noqa: F401
+                     triangle_digest)

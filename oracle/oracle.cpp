@@ -322,6 +322,36 @@ void run_andersen(const u32* addr, u64 n_addr, const u32* assign, u64 n_assign,
   store_binary(r, "PointsTo", full, false);
 }
 
+// Order-independent digest of a tuple set (test infrastructure): per tuple
+// h = m(m(m(t0) ^ t1) ^ t2 ...) with m the splitmix64 finaliser; the digest
+// is (count, sum of h, sum of m(h + c)) mod 2^64.  Two sets with equal
+// digests and equal counts are equal unless the 128-bit sums collide.
+inline u64 mix64(u64 z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+constexpr u64 kDigestC = 0x632be59bd9b4e019ull;
+struct Digest {
+  u64 n = 0, s1 = 0, s2 = 0;
+  void add(u64 h) {
+    ++n;
+    s1 += h;
+    s2 += mix64(h + kDigestC);
+  }
+  void merge(const Digest& o) {
+    n += o.n;
+    s1 += o.s1;
+    s2 += o.s2;
+  }
+};
+inline u64 tuple_hash3(u32 a, u32 b, u32 c) { return mix64(mix64(mix64(a) ^ b) ^ c); }
+
+// Root-key sample of the triangle programs: only keys a with a % stride == 0
+// (1 = every key); set through orc_set_key_stride.
+u32 g_key_stride = 1;
+
 // ---------------------------------------------------------- Triangle ------
 // Triangle(a,b,c) <= Knows(a,b) & Knows(b,c) & Knows(a,c), var order a,b,c.
 // count_only: the rule carries .with_count(); only the count is produced.
@@ -335,7 +365,7 @@ void run_triangle(const u32* rows, u64 m, bool count_only, u32 key_lo, u32 key_h
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t ga = 0; ga < int64_t(g.vals.size()); ++ga) {
     u32 a = g.vals[ga];
-    if (a < key_lo || a >= key_hi) continue;
+    if (a < key_lo || a >= key_hi || a % g_key_stride != 0) continue;
     u64 ab = g.offs[ga], ae = g.offs[ga + 1];
     u64 local = 0;
     auto& out = sink.mine();
@@ -564,6 +594,39 @@ void run_q3(const u32* kr, u64 mk, const u32* lr, u64 ml, const u32* pr, u64 mp,
   r.iterations = 1;
 }
 
+// Digest of the oriented triangle listing Tri(a,b,c) <= Knows(a,b),
+// Knows(b,c), Knows(a,c) without materialising it: the same enumeration as
+// run_triangle (every triangle is listed once, a < b < c on an oriented
+// input, so the listing is a set), each tuple hashed as it is found.
+Digest triangle_digest(const u32* rows, u64 m, int threads) {
+  Keys kn = make_index(rows, m, false);
+  Groups g(kn);
+  std::vector<Digest> part(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t ga = 0; ga < int64_t(g.vals.size()); ++ga) {
+    const u32 a = g.vals[ga];
+    const u64 ab = g.offs[ga], ae = g.offs[ga + 1];
+    Digest& d = part[omp_get_thread_num()];
+    for (u64 i = ab; i < ae; ++i) {
+      const u32 b = lo32(kn[i]);
+      auto [bb, be] = g.find(b);
+      u64 p = bb, q = ab;
+      while (p < be && q < ae) {
+        const u32 cb = lo32(kn[p]), ca = lo32(kn[q]);
+        if (cb < ca) ++p;
+        else if (ca < cb) ++q;
+        else {
+          d.add(tuple_hash3(a, b, cb));
+          ++p, ++q;
+        }
+      }
+    }
+  }
+  Digest tot;
+  for (auto& d : part) tot.merge(d);
+  return tot;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- C ABI ------
@@ -681,5 +744,42 @@ uint64_t orc_rule_count(void* h, const char* rel) {
 }
 double orc_seconds(void* h) { return static_cast<Result*>(h)->seconds; }
 void orc_free(void* h) { delete static_cast<Result*>(h); }
+
+// Root-key sample of the triangle programs (keys a with a % stride == 0).
+void orc_set_key_stride(uint32_t stride) { g_key_stride = stride ? stride : 1; }
+
+// Digest (count, sum1, sum2) of n row-major tuples of arity k (k <= 3; a
+// shorter tuple hashes as if padded on the left by nothing: k = 2 hashes
+// m(m(t0) ^ t1)).  The rows must be distinct for the digest to be a set's.
+void orc_digest_rows(const uint32_t* rows, uint64_t n, uint32_t k, int threads, uint64_t* out) {
+  if (threads <= 0) threads = omp_get_max_threads();
+  std::vector<Digest> part(threads);
+#pragma omp parallel for schedule(static) num_threads(threads)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    const uint32_t* t = rows + uint64_t(i) * k;
+    u64 h = mix64(t[0]);
+    for (uint32_t c = 1; c < k; ++c) h = mix64(h ^ t[c]);
+    part[omp_get_thread_num()].add(h);
+  }
+  Digest tot;
+  for (auto& d : part) tot.merge(d);
+  out[0] = tot.n;
+  out[1] = tot.s1;
+  out[2] = tot.s2;
+}
+
+// Digest of the full triangle listing of the KnowsInput rows (m pairs);
+// *seconds = wall time of the enumeration (index build included).
+void orc_triangle_digest(const uint32_t* rows, uint64_t m, int threads, uint64_t* out,
+                         double* seconds) {
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  const auto t0 = std::chrono::steady_clock::now();
+  const Digest d = triangle_digest(rows, m, threads);
+  *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  out[0] = d.n;
+  out[1] = d.s1;
+  out[2] = d.s2;
+}
 
 }  // extern "C"

@@ -58,8 +58,46 @@ def _load():
     lib.orc_seconds.argtypes = [ctypes.c_void_p]
     lib.orc_free.restype = None
     lib.orc_free.argtypes = [ctypes.c_void_p]
+    lib.orc_set_key_stride.restype = None
+    lib.orc_set_key_stride.argtypes = [ctypes.c_uint32]
+    lib.orc_digest_rows.restype = None
+    lib.orc_digest_rows.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64,
+                                    ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+    lib.orc_triangle_digest.restype = None
+    lib.orc_triangle_digest.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64,
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_double)]
     _lib = lib
   return _lib
+
+
+def digest_rows(rows: np.ndarray, threads: int = 0) -> tuple[int, int, int]:
+  """Order-independent digest (count, sum1, sum2) of distinct tuples."""
+  lib = _load()
+  rows = np.ascontiguousarray(rows, dtype=np.uint32)
+  if rows.ndim == 1:
+    rows = rows.reshape(-1, 1)
+  out = (ctypes.c_uint64 * 3)()
+  lib.orc_digest_rows(rows.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), rows.shape[0],
+                      rows.shape[1], threads, out)
+  return int(out[0]), int(out[1]), int(out[2])
+
+
+def triangle_digest(knows: np.ndarray, threads: int = 0) -> tuple[tuple[int, int, int], float]:
+  """Digest of the oriented triangle listing over the KnowsInput pairs, and
+  the enumeration's wall seconds."""
+  lib = _load()
+  knows = np.ascontiguousarray(knows, dtype=np.uint32)
+  out = (ctypes.c_uint64 * 3)()
+  sec = ctypes.c_double()
+  lib.orc_triangle_digest(knows.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), knows.shape[0],
+                          threads, out, ctypes.byref(sec))
+  return (int(out[0]), int(out[1]), int(out[2])), float(sec.value)
+
+
+def set_key_stride(stride: int) -> None:
+  """Triangle programs: sample the root keys a with a % stride == 0."""
+  _load().orc_set_key_stride(stride)
 
 
 ARITY = {"Triangle": 3, "Reach": 1}

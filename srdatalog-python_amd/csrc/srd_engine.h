@@ -25,6 +25,8 @@ int srd_engine_delta_sizes(struct srd_engine* e, const char* rel, unsigned long 
 int srd_engine_shutdown(struct srd_engine* e);
 double srd_engine_run_seconds(struct srd_engine* e);
 unsigned long long srd_engine_join_tuples(struct srd_engine* e);
+void srd_engine_count_probes(struct srd_engine* e, int on);
+unsigned long long srd_engine_join_searches(struct srd_engine* e);
 void srd_engine_set_quiet(struct srd_engine* e, int quiet);
 void srd_engine_set_grain(struct srd_engine* e, unsigned grain);
 int srd_engine_set_instances(struct srd_engine* e, const uint32_t* bounds, uint64_t n_bounds);

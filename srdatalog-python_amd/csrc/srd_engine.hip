@@ -282,6 +282,12 @@ struct srd_engine {
   // 1024 rows) ran 8 % faster than 571 MB (512 rows).
   uint64_t bitmap_bytes = uint64_t(384) << 20;
   uint64_t join_probes = 0;        // tuples emitted by joins (pre-dedup)
+  // probe runs (srdatalog_count_probes): every pipeline also counts its
+  // searches, root-map lookups and bitmap tests in an extra, untimed-by-the-
+  // caller launch of the probe mode
+  bool count_probes = false;
+  uint64_t join_searches = 0;
+  srd::DevBuf<uint64_t> probe_buf;
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
   bool op_profile = false;         // SRDATALOG_OP_PROFILE: host time per op, printed per run
   std::map<std::string, std::pair<double, uint64_t>> op_times;
@@ -806,6 +812,11 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
     e.key_work.reserve(nk, e.s);
     e.work_start.reserve(nk + 1, e.s);
     e.work_ctr.reserve(SRD_XCDS * SRD_CTR_STRIDE, e.s);
+    if (e.count_probes) {
+      e.probe_buf.reserve(1, e.s);
+      fill_u64(e.probe_buf.p, 0, 1, e.s);
+      a.probe_ctr = reinterpret_cast<unsigned long long*>(e.probe_buf.p);
+    }
     a.key_rng = e.key_rng.p;
     a.key_aux = e.key_aux.p;
     a.key_work = e.key_work.p;
@@ -1001,7 +1012,9 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         std::fprintf(stderr, "[srd]   %s: %llu tuples\n", d->rule, (unsigned long long)total);
         std::fflush(stderr);
       }
+      if (e.count_probes && d->probe) launch(d->probe, "probe");
     }
+    if (e.count_probes) e.join_searches += read_u64(e, e.probe_buf.p);
   }
   e.join_probes += total;
   if (d->count_only) {
@@ -1242,6 +1255,7 @@ static void reset_state(srd_engine& e) {
   e.counts.clear();
   e.iterations = 0;
   e.join_probes = 0;
+  e.join_searches = 0;
 }
 
 static void run_program(srd_engine& e, uint64_t max_iters) {
@@ -1377,6 +1391,8 @@ using namespace srd;
 template <class F>
 static int eguard(srd_engine* e, int code_on_error, F&& f) {
   if (!e) return 1;
+  RunScope rs;
+  e->ws.ticket_order = !rs.alone;
   try {
     f();
     e->err.clear();
@@ -1397,7 +1413,7 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
   auto* e = new srd_engine();
   try {
     pool_init(0);
-    SRD_CHECK(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
+    e->s = stream_acquire();
     const char* tr = std::getenv("SRDATALOG_TRACE");
     e->trace = tr && *tr && *tr != '0';
     const char* op = std::getenv("SRDATALOG_OP_PROFILE");
@@ -1451,10 +1467,8 @@ void srd_engine_destroy(srd_engine* e) {
       r.dev_input.release();
     }
   }
-  if (s) {
-    (void)hipStreamSynchronize(s);
-    (void)hipStreamDestroy(s);
-  }
+  // the stream (and the blocks freed on it) serve the next engine
+  stream_release(s);
 }
 
 const char* srd_engine_last_error(srd_engine* e) { return e ? e->err.c_str() : "no engine"; }
@@ -1570,14 +1584,19 @@ int srd_engine_export_instances(srd_engine* e, const char* rel, uint32_t* out, u
     if (!out) return;
     if (cap_rows < x->n) fail("srdatalog_export_instances: buffer too small");
     if (cuts) {
-      DevBuf<uint64_t> per(e->n_inst, e->s);
+      DevBuf<uint64_t> per(e->n_inst + 2, e->s);
       instance_counts(x->cols.p, x->n, e->inst_bounds.p, e->n_inst, per.p, e->s);
-      std::vector<uint64_t> h(e->n_inst);
-      SRD_CHECK(hipMemcpyAsync(h.data(), per.p, e->n_inst * 8, hipMemcpyDeviceToHost, e->s));
+      outside_counts(x->cols.p, x->n, e->inst_bounds.p, e->n_inst, per.p + e->n_inst, e->s);
+      std::vector<uint64_t> h(e->n_inst + 2);
+      SRD_CHECK(hipMemcpyAsync(h.data(), per.p, (e->n_inst + 2) * 8, hipMemcpyDeviceToHost, e->s));
       SRD_CHECK(hipStreamSynchronize(e->s));
-      // rows below the first bound belong to no instance; they precede cut 0
-      uint64_t acc = x->n;
-      for (uint32_t g = 0; g < e->n_inst; ++g) acc -= h[g];
+      // Rows below the first bound belong to no instance and precede cut 0;
+      // a row at or above the last bound would be rebased into the wrong
+      // range by the export, so it is an error.
+      if (h[e->n_inst + 1] != 0)
+        fail("srdatalog_export_instances: " + std::to_string(h[e->n_inst + 1]) +
+             " rows lie at or above the last instance bound");
+      uint64_t acc = h[e->n_inst];
       for (uint32_t g = 0; g < e->n_inst; ++g) {
         cuts[g] = acc;
         acc += h[g];
@@ -1659,6 +1678,12 @@ long long srd_engine_instance_delta_sizes(srd_engine* e, const char* rel, unsign
 double srd_engine_run_seconds(srd_engine* e) { return e ? e->run_seconds : 0.0; }
 
 unsigned long long srd_engine_join_tuples(srd_engine* e) { return e ? e->join_probes : 0; }
+
+void srd_engine_count_probes(srd_engine* e, int on) {
+  if (e) e->count_probes = on != 0;
+}
+
+unsigned long long srd_engine_join_searches(srd_engine* e) { return e ? e->join_searches : 0; }
 
 void srd_engine_set_quiet(srd_engine* e, int quiet) {
   if (e) e->quiet = quiet != 0;

@@ -21,6 +21,9 @@
 //   count  (0)  counts only (count-only rules; fallback when staging
 //               overflows);
 //   write  (1)  re-runs the traversal writing at the scanned offsets;
+//   probe  (5)  traverses like count but writes nothing except the number
+//               of searches, root-map lookups and bitmap tests it made
+//               (probe_ctr); run once, untimed, to report join probes;
 //   weigh  (4)  rules with two or more nested levels: writes each level-1
 //               item's cost (its level-2 candidates + SRD_ITEM_COST) so the
 //               runtime can cut slices of equal cost instead of equal item
@@ -103,6 +106,9 @@ struct SrdLaunchArgs {
   uint64_t slice_lo;           // the launch takes slices [slice_lo, n_slices)
   uint32_t n_ranges;           // slice ranges, 1..SRD_XCDS (0 is taken as 1)
   uint32_t pad2_;
+  // probe runs only (else nullptr): the root launch and the probe mode add
+  // their lower/upper-bound searches, root-map lookups and bitmap tests
+  unsigned long long* probe_ctr;
 };
 
 // Slices are split into SRD_XCDS contiguous ranges, one per XCD (workgroups
@@ -143,6 +149,7 @@ struct SrdPipelineDesc {
   // next index column, so tuples come out in (root key, level 1, level 2 ...)
   // order with no repeats, in every mode.
   uint32_t sorted_output;
+  SrdLaunchFn probe;                 // mode 5 (search counting)
 };
 
 #ifdef __cplusplus

@@ -8,19 +8,22 @@
 //  * sort = LSD radix over the tightly packed key (columns concatenated at
 //    their significant bit widths, column 0 most significant) with 8-bit
 //    digits.  The pack kernel also builds every pass's digit histogram, and
-//    each pass is ONE kernel: tile ids are taken in launch order from an
-//    atomic counter, each tile ranks its keys with a 8-ballot digit match,
-//    publishes its per-digit count and resolves its global offset with a
-//    decoupled look-back over earlier tiles, then writes its keys from an
-//    LDS-sorted copy so that each digit run is a coalesced store.
+//    each pass is ONE kernel: each tile ranks its keys with an 8-ballot
+//    digit match, publishes its per-digit count and resolves its global
+//    offset with a decoupled look-back over earlier tiles, then writes its
+//    keys from an LDS-sorted copy so that each digit run is a coalesced
+//    store.  Tile order is the block id when the run is alone on the GPU and
+//    a ticket from an atomic counter when other runs may share it.
 //  * dedup unpacks the sorted keys straight into the index columns and
 //    emits the distinct column-0 values with their row offsets in the same
 //    pass.
-//  * set difference and merge stage, per 4096-row tile, the matching
-//    window of the other sorted input in LDS and binary-search it there.
+//  * merge and the dense set difference are merge-path tiles (diagonal
+//    split points, both pieces staged in LDS); the sparse set difference
+//    stages, per 4096-row tile of NEW, the window of FULL it covers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -40,10 +43,31 @@ void check(hipError_t e, const char* what) {
 }
 
 // ================================================================ pool ====
+// A caching allocator over hipMalloc, in the manner of the reference's RMM
+// pool (gpu/device_array_rmm_impl.h:102-160, memory is never handed back
+// while the process runs).  Freed blocks go to a free list tagged with the
+// stream they were freed on; a later allocation on the same stream reuses
+// one without synchronising (stream order makes the reuse safe), blocks of a
+// stream that has been retired (synchronised) serve any stream.  Steady-state
+// runs therefore never call into the driver's allocator: the stream-ordered
+// hipMallocAsync pool this replaced blocked the host for 0.1-5 s per run once
+// many blocks had passed through it.  Only when hipMalloc fails are the
+// cached blocks released (after a device synchronisation) and the request
+// retried.
 namespace {
 std::mutex g_pool_mu;
 PoolStats g_stats;
 bool g_pool_ready = false;
+int g_device = 0;
+
+struct FreeBlock {
+  void* p;
+  hipStream_t s;
+  bool any_stream;  // its stream was synchronised when retired
+};
+std::multimap<uint64_t, FreeBlock> g_free;    // block bytes -> block
+std::map<void*, uint64_t> g_live;             // live pointer -> block bytes
+uint64_t g_cached = 0;                        // bytes held in g_free
 
 uint64_t env_bytes(const char* a, const char* b) {
   const char* v = std::getenv(a);
@@ -51,65 +75,111 @@ uint64_t env_bytes(const char* a, const char* b) {
   if (!v || !*v) return 0;
   return std::strtoull(v, nullptr, 10);
 }
+
+// Size classes: powers of two up to 1 MiB, 2 MiB multiples beyond.
+uint64_t block_bytes(uint64_t bytes) {
+  if (bytes <= (uint64_t(1) << 20)) {
+    uint64_t b = 512;
+    while (b < bytes) b <<= 1;
+    return b;
+  }
+  const uint64_t g = uint64_t(2) << 20;
+  return (bytes + g - 1) / g * g;
+}
+
+// Largest cached block a request of `b` bytes may take (bounds the waste of
+// grow-only buffers handed blocks far above their size).
+uint64_t fit_limit(uint64_t b) {
+  if (b <= (uint64_t(1) << 20)) return b;
+  return b + std::max<uint64_t>(b / 4, uint64_t(8) << 20);
+}
+
+// Take a cached block for `b` bytes on stream s (lock held), or nullptr.
+void* take_cached(uint64_t b, hipStream_t s, uint64_t* got) {
+  const uint64_t lim = fit_limit(b);
+  for (auto it = g_free.lower_bound(b); it != g_free.end() && it->first <= lim; ++it) {
+    if (it->second.s == s || it->second.any_stream) {
+      void* p = it->second.p;
+      *got = it->first;
+      g_cached -= it->first;
+      g_free.erase(it);
+      return p;
+    }
+  }
+  return nullptr;
+}
 }  // namespace
 
 void pool_init(int device) {
   std::lock_guard<std::mutex> g(g_pool_mu);
   if (g_pool_ready) return;
   SRD_CHECK(hipSetDevice(device));
-  hipMemPool_t pool;
-  SRD_CHECK(hipDeviceGetDefaultMemPool(&pool, device));
-  uint64_t thr = ~uint64_t(0);
-  SRD_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+  g_device = device;
   g_stats.budget = env_bytes("SRDATALOG_POOL_MAX_SIZE", "SRDATALOG_RMM_POOL_MAX_SIZE");
   g_pool_ready = true;
 }
 
 void* pool_alloc(uint64_t bytes, hipStream_t s) {
   if (!g_pool_ready) pool_init(0);
-  {
-    std::lock_guard<std::mutex> g(g_pool_mu);
-    if (g_stats.budget && g_stats.in_use + bytes > g_stats.budget)
-      fail("srdatalog pool: allocation of " + std::to_string(bytes) + " bytes exceeds the budget of " +
-           std::to_string(g_stats.budget) + " bytes (" + std::to_string(g_stats.in_use) + " in use)");
-    g_stats.in_use += bytes;
-    g_stats.peak = std::max(g_stats.peak, g_stats.in_use);
-    g_stats.allocs++;
+  const uint64_t b = block_bytes(std::max<uint64_t>(bytes, 1));
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  if (g_stats.budget && g_stats.in_use + bytes > g_stats.budget)
+    fail("srdatalog pool: allocation of " + std::to_string(bytes) + " bytes exceeds the budget of " +
+         std::to_string(g_stats.budget) + " bytes (" + std::to_string(g_stats.in_use) + " in use)");
+  uint64_t got = 0;
+  void* p = take_cached(b, s, &got);
+  if (!p) {
+    got = b;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      // Hand every cached block back to the device once the streams that
+      // freed them are idle, then try once more.
+      (void)hipGetLastError();
+      (void)hipDeviceSynchronize();
+      for (auto& kv : g_free) (void)hipFree(kv.second.p);
+      g_free.clear();
+      g_cached = 0;
+      g_stats.trims++;
+      p = nullptr;
+      e = hipMalloc(&p, b);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail("srdatalog pool: hipMalloc(" + std::to_string(b) + ") failed: " + hipGetErrorString(e));
+    }
+    g_stats.reserved += b;
+    g_stats.driver_allocs++;
   }
-  void* p = nullptr;
-  hipError_t e = hipMallocAsync(&p, bytes, s);
-  if (e != hipSuccess) {
-    // The pool keeps freed blocks reserved; a large request may not fit any
-    // of them while their sum would.  Hand the unused reservations back to
-    // the device and try once more.
-    (void)hipGetLastError();
-    (void)hipStreamSynchronize(s);
-    int dev = 0;
-    hipMemPool_t pool;
-    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
-      (void)hipMemPoolTrimTo(pool, 0);
-    p = nullptr;
-    e = hipMallocAsync(&p, bytes, s);
-  }
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    std::lock_guard<std::mutex> g(g_pool_mu);
-    g_stats.in_use -= bytes;
-    fail("srdatalog pool: hipMallocAsync(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
-  }
+  g_live[p] = got;
+  g_stats.in_use += bytes;
+  g_stats.peak = std::max(g_stats.peak, g_stats.in_use);
+  g_stats.allocs++;
   return p;
 }
 
 void pool_free(void* p, uint64_t bytes, hipStream_t s) {
   if (!p) return;
-  (void)hipFreeAsync(p, s);
   std::lock_guard<std::mutex> g(g_pool_mu);
+  auto it = g_live.find(p);
+  if (it == g_live.end()) return;  // not from this pool
+  g_free.emplace(it->second, FreeBlock{p, s, false});
+  g_cached += it->second;
+  g_live.erase(it);
   g_stats.in_use -= bytes;
+}
+
+void pool_retire_stream(hipStream_t s) {
+  (void)hipStreamSynchronize(s);
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  for (auto& kv : g_free)
+    if (kv.second.s == s) kv.second.any_stream = true;
 }
 
 PoolStats pool_stats() {
   std::lock_guard<std::mutex> g(g_pool_mu);
-  return g_stats;
+  PoolStats st = g_stats;
+  st.cached = g_cached;
+  return st;
 }
 
 void pool_set_budget(uint64_t bytes) {
@@ -122,41 +192,107 @@ void pool_reset_peak() {
   g_stats.peak = g_stats.in_use;
 }
 
+// ------------------------------------------------- streams, pinned memory --
+// Engines come and go (one per session, several per serving process); their
+// streams and pinned host buffers are kept for the next one instead of being
+// destroyed and freed, which synchronises and unmaps.
+namespace {
+std::mutex g_res_mu;
+std::vector<hipStream_t> g_streams;
+std::multimap<uint64_t, void*> g_pinned;  // bytes -> pinned block
+std::atomic<int> g_running{0};
+}  // namespace
+
+hipStream_t stream_acquire() {
+  {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    if (!g_streams.empty()) {
+      hipStream_t s = g_streams.back();
+      g_streams.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s;
+  SRD_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+void stream_release(hipStream_t s) {
+  if (!s) return;
+  pool_retire_stream(s);
+  std::lock_guard<std::mutex> g(g_res_mu);
+  g_streams.push_back(s);
+}
+
+void* pinned_acquire(uint64_t bytes, uint64_t* got) {
+  {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    auto it = g_pinned.lower_bound(bytes);
+    if (it != g_pinned.end() && it->first <= 2 * bytes + 4096) {
+      void* p = it->second;
+      *got = it->first;
+      g_pinned.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  SRD_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  *got = bytes;
+  return p;
+}
+
+void pinned_release(void* p, uint64_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(g_res_mu);
+  g_pinned.emplace(bytes, p);
+}
+
+bool run_enter() { return g_running.fetch_add(1) == 0; }
+void run_exit() { g_running.fetch_sub(1); }
+
 Workspace::~Workspace() {
-  if (pinned) (void)hipHostFree(pinned);
-  if (stage_host) (void)hipHostFree(stage_host);
+  pinned_release(pinned, pinned_words * 4);
+  pinned_release(stage_host, stage_host_bytes);
 }
 
 uint32_t* pinned_words(Workspace& ws, uint64_t words) {
   if (words > ws.pinned_words) {
-    if (ws.pinned) SRD_CHECK(hipHostFree(ws.pinned));
-    uint64_t w = std::max<uint64_t>(words, 1024);
-    SRD_CHECK(hipHostMalloc((void**)&ws.pinned, w * 4, hipHostMallocDefault));
-    ws.pinned_words = w;
+    pinned_release(ws.pinned, ws.pinned_words * 4);
+    ws.pinned = nullptr;
+    ws.pinned_words = 0;
+    uint64_t got = 0;
+    ws.pinned = static_cast<uint32_t*>(pinned_acquire(std::max<uint64_t>(words, 1024) * 4, &got));
+    ws.pinned_words = got / 4;
   }
   return ws.pinned;
 }
 
 // ============================================================== timing ====
+// Several engines may run on their own threads (serving); the scope nesting
+// is per thread, the collected times are shared under a lock.
 namespace {
 struct PendingTime {
   std::string name;
   hipEvent_t a, b;
   double bytes;
 };
-bool g_timing = false;
+std::atomic<bool> g_timing{false};
+std::mutex g_time_mu;
 std::vector<PendingTime> g_pending;
 std::vector<hipEvent_t> g_free_events;
 std::map<std::string, KernelTime> g_times;
-std::string g_cur_name;
-hipEvent_t g_cur_start;
-int g_depth = 0;  // nested scopes: only the outermost one is recorded
+thread_local std::string t_cur_name;
+thread_local hipEvent_t t_cur_start;
+thread_local int t_depth = 0;  // nested scopes: only the outermost one is recorded
 
 hipEvent_t get_event() {
-  if (!g_free_events.empty()) {
-    hipEvent_t e = g_free_events.back();
-    g_free_events.pop_back();
-    return e;
+  {
+    std::lock_guard<std::mutex> g(g_time_mu);
+    if (!g_free_events.empty()) {
+      hipEvent_t e = g_free_events.back();
+      g_free_events.pop_back();
+      return e;
+    }
   }
   hipEvent_t e;
   SRD_CHECK(hipEventCreate(&e));
@@ -166,27 +302,29 @@ hipEvent_t get_event() {
 
 void timing_enable(bool on) {
   g_timing = on;
-  g_depth = 0;
+  t_depth = 0;
 }
 bool timing_enabled() { return g_timing; }
 
 void timing_begin(const char* name, hipStream_t s) {
   if (!g_timing) return;
-  if (g_depth++ > 0) return;  // inside another timed scope: that one holds the time
-  g_cur_name = name;
-  g_cur_start = get_event();
-  SRD_CHECK(hipEventRecord(g_cur_start, s));
+  if (t_depth++ > 0) return;  // inside another timed scope: that one holds the time
+  t_cur_name = name;
+  t_cur_start = get_event();
+  SRD_CHECK(hipEventRecord(t_cur_start, s));
 }
 
 void timing_end(hipStream_t s, double alg_bytes) {
-  if (!g_timing || g_depth == 0) return;
-  if (--g_depth > 0) return;
+  if (!g_timing || t_depth == 0) return;
+  if (--t_depth > 0) return;
   hipEvent_t e = get_event();
   SRD_CHECK(hipEventRecord(e, s));
-  g_pending.push_back({g_cur_name, g_cur_start, e, alg_bytes});
+  std::lock_guard<std::mutex> g(g_time_mu);
+  g_pending.push_back({t_cur_name, t_cur_start, e, alg_bytes});
 }
 
 std::vector<KernelTime> timing_collect() {
+  std::lock_guard<std::mutex> g(g_time_mu);
   for (auto& p : g_pending) {
     SRD_CHECK(hipEventSynchronize(p.b));
     float ms = 0;
@@ -207,6 +345,7 @@ std::vector<KernelTime> timing_collect() {
 
 void timing_reset() {
   timing_collect();
+  std::lock_guard<std::mutex> g(g_time_mu);
   g_times.clear();
 }
 
@@ -532,6 +671,22 @@ void build_root_pos(const Index& x, uint64_t max_entries, hipStream_t s) {
   m.ok = true;
 }
 
+// out[0] = rows below bounds[0], out[1] = rows at or above bounds[n_inst].
+__global__ void outside_count_kernel(const uint32_t* __restrict__ col0, uint32_t n,
+                                     const uint32_t* __restrict__ bounds, uint32_t n_inst,
+                                     uint64_t* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  out[0] = n ? lower_bound(col0, 0, n, bounds[0]) : 0;
+  out[1] = n ? n - lower_bound(col0, 0, n, bounds[n_inst]) : 0;
+}
+
+void outside_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
+                    uint64_t* out, hipStream_t s) {
+  if (n > 0xffffffffull) fail("outside_counts: more than 2^32 rows");
+  outside_count_kernel<<<1, 64, 0, s>>>(col0, uint32_t(n), bounds, n_inst, out);
+  SRD_CHECK(hipGetLastError());
+}
+
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t s) {
   if (n_inst == 0) return;
@@ -574,7 +729,8 @@ static void copy_out(uint8_t* dst, const uint8_t* src, uint64_t bytes, Workspace
     SRD_CHECK(hipStreamSynchronize(s));
     return;
   }
-  if (!ws.stage_host) SRD_CHECK(hipHostMalloc((void**)&ws.stage_host, 2 * kChunk, hipHostMallocDefault));
+  if (!ws.stage_host)
+    ws.stage_host = static_cast<uint8_t*>(pinned_acquire(2 * kChunk, &ws.stage_host_bytes));
   const unsigned hw = std::thread::hardware_concurrency();
   const unsigned nt = std::max(1u, std::min(8u, hw ? hw / 2 : 4u));
   hipEvent_t ev[2];
@@ -721,6 +877,20 @@ struct PackSpec {
   uint32_t shift[kMaxCols];  // bit offset of column c in the 128-bit key
   uint32_t k;
   uint32_t passes;
+};
+
+// State of a device-planned sort (build_index_dev): the row count, key
+// layout and pass schedule are computed on the device, so the sort needs
+// no host read-back and its launches can be enqueued or captured blind.
+struct DevPlan {
+  uint64_t n;                  // rows to sort
+  uint64_t u, r;               // outputs: distinct rows, distinct column-0 values
+  uint32_t shift[kMaxCols];    // bit offset of column c in the packed key
+  uint32_t bits[kMaxCols];     // significant bits of column c
+  uint32_t k, passes, flags, err;
+  uint32_t pass[kMaxPasses];   // bit 0: pass runs, bit 1: it reads the second buffer
+  uint32_t final_buf;          // buffer holding the sorted keys
+  uint32_t pad_;
 };
 
 // Block-level reduction of per-thread column maxima and order flags, then one
@@ -872,8 +1042,10 @@ __global__ __launch_bounds__(kBlock) void pack_hist_kernel(PackSpec spec, uint64
 constexpr uint64_t kLbAgg = 1, kLbPrefix = 2;
 constexpr uint64_t kLbMask = (uint64_t(1) << 46) - 1;
 // Polls before a look-back gives up and raises the error flag.  Tiles wait
-// only on tiles with smaller block ids, which every XCD dispatches first, so
-// a wait is short; the limit bounds a broken invariant to seconds.
+// only on tiles of smaller order -- smaller block ids, which every XCD
+// dispatches first, when the run is alone on the GPU, else smaller tickets,
+// which are held only by running blocks -- so a wait is short; the limit
+// bounds a broken invariant to seconds.
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int kLbWin = 8;  // onesweep look-back words loaded per round trip
 
@@ -949,24 +1121,46 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const uint64_t* __restrict__ in_lo, const uint64_t* __restrict__ in_hi,
     uint64_t* __restrict__ out_lo, uint64_t* __restrict__ out_hi, uint32_t n, uint32_t shift,
     const uint32_t* __restrict__ bin_start, uint64_t* __restrict__ lookback,
-    uint32_t* __restrict__ tile_ctr, uint32_t epoch, uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ tile_ctr, uint32_t epoch, uint32_t* __restrict__ err, bool ticket,
+    const DevPlan* __restrict__ dp, uint32_t pass_no) {
   constexpr int ITEMS = os_items(W);
   constexpr uint32_t TILE = kBlock * ITEMS;
+  if (dp) {
+    // device-planned sort: a skipped pass leaves the keys where they are;
+    // an active one reads the buffer the schedule names
+    const uint32_t ps = dp->pass[pass_no];
+    if (!(ps & 1u)) return;
+    n = uint32_t(dp->n);
+    if (ps & 2u) {
+      const uint64_t* t0 = in_lo;
+      const uint64_t* t1 = in_hi;
+      in_lo = out_lo;
+      in_hi = out_hi;
+      out_lo = const_cast<uint64_t*>(t0);
+      out_hi = const_cast<uint64_t*>(t1);
+    }
+  }
   __shared__ uint64_t s_keys[W][TILE];
   __shared__ uint32_t s_wcnt[kWavesPerBlock][kRadix];
   __shared__ uint32_t s_local[kRadix];
   __shared__ uint64_t s_global[kRadix];
   __shared__ uint32_t s_wsum[kWavesPerBlock];
 
+  __shared__ uint32_t s_tile;
   const uint32_t tid = threadIdx.x, w = tid / kWave, l = lane_id();
-  // tile order = block order: blocks are dispatched in id order, so a
-  // tile's predecessors are running or done (a ticket atomic per tile costs
-  // ~10 ns of same-address serialisation)
-  (void)tile_ctr;
+  // Tile order.  Alone on the GPU, tile = block id: each XCD dispatches its
+  // blocks in id order, so every predecessor of a running tile is running,
+  // done, or next in its XCD's queue behind blocks that finish.  When other
+  // runs' kernels may hold CUs meanwhile, a block takes a ticket once it
+  // runs instead (tile_ctr, zeroed by pack_hist_kernel), so a tile only ever
+  // waits on tiles that are already running; the ticket's same-address
+  // atomic costs ~10 ns per tile, which is why it is not always taken.
   for (uint32_t i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&s_wcnt[0][0])[i] = 0;
+  if (ticket && tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
   __syncthreads();
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = ticket ? s_tile : blockIdx.x;
   const uint64_t base = uint64_t(tile) * TILE;
+  if (base >= n) return;  // launched for a capacity above the rows (device-planned sorts)
   const uint32_t count = uint32_t(min<uint64_t>(TILE, n - base));
 
   uint64_t klo[ITEMS], khi[ITEMS];
@@ -1341,11 +1535,13 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
       if (W == 1)
         onesweep_kernel<1><<<uint32_t(tiles), kBlock, 0, s>>>(ilo, ihi, olo, ohi, uint32_t(n),
                                                                p * kRadixBits, starts + p * kRadix,
-                                                               ws.lookback.p, ctr + p, ws.epoch, err);
+                                                               ws.lookback.p, ctr + p, ws.epoch, err,
+                                                               ws.ticket_order, nullptr, 0);
       else
         onesweep_kernel<2><<<uint32_t(tiles), kBlock, 0, s>>>(ilo, ihi, olo, ohi, uint32_t(n),
                                                                p * kRadixBits, starts + p * kRadix,
-                                                               ws.lookback.p, ctr + p, ws.epoch, err);
+                                                               ws.lookback.p, ctr + p, ws.epoch, err,
+                                                               ws.ticket_order, nullptr, 0);
       SRD_CHECK(hipGetLastError());
       cur ^= 1;
     }
@@ -1414,6 +1610,450 @@ void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index
   }
   bool known = src.col_max.size() == src.arity;
   build_index(in.data(), src.arity, src.n, out, ws, s, known ? &mx : nullptr);
+}
+
+// ======================================== device-planned sort + dedup ====
+// build_index_dev: the same radix sort + dedup as build_index, with no host
+// read-back and no allocation.  The row count may live in device memory;
+// column maxima, the packed key layout, which passes run and which buffer
+// each reads are decided by device kernels (DevPlan), and every launch is
+// shaped by a host-known row capacity, so the whole sequence can be
+// enqueued (or captured) without knowing the data.  Passes whose digit is
+// constant and every pass over already-ordered input are skipped on the
+// device; a skipped pass leaves the keys in place, so the buffer the next
+// pass reads is the parity of the passes run before it.
+
+// Scratch carved from one caller-provided buffer (see devsort_scratch_bytes).
+struct DevSortScratch {
+  DevPlan* plan;
+  uint64_t* keys[2][2];     // [buffer][word], cap entries each
+  uint32_t* hist;           // kMaxPasses * kRadix
+  uint32_t* starts;         // kMaxPasses * kRadix
+  uint32_t* colmax;         // kMaxCols + 1 (order flags last)
+  uint32_t* tickets;        // kMaxPasses onesweep tile tickets
+  uint64_t* lookback;       // os_tiles * kRadix
+  uint64_t lookback_words;
+  uint32_t* tile_u;         // dedup tiles
+  uint32_t* tile_r;
+  uint64_t* scan_u;         // dedup tiles + 1
+  uint64_t* scan_r;
+  uint64_t* scan_tmp;       // multi-level scan partials (scan_noalloc_words)
+};
+
+static uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
+
+// u64 words of partial sums a scan of n entries needs beyond its output.
+static uint64_t scan_noalloc_words(uint64_t n) {
+  if (n <= kTile) return 0;
+  const uint64_t tiles = (n + kTile - 1) / kTile;
+  return 2 * tiles + 1 + scan_noalloc_words(tiles);
+}
+
+// Exclusive scan (out[n] = total) using `tmp` for the partial sums: the
+// multi-level scan_impl without its allocations.
+template <class T>
+static void scan_noalloc(const T* in, uint64_t* out, uint64_t n, uint64_t* tmp, hipStream_t s) {
+  if (n <= kTile) {
+    scan_tile_kernel<T><<<1, kBlock, 0, s>>>(in, n, nullptr, out);
+    SRD_CHECK(hipGetLastError());
+    return;
+  }
+  const uint64_t tiles = (n + kTile - 1) / kTile;
+  uint64_t* partial = tmp;
+  uint64_t* carry = tmp + tiles;
+  scan_reduce_kernel<T><<<uint32_t(tiles), kBlock, 0, s>>>(in, n, partial);
+  SRD_CHECK(hipGetLastError());
+  scan_noalloc<uint64_t>(partial, carry, tiles, tmp + 2 * tiles + 1, s);
+  scan_tile_kernel<T><<<uint32_t(tiles), kBlock, 0, s>>>(in, n, carry, out);
+  SRD_CHECK(hipGetLastError());
+}
+
+uint64_t devsort_scratch_bytes(uint64_t cap, uint32_t w) {
+  const uint64_t os_tiles = (cap + uint64_t(kBlock) * os_items(2) - 1) / (uint64_t(kBlock) * os_items(2));
+  const uint64_t os_tiles1 = (cap + uint64_t(kBlock) * os_items(1) - 1) / (uint64_t(kBlock) * os_items(1));
+  const uint64_t dt = (cap + kTile - 1) / kTile;
+  uint64_t b = align256(sizeof(DevPlan));
+  b += 2 * uint64_t(w) * align256(cap * 8);
+  b += 2 * align256(uint64_t(kMaxPasses) * kRadix * 4);
+  b += align256((kMaxCols + 1) * 4) + align256(kMaxPasses * 4);
+  b += align256(std::max(os_tiles, os_tiles1) * kRadix * 8);
+  b += 2 * align256(dt * 4) + 2 * align256((dt + 1) * 8);
+  b += align256(std::max<uint64_t>(scan_noalloc_words(dt), 1) * 8);
+  return b;
+}
+
+static DevSortScratch carve_devsort(void* base, uint64_t cap, uint32_t w) {
+  uint8_t* p = static_cast<uint8_t*>(base);
+  auto take = [&](uint64_t bytes) {
+    uint8_t* q = p;
+    p += align256(bytes);
+    return q;
+  };
+  const uint64_t os_tiles = (cap + uint64_t(kBlock) * os_items(2) - 1) / (uint64_t(kBlock) * os_items(2));
+  const uint64_t os_tiles1 = (cap + uint64_t(kBlock) * os_items(1) - 1) / (uint64_t(kBlock) * os_items(1));
+  const uint64_t dt = (cap + kTile - 1) / kTile;
+  DevSortScratch x{};
+  x.plan = reinterpret_cast<DevPlan*>(take(sizeof(DevPlan)));
+  for (int b = 0; b < 2; ++b)
+    for (uint32_t wd = 0; wd < 2; ++wd)
+      x.keys[b][wd] = wd < w ? reinterpret_cast<uint64_t*>(take(cap * 8)) : nullptr;
+  x.hist = reinterpret_cast<uint32_t*>(take(uint64_t(kMaxPasses) * kRadix * 4));
+  x.starts = reinterpret_cast<uint32_t*>(take(uint64_t(kMaxPasses) * kRadix * 4));
+  x.colmax = reinterpret_cast<uint32_t*>(take((kMaxCols + 1) * 4));
+  x.tickets = reinterpret_cast<uint32_t*>(take(kMaxPasses * 4));
+  x.lookback_words = std::max(os_tiles, os_tiles1) * kRadix;
+  x.lookback = reinterpret_cast<uint64_t*>(take(x.lookback_words * 8));
+  x.tile_u = reinterpret_cast<uint32_t*>(take(dt * 4));
+  x.tile_r = reinterpret_cast<uint32_t*>(take(dt * 4));
+  x.scan_u = reinterpret_cast<uint64_t*>(take((dt + 1) * 8));
+  x.scan_r = reinterpret_cast<uint64_t*>(take((dt + 1) * 8));
+  x.scan_tmp = reinterpret_cast<uint64_t*>(take(std::max<uint64_t>(scan_noalloc_words(dt), 1) * 8));
+  return x;
+}
+
+// Row count, cleared histograms, maxima and tickets.  n_dev (may be null)
+// overrides n_host and is clamped to the capacity (err bit 1 when above).
+// Error bits of a plan: 0 look-back did not resolve, 1 rows above the
+// capacity, 2 key wider than the launch's key words.
+__global__ __launch_bounds__(kBlock) void devsort_init_kernel(DevPlan* __restrict__ plan,
+                                                             const uint64_t* __restrict__ n_dev,
+                                                             uint64_t n_host, uint64_t cap, uint32_t k,
+                                                             uint32_t* __restrict__ hist,
+                                                             uint32_t* __restrict__ colmax,
+                                                             uint32_t* __restrict__ tickets) {
+  for (uint32_t i = threadIdx.x; i < uint32_t(kMaxPasses) * kRadix; i += kBlock) hist[i] = 0;
+  if (threadIdx.x <= uint32_t(kMaxCols)) colmax[threadIdx.x] = 0;
+  if (threadIdx.x < uint32_t(kMaxPasses)) tickets[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    uint64_t n = n_dev ? *n_dev : n_host;
+    uint32_t err = 0;
+    if (n > cap) {
+      n = cap;
+      err = 2;
+    }
+    plan->n = n;
+    plan->u = plan->r = 0;
+    plan->k = k;
+    plan->passes = 0;
+    plan->flags = 0;
+    plan->err = err;
+    plan->final_buf = 0;
+    for (int p = 0; p < kMaxPasses; ++p) plan->pass[p] = 0;
+  }
+}
+
+// Column maxima and order flags (as col_max_kernel) over the planned rows.
+__global__ __launch_bounds__(kBlock) void devsort_colmax_kernel(PackSpec spec,
+                                                               const DevPlan* __restrict__ plan,
+                                                               uint32_t* __restrict__ out_max) {
+  const uint64_t n = plan->n;
+  uint32_t m[kMaxCols];
+  for (uint32_t c = 0; c < spec.k; ++c) m[c] = 0;
+  bool unsorted = false, dup = false;
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    int order = 0;
+    for (uint32_t c = 0; c < spec.k; ++c) {
+      const uint32_t v = spec.in[c][i];
+      m[c] = max(m[c], v);
+      if (i > 0 && order == 0) {
+        const uint32_t pv = spec.in[c][i - 1];
+        order = v < pv ? -1 : (v > pv ? 1 : 0);
+      }
+    }
+    unsorted |= order < 0;
+    dup |= (i > 0 && order == 0);
+  }
+  colmax_flush(m, spec.k, unsorted, dup, out_max);
+}
+
+// Key layout from the maxima (column k-1 in the lowest bits); err bit 2 when
+// the key is wider than the W words the launch was built for.
+__global__ void devsort_layout_kernel(DevPlan* __restrict__ plan,
+                                      const uint32_t* __restrict__ colmax, uint32_t w) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t k = plan->k;
+  uint32_t total = 0;
+  for (int c = int(k) - 1; c >= 0; --c) {
+    const uint32_t v = colmax[c];
+    const uint32_t b = v ? 32u - uint32_t(__builtin_clz(v)) : 0u;
+    plan->shift[c] = total;
+    plan->bits[c] = b;
+    total += b;
+  }
+  if (total > 64u * w) plan->err |= 4u;
+  plan->passes = total > 64u * w ? 0u : (total + kRadixBits - 1) / kRadixBits;
+  plan->flags = colmax[kMaxCols];
+}
+
+// Pack the planned rows into keys and build every pass's histogram.
+template <int W>
+__global__ __launch_bounds__(kBlock) void devsort_pack_kernel(PackSpec spec,
+                                                             const DevPlan* __restrict__ plan,
+                                                             uint64_t* __restrict__ lo_out,
+                                                             uint64_t* __restrict__ hi_out,
+                                                             uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kMaxPasses * kRadix];
+  const uint64_t n = plan->n;
+  const uint32_t passes = plan->passes;
+  uint32_t shift[kMaxCols];
+  for (uint32_t c = 0; c < spec.k; ++c) shift[c] = plan->shift[c];
+  for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0;
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * kBlock) {
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t c = 0; c < spec.k; ++c) {
+      const uint64_t v = spec.in[c][i];
+      const uint32_t sh = shift[c];
+      if (sh >= 64) {
+        hi |= v << (sh - 64);
+      } else {
+        lo |= v << sh;
+        if (W == 2 && sh > 32) hi |= v >> (64 - sh);
+      }
+    }
+    lo_out[i] = lo;
+    if (W == 2) hi_out[i] = hi;
+    for (uint32_t p = 0; p < passes; ++p)
+      atomicAdd(&h[p * kRadix + digit_of<W>(lo, hi, p * kRadixBits)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Digit starts of every pass, and the schedule: a pass runs unless its
+// digit is constant over the keys or the rows were already in order; the
+// buffer it reads alternates with the passes that run.
+__global__ __launch_bounds__(kRadix) void devsort_schedule_kernel(DevPlan* __restrict__ plan,
+                                                                 const uint32_t* __restrict__ hist,
+                                                                 uint32_t* __restrict__ starts) {
+  __shared__ uint32_t wsum[kWavesPerBlock];
+  __shared__ uint32_t s_trivial[kMaxPasses];
+  const uint32_t passes = plan->passes;
+  const uint64_t n = plan->n;
+  const bool ordered = (plan->flags & 1u) == 0;
+  if (threadIdx.x < uint32_t(kMaxPasses)) s_trivial[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t v = hist[p * kRadix + threadIdx.x];
+    uint32_t tot;
+    starts[p * kRadix + threadIdx.x] = block_excl_scan<uint32_t>(v, wsum, &tot);
+    if (uint64_t(v) == n) s_trivial[p] = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t buf = 0;
+    for (uint32_t p = 0; p < uint32_t(kMaxPasses); ++p) {
+      const bool run = p < passes && n > 0 && !ordered && !s_trivial[p];
+      plan->pass[p] = (run ? 1u : 0u) | (buf << 1);
+      if (run) buf ^= 1u;
+    }
+    plan->final_buf = buf;
+  }
+}
+
+// dedup_count_kernel / dedup_write_kernel over the planned rows and layout.
+template <int W>
+__global__ __launch_bounds__(kBlock) void devsort_dedup_count_kernel(
+    const uint64_t* __restrict__ lo0, const uint64_t* __restrict__ hi0,
+    const uint64_t* __restrict__ lo1, const uint64_t* __restrict__ hi1,
+    const DevPlan* __restrict__ plan, uint32_t* __restrict__ tile_u,
+    uint32_t* __restrict__ tile_r) {
+  __shared__ uint32_t su[kWavesPerBlock], sr[kWavesPerBlock];
+  const uint64_t n = plan->n;
+  const bool b1 = plan->final_buf != 0;
+  const uint64_t* lo = b1 ? lo1 : lo0;
+  const uint64_t* hi = b1 ? hi1 : hi0;
+  const uint32_t s0 = plan->shift[0], bits0 = plan->bits[0];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint32_t cu = 0, cr = 0;
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t idx = base + tile_pos(w, i, l);
+    bool fu = false, fr = false;
+    if (idx < n) {
+      uint64_t a, b, pa = 0, pb = 0;
+      load_key<W>(lo, hi, idx, a, b);
+      if (idx > 0) load_key<W>(lo, hi, idx - 1, pa, pb);
+      fu = idx == 0 || a != pa || b != pb;
+      fr = idx == 0 || key_col<W>(a, b, s0, bits0) != key_col<W>(pa, pb, s0, bits0);
+    }
+    cu += __popcll(ballot(fu));
+    cr += __popcll(ballot(fr));
+  }
+  if (l == 0) su[w] = cu, sr[w] = cr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t a = 0, b = 0;
+    for (int j = 0; j < kWavesPerBlock; ++j) a += su[j], b += sr[j];
+    tile_u[blockIdx.x] = a;
+    tile_r[blockIdx.x] = b;
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void devsort_dedup_write_kernel(
+    const uint64_t* __restrict__ lo0, const uint64_t* __restrict__ hi0,
+    const uint64_t* __restrict__ lo1, const uint64_t* __restrict__ hi1,
+    const DevPlan* __restrict__ plan, const uint64_t* __restrict__ uoff,
+    const uint64_t* __restrict__ roff, uint32_t* __restrict__ cols, uint64_t stride,
+    uint32_t* __restrict__ root_vals, uint32_t* __restrict__ root_offs) {
+  __shared__ uint32_t su[kWavesPerBlock], sr[kWavesPerBlock];
+  const uint64_t n = plan->n;
+  const bool b1 = plan->final_buf != 0;
+  const uint64_t* lo = b1 ? lo1 : lo0;
+  const uint64_t* hi = b1 ? hi1 : hi0;
+  const uint32_t k = plan->k;
+  uint32_t sh[kMaxCols], bt[kMaxCols];
+  for (uint32_t c = 0; c < k; ++c) sh[c] = plan->shift[c], bt[c] = plan->bits[c];
+  const uint32_t w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
+  uint64_t a[kItems], b[kItems], mu[kItems], mr[kItems];
+  uint32_t cu = 0, cr = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t idx = base + tile_pos(w, i, l);
+    bool fu = false, fr = false;
+    a[i] = b[i] = 0;
+    if (idx < n) {
+      uint64_t pa = 0, pb = 0;
+      load_key<W>(lo, hi, idx, a[i], b[i]);
+      if (idx > 0) load_key<W>(lo, hi, idx - 1, pa, pb);
+      fu = idx == 0 || a[i] != pa || b[i] != pb;
+      fr = idx == 0 || key_col<W>(a[i], b[i], sh[0], bt[0]) != key_col<W>(pa, pb, sh[0], bt[0]);
+    }
+    mu[i] = ballot(fu);
+    mr[i] = ballot(fr);
+    cu += __popcll(mu[i]);
+    cr += __popcll(mr[i]);
+  }
+  if (l == 0) su[w] = cu, sr[w] = cr;
+  __syncthreads();
+  uint64_t pu = uoff[blockIdx.x], pr = roff[blockIdx.x];
+  for (uint32_t j = 0; j < w; ++j) pu += su[j], pr += sr[j];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if ((mu[i] >> l) & 1) {
+      const uint64_t row = pu + popc_below(mu[i]);
+      for (uint32_t c = 0; c < k; ++c)
+        cols[uint64_t(c) * stride + row] = key_col<W>(a[i], b[i], sh[c], bt[c]);
+      if ((mr[i] >> l) & 1) {
+        const uint64_t g = pr + popc_below(mr[i]);
+        root_vals[g] = key_col<W>(a[i], b[i], sh[0], bt[0]);
+        root_offs[g] = uint32_t(row);
+      }
+    }
+    pu += __popcll(mu[i]);
+    pr += __popcll(mr[i]);
+  }
+}
+
+// Totals into the plan, root_offs[r] = u, and the caller's count words.
+__global__ void devsort_finish_kernel(DevPlan* __restrict__ plan,
+                                      const uint64_t* __restrict__ su,
+                                      const uint64_t* __restrict__ sr, uint64_t tiles,
+                                      uint32_t* __restrict__ root_offs,
+                                      uint64_t* __restrict__ counts_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint64_t u = su[tiles], r = sr[tiles];
+  plan->u = u;
+  plan->r = r;
+  root_offs[r] = uint32_t(u);
+  if (counts_out) {
+    counts_out[0] = u;
+    counts_out[1] = r;
+    counts_out[2] = plan->err;
+  }
+}
+
+void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const uint64_t* n_dev,
+                     uint64_t n_host, uint32_t w, void* scratch, uint32_t* out_cols,
+                     uint64_t out_stride, uint32_t* root_vals, uint32_t* root_offs,
+                     uint64_t* counts_out, uint32_t* epoch, bool ticket, hipStream_t s) {
+  if (k == 0 || k > 4) fail("build_index_dev: arity must be in [1, 4]");
+  if (w != 1 && w != 2) fail("build_index_dev: key words must be 1 or 2");
+  if (cap == 0) fail("build_index_dev: zero capacity");
+  if (cap >= (uint64_t(1) << 32)) fail("build_index_dev: capacity of 2^32 rows or more");
+  DevSortScratch x = carve_devsort(scratch, cap, w);
+  PackSpec ps{};
+  ps.k = k;
+  for (uint32_t c = 0; c < k; ++c) ps.in[c] = in[c];
+  devsort_init_kernel<<<1, kBlock, 0, s>>>(x.plan, n_dev, n_host, cap, k, x.hist, x.colmax,
+                                           x.tickets);
+  SRD_CHECK(hipGetLastError());
+  {
+    TimeScope t("sort.colmax", s);
+    devsort_colmax_kernel<<<grid_for(cap, kBlock * 16, 1024), kBlock, 0, s>>>(ps, x.plan, x.colmax);
+    SRD_CHECK(hipGetLastError());
+  }
+  devsort_layout_kernel<<<1, 64, 0, s>>>(x.plan, x.colmax, w);
+  SRD_CHECK(hipGetLastError());
+  {
+    TimeScope t("sort.pack", s);
+    const uint32_t g = grid_for(cap, kBlock * 8, 4096);
+    if (w == 1)
+      devsort_pack_kernel<1><<<g, kBlock, 0, s>>>(ps, x.plan, x.keys[0][0], nullptr, x.hist);
+    else
+      devsort_pack_kernel<2><<<g, kBlock, 0, s>>>(ps, x.plan, x.keys[0][0], x.keys[0][1], x.hist);
+    SRD_CHECK(hipGetLastError());
+  }
+  devsort_schedule_kernel<<<1, kRadix, 0, s>>>(x.plan, x.hist, x.starts);
+  SRD_CHECK(hipGetLastError());
+  // every pass the widest layout could need is launched; the plan skips
+  // the ones the data does not
+  const uint32_t max_passes = (32u * k + kRadixBits - 1) / kRadixBits;
+  const uint64_t tile = uint64_t(kBlock) * os_items(int(w));
+  const uint64_t tiles = (cap + tile - 1) / tile;
+  for (uint32_t p = 0; p < std::min<uint32_t>(max_passes, uint32_t(w) * 64 / kRadixBits); ++p) {
+    if (++*epoch >= 0xFFFF) {
+      fill_u64(x.lookback, 0, x.lookback_words, s);
+      *epoch = 1;
+    }
+    TimeScope t("sort.onesweep", s);
+    if (w == 1)
+      onesweep_kernel<1><<<uint32_t(tiles), kBlock, 0, s>>>(
+          x.keys[0][0], nullptr, x.keys[1][0], nullptr, uint32_t(cap), p * kRadixBits,
+          x.starts + p * kRadix, x.lookback, x.tickets + p, *epoch, &x.plan->err, ticket, x.plan, p);
+    else
+      onesweep_kernel<2><<<uint32_t(tiles), kBlock, 0, s>>>(
+          x.keys[0][0], x.keys[0][1], x.keys[1][0], x.keys[1][1], uint32_t(cap), p * kRadixBits,
+          x.starts + p * kRadix, x.lookback, x.tickets + p, *epoch, &x.plan->err, ticket, x.plan, p);
+    SRD_CHECK(hipGetLastError());
+  }
+  const uint64_t dt = (cap + kTile - 1) / kTile;
+  {
+    TimeScope t("sort.dedup_count", s);
+    if (w == 1)
+      devsort_dedup_count_kernel<1><<<uint32_t(dt), kBlock, 0, s>>>(
+          x.keys[0][0], nullptr, x.keys[1][0], nullptr, x.plan, x.tile_u, x.tile_r);
+    else
+      devsort_dedup_count_kernel<2><<<uint32_t(dt), kBlock, 0, s>>>(
+          x.keys[0][0], x.keys[0][1], x.keys[1][0], x.keys[1][1], x.plan, x.tile_u, x.tile_r);
+    SRD_CHECK(hipGetLastError());
+  }
+  scan_noalloc<uint32_t>(x.tile_u, x.scan_u, dt, x.scan_tmp, s);
+  scan_noalloc<uint32_t>(x.tile_r, x.scan_r, dt, x.scan_tmp, s);
+  {
+    TimeScope t("sort.dedup_write", s);
+    if (w == 1)
+      devsort_dedup_write_kernel<1><<<uint32_t(dt), kBlock, 0, s>>>(
+          x.keys[0][0], nullptr, x.keys[1][0], nullptr, x.plan, x.scan_u, x.scan_r, out_cols,
+          out_stride, root_vals, root_offs);
+    else
+      devsort_dedup_write_kernel<2><<<uint32_t(dt), kBlock, 0, s>>>(
+          x.keys[0][0], x.keys[0][1], x.keys[1][0], x.keys[1][1], x.plan, x.scan_u, x.scan_r,
+          out_cols, out_stride, root_vals, root_offs);
+    SRD_CHECK(hipGetLastError());
+  }
+  devsort_finish_kernel<<<1, 64, 0, s>>>(x.plan, x.scan_u, x.scan_r, dt, root_offs, counts_out);
+  SRD_CHECK(hipGetLastError());
+}
+
+void devsort_clear_lookback(void* scratch, uint64_t cap, uint32_t w, hipStream_t s) {
+  DevSortScratch x = carve_devsort(scratch, cap, w);
+  fill_u64(x.lookback, 0, x.lookback_words, s);
 }
 
 // ============================================================== roots ====
@@ -1728,13 +2368,21 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
     uint64_t sb, uint64_t nb, const uint64_t* __restrict__ split, uint32_t* __restrict__ out,
     uint64_t so, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctr, uint32_t epoch,
     uint64_t* __restrict__ total_out, uint32_t* __restrict__ tile_cnt,
-    const uint64_t* __restrict__ tile_off) {
+    const uint64_t* __restrict__ tile_off, bool ticket) {
   constexpr bool DIFF = MODE != 1;
   __shared__ uint32_t st[K][kMpLds];
   __shared__ uint32_t s_wsum[kWavesPerBlock];
   __shared__ uint64_t s_prefix;
+  __shared__ uint32_t s_tile;
   const uint32_t tid = threadIdx.x;
-  const uint32_t tile = blockIdx.x;  // look-back order = block dispatch order
+  // look-back order: block id, or a ticket when other runs share the GPU
+  // (the reasoning of onesweep_kernel); ctr[0] is zeroed before the launch
+  uint32_t tile = blockIdx.x;
+  if (MODE == 0 && ticket) {
+    if (tid == 0) s_tile = atomicAdd(&ctr[0], 1u);
+    __syncthreads();
+    tile = s_tile;
+  }
   const uint64_t d0 = uint64_t(tile) * kMpTile;
   const uint64_t d1 = min<uint64_t>(d0 + kMpTile, na + nb);
   const uint64_t i0 = split[tile], i1 = split[tile + 1];
@@ -1862,7 +2510,7 @@ static uint64_t merge_path(const Index& a, const Index& b, uint32_t* out, uint64
   mp_tile_kernel<K, MODE><<<uint32_t(tiles), kBlock, 0, s>>>(
       a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, out, so,
       MODE == 0 ? ws.mp_lb.p : nullptr, MODE == 0 ? ws.mp_ctr.p : nullptr, ws.mp_epoch,
-      MODE == 0 ? ws.mp_total.p : nullptr, nullptr, nullptr);
+      MODE == 0 ? ws.mp_total.p : nullptr, nullptr, nullptr, ws.ticket_order);
   SRD_CHECK(hipGetLastError());
   return tiles;
 }
@@ -1904,7 +2552,7 @@ static void set_difference_gathered(const Index& a, const Index& b, uint32_t* ou
   DevBuf<uint64_t> off(tiles + 1, s);
   mp_tile_kernel<K, 4><<<uint32_t(tiles), kBlock, 0, s>>>(
       a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, tmp.p, ts, nullptr,
-      nullptr, 0, nullptr, cnt.p, nullptr);
+      nullptr, 0, nullptr, cnt.p, nullptr, false);
   SRD_CHECK(hipGetLastError());
   scan_impl<uint32_t>(cnt.p, off.p, tiles, s);
   mp_gather_kernel<K><<<uint32_t(tiles), kBlock, 0, s>>>(tmp.p, ts, ws.mp_split.p, cnt.p, off.p,
@@ -1930,12 +2578,12 @@ static void set_difference_two_pass(const Index& a, const Index& b, uint32_t* ou
   DevBuf<uint64_t> off(tiles + 1, s);
   mp_tile_kernel<K, 2><<<uint32_t(tiles), kBlock, 0, s>>>(
       a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, out, so, nullptr, nullptr,
-      0, nullptr, cnt.p, nullptr);
+      0, nullptr, cnt.p, nullptr, false);
   SRD_CHECK(hipGetLastError());
   scan_impl<uint32_t>(cnt.p, off.p, tiles, s);
   mp_tile_kernel<K, 3><<<uint32_t(tiles), kBlock, 0, s>>>(
       a.cols.p, a.stride, a.n, b.cols.p, b.stride, b.n, ws.mp_split.p, out, so, nullptr, nullptr,
-      0, nullptr, nullptr, off.p);
+      0, nullptr, nullptr, off.p, false);
   SRD_CHECK(hipGetLastError());
   ws.mp_total.reserve(1, s);
   ws.mp_ctr.reserve(2, s);

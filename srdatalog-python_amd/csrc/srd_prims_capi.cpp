@@ -15,10 +15,20 @@ namespace {
 
 thread_local std::string g_err;
 
+}  // namespace
+
+namespace srd {
+// Message returned by srd_last_error() on this thread (other C-ABI sources
+// of the library report through it).
+void capi_set_error(const std::string& msg) { g_err = msg; }
+}  // namespace srd
+
+namespace {
+
 struct Ctx {
   hipStream_t s = nullptr;
   srd::Workspace ws;
-  Ctx() { SRD_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  Ctx() { s = srd::stream_acquire(); }
 };
 
 Ctx& ctx() {
@@ -32,7 +42,9 @@ Ctx& ctx() {
 
 template <class F>
 int guard(F&& f) {
+  srd::RunScope rs;
   try {
+    ctx().ws.ticket_order = !rs.alone;
     f();
     g_err.clear();
     return 0;
@@ -246,6 +258,15 @@ int srd_calib_copy(uint64_t bytes, int reps, double* ms_per_rep) {
   });
 }
 
+int srd_pool_stats(uint64_t* out, uint32_t cap) {
+  return guard([&] {
+    const srd::PoolStats st = srd::pool_stats();
+    const uint64_t v[8] = {st.in_use, st.peak,     st.budget, st.allocs,
+                           st.driver_allocs, st.reserved, st.cached, st.trims};
+    for (uint32_t i = 0; i < cap && i < 8; ++i) out[i] = v[i];
+  });
+}
+
 int srd_timing_enable(int on) {
   srd::timing_enable(on != 0);
   return 0;
@@ -279,3 +300,4 @@ int srd_timing_reset(void) {
 }
 
 }  // extern "C"
+

@@ -28,23 +28,50 @@ void check(hipError_t e, const char* what);
 #define SRD_CHECK(x) ::srd::check((x), #x)
 
 // ------------------------------------------------------------------ pool --
-// Stream-ordered pool over hipMallocAsync.  Memory returned to the pool is
-// cached (release threshold = max) so steady-state iterations do not call
-// into the driver.  `budget` (bytes, 0 = unlimited) makes allocations beyond
-// it throw, which is how admission control sees memory pressure.
+// Caching device allocator over hipMalloc: freed blocks are kept on a free
+// list tagged with the stream that freed them and reused by later
+// allocations on that stream (or on any stream once it is retired), so
+// steady-state iterations and later sessions do not call into the driver.
+// `budget` (bytes, 0 = unlimited) makes allocations beyond it throw, which
+// is how admission control sees memory pressure.
 struct PoolStats {
-  uint64_t in_use = 0;
+  uint64_t in_use = 0;         // bytes requested by live buffers
   uint64_t peak = 0;
   uint64_t budget = 0;
-  uint64_t allocs = 0;
+  uint64_t allocs = 0;         // allocations served
+  uint64_t driver_allocs = 0;  // of which went to hipMalloc
+  uint64_t reserved = 0;       // bytes obtained from hipMalloc so far
+  uint64_t cached = 0;         // bytes held on the free list
+  uint64_t trims = 0;          // times the free list was handed back
 };
 
 void pool_init(int device);
 void* pool_alloc(uint64_t bytes, hipStream_t s);
 void pool_free(void* p, uint64_t bytes, hipStream_t s);
+// Synchronise s and let every block it freed serve any stream.
+void pool_retire_stream(hipStream_t s);
 PoolStats pool_stats();
 void pool_set_budget(uint64_t bytes);
 void pool_reset_peak();
+
+// Process-wide reuse of streams (default priority, non-blocking) and of
+// pinned host blocks across engine sessions.
+hipStream_t stream_acquire();
+void stream_release(hipStream_t s);  // retires s in the pool first
+void* pinned_acquire(uint64_t bytes, uint64_t* got);
+void pinned_release(void* p, uint64_t bytes);
+
+// Runs in flight: run_enter() returns true when no other run is in flight.
+// Only such a run may take inter-tile look-back order from the block index;
+// runs that overlap another take their tile order from a ticket, so a tile
+// never waits on one that has not been dispatched (see onesweep_kernel).
+bool run_enter();
+void run_exit();
+struct RunScope {
+  bool alone;
+  RunScope() : alone(run_enter()) {}
+  ~RunScope() { run_exit(); }
+};
 
 // Device buffer allocated from the pool.  Copies share the allocation (the
 // last owner returns it to the pool on its stream): indexes are immutable
@@ -156,9 +183,13 @@ struct Workspace {
   DevBuf<uint32_t> mp_ctr;       // [tile ticket, error flag]
   DevBuf<uint64_t> mp_total;     // rows the set difference kept
   uint32_t mp_epoch = 0;
+  // look-back kernels take tile order from a ticket (other runs may share
+  // the GPU) instead of the block id; set per run from RunScope
+  bool ticket_order = false;
   uint32_t* pinned = nullptr;    // pinned host words for read-backs
   uint64_t pinned_words = 0;
   uint8_t* stage_host = nullptr; // two pinned chunks for exports to pageable memory
+  uint64_t stage_host_bytes = 0;
   ~Workspace();
 };
 
@@ -180,6 +211,25 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
 // Same, with the source columns permuted: out column c = src column perm[c].
 void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index& out,
                       Workspace& ws, hipStream_t s);
+
+// Device-planned sort + dedup: build_index without host read-backs or
+// allocations.  Rows: n_dev (device word, may be null) or n_host, at most
+// `cap` (launch shapes come from cap).  w = key words (1 suffices for k <= 2;
+// 2 for k <= 4 columns of any width).  scratch: devsort_scratch_bytes(cap, w)
+// bytes, 256-byte aligned.  Outputs: out_cols (k columns, out_stride apart,
+// cap rows), root_vals (cap), root_offs (cap + 1) and, if counts_out is not
+// null, counts_out[0..2] = distinct rows, distinct column-0 values, error
+// bits (1: a look-back did not resolve, 2: rows above cap, 4: key wider
+// than w words).  epoch: the caller's
+// look-back epoch for this scratch (advanced here); ticket: take tile order
+// from tickets (required when other work may share the GPU).
+uint64_t devsort_scratch_bytes(uint64_t cap, uint32_t w);
+void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const uint64_t* n_dev,
+                     uint64_t n_host, uint32_t w, void* scratch, uint32_t* out_cols,
+                     uint64_t out_stride, uint32_t* root_vals, uint32_t* root_offs,
+                     uint64_t* counts_out, uint32_t* epoch, bool ticket, hipStream_t s);
+// Zero the look-back words of a devsort scratch (once before first use).
+void devsort_clear_lookback(void* scratch, uint64_t cap, uint32_t w, hipStream_t s);
 
 // out = a - b (both sorted and duplicate-free, same arity and column order).
 void set_difference(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s);
@@ -230,6 +280,10 @@ void export_rows(const Index& x, uint32_t* host_out, const uint32_t* dev_bounds,
 // [bounds[g], bounds[g+1]), g < n_inst (bounds has n_inst+1 entries).
 void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
                      uint64_t* out, hipStream_t s);
+// out[0] = rows of col0 below bounds[0], out[1] = rows at or above
+// bounds[n_inst] (rows that belong to no instance).
+void outside_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
+                    uint64_t* out, hipStream_t s);
 
 // Membership bitmaps for the keys of two-column index x whose ranges hold at
 // least min_len rows (min_len doubled until they fit max_bytes): bitmap i

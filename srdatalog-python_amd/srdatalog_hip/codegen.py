@@ -165,7 +165,8 @@ class PipelineGen:
     """Wave-cooperative emission of this lane's tuple when `cond` holds.
 
     MODE 0 counts, 1 writes at the slice's scanned offset, 2 appends to the
-    slice's staging segment, 3 writes at the item's own row (direct)."""
+    slice's staging segment, 3 writes at the item's own row (direct), 5
+    counts the searches only (probe runs)."""
     lines = [f"{ind}{{",
              f"{ind}  const uint64_t m_ = srd::ballot({cond});",
              f"{ind}  const uint32_t c_ = __popcll(m_);",
@@ -287,6 +288,8 @@ class PipelineGen:
     self._nr = getattr(self, "_nr", 0) + 1
     nm = f"nr_{tag}_{v}_{len(prefix)}_{self._nr}"
     code.append(f"{ind}const srd::Rng {nm} = {self.find_expr(parent, self.var(prefix[-1]))};")
+    if getattr(self, "_in_run", False):
+      code.append(f"{ind}if constexpr (MODE == 5) probes_ += 1u;")
     r = Rg(v, len(prefix), f"{nm}.b", f"{nm}.e")
     reg[key] = r
     return r
@@ -305,6 +308,7 @@ class PipelineGen:
     return slots
 
   def gen_root(self) -> str:
+    self._in_run = False
     out = [f"__global__ __launch_bounds__(256) void root_k(SrdLaunchArgs A) {{",
            "  const SrdViews& V = A.views;",
            "  const uint32_t k = blockIdx.x * 256u + threadIdx.x;",
@@ -347,6 +351,10 @@ class PipelineGen:
           body.append(f"    work *= uint64_t({self.size_expr(r, True)});")
     out += body
     out.append("  }")
+    # probe runs: root_find of every slot but the driver's, and the range
+    # narrowings of a key that has level-1 work
+    n_nr = sum(1 for line in body if "const srd::Rng nr_" in line)
+    out.append(f"  if (A.probe_ctr) atomicAdd(A.probe_ctr, {len(slots) - 1}ull + (ok ? {n_nr}ull : 0ull));")
     for si in range(len(slots)):
       out.append(f"  A.key_rng[uint64_t(k) * {2 * len(slots)} + {2 * si}] = rs{si}.b;")
       out.append(f"  A.key_rng[uint64_t(k) * {2 * len(slots)} + {2 * si + 1}] = rs{si}.e;")
@@ -397,6 +405,7 @@ class PipelineGen:
         code.append(f"{i2}const uint32_t e_ = {pick([f'uint32_t({r.e})' for r in rngs])};")
         code.append(f"{i2}{tag}on = pos_ == b_ || col_[pos_ - 1] != val_;")
         code.append(f"{i2}const srd::Rng cd_ = srdj::run_at(col_, pos_, e_, val_, false);")
+        code.append(f"{i2}if constexpr (MODE == 5) probes_ += 1u;")
       code.append(f"{i2}{var} = val_;")
       for j in range(len(rngs)):
         code.append(f"{i2}if ({drv_expr} == {j}u) {{ {tag}c{j}b = cd_.b; {tag}c{j}e = cd_.e; }}")
@@ -425,6 +434,7 @@ class PipelineGen:
           else:
             code.append(f"{i2}const bool first_ = pos_ == {r.b} || col_[pos_ - 1] != val_;")
             code.append(f"{i2}const srd::Rng cd_ = srdj::run_at(col_, pos_, {r.e}, val_, false);")
+            code.append(f"{i2}if constexpr (MODE == 5) probes_ += 1u;")
         code.append(f"{i2}{tag}on = first_;")
         code.append(f"{i2}{var} = val_;")
         code.append(f"{i2}{tag}c{j}b = cd_.b; {tag}c{j}e = cd_.e;")
@@ -450,7 +460,8 @@ class PipelineGen:
                   f"{tag}c{m}e = V.v[{rm.view}].root_offs[lb_ + 1];"]
         if rm.depth == self.view_arity(rm.view) - 1:
           return [f"{i3}{tag}c{m}b = lb_; {tag}c{m}e = lb_ + 1;"]
-        return [f"{i3}{tag}c{m}b = lb_; {tag}c{m}e = srd::upper_bound_la(col_, lb_ + 1, e_, {var});"]
+        return [f"{i3}{tag}c{m}b = lb_; {tag}c{m}e = srd::upper_bound_la(col_, lb_ + 1, e_, {var});",
+                f"{i3}if constexpr (MODE == 5) probes_ += 1u;"]
 
       for t in range(len(rngs) - 1):
         (ca, ba, ea), (cb, bb, eb) = src(t), src(t + 1)
@@ -464,6 +475,7 @@ class PipelineGen:
         va, vb = rngs[t].view, rngs[t + 1].view
         code.append(f"{i2}bool hit_ = false;")
         code.append(f"{i2}uint32_t lb_ = 0u;")
+        code.append(f"{i2}if constexpr (MODE == 5) probes_ += {tag}on ? 1u : 0u;")
         search = [f"lb_ = {tag}on ? srd::lower_bound_la(col_, b_, e_, {var}) : 0u;",
                   f"hit_ = {tag}on && lb_ < e_ && col_[lb_] == {var};"]
         if rngs[t].full0 or rngs[t + 1].full0:
@@ -507,6 +519,7 @@ class PipelineGen:
         code.append(f"{ind}{{  // search source {m}")
         i2 = ind + "  "
         code.append(f"{i2}const bool on_ = {tag}on && {drv_expr} != {m}u;")
+        code.append(f"{i2}if constexpr (MODE == 5) probes_ += on_ ? 1u : 0u;")
         if rm.full0:
           code.append(f"{i2}const uint32_t* col_ = V.v[{rm.view}].root_vals;")
           code.append(f"{i2}const uint32_t b_ = 0u, e_ = V.v[{rm.view}].r;")
@@ -528,6 +541,7 @@ class PipelineGen:
         else:
           code.append(f"{i2}  if (hit_) {{ {tag}c{m}b = lb_; "
                       f"{tag}c{m}e = srd::upper_bound_la(col_, lb_ + 1, e_, {var}); }}")
+          code.append(f"{i2}  if constexpr (MODE == 5) probes_ += hit_ ? 1u : 0u;")
         code.append(f"{i2}}}")
         code.append(f"{ind}}}")
     code.append(f"{ind}{hit} = {tag}on;")
@@ -559,6 +573,7 @@ class PipelineGen:
 
   # ------------------------------------------------------- slice kernel ---
   def gen_run(self) -> str:
+    self._in_run = True
     o = ["template <int MODE>",
          # eight waves per SIMD: the persistent grid launches 8 blocks per CU,
          # and with more than ~100 SGPRs a wave the eighth block could only
@@ -606,6 +621,7 @@ class PipelineGen:
     ns = len(slots)
     o += ["  // fused mode: the wave's open staging segment, kept across slices (a",
           "  // slice's rows in it are one record; the next slice goes on behind them)",
+          "  uint32_t probes_ = 0;  // MODE 5: searches, map lookups and bitmap tests of this lane",
           "  uint64_t seg_start = ~uint64_t(0);",
           "  uint32_t seg_room = 0;",
           "  bool seg_dead = false;  // staging exhausted: count only (host re-runs the write pass)",
@@ -689,7 +705,11 @@ class PipelineGen:
           "  }"]
     if self.ring:
       o += ["  if (MODE == 2 && seg_start != ~uint64_t(0)) ring_out(seg_start, false);"]
-    o += ["}"]
+    o += ["  if constexpr (MODE == 5) {",
+          "    const uint64_t ps_ = srd::wave_sum<uint64_t>(uint64_t(probes_));",
+          "    if (lane == 0 && ps_) atomicAdd(A.probe_ctr, (unsigned long long)ps_);",
+          "  }",
+          "}"]
     lds_bytes = 0
     for d in self._lds:
       dims = [int(x) for x in re.findall(r"\[(\d+)\]", d)]
@@ -880,6 +900,9 @@ class PipelineGen:
            "}",
            "void launch_direct(const SrdLaunchArgs* a, hipStream_t s) {",
            "  run_k<3><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}",
+           "void launch_probe(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<5><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}"] + ([
            "void launch_weigh(const SrdLaunchArgs* a, hipStream_t s) {",
            "  run_k<4><<<a->grid_blocks, 256, 0, s>>>(*a);",
@@ -911,7 +934,7 @@ class PipelineGen:
             f"&{self.ns}::launch_root, &{self.ns}::launch_count, &{self.ns}::launch_write, "
             f"&{self.ns}::launch_fused, &{self.ns}::launch_direct, "
             f"{'&' + self.ns + '::launch_weigh' if len(self.levels) >= 2 else 'nullptr'}, "
-            f"{1 if self.sorted_output() else 0}" + "}")
+            f"{1 if self.sorted_output() else 0}, &{self.ns}::launch_probe" + "}")
 
 
 @dataclass
@@ -1014,6 +1037,12 @@ int srdatalog_prepare(void) {
 double srdatalog_run_seconds(void) { return g_eng ? srd_engine_run_seconds(g_eng) : 0.0; }
 
 unsigned long long srdatalog_join_tuples(void) { return g_eng ? srd_engine_join_tuples(g_eng) : 0; }
+
+void srdatalog_count_probes(int on) {
+  if (ensure_engine()) srd_engine_count_probes(g_eng, on);
+}
+
+unsigned long long srdatalog_join_probes(void) { return g_eng ? srd_engine_join_searches(g_eng) : 0; }
 
 void srdatalog_set_grain(unsigned grain) {
   if (ensure_engine()) srd_engine_set_grain(g_eng, grain);

@@ -23,7 +23,7 @@ LIB_DIR = os.path.join(PKG_DIR, "_lib")
 RUNTIME_LIB = os.path.join(LIB_DIR, "libsrdatalog_hip.so")
 
 OFFLOAD_ARCH = "gfx950"
-RUNTIME_SOURCES = ["srd_prims.hip", "srd_engine.hip", "srd_prims_capi.cpp", "srd_engine_capi.cpp"]
+RUNTIME_SOURCES = ["srd_prims.hip", "srd_engine.hip", "srd_prims_capi.cpp", "srd_noalloc.hip"]
 
 
 def hipcc() -> str:
@@ -107,9 +107,22 @@ def _bind_prims(lib) -> None:
   lib.srd_calib_copy.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
   lib.srd_prim_bench.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
                                  ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), _u64p]
+  lib.srd_pool_stats.argtypes = [_u64p, ctypes.c_uint32]
   lib.srd_timing_enable.argtypes = [ctypes.c_int]
   lib.srd_timing_report.restype = ctypes.c_uint64
   lib.srd_timing_report.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
+
+
+POOL_FIELDS = ("in_use", "peak", "budget", "allocs", "driver_allocs", "reserved", "cached",
+               "trims")
+
+
+def pool_stats() -> dict:
+  """Counters of the runtime's device memory pool (bytes and counts)."""
+  lib = load_runtime()
+  buf = (ctypes.c_uint64 * len(POOL_FIELDS))()
+  _check(lib, lib.srd_pool_stats(buf, len(POOL_FIELDS)), "pool_stats")
+  return {k: int(buf[i]) for i, k in enumerate(POOL_FIELDS)}
 
 
 def calib_copy(nbytes: int, reps: int = 5) -> float:

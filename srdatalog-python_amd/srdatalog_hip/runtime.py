@@ -48,6 +48,9 @@ class ProgramLib:
     lib.srdatalog_prepare.restype = ctypes.c_int
     lib.srdatalog_run_seconds.restype = ctypes.c_double
     lib.srdatalog_join_tuples.restype = ctypes.c_ulonglong
+    lib.srdatalog_count_probes.argtypes = [ctypes.c_int]
+    lib.srdatalog_count_probes.restype = None
+    lib.srdatalog_join_probes.restype = ctypes.c_ulonglong
     lib.srdatalog_set_grain.argtypes = [ctypes.c_uint]
     lib.srdatalog_set_grain.restype = None
     lib.srdatalog_set_instances.argtypes = [_u32p, ctypes.c_uint64]
@@ -120,6 +123,15 @@ class ProgramLib:
 
   def join_tuples(self) -> int:
     return int(self.lib.srdatalog_join_tuples())
+
+  def count_probes(self, on: bool) -> None:
+    """While on, every run also counts the joins' searches (an extra launch
+    per pipeline; leave it off in timed runs)."""
+    self.lib.srdatalog_count_probes(1 if on else 0)
+
+  def join_probes(self) -> int:
+    """Searches, root-map lookups and bitmap tests of the last probe run."""
+    return int(self.lib.srdatalog_join_probes())
 
   def set_grain(self, grain: int) -> None:
     self.lib.srdatalog_set_grain(grain)

@@ -27,7 +27,18 @@ void pool_free(void* p, uint64_t bytes, hipStream_t) {
   g_stats.in_use -= bytes;
   std::free(p);
 }
+void pool_retire_stream(hipStream_t) {}
 PoolStats pool_stats() { return g_stats; }
+hipStream_t stream_acquire() { return (void*)1; }
+void stream_release(hipStream_t) {}
+void* pinned_acquire(uint64_t bytes, uint64_t* got) {
+  *got = bytes;
+  return std::malloc(bytes ? bytes : 1);
+}
+void pinned_release(void* p, uint64_t) { std::free(p); }
+static int g_running = 0;
+bool run_enter() { return g_running++ == 0; }
+void run_exit() { --g_running; }
 void pool_set_budget(uint64_t b) { g_stats.budget = b; }
 void pool_reset_peak() { g_stats.peak = g_stats.in_use; }
 
@@ -217,6 +228,12 @@ void instance_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, u
     const uint32_t* hi = col0 ? std::lower_bound(lo, col0 + n, bounds[g + 1]) : nullptr;
     out[g] = uint64_t(hi - lo);
   }
+}
+
+void outside_counts(const uint32_t* col0, uint64_t n, const uint32_t* bounds, uint32_t n_inst,
+                    uint64_t* out, hipStream_t) {
+  out[0] = col0 ? uint64_t(std::lower_bound(col0, col0 + n, bounds[0]) - col0) : 0;
+  out[1] = col0 ? uint64_t(col0 + n - std::lower_bound(col0, col0 + n, bounds[n_inst])) : 0;
 }
 
 void gather_segments(const uint64_t* recs, const uint32_t* rec_ctr, uint64_t rec_cap,

@@ -36,6 +36,7 @@ def test_generated_desc_carries_the_flags():
   src = codegen.generate_program_source("triangle", compiler.relation_decls(programs.PROGRAMS["triangle"]()),
                                         compiler.compile_to_mir(programs.PROGRAMS["triangle"]()))
   line = next(l for l in src.splitlines() if l.lstrip().startswith('{"TriangleJoin"'))
-  assert line.rstrip().endswith("launch_weigh, 1}")
+  # weigh launcher, sorted_output = 1, probe launcher
+  assert line.rstrip().endswith("launch_weigh, 1, &srdp_1_TriangleJoin::launch_probe}")
   # want_bitmaps, want_root_pos
   assert '{"Knows", 2, {0, 1}, SRD_FULL, 1, 1}' in line
