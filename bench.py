@@ -81,8 +81,11 @@ def parse():
   return p.parse_args()
 
 
+_T0 = time.time()
+
+
 def log(msg: str) -> None:
-  print(f"[bench] {msg}", file=sys.stderr, flush=True)
+  print(f"[bench {time.time() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def primitives(native, n_sort: int, reps: int) -> dict:

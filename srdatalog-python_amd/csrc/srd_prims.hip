@@ -349,11 +349,17 @@ void timing_reset() {
   g_times.clear();
 }
 
+// A null name makes the scope a no-op (callers that must not create events).
 struct TimeScope {
   hipStream_t s;
   double bytes;
-  TimeScope(const char* n, hipStream_t st, double b = 0) : s(st), bytes(b) { timing_begin(n, st); }
-  ~TimeScope() { timing_end(s, bytes); }
+  bool on;
+  TimeScope(const char* n, hipStream_t st, double b = 0) : s(st), bytes(b), on(n != nullptr) {
+    if (on) timing_begin(n, st);
+  }
+  ~TimeScope() {
+    if (on) timing_end(s, bytes);
+  }
 };
 
 // ================================================================ fill ====
@@ -890,6 +896,8 @@ struct DevPlan {
   uint32_t k, passes, flags, err;
   uint32_t pass[kMaxPasses];   // bit 0: pass runs, bit 1: it reads the second buffer
   uint32_t final_buf;          // buffer holding the sorted keys
+  uint32_t epoch0;             // look-back epoch of pass p: epoch0 + p
+  uint32_t wrap;               // the epochs wrapped: the look-back words are cleared first
   uint32_t pad_;
 };
 
@@ -1131,6 +1139,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const uint32_t ps = dp->pass[pass_no];
     if (!(ps & 1u)) return;
     n = uint32_t(dp->n);
+    epoch = dp->epoch0 + pass_no;
     if (ps & 2u) {
       const uint64_t* t0 = in_lo;
       const uint64_t* t1 = in_hi;
@@ -1631,6 +1640,8 @@ struct DevSortScratch {
   uint32_t* starts;         // kMaxPasses * kRadix
   uint32_t* colmax;         // kMaxCols + 1 (order flags last)
   uint32_t* tickets;        // kMaxPasses onesweep tile tickets
+  uint32_t* epoch_ctr;      // last look-back epoch handed out (device-resident, so
+                            // replays of a captured sort never reuse one)
   uint64_t* lookback;       // os_tiles * kRadix
   uint64_t lookback_words;
   uint32_t* tile_u;         // dedup tiles
@@ -1668,21 +1679,24 @@ static void scan_noalloc(const T* in, uint64_t* out, uint64_t n, uint64_t* tmp, 
   SRD_CHECK(hipGetLastError());
 }
 
-uint64_t devsort_scratch_bytes(uint64_t cap, uint32_t w) {
+// The layout depends on the capacity only (both key words are always laid
+// out), so one scratch serves every key width without moving its look-back
+// words: a look-back word is only ever written as one.
+uint64_t devsort_scratch_bytes(uint64_t cap) {
   const uint64_t os_tiles = (cap + uint64_t(kBlock) * os_items(2) - 1) / (uint64_t(kBlock) * os_items(2));
   const uint64_t os_tiles1 = (cap + uint64_t(kBlock) * os_items(1) - 1) / (uint64_t(kBlock) * os_items(1));
   const uint64_t dt = (cap + kTile - 1) / kTile;
   uint64_t b = align256(sizeof(DevPlan));
-  b += 2 * uint64_t(w) * align256(cap * 8);
+  b += 4 * align256(cap * 8);
   b += 2 * align256(uint64_t(kMaxPasses) * kRadix * 4);
-  b += align256((kMaxCols + 1) * 4) + align256(kMaxPasses * 4);
+  b += align256((kMaxCols + 1) * 4) + align256(kMaxPasses * 4) + align256(4);
   b += align256(std::max(os_tiles, os_tiles1) * kRadix * 8);
   b += 2 * align256(dt * 4) + 2 * align256((dt + 1) * 8);
   b += align256(std::max<uint64_t>(scan_noalloc_words(dt), 1) * 8);
   return b;
 }
 
-static DevSortScratch carve_devsort(void* base, uint64_t cap, uint32_t w) {
+static DevSortScratch carve_devsort(void* base, uint64_t cap) {
   uint8_t* p = static_cast<uint8_t*>(base);
   auto take = [&](uint64_t bytes) {
     uint8_t* q = p;
@@ -1696,11 +1710,12 @@ static DevSortScratch carve_devsort(void* base, uint64_t cap, uint32_t w) {
   x.plan = reinterpret_cast<DevPlan*>(take(sizeof(DevPlan)));
   for (int b = 0; b < 2; ++b)
     for (uint32_t wd = 0; wd < 2; ++wd)
-      x.keys[b][wd] = wd < w ? reinterpret_cast<uint64_t*>(take(cap * 8)) : nullptr;
+      x.keys[b][wd] = reinterpret_cast<uint64_t*>(take(cap * 8));
   x.hist = reinterpret_cast<uint32_t*>(take(uint64_t(kMaxPasses) * kRadix * 4));
   x.starts = reinterpret_cast<uint32_t*>(take(uint64_t(kMaxPasses) * kRadix * 4));
   x.colmax = reinterpret_cast<uint32_t*>(take((kMaxCols + 1) * 4));
   x.tickets = reinterpret_cast<uint32_t*>(take(kMaxPasses * 4));
+  x.epoch_ctr = reinterpret_cast<uint32_t*>(take(4));
   x.lookback_words = std::max(os_tiles, os_tiles1) * kRadix;
   x.lookback = reinterpret_cast<uint64_t*>(take(x.lookback_words * 8));
   x.tile_u = reinterpret_cast<uint32_t*>(take(dt * 4));
@@ -1720,7 +1735,8 @@ __global__ __launch_bounds__(kBlock) void devsort_init_kernel(DevPlan* __restric
                                                              uint64_t n_host, uint64_t cap, uint32_t k,
                                                              uint32_t* __restrict__ hist,
                                                              uint32_t* __restrict__ colmax,
-                                                             uint32_t* __restrict__ tickets) {
+                                                             uint32_t* __restrict__ tickets,
+                                                             uint32_t* __restrict__ epoch_ctr) {
   for (uint32_t i = threadIdx.x; i < uint32_t(kMaxPasses) * kRadix; i += kBlock) hist[i] = 0;
   if (threadIdx.x <= uint32_t(kMaxCols)) colmax[threadIdx.x] = 0;
   if (threadIdx.x < uint32_t(kMaxPasses)) tickets[threadIdx.x] = 0;
@@ -1739,7 +1755,26 @@ __global__ __launch_bounds__(kBlock) void devsort_init_kernel(DevPlan* __restric
     plan->err = err;
     plan->final_buf = 0;
     for (int p = 0; p < kMaxPasses; ++p) plan->pass[p] = 0;
+    // kMaxPasses epochs per sort; epochs live in 16 bits of a look-back word
+    uint32_t e = *epoch_ctr;
+    plan->wrap = 0;
+    if (e + 2u * kMaxPasses >= 0xFFFFu) {
+      e = 0;
+      plan->wrap = 1;
+    }
+    plan->epoch0 = e + 1u;
+    *epoch_ctr = e + uint32_t(kMaxPasses);
   }
+}
+
+// Clear the look-back words when the plan's epochs wrapped (else nothing).
+__global__ __launch_bounds__(kBlock) void devsort_wrap_clear_kernel(const DevPlan* __restrict__ plan,
+                                                                   uint64_t* __restrict__ lookback,
+                                                                   uint64_t words) {
+  if (!plan->wrap) return;
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < words;
+       i += uint64_t(gridDim.x) * kBlock)
+    lookback[i] = 0;
 }
 
 // Column maxima and order flags (as col_max_kernel) over the planned rows.
@@ -1971,27 +2006,31 @@ __global__ void devsort_finish_kernel(DevPlan* __restrict__ plan,
 void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const uint64_t* n_dev,
                      uint64_t n_host, uint32_t w, void* scratch, uint32_t* out_cols,
                      uint64_t out_stride, uint32_t* root_vals, uint32_t* root_offs,
-                     uint64_t* counts_out, uint32_t* epoch, bool ticket, hipStream_t s) {
+                     uint64_t* counts_out, bool ticket, hipStream_t s, bool timed) {
   if (k == 0 || k > 4) fail("build_index_dev: arity must be in [1, 4]");
   if (w != 1 && w != 2) fail("build_index_dev: key words must be 1 or 2");
   if (cap == 0) fail("build_index_dev: zero capacity");
   if (cap >= (uint64_t(1) << 32)) fail("build_index_dev: capacity of 2^32 rows or more");
-  DevSortScratch x = carve_devsort(scratch, cap, w);
+  DevSortScratch x = carve_devsort(scratch, cap);
+  auto tname = [&](const char* nm) { return timed ? nm : nullptr; };
   PackSpec ps{};
   ps.k = k;
   for (uint32_t c = 0; c < k; ++c) ps.in[c] = in[c];
   devsort_init_kernel<<<1, kBlock, 0, s>>>(x.plan, n_dev, n_host, cap, k, x.hist, x.colmax,
-                                           x.tickets);
+                                           x.tickets, x.epoch_ctr);
+  SRD_CHECK(hipGetLastError());
+  devsort_wrap_clear_kernel<<<grid_for(x.lookback_words, kBlock * 16, 1024), kBlock, 0, s>>>(
+      x.plan, x.lookback, x.lookback_words);
   SRD_CHECK(hipGetLastError());
   {
-    TimeScope t("sort.colmax", s);
+    TimeScope t(tname("sort.colmax"), s);
     devsort_colmax_kernel<<<grid_for(cap, kBlock * 16, 1024), kBlock, 0, s>>>(ps, x.plan, x.colmax);
     SRD_CHECK(hipGetLastError());
   }
   devsort_layout_kernel<<<1, 64, 0, s>>>(x.plan, x.colmax, w);
   SRD_CHECK(hipGetLastError());
   {
-    TimeScope t("sort.pack", s);
+    TimeScope t(tname("sort.pack"), s);
     const uint32_t g = grid_for(cap, kBlock * 8, 4096);
     if (w == 1)
       devsort_pack_kernel<1><<<g, kBlock, 0, s>>>(ps, x.plan, x.keys[0][0], nullptr, x.hist);
@@ -2007,24 +2046,20 @@ void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const 
   const uint64_t tile = uint64_t(kBlock) * os_items(int(w));
   const uint64_t tiles = (cap + tile - 1) / tile;
   for (uint32_t p = 0; p < std::min<uint32_t>(max_passes, uint32_t(w) * 64 / kRadixBits); ++p) {
-    if (++*epoch >= 0xFFFF) {
-      fill_u64(x.lookback, 0, x.lookback_words, s);
-      *epoch = 1;
-    }
-    TimeScope t("sort.onesweep", s);
+    TimeScope t(tname("sort.onesweep"), s);
     if (w == 1)
       onesweep_kernel<1><<<uint32_t(tiles), kBlock, 0, s>>>(
           x.keys[0][0], nullptr, x.keys[1][0], nullptr, uint32_t(cap), p * kRadixBits,
-          x.starts + p * kRadix, x.lookback, x.tickets + p, *epoch, &x.plan->err, ticket, x.plan, p);
+          x.starts + p * kRadix, x.lookback, x.tickets + p, 0u, &x.plan->err, ticket, x.plan, p);
     else
       onesweep_kernel<2><<<uint32_t(tiles), kBlock, 0, s>>>(
           x.keys[0][0], x.keys[0][1], x.keys[1][0], x.keys[1][1], uint32_t(cap), p * kRadixBits,
-          x.starts + p * kRadix, x.lookback, x.tickets + p, *epoch, &x.plan->err, ticket, x.plan, p);
+          x.starts + p * kRadix, x.lookback, x.tickets + p, 0u, &x.plan->err, ticket, x.plan, p);
     SRD_CHECK(hipGetLastError());
   }
   const uint64_t dt = (cap + kTile - 1) / kTile;
   {
-    TimeScope t("sort.dedup_count", s);
+    TimeScope t(tname("sort.dedup_count"), s);
     if (w == 1)
       devsort_dedup_count_kernel<1><<<uint32_t(dt), kBlock, 0, s>>>(
           x.keys[0][0], nullptr, x.keys[1][0], nullptr, x.plan, x.tile_u, x.tile_r);
@@ -2036,7 +2071,7 @@ void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const 
   scan_noalloc<uint32_t>(x.tile_u, x.scan_u, dt, x.scan_tmp, s);
   scan_noalloc<uint32_t>(x.tile_r, x.scan_r, dt, x.scan_tmp, s);
   {
-    TimeScope t("sort.dedup_write", s);
+    TimeScope t(tname("sort.dedup_write"), s);
     if (w == 1)
       devsort_dedup_write_kernel<1><<<uint32_t(dt), kBlock, 0, s>>>(
           x.keys[0][0], nullptr, x.keys[1][0], nullptr, x.plan, x.scan_u, x.scan_r, out_cols,
@@ -2051,9 +2086,10 @@ void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const 
   SRD_CHECK(hipGetLastError());
 }
 
-void devsort_clear_lookback(void* scratch, uint64_t cap, uint32_t w, hipStream_t s) {
-  DevSortScratch x = carve_devsort(scratch, cap, w);
+void devsort_clear_lookback(void* scratch, uint64_t cap, hipStream_t s) {
+  DevSortScratch x = carve_devsort(scratch, cap);
   fill_u64(x.lookback, 0, x.lookback_words, s);
+  fill_u32(x.epoch_ctr, 0, 1, s);
 }
 
 // ============================================================== roots ====

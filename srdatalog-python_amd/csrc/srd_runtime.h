@@ -215,21 +215,25 @@ void build_index_from(const Index& src, const std::vector<uint32_t>& perm, Index
 // Device-planned sort + dedup: build_index without host read-backs or
 // allocations.  Rows: n_dev (device word, may be null) or n_host, at most
 // `cap` (launch shapes come from cap).  w = key words (1 suffices for k <= 2;
-// 2 for k <= 4 columns of any width).  scratch: devsort_scratch_bytes(cap, w)
-// bytes, 256-byte aligned.  Outputs: out_cols (k columns, out_stride apart,
+// 2 for k <= 4 columns of any width).  scratch: devsort_scratch_bytes(cap)
+// bytes, 256-byte aligned; its layout depends on cap alone, so a scratch
+// reused across calls must always be given the same cap (its look-back
+// words would otherwise overlap earlier calls' keys).  Outputs: out_cols (k columns, out_stride apart,
 // cap rows), root_vals (cap), root_offs (cap + 1) and, if counts_out is not
 // null, counts_out[0..2] = distinct rows, distinct column-0 values, error
 // bits (1: a look-back did not resolve, 2: rows above cap, 4: key wider
-// than w words).  epoch: the caller's
-// look-back epoch for this scratch (advanced here); ticket: take tile order
-// from tickets (required when other work may share the GPU).
-uint64_t devsort_scratch_bytes(uint64_t cap, uint32_t w);
+// than w words).  The look-back epochs are kept in the scratch itself and
+// advanced by a kernel, so a captured sort can be replayed; ticket: take tile order
+// from tickets (required when other work may share the GPU); timed: record
+// the launches in the kernel timing (false: no events are created).
+uint64_t devsort_scratch_bytes(uint64_t cap);
 void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const uint64_t* n_dev,
                      uint64_t n_host, uint32_t w, void* scratch, uint32_t* out_cols,
                      uint64_t out_stride, uint32_t* root_vals, uint32_t* root_offs,
-                     uint64_t* counts_out, uint32_t* epoch, bool ticket, hipStream_t s);
-// Zero the look-back words of a devsort scratch (once before first use).
-void devsort_clear_lookback(void* scratch, uint64_t cap, uint32_t w, hipStream_t s);
+                     uint64_t* counts_out, bool ticket, hipStream_t s, bool timed = true);
+// Zero the look-back words and epoch of a devsort scratch (once, before its
+// first use).
+void devsort_clear_lookback(void* scratch, uint64_t cap, hipStream_t s);
 
 // out = a - b (both sorted and duplicate-free, same arity and column order).
 void set_difference(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s);
