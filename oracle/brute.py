@@ -129,3 +129,23 @@ def q3(knows, located, part_of):
          for (x3, c3) in loc if x3 == p3 for (d1, co) in part if d1 == c1
          if (c2, co) in part and (c3, co) in part}
   return {"Triangle": tri}
+
+
+def split_rec(seed, filt, edge):
+  """R(x,y) <= Seed(x,y); R(x,z) <= R(x,y), !Filter(y), Edge(y,z)."""
+  seed = {(int(a), int(b)) for a, b in seed}
+  blocked = {int(v[0]) if hasattr(v, "__len__") else int(v) for v in filt}
+  succ = {}
+  for a, b in edge:
+    succ.setdefault(int(a), set()).add(int(b))
+  r, deltas = _fix(set(seed), lambda p: {(x, z) for (x, y) in p if y not in blocked
+                                         for z in succ.get(y, ())})
+  return {"R": r}, {"R": deltas}
+
+
+def split_nonrec(a, b, c):
+  """Q(x,z) <= A(x,y), !B(y), C(x,z): one pass, no iteration."""
+  blocked = {int(v[0]) if hasattr(v, "__len__") else int(v) for v in b}
+  keep = {int(x) for x, y in a if int(y) not in blocked}
+  q = {(int(x), int(z)) for x, z in c if int(x) in keep}
+  return {"Q": q}, {}

@@ -33,12 +33,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <iostream>
 #include <map>
 #include <memory>
 #include <string>
 #include <vector>
 
+#include "srd_devloop.h"
 #include "srd_engine.h"
 #include "srd_pipeline.h"
 #include "srd_runtime.h"
@@ -150,6 +152,7 @@ enum class OpKind {
   ComputeDelta,
   MergeIndex,
   RebuildFromIndex,
+  FlatView,
   Reconstruct,
   Hook,
 };
@@ -294,6 +297,11 @@ struct srd_engine {
   // instance table of a packed batch: n_inst+1 ascending column-0 bounds
   srd::DevBuf<uint32_t> inst_bounds;
   uint32_t n_inst = 0;
+  // device-resident iterations of eligible recursive steps
+  bool devloop = true;             // SRDATALOG_DEVLOOP=0: host-driven loop everywhere
+  bool devloop_graph = true;       // SRDATALOG_DEVLOOP_GRAPH=0: enqueue without capturing
+  uint64_t devloop_batch = 4;      // SRDATALOG_DEVLOOP_BATCH: iterations per control read
+  uint64_t devloop_caps = 0;       // SRDATALOG_DEVLOOP_CAPS: initial capacities (tests)
 };
 
 namespace srd {
@@ -351,6 +359,10 @@ static Op parse_op(srd_engine& e, const SNode& n) {
     parse_index_ref(e, n.kw(":source"), op.rel, op.index);
     parse_index_ref(e, n.kw(":target"), r2, op.index2);
     if (r2 != op.rel) fail("mir: rebuild-index-from-index across relations");
+    op.ver = parse_ver(n.kw(":ver"));
+  } else if (h == "create-flat-view") {
+    op.kind = OpKind::FlatView;
+    parse_index_ref(e, n.kw(":index"), op.rel, op.index);
     op.ver = parse_ver(n.kw(":ver"));
   } else if (h == "post-stratum-reconstruct-intern-cols") {
     op.kind = OpKind::Reconstruct;
@@ -1041,6 +1053,7 @@ static const char* op_name(OpKind k) {
     case OpKind::ComputeDelta: return "compute-delta-index";
     case OpKind::MergeIndex: return "merge-index";
     case OpKind::RebuildFromIndex: return "rebuild-index-from-index";
+    case OpKind::FlatView: return "create-flat-view";
     case OpKind::Reconstruct: return "reconstruct";
     case OpKind::Hook: return "hook";
   }
@@ -1226,6 +1239,26 @@ static void exec_op_body(srd_engine& e, const Op& op, uint64_t iteration, int st
       }
       break;
     }
+    case OpKind::FlatView: {
+      // A split rule's temp relation (written by its first pipeline, read
+      // by its second, no maintenance of its own): the rows become its NEW
+      // index in the given order and, as the same index, its FULL one --
+      // the second pipeline scans it as NEW in a recursive step and as FULL
+      // in a non-recursive one (gpu_mir_helpers.h:436-466 builds the index
+      // the scan iterates).
+      Relation& r = e.rels[op.rel];
+      r.newv.erase(op.index);
+      const Index& nw = new_index(e, r, op.index);
+      Versioned v;
+      copy_index(nw, v.idx, e.s);
+      r.full.clear();
+      r.segs.clear();
+      r.full_version += 1;
+      r.delta_seq = r.full_version;
+      v.version = r.full_version;
+      r.full[op.index] = std::move(v);
+      break;
+    }
     case OpKind::Reconstruct:
     case OpKind::Hook:
       break;
@@ -1258,6 +1291,679 @@ static void reset_state(srd_engine& e) {
   e.join_searches = 0;
 }
 
+}  // namespace srd
+
+// ================================================= device-resident loop ====
+// A recursive step whose iteration the device can size by itself runs
+// without host read-backs: every buffer is reserved at a capacity, row
+// counts live in a device control block (DevCtl), each pipeline's launches
+// read their views and sizes from a device SrdDyn that a bind kernel fills,
+// NEW is sorted by the device-planned radix sort, and DELTA = NEW - FULL and
+// FULL u DELTA are merge-path kernels sized from device counts.  One
+// iteration is captured into a HIP graph -- two, since FULL alternates
+// between two buffers -- and replayed; the host reads the control block
+// every few iterations.  The reference drives the same loop from the host
+// (codegen/jit/orchestrator_jit.py:600-739) with a blocking size read per
+// rule and phase (codegen/jit/complete_runner.py:1072-1112).
+//
+// Iteration 0 reads FULL wherever a rule reads DELTA
+// (complete_runner.py:834-838); the loop starts with DELTA := FULL, which
+// is the same computation.  Capacities are guesses: a kernel that finds one
+// too small raises `overflow` with the size it needed and sets `skip`, which
+// turns every later launch into a no-op.  All capacity checks of an
+// iteration come before its first write of DELTA or FULL, so the host grows
+// the buffers and runs the abandoned iteration again.
+namespace srd {
+
+struct DevRelState {
+  int rel = -1;
+  uint32_t k = 0;
+  Order canon;          // index order of NEW, DELTA and FULL
+  Order raw_order;      // column order the pipelines write NEW in
+  DevBuf<uint32_t> raw, nsort, nroot_v, nroot_o, delta, droot_v, droot_o, full[2];
+  DevBuf<uint8_t> sort_scratch;
+  DevBuf<uint64_t> sd_split, sd_off, sd_scan, mg_split, rt_off, rt_scan, deltas, inst;
+  DevBuf<uint32_t> sd_tmp, sd_cnt, rt_pos, rt_cnt;
+  uint32_t bound = 0;   // upper bound of every column value (for col_max)
+};
+
+struct DevPipeState {
+  const SrdPipelineDesc* d = nullptr;
+  SrdLaunchArgs a;
+  SrdDyn dyn_host;
+  DevBuf<uint8_t> dyn;
+  DevBuf<uint32_t> key_rng, key_aux, work_ctr, counts;
+  DevBuf<uint64_t> key_work, work_start, offsets, scan_keys, scan_slices;
+  DlBind bind;
+  DlOut out;
+  int32_t vrel[SRD_MAX_VIEWS];
+  bool direct = false;
+};
+
+struct DevLoop {
+  std::vector<DevRelState> rels;
+  std::vector<DevPipeState> pipes;
+  DevBuf<uint8_t> ctl_buf;
+  DevCtl* ctl = nullptr;
+  uint64_t cap_new = 0, cap_full = 0, cap_keys = 0, cap_slices = 0, iter_cap = 0;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  ~DevLoop() { drop_graphs(); }
+  void drop_graphs() {
+    for (auto& g : exec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
+};
+
+}  // namespace srd
+
+// ------------------------------------------------------- host side ----
+namespace srd {
+
+static void flatten_ops(const std::vector<Op>& ops, std::vector<const Op*>& out) {
+  for (auto& op : ops) {
+    if (op.kind == OpKind::Parallel) flatten_ops(op.kids, out);
+    else out.push_back(&op);
+  }
+}
+
+// Whether the device can size every op of the step: linear rules over the
+// step's DELTAs (canonical order) and FULLs of relations the step does not
+// write, every relation rebuilding its NEW once before any DELTA or FULL
+// update, at most kDevRels relations of arity <= 4 and kDevPipes pipelines.
+static bool devloop_eligible(srd_engine& e, const Step& st, std::string& why) {
+  std::vector<const Op*> ops;
+  flatten_ops(st.ops, ops);
+  const std::vector<int>& rels = st.canonical_rels;
+  auto in_step = [&](int r) { return std::find(rels.begin(), rels.end(), r) != rels.end(); };
+  if (rels.empty() || rels.size() > size_t(kDevRels)) return why = "relation count", false;
+  for (int ri : rels)
+    if (e.rels[ri].arity == 0 || e.rels[ri].arity > 4) return why = "arity", false;
+  bool updated = false;
+  size_t npipes = 0;
+  std::map<int, int> rebuilds;
+  std::map<int, Order> raw_order;
+  for (const Op* op : ops) {
+    switch (op->kind) {
+      case OpKind::Pipeline: {
+        if (updated) return why = "pipeline after a DELTA/FULL update", false;
+        const SrdPipelineDesc* d = op->pipe;
+        if (d->count_only || !d->root_dyn || !d->count_dyn || !d->write_dyn)
+          return why = "pipeline without device-sized launches", false;
+        for (uint32_t v = 0; v < d->n_views; ++v) {
+          const SrdViewSpec& vs = d->views[v];
+          const int ri = rel_id(e, vs.rel);
+          Order order(vs.cols, vs.cols + vs.ncols);
+          if (in_step(ri)) {
+            if (vs.ver != SRD_DELTA || order != e.rels[ri].canonical)
+              return why = "view of a step relation other than its canonical DELTA", false;
+          } else if (vs.ver != SRD_FULL) {
+            return why = "DELTA/NEW view of a relation outside the step", false;
+          }
+        }
+        for (uint32_t k = 0; k < d->n_dest; ++k) {
+          const int ri = rel_id(e, d->dest[k].rel);
+          if (!in_step(ri)) return why = "destination outside the step", false;
+          Order order(d->dest[k].cols, d->dest[k].cols + d->dest[k].arity);
+          auto it = raw_order.find(ri);
+          if (it != raw_order.end() && it->second != order)
+            return why = "destinations in different orders", false;
+          raw_order[ri] = order;
+        }
+        ++npipes;
+        break;
+      }
+      case OpKind::RebuildIndex:
+        if (op->ver != SRD_NEW || !in_step(op->rel) || op->index != e.rels[op->rel].canonical)
+          return why = "rebuild-index other than NEW in canonical order", false;
+        if (updated) return why = "rebuild after a DELTA/FULL update", false;
+        rebuilds[op->rel]++;
+        break;
+      case OpKind::ComputeDelta:
+      case OpKind::MergeIndex:
+        if (!in_step(op->rel) || op->index != e.rels[op->rel].canonical)
+          return why = "delta/merge other than canonical", false;
+        updated = true;
+        break;
+      case OpKind::ClearRelation:
+      case OpKind::CheckSize:
+        break;
+      default:
+        return why = std::string("op ") + op_name(op->kind), false;
+    }
+  }
+  if (npipes == 0 || npipes > size_t(kDevPipes)) return why = "pipeline count", false;
+  for (int ri : rels)
+    if (rebuilds[ri] != 1 || !raw_order.count(ri)) return why = "NEW not rebuilt exactly once", false;
+  return true;
+}
+
+struct DevCaps {
+  uint64_t new_rows, full_rows, keys, slices, iters;
+};
+
+// IndexView of a static (FULL, not written by the step) view, with the
+// bitmaps and root map the pipeline asks for, as run_pipeline prepares them.
+static IndexView static_view(srd_engine& e, const SrdViewSpec& vs, std::vector<uint32_t>& vmax) {
+  Relation& r = e.rels[rel_id(e, vs.rel)];
+  Order order(vs.cols, vs.cols + vs.ncols);
+  const Index& x = full_index(e, r, order);
+  vmax = x.col_max;
+  if (vs.want_bitmaps && x.arity == 2 && e.bitmap_bytes) {
+    const HeavyBits* h = x.heavy.get();
+    if (!h || h->built_for != x.cols.p || h->built_n != x.n)
+      build_heavy_bitmaps(x, e.bitmap_min, e.bitmap_bytes, e.ws, e.s);
+  }
+  if (vs.want_root_pos && e.root_pos) {
+    const RootPos* m = x.rootpos.get();
+    if (!m || m->built_for != x.root_vals.p || m->built_r != x.r)
+      build_root_pos(x, std::max<uint64_t>(8ull * x.r, uint64_t(1) << 20), e.s);
+  }
+  return x.view();
+}
+
+static void copy_cols(uint32_t* dst, uint64_t ds, const uint32_t* src, uint64_t ss, uint32_t k,
+                      uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  for (uint32_t c = 0; c < k; ++c)
+    SRD_CHECK(hipMemcpyAsync(dst + uint64_t(c) * ds, src + uint64_t(c) * ss, n * 4,
+                             hipMemcpyDeviceToDevice, s));
+}
+
+// Allocate every buffer of the loop at the given capacities and fill the
+// launch templates.  The iteration's state (FULL, DELTA with its root
+// arrays, recorded deltas) is then copied in by the caller.
+static void devloop_alloc(srd_engine& e, const Step& st, DevLoop& L, const DevCaps& cp) {
+  hipStream_t s = e.s;
+  L.cap_new = round_stride(cp.new_rows);
+  L.cap_full = round_stride(cp.full_rows);
+  L.cap_keys = cp.keys;
+  L.cap_slices = cp.slices;
+  L.iter_cap = cp.iters;
+  const uint64_t cn = L.cap_new, cf = L.cap_full;
+  const uint64_t sd_tiles = std::max<uint64_t>(mp_tiles_for(cn + cf), 1);
+  const uint64_t mg_tiles = std::max<uint64_t>(mp_tiles_for(cf), 1);
+  const uint64_t rt_tiles = roots_dev_tiles(cn);
+  for (auto& R : L.rels) {
+    const uint32_t k = R.k;
+    R.raw.alloc(k * cn, s);
+    R.nsort.alloc(k * cn, s);
+    R.nroot_v.alloc(cn, s);
+    R.nroot_o.alloc(cn + 1, s);
+    R.sort_scratch.alloc(devsort_scratch_bytes(cn), s);
+    devsort_clear_lookback(R.sort_scratch.p, cn, s);
+    R.delta.alloc(k * cn, s);
+    R.droot_v.alloc(cn, s);
+    R.droot_o.alloc(cn + 1, s);
+    R.full[0].alloc(k * cf, s);
+    R.full[1].alloc(k * cf, s);
+    R.sd_split.alloc(sd_tiles + 1, s);
+    R.sd_tmp.alloc(k * cn, s);
+    R.sd_cnt.alloc(sd_tiles, s);
+    R.sd_off.alloc(sd_tiles + 1, s);
+    R.sd_scan.alloc(scan_dev_words(sd_tiles), s);
+    R.mg_split.alloc(mg_tiles + 1, s);
+    R.rt_pos.alloc(cn, s);
+    R.rt_cnt.alloc(rt_tiles, s);
+    R.rt_off.alloc(rt_tiles + 1, s);
+    R.rt_scan.alloc(scan_dev_words(rt_tiles), s);
+    R.deltas.alloc(L.iter_cap, s);
+    if (e.n_inst) R.inst.alloc(L.iter_cap * e.n_inst, s);
+  }
+  auto slot_of = [&](int ri) {
+    for (size_t j = 0; j < L.rels.size(); ++j)
+      if (L.rels[j].rel == ri) return int(j);
+    return -1;
+  };
+  for (size_t pi = 0; pi < L.pipes.size(); ++pi) {
+    DevPipeState& P = L.pipes[pi];
+    const SrdPipelineDesc* d = P.d;
+    P.key_rng.alloc(L.cap_keys * 2 * SRD_MAX_VIEWS, s);
+    P.key_aux.alloc(L.cap_keys, s);
+    P.key_work.alloc(L.cap_keys, s);
+    P.work_start.alloc(L.cap_keys + 1, s);
+    P.scan_keys.alloc(scan_dev_words(L.cap_keys), s);
+    P.counts.alloc(L.cap_slices, s);
+    P.offsets.alloc(L.cap_slices + 1, s);
+    P.scan_slices.alloc(scan_dev_words(L.cap_slices), s);
+    P.work_ctr.alloc(SRD_XCDS * SRD_CTR_STRIDE, s);
+    P.dyn.alloc(sizeof(SrdDyn), s);
+    std::memset(&P.a, 0, sizeof(P.a));
+    P.a.grain = e.grain_override ? e.grain_override : d->default_grain;
+    P.a.grid_blocks = 2048;
+    P.a.n_ranges = e.xcd_ranges;
+    P.a.key_rng = P.key_rng.p;
+    P.a.key_aux = P.key_aux.p;
+    P.a.key_work = P.key_work.p;
+    P.a.work_start = P.work_start.p;
+    P.a.counts = P.counts.p;
+    P.a.offsets = P.offsets.p;
+    P.a.work_ctr = P.work_ctr.p;
+    P.a.dyn = reinterpret_cast<const SrdDyn*>(P.dyn.p);
+    for (uint32_t k = 0; k < d->n_dest; ++k) {
+      const int j = slot_of(rel_id(e, d->dest[k].rel));
+      P.a.out[k].cols = L.rels[j].raw.p;
+      P.a.out[k].stride = cn;
+      P.a.out[k].base = 0;
+      P.out.rel[k] = j;
+    }
+    P.out.n_dest = d->n_dest;
+    P.out.pipe = uint32_t(pi);
+    P.out.grain = P.a.grain;
+    P.out.direct = P.direct ? 1u : 0u;
+    P.out.cap_slices = L.cap_slices;
+    P.out.cap_new = cn;
+    P.bind.cap_keys = L.cap_keys;
+    for (uint32_t v = 0; v < d->n_views; ++v) {
+      if (P.vrel[v] < 0) continue;  // static: prepared once in dyn_host
+      const DevRelState& R = L.rels[P.vrel[v]];
+      IndexView iv;
+      std::memset(&iv, 0, sizeof(iv));
+      iv.cols = R.delta.p;
+      iv.root_vals = R.droot_v.p;
+      iv.root_offs = R.droot_o.p;
+      iv.stride = cn;
+      iv.arity = R.k;
+      iv.bm_min = ~0u;
+      P.dyn_host.views.v[v] = iv;
+    }
+    P.dyn_host.skip = 0;
+    SRD_CHECK(hipMemcpyAsync(P.dyn.p, &P.dyn_host, sizeof(SrdDyn), hipMemcpyHostToDevice, s));
+  }
+  (void)st;
+}
+
+// Enqueue one iteration reading FULL from full[parity] (merged into the
+// other buffer).  No allocation, no host synchronisation: it may be captured.
+static void devloop_enqueue(srd_engine& e, const Step& st, DevLoop& L, int parity) {
+  hipStream_t s = e.s;
+  DevCtl* c = L.ctl;
+  const uint32_t nrels = uint32_t(L.rels.size());
+  const uint64_t cn = L.cap_new, cf = L.cap_full;
+  auto slot_of = [&](int ri) {
+    for (size_t j = 0; j < L.rels.size(); ++j)
+      if (L.rels[j].rel == ri) return uint32_t(j);
+    fail("device loop: relation outside the step");
+  };
+  std::vector<const Op*> ops;
+  flatten_ops(st.ops, ops);
+  size_t last_rebuild = 0;
+  for (size_t i = 0; i < ops.size(); ++i)
+    if (ops[i]->kind == OpKind::RebuildIndex) last_rebuild = i;
+  dl_iter_begin(c, nrels, s);
+  size_t pi = 0;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const Op* op = ops[i];
+    switch (op->kind) {
+      case OpKind::Pipeline: {
+        DevPipeState& P = L.pipes[pi];
+        const SrdPipelineDesc* d = P.d;
+        SrdDyn* dy = reinterpret_cast<SrdDyn*>(P.dyn.p);
+        dl_bind(c, dy, P.bind, s);
+        d->root_dyn(&P.a, s);
+        SRD_CHECK(hipGetLastError());
+        exclusive_scan_u64_dev(P.key_work.p, P.work_start.p, L.cap_keys, &c->keys[pi], P.scan_keys.p, s);
+        dl_plan(c, dy, P.work_start.p, P.out, s);
+        fill_u32(P.work_ctr.p, 0, SRD_XCDS * SRD_CTR_STRIDE, s);
+        if (P.direct) {
+          d->direct_dyn(&P.a, s);
+          SRD_CHECK(hipGetLastError());
+        } else {
+          d->count_dyn(&P.a, s);
+          SRD_CHECK(hipGetLastError());
+          exclusive_scan_u32_dev(P.counts.p, P.offsets.p, L.cap_slices, &c->slices[pi],
+                                 P.scan_slices.p, s);
+          dl_out(c, dy, P.offsets.p, P.out, s);
+          fill_u32(P.work_ctr.p, 0, SRD_XCDS * SRD_CTR_STRIDE, s);
+          d->write_dyn(&P.a, s);
+          SRD_CHECK(hipGetLastError());
+        }
+        ++pi;
+        break;
+      }
+      case OpKind::RebuildIndex: {
+        const uint32_t j = slot_of(op->rel);
+        DevRelState& R = L.rels[j];
+        const Order p = perm_between(R.raw_order, R.canon);
+        std::vector<const uint32_t*> in(R.k);
+        for (uint32_t c2 = 0; c2 < R.k; ++c2) in[c2] = R.raw.p + uint64_t(p[c2]) * cn;
+        build_index_dev(in.data(), R.k, cn, &c->new_n[j], 0, R.k <= 2 ? 1u : 2u,
+                        R.sort_scratch.p, R.nsort.p, cn, R.nroot_v.p, R.nroot_o.p, &c->newc[j][0],
+                        /*ticket=*/true, s, /*timed=*/false);
+        if (i == last_rebuild) {
+          dl_check(c, nrels, cf, s);
+        }
+        break;
+      }
+      case OpKind::ComputeDelta: {
+        const uint32_t j = slot_of(op->rel);
+        DevRelState& R = L.rels[j];
+        dl_sd_prep(c, j, s);
+        SetDiffDevScratch x{R.sd_split.p, R.sd_tmp.p, cn, R.sd_cnt.p, R.sd_off.p, R.sd_scan.p};
+        set_difference_dev(R.k, R.nsort.p, cn, R.full[parity].p, cf, cn + cf, c->sd_nab[j], R.delta.p,
+                           cn, x, &c->sd_total[j], s);
+        dl_sd_fin(c, j, s);
+        build_roots_dev(R.delta.p, cn, &c->delta_n[j], R.rt_pos.p, R.rt_cnt.p, R.rt_off.p,
+                        R.rt_scan.p, R.droot_v.p, R.droot_o.p, &c->delta_r[j], s);
+        if (e.n_inst) {
+          dl_inst(c, j, R.delta.p, e.inst_bounds.p, e.n_inst, R.inst.p, s);
+        }
+        break;
+      }
+      case OpKind::MergeIndex: {
+        const uint32_t j = slot_of(op->rel);
+        DevRelState& R = L.rels[j];
+        dl_mg_prep(c, j, s);
+        merge_dev(R.k, R.full[parity].p, cf, R.delta.p, cn, cf, c->mg_nab[j], R.full[parity ^ 1].p,
+                  cf, R.mg_split.p, s);
+        dl_mg_fin(c, j, s);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  DlEnd end{};
+  end.nrels = nrels;
+  for (uint32_t j = 0; j < nrels; ++j) end.deltas[j] = L.rels[j].deltas.p;
+  dl_iter_end(c, end, s);
+}
+
+static void devloop_capture(srd_engine& e, const Step& st, DevLoop& L) {
+  L.drop_graphs();
+  for (int p = 0; p < 2; ++p) {
+    hipGraph_t g = nullptr;
+    SRD_CHECK(hipStreamBeginCapture(e.s, hipStreamCaptureModeRelaxed));
+    try {
+      devloop_enqueue(e, st, L, p);
+    } catch (...) {
+      (void)hipStreamEndCapture(e.s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    SRD_CHECK(hipStreamEndCapture(e.s, &g));
+    hipError_t err = hipGraphInstantiate(&L.exec[p], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SRD_CHECK(err);
+  }
+}
+
+}  // namespace srd
+
+namespace srd {
+
+// Copy the loop state (FULL in full[parity], DELTA and its root arrays,
+// the recorded per-iteration sizes) from `from` into the fresh buffers of `to`.
+static void devloop_copy_state(srd_engine& e, const DevLoop& from, int parity, DevLoop& to,
+                               const DevCtl& h) {
+  hipStream_t s = e.s;
+  for (size_t j = 0; j < from.rels.size(); ++j) {
+    const DevRelState& A = from.rels[j];
+    DevRelState& B = to.rels[j];
+    copy_cols(B.full[0].p, to.cap_full, A.full[parity].p, from.cap_full, A.k, h.full_n[j], s);
+    copy_cols(B.delta.p, to.cap_new, A.delta.p, from.cap_new, A.k, h.delta_n[j], s);
+    if (h.delta_n[j]) {
+      SRD_CHECK(hipMemcpyAsync(B.droot_v.p, A.droot_v.p, h.delta_r[j] * 4, hipMemcpyDeviceToDevice, s));
+      SRD_CHECK(hipMemcpyAsync(B.droot_o.p, A.droot_o.p, (h.delta_r[j] + 1) * 4,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    if (h.iter) {
+      SRD_CHECK(hipMemcpyAsync(B.deltas.p, A.deltas.p, h.iter * 8, hipMemcpyDeviceToDevice, s));
+      if (e.n_inst)
+        SRD_CHECK(hipMemcpyAsync(B.inst.p, A.inst.p, h.iter * e.n_inst * 8, hipMemcpyDeviceToDevice, s));
+    }
+  }
+}
+
+// Run the step's iterations on the device.  Returns the iterations run;
+// *converged tells whether the fixpoint was reached (else the host loop
+// continues from the returned iteration with the state handed back).
+static uint64_t run_step_device(srd_engine& e, const Step& st, int step_no, uint64_t max_iters,
+                                bool* converged) {
+  hipStream_t s = e.s;
+  *converged = false;
+  std::vector<const Op*> ops;
+  flatten_ops(st.ops, ops);
+  auto L = std::make_unique<DevLoop>();
+  // relations and their entry state: FULL as built by the base step
+  uint64_t full_max = 0;
+  std::vector<const Index*> full0;
+  for (int ri : st.canonical_rels) {
+    Relation& r = e.rels[ri];
+    DevRelState R;
+    R.rel = ri;
+    R.k = r.arity;
+    R.canon = r.canonical;
+    const Index& f = full_index(e, r, r.canonical);
+    full0.push_back(&f);
+    full_max = std::max<uint64_t>(full_max, f.n);
+    for (uint32_t m : f.col_max) R.bound = std::max(R.bound, m);
+    L->rels.push_back(std::move(R));
+  }
+  auto slot_of = [&](int ri) {
+    for (size_t j = 0; j < L->rels.size(); ++j)
+      if (L->rels[j].rel == ri) return int(j);
+    return -1;
+  };
+  uint32_t vbound = 0;
+  for (const Op* op : ops) {
+    if (op->kind != OpKind::Pipeline) continue;
+    DevPipeState P;
+    P.d = op->pipe;
+    const SrdPipelineDesc* d = P.d;
+    P.direct = d->direct_ok && d->direct_dyn && !e.two_pass;
+    std::memset(&P.dyn_host, 0, sizeof(P.dyn_host));
+    std::memset(&P.bind, 0, sizeof(P.bind));
+    std::memset(&P.out, 0, sizeof(P.out));
+    for (uint32_t v = 0; v < d->n_views; ++v) {
+      const int j = slot_of(rel_id(e, d->views[v].rel));
+      P.vrel[v] = j;
+      P.bind.vrel[v] = j;
+      if (j < 0) {
+        std::vector<uint32_t> vm;
+        P.dyn_host.views.v[v] = static_view(e, d->views[v], vm);
+        for (uint32_t m : vm) vbound = std::max(vbound, m);
+      }
+    }
+    P.bind.n_views = d->n_views;
+    P.bind.root_kind = d->root_kind;
+    P.bind.n_root_src = d->root_kind == SRD_ROOT_JOIN ? d->n_root_src : 1;
+    for (uint32_t i = 0; i < SRD_MAX_VIEWS; ++i) P.bind.root_view[i] = d->root_view[i];
+    P.bind.pipe = uint32_t(L->pipes.size());
+    for (uint32_t k = 0; k < d->n_dest; ++k) {
+      const int j = slot_of(rel_id(e, d->dest[k].rel));
+      L->rels[j].raw_order = Order(d->dest[k].cols, d->dest[k].cols + d->dest[k].arity);
+    }
+    L->pipes.push_back(std::move(P));
+  }
+  for (auto& R : L->rels) R.bound = std::max(R.bound, vbound);
+
+  DevCaps cp;
+  cp.full_rows = std::max<uint64_t>(uint64_t(1) << 20, 4 * full_max);
+  cp.new_rows = std::max<uint64_t>(uint64_t(1) << 20, 2 * full_max);
+  cp.keys = uint64_t(1) << 20;
+  cp.slices = uint64_t(1) << 16;
+  cp.iters = 1024;
+  if (e.devloop_caps) {
+    cp.full_rows = cp.new_rows = cp.keys = cp.slices = e.devloop_caps;
+    cp.iters = std::max<uint64_t>(e.devloop_caps, 2);
+  }
+  // DELTA starts as FULL: both must hold the base step's FULL
+  cp.full_rows = std::max<uint64_t>(cp.full_rows, std::max<uint64_t>(full_max, 1));
+  cp.new_rows = std::max<uint64_t>(cp.new_rows, std::max<uint64_t>(full_max, 1));
+  devloop_alloc(e, st, *L, cp);
+
+  // DELTA := FULL (iteration 0 reads FULL where the rules read DELTA)
+  DevCtl h;
+  std::memset(&h, 0, sizeof(h));
+  h.max_iter = max_iters;
+  h.iter_cap = L->iter_cap;
+  for (size_t j = 0; j < L->rels.size(); ++j) {
+    DevRelState& R = L->rels[j];
+    const Index& f = *full0[j];
+    if (f.n > L->cap_full || f.n > L->cap_new) fail("device loop: initial capacities below FULL");
+    copy_cols(R.full[0].p, L->cap_full, f.cols.p, f.stride, R.k, f.n, s);
+    copy_cols(R.delta.p, L->cap_new, f.cols.p, f.stride, R.k, f.n, s);
+    if (f.n) {
+      SRD_CHECK(hipMemcpyAsync(R.droot_v.p, f.root_vals.p, uint64_t(f.r) * 4, hipMemcpyDeviceToDevice, s));
+      SRD_CHECK(hipMemcpyAsync(R.droot_o.p, f.root_offs.p, (uint64_t(f.r) + 1) * 4,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    h.full_n[j] = h.delta_n[j] = f.n;
+    h.delta_r[j] = f.r;
+  }
+  L->ctl_buf.alloc(sizeof(DevCtl), s);
+  L->ctl = reinterpret_cast<DevCtl*>(L->ctl_buf.p);
+  uint64_t pin_bytes = 0;
+  DevCtl* hp = static_cast<DevCtl*>(pinned_acquire(sizeof(DevCtl), &pin_bytes));
+  std::unique_ptr<DevCtl, std::function<void(DevCtl*)>> pin_guard(
+      hp, [pin_bytes](DevCtl* q) { pinned_release(q, pin_bytes); });
+  *hp = h;
+  SRD_CHECK(hipMemcpyAsync(L->ctl, hp, sizeof(DevCtl), hipMemcpyHostToDevice, s));
+  SRD_CHECK(hipStreamSynchronize(s));
+  if (e.devloop_graph) devloop_capture(e, st, *L);
+
+  int parity = 0;
+  uint64_t done_iters = 0;
+  const uint64_t batch = e.devloop_batch;
+  for (;;) {
+    for (uint64_t b = 0; b < batch; ++b) {
+      const int p = int((parity + b) & 1);
+      if (e.devloop_graph) SRD_CHECK(hipGraphLaunch(L->exec[p], s));
+      else devloop_enqueue(e, st, *L, p);
+    }
+    SRD_CHECK(hipMemcpyAsync(hp, L->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, s));
+    SRD_CHECK(hipStreamSynchronize(s));
+    h = *hp;
+    parity ^= int((h.iter - done_iters) & 1);
+    done_iters = h.iter;
+    if (e.trace) {
+      std::fprintf(stderr, "[srd]   device loop: %llu iterations, overflow %u, done %u\n",
+                   (unsigned long long)h.iter, h.overflow, h.done);
+      std::fflush(stderr);
+    }
+    if (h.err) fail("device loop: a device-planned sort reported an error");
+    if (h.overflow) {
+      DevCaps nc{L->cap_new, L->cap_full, L->cap_keys, L->cap_slices, L->iter_cap};
+      auto grow = [](uint64_t cur, uint64_t need) {
+        return std::max<uint64_t>(2 * cur, need + need / 4);
+      };
+      if (h.overflow & kOvfNew) nc.new_rows = grow(nc.new_rows, h.need_new);
+      if (h.overflow & kOvfFull) nc.full_rows = grow(nc.full_rows, h.need_full);
+      if (h.overflow & kOvfKeys) nc.keys = grow(nc.keys, h.need_keys);
+      if (h.overflow & kOvfSlices) nc.slices = grow(nc.slices, h.need_slices);
+      if (h.overflow & kOvfIters) nc.iters = 2 * nc.iters;
+      if (e.trace) {
+        std::fprintf(stderr, "[srd]   device loop: grow (overflow %u): new %llu full %llu keys %llu "
+                     "slices %llu iters %llu\n", h.overflow, (unsigned long long)nc.new_rows,
+                     (unsigned long long)nc.full_rows, (unsigned long long)nc.keys,
+                     (unsigned long long)nc.slices, (unsigned long long)nc.iters);
+        std::fflush(stderr);
+      }
+      auto L2 = std::make_unique<DevLoop>();
+      for (auto& R : L->rels) {
+        DevRelState B;
+        B.rel = R.rel;
+        B.k = R.k;
+        B.canon = R.canon;
+        B.raw_order = R.raw_order;
+        B.bound = R.bound;
+        L2->rels.push_back(std::move(B));
+      }
+      for (auto& P : L->pipes) {
+        DevPipeState Q;
+        Q.d = P.d;
+        Q.direct = P.direct;
+        Q.dyn_host = P.dyn_host;
+        Q.bind = P.bind;
+        std::memcpy(Q.vrel, P.vrel, sizeof(Q.vrel));
+        L2->pipes.push_back(std::move(Q));
+      }
+      devloop_alloc(e, st, *L2, nc);
+      devloop_copy_state(e, *L, parity, *L2, h);
+      L2->ctl_buf = L->ctl_buf;
+      L2->ctl = L->ctl;
+      h.overflow = 0;
+      h.skip = 0;
+      h.need_new = h.need_full = h.need_keys = h.need_slices = 0;
+      h.iter_cap = L2->iter_cap;
+      *hp = h;
+      SRD_CHECK(hipMemcpyAsync(L2->ctl, hp, sizeof(DevCtl), hipMemcpyHostToDevice, s));
+      SRD_CHECK(hipStreamSynchronize(s));
+      L = std::move(L2);
+      parity = 0;
+      if (e.devloop_graph) devloop_capture(e, st, *L);
+      continue;
+    }
+    if (h.done || h.iter >= max_iters) break;
+  }
+
+  // hand the state back to the relations
+  std::vector<uint64_t> dl(h.iter);
+  for (size_t j = 0; j < L->rels.size(); ++j) {
+    DevRelState& R = L->rels[j];
+    Relation& r = e.rels[R.rel];
+    if (h.iter) {
+      SRD_CHECK(hipMemcpyAsync(dl.data(), R.deltas.p, h.iter * 8, hipMemcpyDeviceToHost, s));
+      SRD_CHECK(hipStreamSynchronize(s));
+    }
+    if (r.deltas_step != step_no) {
+      r.deltas.clear();
+      r.inst_deltas.clear();
+      r.deltas_step = step_no;
+    }
+    for (uint64_t i = 0; i < h.iter; ++i) {
+      r.deltas.push_back(dl[i]);
+      if (e.n_inst) {
+        DevBuf<uint64_t> per;
+        per.p = R.inst.p + i * e.n_inst;
+        per.n = e.n_inst;
+        per.s = s;
+        per.owner = R.inst.owner;
+        r.inst_deltas.push_back(per);
+      }
+    }
+    Index f;
+    f.arity = R.k;
+    f.n = h.full_n[j];
+    f.stride = L->cap_full;
+    f.cols = R.full[parity];
+    f.col_max.assign(R.k, f.n ? R.bound : 0);
+    build_roots(f, e.ws, e.s);
+    Index d;
+    d.arity = R.k;
+    d.n = h.delta_n[j];
+    d.r = uint32_t(h.delta_r[j]);
+    d.stride = L->cap_new;
+    d.col_max.assign(R.k, d.n ? R.bound : 0);
+    if (d.n) {
+      d.cols = R.delta;
+      d.root_vals = R.droot_v;
+      d.root_offs = R.droot_o;
+    }
+    r.segs.clear();
+    r.full.clear();
+    r.full_version += h.iter;
+    r.delta_seq = r.full_version;
+    Versioned v;
+    v.idx = std::move(f);
+    v.version = r.full_version;
+    r.full[r.canonical] = std::move(v);
+    r.delta.clear();
+    r.delta[r.canonical] = std::move(d);
+    clear_new(r);
+  }
+  e.join_probes += h.join_rows;
+  *converged = h.done != 0;
+  return h.iter;
+}
+
+}  // namespace srd
+
+namespace srd {
+
 static void run_program(srd_engine& e, uint64_t max_iters) {
   reset_state(e);
   for (auto& r : e.rels) upload_input(e, r);
@@ -1270,7 +1976,16 @@ static void run_program(srd_engine& e, uint64_t max_iters) {
     } else {
       for (int ri : st.segmented_rels) e.rels[ri].segmented = true;
       uint64_t it = 0;
-      for (; it < max_iters; ++it) {
+      bool converged = false;
+      std::string why;
+      if (e.devloop && !e.count_probes && devloop_eligible(e, st, why)) {
+        it = run_step_device(e, st, int(si), max_iters, &converged);
+        converged = converged || it >= max_iters;
+      } else if (e.trace && e.devloop) {
+        std::fprintf(stderr, "[srd] step %zu: host-driven loop (%s)\n", si, why.c_str());
+        std::fflush(stderr);
+      }
+      for (; !converged && it < max_iters; ++it) {
         if (it > 0) {
           uint64_t tot = 0;
           for (int ri : st.canonical_rels) {
@@ -1442,6 +2157,13 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
       e->min_slices_per_wave = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_STAGE_MAX_BYTES"))
       e->stage_max_bytes = std::max<uint64_t>(std::strtoull(c, nullptr, 10), uint64_t(1) << 20);
+    if (const char* c = std::getenv("SRDATALOG_DEVLOOP")) e->devloop = !(*c == '0' && c[1] == 0);
+    if (const char* c = std::getenv("SRDATALOG_DEVLOOP_GRAPH"))
+      e->devloop_graph = !(*c == '0' && c[1] == 0);
+    if (const char* c = std::getenv("SRDATALOG_DEVLOOP_BATCH"))
+      e->devloop_batch = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);
+    if (const char* c = std::getenv("SRDATALOG_DEVLOOP_CAPS"))
+      e->devloop_caps = std::strtoull(c, nullptr, 10);
     if (const char* m = std::getenv("SRDATALOG_JOIN_MODE")) {
       e->two_pass = std::string(m) == "twopass";
       e->no_direct = std::string(m) == "fused";

@@ -54,8 +54,8 @@ __device__ __forceinline__ Rng run_at(const uint32_t* __restrict__ col, uint32_t
 // Slices [slice_lo, n_slices) are split into SRD_XCDS ranges; a wave takes
 // the next slice of its own XCD's range, then of the following ranges in
 // turn; it returns n_slices when every range is exhausted.
-__device__ __forceinline__ uint64_t next_slice(const SrdLaunchArgs& A) {
-  const uint64_t lo = A.slice_lo, n = A.n_slices;
+__device__ __forceinline__ uint64_t next_slice(const SrdLaunchArgs& A, uint64_t n) {
+  const uint64_t lo = A.slice_lo;
   if (n <= lo) return n;
   const uint64_t span = n - lo;
   const uint32_t nr = A.n_ranges ? A.n_ranges : 1u;

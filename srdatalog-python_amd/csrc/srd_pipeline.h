@@ -75,6 +75,8 @@ struct SrdOut {
   uint64_t base;    // first row this launch writes
 };
 
+struct SrdDyn;
+
 struct SrdLaunchArgs {
   SrdViews views;
   uint32_t n_root;             // driver root keys (join) or 1 (scan)
@@ -109,6 +111,24 @@ struct SrdLaunchArgs {
   // probe runs only (else nullptr): the root launch and the probe mode add
   // their lower/upper-bound searches, root-map lookups and bitmap tests
   unsigned long long* probe_ctr;
+  // device-sized launches (the *_dyn entry points) only: views, key and
+  // slice counts and output bases are read from this device struct, which
+  // kernels of the same iteration fill in, instead of the fields above
+  const SrdDyn* dyn;
+};
+
+// Device-resident part of a device-sized launch.  skip != 0 turns every
+// launch reading it into a no-op (an iteration after convergence, or one
+// abandoned because a buffer would overflow).
+struct SrdDyn {
+  SrdViews views;
+  uint64_t total_work;
+  uint64_t n_slices;
+  uint64_t out_base[SRD_MAX_DESTS];
+  uint32_t n_root;
+  uint32_t drv_view;
+  uint32_t skip;
+  uint32_t pad_;
 };
 
 // Slices are split into SRD_XCDS contiguous ranges, one per XCD (workgroups
@@ -150,6 +170,12 @@ struct SrdPipelineDesc {
   // order with no repeats, in every mode.
   uint32_t sorted_output;
   SrdLaunchFn probe;                 // mode 5 (search counting)
+  // device-sized variants (SrdLaunchArgs::dyn set; grid_blocks fixed);
+  // nullptr outside recursive steps, and direct_dyn unless direct_ok
+  SrdLaunchFn root_dyn;
+  SrdLaunchFn count_dyn;
+  SrdLaunchFn write_dyn;
+  SrdLaunchFn direct_dyn;
 };
 
 #ifdef __cplusplus

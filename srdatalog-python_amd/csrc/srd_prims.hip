@@ -32,6 +32,7 @@
 #include <thread>
 #include <vector>
 
+#include "srd_devloop.h"
 #include "srd_runtime.h"
 
 namespace srd {
@@ -790,8 +791,10 @@ __device__ __forceinline__ uint32_t tile_pos(uint32_t w, uint32_t i, uint32_t l)
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void scan_reduce_kernel(const T* __restrict__ in, uint64_t n,
-                                                             uint64_t* __restrict__ partial) {
+                                                             uint64_t* __restrict__ partial,
+                                                             const uint64_t* __restrict__ n_dev = nullptr) {
   __shared__ uint64_t wsum[kWavesPerBlock];
+  if (n_dev) n = min(n, *n_dev);  // device-sized: entries past the count are 0
   const uint64_t base = uint64_t(blockIdx.x) * kTile;
   uint64_t s = 0;
 #pragma unroll
@@ -814,8 +817,10 @@ __global__ __launch_bounds__(kBlock) void scan_reduce_kernel(const T* __restrict
 template <class T>
 __global__ __launch_bounds__(kBlock) void scan_tile_kernel(const T* __restrict__ in, uint64_t n,
                                                            const uint64_t* __restrict__ carry,
-                                                           uint64_t* __restrict__ out) {
+                                                           uint64_t* __restrict__ out,
+                                                           const uint64_t* __restrict__ n_dev = nullptr) {
   __shared__ uint64_t wtot[kWavesPerBlock];
+  if (n_dev) n = min(n, *n_dev);
   const uint32_t w = threadIdx.x / kWave, l = lane_id();
   const uint64_t base = uint64_t(blockIdx.x) * kTile;
   uint64_t v[kItems];
@@ -2099,8 +2104,10 @@ void devsort_clear_lookback(void* scratch, uint64_t cap, hipStream_t s) {
 __global__ __launch_bounds__(kBlock) void root_mark_kernel(const uint32_t* __restrict__ c0,
                                                            uint64_t n,
                                                            uint32_t* __restrict__ pos,
-                                                           uint32_t* __restrict__ tile_r) {
+                                                           uint32_t* __restrict__ tile_r,
+                                                           const uint64_t* __restrict__ n_dev = nullptr) {
   __shared__ uint32_t sr[kWavesPerBlock];
+  if (n_dev) n = min(n, *n_dev);  // tiles past the count record 0 starts
   const uint32_t w = threadIdx.x / kWave, l = lane_id();
   const uint64_t base = uint64_t(blockIdx.x) * kTile;
   uint64_t m[kItems];
@@ -2371,7 +2378,9 @@ __device__ __forceinline__ uint64_t mp_split(const uint32_t* a, uint64_t sa, uin
 template <int K>
 __global__ __launch_bounds__(kBlock) void mp_partition_kernel(
     const uint32_t* __restrict__ a, uint64_t sa, uint64_t na, const uint32_t* __restrict__ b,
-    uint64_t sb, uint64_t nb, uint64_t tiles, uint64_t* __restrict__ split) {
+    uint64_t sb, uint64_t nb, uint64_t tiles, uint64_t* __restrict__ split,
+    const uint64_t* __restrict__ nab = nullptr) {
+  if (nab) na = nab[0], nb = nab[1];  // device-sized: split points past the rows are na
   const uint64_t t = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
   if (t > tiles) return;
   const uint64_t d = min<uint64_t>(t * kMpTile, na + nb);
@@ -2404,7 +2413,7 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
     uint64_t sb, uint64_t nb, const uint64_t* __restrict__ split, uint32_t* __restrict__ out,
     uint64_t so, uint64_t* __restrict__ lookback, uint32_t* __restrict__ ctr, uint32_t epoch,
     uint64_t* __restrict__ total_out, uint32_t* __restrict__ tile_cnt,
-    const uint64_t* __restrict__ tile_off, bool ticket) {
+    const uint64_t* __restrict__ tile_off, bool ticket, const uint64_t* __restrict__ nab = nullptr) {
   constexpr bool DIFF = MODE != 1;
   __shared__ uint32_t st[K][kMpLds];
   __shared__ uint32_t s_wsum[kWavesPerBlock];
@@ -2418,6 +2427,15 @@ __global__ __launch_bounds__(kBlock) void mp_tile_kernel(
     if (tid == 0) s_tile = atomicAdd(&ctr[0], 1u);
     __syncthreads();
     tile = s_tile;
+  }
+  if (nab) {
+    // device-sized launch (MODE 1 and 4): tiles past the rows do nothing
+    na = nab[0];
+    nb = nab[1];
+    if (uint64_t(tile) * kMpTile >= na + nb) {
+      if ((MODE == 2 || MODE == 4) && tid == 0) tile_cnt[tile] = 0;
+      return;
+    }
   }
   const uint64_t d0 = uint64_t(tile) * kMpTile;
   const uint64_t d1 = min<uint64_t>(d0 + kMpTile, na + nb);
@@ -2808,6 +2826,195 @@ void merge_disjoint(const Index& a, const Index& b, Index& out, Workspace& ws, h
     case 4: merge_k<4>(a, b, out, ws, s); break;
     default: fail("merge_disjoint: arity > 4 not supported");
   }
+}
+
+
+// ============================================== device-sized primitives ====
+// For device-resident iterations: row counts are read from device words,
+// launch shapes come from host capacities, nothing is read back and
+// nothing is allocated; blocks past the counted rows exit at once.
+
+uint64_t scan_dev_words(uint64_t cap) { return std::max<uint64_t>(scan_noalloc_words(cap), 1); }
+
+template <class T>
+static void scan_dev(const T* in, uint64_t* out, uint64_t cap, const uint64_t* n_dev, uint64_t* tmp,
+                     hipStream_t s) {
+  if (cap <= kTile) {
+    scan_tile_kernel<T><<<1, kBlock, 0, s>>>(in, cap, nullptr, out, n_dev);
+    SRD_CHECK(hipGetLastError());
+    return;
+  }
+  const uint64_t tiles = (cap + kTile - 1) / kTile;
+  uint64_t* partial = tmp;
+  uint64_t* carry = tmp + tiles;
+  scan_reduce_kernel<T><<<uint32_t(tiles), kBlock, 0, s>>>(in, cap, partial, n_dev);
+  SRD_CHECK(hipGetLastError());
+  // every tile past the counted rows contributes 0: the partials scan at
+  // the capacity's tile count
+  scan_noalloc<uint64_t>(partial, carry, tiles, tmp + 2 * tiles + 1, s);
+  scan_tile_kernel<T><<<uint32_t(tiles), kBlock, 0, s>>>(in, cap, carry, out, n_dev);
+  SRD_CHECK(hipGetLastError());
+}
+
+void exclusive_scan_u32_dev(const uint32_t* in, uint64_t* out, uint64_t cap, const uint64_t* n_dev,
+                            uint64_t* tmp, hipStream_t s) {
+  scan_dev<uint32_t>(in, out, cap, n_dev, tmp, s);
+}
+
+void exclusive_scan_u64_dev(const uint64_t* in, uint64_t* out, uint64_t cap, const uint64_t* n_dev,
+                            uint64_t* tmp, hipStream_t s) {
+  scan_dev<uint64_t>(in, out, cap, n_dev, tmp, s);
+}
+
+__global__ void copy_word_kernel(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
+uint64_t mp_tiles_for(uint64_t rows) { return (rows + kMpTile - 1) / kMpTile; }
+
+template <int K>
+static void set_difference_dev_k(const uint32_t* a, uint64_t sa, const uint32_t* b, uint64_t sb,
+                                 uint64_t cap_ab, const uint64_t* nab, uint32_t* out, uint64_t so,
+                                 const SetDiffDevScratch& x, uint64_t* total_out, hipStream_t s) {
+  const uint64_t tiles = std::max<uint64_t>(mp_tiles_for(cap_ab), 1);
+  mp_partition_kernel<K><<<uint32_t((tiles + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      a, sa, 0, b, sb, 0, tiles, x.split, nab);
+  SRD_CHECK(hipGetLastError());
+  mp_tile_kernel<K, 4><<<uint32_t(tiles), kBlock, 0, s>>>(a, sa, 0, b, sb, 0, x.split, x.tmp,
+                                                          x.tmp_stride, nullptr, nullptr, 0, nullptr,
+                                                          x.cnt, nullptr, false, nab);
+  SRD_CHECK(hipGetLastError());
+  scan_dev<uint32_t>(x.cnt, x.off, tiles, nullptr, x.scan_tmp, s);
+  mp_gather_kernel<K><<<uint32_t(tiles), kBlock, 0, s>>>(x.tmp, x.tmp_stride, x.split, x.cnt, x.off,
+                                                         out, so);
+  SRD_CHECK(hipGetLastError());
+  copy_word_kernel<<<1, 64, 0, s>>>(x.off + tiles, total_out);
+  SRD_CHECK(hipGetLastError());
+}
+
+void set_difference_dev(uint32_t k, const uint32_t* a, uint64_t sa, const uint32_t* b, uint64_t sb,
+                        uint64_t cap_ab, const uint64_t* nab, uint32_t* out, uint64_t so,
+                        const SetDiffDevScratch& x, uint64_t* total_out, hipStream_t s) {
+  switch (k) {
+    case 1: set_difference_dev_k<1>(a, sa, b, sb, cap_ab, nab, out, so, x, total_out, s); break;
+    case 2: set_difference_dev_k<2>(a, sa, b, sb, cap_ab, nab, out, so, x, total_out, s); break;
+    case 3: set_difference_dev_k<3>(a, sa, b, sb, cap_ab, nab, out, so, x, total_out, s); break;
+    case 4: set_difference_dev_k<4>(a, sa, b, sb, cap_ab, nab, out, so, x, total_out, s); break;
+    default: fail("set_difference_dev: arity must be in [1, 4]");
+  }
+}
+
+template <int K>
+static void merge_dev_k(const uint32_t* a, uint64_t sa, const uint32_t* b, uint64_t sb,
+                        uint64_t cap_ab, const uint64_t* nab, uint32_t* out, uint64_t so,
+                        uint64_t* split, hipStream_t s) {
+  const uint64_t tiles = std::max<uint64_t>(mp_tiles_for(cap_ab), 1);
+  mp_partition_kernel<K><<<uint32_t((tiles + 1 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      a, sa, 0, b, sb, 0, tiles, split, nab);
+  SRD_CHECK(hipGetLastError());
+  mp_tile_kernel<K, 1><<<uint32_t(tiles), kBlock, 0, s>>>(a, sa, 0, b, sb, 0, split, out, so,
+                                                          nullptr, nullptr, 0, nullptr, nullptr,
+                                                          nullptr, false, nab);
+  SRD_CHECK(hipGetLastError());
+}
+
+void merge_dev(uint32_t k, const uint32_t* a, uint64_t sa, const uint32_t* b, uint64_t sb,
+               uint64_t cap_ab, const uint64_t* nab, uint32_t* out, uint64_t so, uint64_t* split,
+               hipStream_t s) {
+  switch (k) {
+    case 1: merge_dev_k<1>(a, sa, b, sb, cap_ab, nab, out, so, split, s); break;
+    case 2: merge_dev_k<2>(a, sa, b, sb, cap_ab, nab, out, so, split, s); break;
+    case 3: merge_dev_k<3>(a, sa, b, sb, cap_ab, nab, out, so, split, s); break;
+    case 4: merge_dev_k<4>(a, sa, b, sb, cap_ab, nab, out, so, split, s); break;
+    default: fail("merge_dev: arity must be in [1, 4]");
+  }
+}
+
+__global__ void roots_dev_finish_kernel(const uint64_t* __restrict__ n_dev,
+                                        const uint64_t* __restrict__ total,
+                                        uint32_t* __restrict__ root_offs, uint64_t* __restrict__ r_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint64_t r = *total;
+  root_offs[r] = uint32_t(*n_dev);
+  *r_out = r;
+}
+
+void build_roots_dev(const uint32_t* c0, uint64_t cap, const uint64_t* n_dev, uint32_t* pos,
+                     uint32_t* tile_r, uint64_t* tile_off, uint64_t* scan_tmp, uint32_t* root_vals,
+                     uint32_t* root_offs, uint64_t* r_out, hipStream_t s) {
+  const uint64_t tiles = std::max<uint64_t>((cap + kTile - 1) / kTile, 1);
+  root_mark_kernel<<<uint32_t(tiles), kBlock, 0, s>>>(c0, cap, pos, tile_r, n_dev);
+  SRD_CHECK(hipGetLastError());
+  scan_dev<uint32_t>(tile_r, tile_off, tiles, nullptr, scan_tmp, s);
+  root_gather_kernel<<<uint32_t(tiles), kBlock, 0, s>>>(c0, pos, tile_r, tile_off, root_vals,
+                                                        root_offs);
+  SRD_CHECK(hipGetLastError());
+  roots_dev_finish_kernel<<<1, 64, 0, s>>>(n_dev, tile_off + tiles, root_offs, r_out);
+  SRD_CHECK(hipGetLastError());
+}
+
+uint64_t roots_dev_tiles(uint64_t cap) { return std::max<uint64_t>((cap + kTile - 1) / kTile, 1); }
+
+
+// ====================================== device-resident iteration steps ====
+// One-thread kernels running the bookkeeping steps of srd_devloop.h.
+__global__ void dl_iter_begin_kernel(DevCtl* c, uint32_t nrels) { dl_iter_begin_step(c, nrels); }
+__global__ void dl_bind_kernel(DevCtl* c, SrdDyn* d, DlBind b) { dl_bind_step(c, d, b); }
+__global__ void dl_plan_kernel(DevCtl* c, SrdDyn* d, const uint64_t* ws, DlOut o) {
+  dl_plan_step(c, d, ws, o);
+}
+__global__ void dl_out_kernel(DevCtl* c, SrdDyn* d, const uint64_t* off, DlOut o) {
+  dl_out_step(c, d, off, o);
+}
+__global__ void dl_check_kernel(DevCtl* c, uint32_t nrels, uint64_t cap_full) {
+  dl_check_step(c, nrels, cap_full);
+}
+__global__ void dl_sd_prep_kernel(DevCtl* c, uint32_t r) { dl_sd_prep_step(c, r); }
+__global__ void dl_sd_fin_kernel(DevCtl* c, uint32_t r) { dl_sd_fin_step(c, r); }
+__global__ void dl_mg_prep_kernel(DevCtl* c, uint32_t r) { dl_mg_prep_step(c, r); }
+__global__ void dl_mg_fin_kernel(DevCtl* c, uint32_t r) { dl_mg_fin_step(c, r); }
+__global__ void dl_iter_end_kernel(DevCtl* c, DlEnd e) { dl_iter_end_step(c, e); }
+
+__global__ __launch_bounds__(kBlock) void dl_inst_kernel(const DevCtl* c, uint32_t r,
+                                                         const uint32_t* __restrict__ col0,
+                                                         const uint32_t* __restrict__ bounds,
+                                                         uint32_t n_inst, uint64_t* __restrict__ out) {
+  if (c->skip) return;
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= n_inst) return;
+  const uint32_t n = uint32_t(c->delta_n[r]);
+  const uint32_t lo = n ? lower_bound(col0, 0, n, bounds[g]) : 0;
+  const uint32_t hi = n ? lower_bound(col0, lo, n, bounds[g + 1]) : 0;
+  out[c->iter * n_inst + g] = hi - lo;
+}
+
+#define SRD_DL_LAUNCH(kern, ...)                  \
+  do {                                            \
+    kern<<<1, 64, 0, s>>>(__VA_ARGS__);           \
+    SRD_CHECK(hipGetLastError());                 \
+  } while (0)
+
+void dl_iter_begin(DevCtl* c, uint32_t nrels, hipStream_t s) { SRD_DL_LAUNCH(dl_iter_begin_kernel, c, nrels); }
+void dl_bind(DevCtl* c, SrdDyn* d, const DlBind& b, hipStream_t s) { SRD_DL_LAUNCH(dl_bind_kernel, c, d, b); }
+void dl_plan(DevCtl* c, SrdDyn* d, const uint64_t* work_start, const DlOut& o, hipStream_t s) {
+  SRD_DL_LAUNCH(dl_plan_kernel, c, d, work_start, o);
+}
+void dl_out(DevCtl* c, SrdDyn* d, const uint64_t* offsets, const DlOut& o, hipStream_t s) {
+  SRD_DL_LAUNCH(dl_out_kernel, c, d, offsets, o);
+}
+void dl_check(DevCtl* c, uint32_t nrels, uint64_t cap_full, hipStream_t s) {
+  SRD_DL_LAUNCH(dl_check_kernel, c, nrels, cap_full);
+}
+void dl_sd_prep(DevCtl* c, uint32_t r, hipStream_t s) { SRD_DL_LAUNCH(dl_sd_prep_kernel, c, r); }
+void dl_sd_fin(DevCtl* c, uint32_t r, hipStream_t s) { SRD_DL_LAUNCH(dl_sd_fin_kernel, c, r); }
+void dl_mg_prep(DevCtl* c, uint32_t r, hipStream_t s) { SRD_DL_LAUNCH(dl_mg_prep_kernel, c, r); }
+void dl_mg_fin(DevCtl* c, uint32_t r, hipStream_t s) { SRD_DL_LAUNCH(dl_mg_fin_kernel, c, r); }
+void dl_iter_end(DevCtl* c, const DlEnd& e, hipStream_t s) { SRD_DL_LAUNCH(dl_iter_end_kernel, c, e); }
+void dl_inst(const DevCtl* c, uint32_t r, const uint32_t* col0, const uint32_t* bounds,
+             uint32_t n_inst, uint64_t* out, hipStream_t s) {
+  if (n_inst == 0) return;
+  dl_inst_kernel<<<(n_inst + kBlock - 1) / kBlock, kBlock, 0, s>>>(c, r, col0, bounds, n_inst, out);
+  SRD_CHECK(hipGetLastError());
 }
 
 }  // namespace srd

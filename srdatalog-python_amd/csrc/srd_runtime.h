@@ -235,6 +235,48 @@ void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const 
 // first use).
 void devsort_clear_lookback(void* scratch, uint64_t cap, hipStream_t s);
 
+// ------------------------------------------------ device-sized primitives --
+// For device-resident iterations (a recursive step whose iterations are
+// enqueued or captured without host read-backs): row counts are read from
+// device words, launch shapes come from host capacities; nothing is read
+// back or allocated.  Scratch sizes: scan_dev_words(entries) u64 words for a
+// scan, mp_tiles_for(rows) merge-path tiles, roots_dev_tiles(rows) tiles.
+uint64_t scan_dev_words(uint64_t cap);
+// out[0..n] = exclusive prefix sums of in[0..n), n = *n_dev <= cap (n_dev
+// null: n = cap), out[n] = total.
+void exclusive_scan_u32_dev(const uint32_t* in, uint64_t* out, uint64_t cap, const uint64_t* n_dev,
+                            uint64_t* tmp, hipStream_t s);
+void exclusive_scan_u64_dev(const uint64_t* in, uint64_t* out, uint64_t cap, const uint64_t* n_dev,
+                            uint64_t* tmp, hipStream_t s);
+uint64_t mp_tiles_for(uint64_t rows);
+uint64_t roots_dev_tiles(uint64_t cap);
+struct SetDiffDevScratch {
+  uint64_t* split;      // tiles + 1
+  uint32_t* tmp;        // k columns of tmp_stride >= |a| rows
+  uint64_t tmp_stride;
+  uint32_t* cnt;        // tiles
+  uint64_t* off;        // tiles + 1
+  uint64_t* scan_tmp;   // scan_dev_words(tiles)
+};
+// out = a - b for sorted duplicate-free k-column inputs (k <= 4); nab =
+// device words {|a|, |b|} with |a| + |b| <= cap_ab (tiles =
+// mp_tiles_for(cap_ab)); *total_out = rows kept.
+void set_difference_dev(uint32_t k, const uint32_t* a, uint64_t sa, const uint32_t* b, uint64_t sb,
+                        uint64_t cap_ab, const uint64_t* nab, uint32_t* out, uint64_t so,
+                        const SetDiffDevScratch& x, uint64_t* total_out, hipStream_t s);
+// out = a u b for disjoint inputs, |a| + |b| (device words nab) <= cap_ab;
+// split: mp_tiles_for(cap_ab) + 1 words.
+void merge_dev(uint32_t k, const uint32_t* a, uint64_t sa, const uint32_t* b, uint64_t sb,
+               uint64_t cap_ab, const uint64_t* nab, uint32_t* out, uint64_t so, uint64_t* split,
+               hipStream_t s);
+// Root arrays of the *n_dev sorted rows whose column 0 is c0 (capacity
+// cap): *r_out = distinct values, root_offs[*r_out] = *n_dev.  pos: cap
+// words; tile_r: roots_dev_tiles(cap); tile_off: tiles + 1;
+// scan_tmp: scan_dev_words(tiles).
+void build_roots_dev(const uint32_t* c0, uint64_t cap, const uint64_t* n_dev, uint32_t* pos,
+                     uint32_t* tile_r, uint64_t* tile_off, uint64_t* scan_tmp, uint32_t* root_vals,
+                     uint32_t* root_offs, uint64_t* r_out, hipStream_t s);
+
 // out = a - b (both sorted and duplicate-free, same arity and column order).
 void set_difference(const Index& a, const Index& b, Index& out, Workspace& ws, hipStream_t s);
 

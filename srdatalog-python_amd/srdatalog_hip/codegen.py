@@ -75,8 +75,10 @@ class PipelineGen:
   """Emits the kernels of one pipeline."""
 
   def __init__(self, ep: M.ExecutePipeline, arity: dict[str, int], uid: int,
-               coop_search: bool = False, ring: bool = True):
+               coop_search: bool = False, ring: bool = True, dyn: bool = False):
     self.ep = ep
+    # also emit the device-sized launches (pipelines of recursive steps)
+    self.dyn = dyn and not ep.count
     # run-narrowed cooperative searches (srdj::seg_lower_bound) instead of one
     # independent binary search per lane; measured slower on the triangle
     # join, whose searches are latency-bound, so off by default
@@ -95,8 +97,6 @@ class PipelineGen:
     if not isinstance(self.root, (M.Scan, M.ColumnJoin)):
       raise CodegenError(f"{ep.rule}: pipeline must start with a scan or a column-join")
     self.levels = [op for op in self.ops[1:] if isinstance(op, (M.ColumnJoin, M.CartesianJoin))]
-    if isinstance(self.root, M.Scan) and self.levels:
-      raise CodegenError(f"{ep.rule}: joins after a scan root are not supported")
     for i, lv in enumerate(self.levels):
       if isinstance(lv, M.CartesianJoin) and i != len(self.levels) - 1:
         raise CodegenError(f"{ep.rule}: cartesian join must be the last join level")
@@ -174,13 +174,13 @@ class PipelineGen:
              f"{ind}    const uint64_t o_ = opos + srd::popc_below(m_);"]
     for di, d in enumerate(self.dests):
       for c, relcol in enumerate(d.index):
-        lines.append(f"{ind}    A.out[{di}].cols[{c} * A.out[{di}].stride + A.out[{di}].base + o_]"
+        lines.append(f"{ind}    A.out[{di}].cols[{c} * A.out[{di}].stride + ob{di}_ + o_]"
                      f" = {self.var(d.terms[relcol])};")
     lines += [f"{ind}  }}",
               f"{ind}  if (MODE == 3 && ({cond})) {{"]
     for di, d in enumerate(self.dests):
       for c, relcol in enumerate(d.index):
-        lines.append(f"{ind}    A.out[{di}].cols[{c} * A.out[{di}].stride + A.out[{di}].base + item]"
+        lines.append(f"{ind}    A.out[{di}].cols[{c} * A.out[{di}].stride + ob{di}_ + item]"
                      f" = {self.var(d.terms[relcol])};")
     lines += [f"{ind}  }}",
               f"{ind}  if (MODE == 2 && c_) {{",
@@ -229,7 +229,7 @@ class PipelineGen:
     if self.filters:
       return False
     if isinstance(self.root, M.Scan):
-      return True
+      return not self.levels
     if not self.levels:
       return True
     return len(self.levels) == 1 and isinstance(self.levels[0], M.CartesianJoin)
@@ -309,22 +309,31 @@ class PipelineGen:
 
   def gen_root(self) -> str:
     self._in_run = False
-    out = [f"__global__ __launch_bounds__(256) void root_k(SrdLaunchArgs A) {{",
-           "  const SrdViews& V = A.views;",
-           "  const uint32_t k = blockIdx.x * 256u + threadIdx.x;",
-           "  if (k >= A.n_root) return;"]
+    # DYN: sizes and views come from the device-resident A.dyn (device-sized
+    # iterations, whose launches are enqueued or captured without knowing
+    # them); the grid is fixed and strides over the keys
+    out = ["template <bool DYN>",
+           f"__global__ __launch_bounds__(256) void root_k(SrdLaunchArgs A) {{",
+           "  const SrdViews& V = DYN ? A.dyn->views : A.views;",
+           "  if (DYN && A.dyn->skip) return;",
+           "  const uint32_t nroot_ = DYN ? A.dyn->n_root : A.n_root;",
+           "  const uint32_t drv_ = DYN ? A.dyn->drv_view : A.drv_view;",
+           "  (void)drv_;",
+           "  const uint32_t kstep_ = DYN ? gridDim.x * 256u : nroot_;",
+           "  for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < nroot_;"
+           " k = (nroot_ - k > kstep_) ? k + kstep_ : nroot_) {"]
     if isinstance(self.root, M.Scan):
-      out += [f"  A.key_work[k] = V.v[{self.vid(self.root)}].n;", "  A.key_aux[k] = 0;", "}"]
+      out += [f"  A.key_work[k] = V.v[{self.vid(self.root)}].n;", "  A.key_aux[k] = 0;", "  }", "}"]
       return "\n".join(out)
     v0 = self.root.var
     slots = self.root_slots()
-    out += ["  const srd::IndexView& D = V.v[A.drv_view];",
+    out += ["  const srd::IndexView& D = V.v[drv_];",
             "  const uint32_t key = D.root_vals[k];",
             f"  const uint32_t {self.var(v0)} = key;",
             "  bool ok = true;"]
     reg = {}
     for si, (v, pre) in enumerate(slots):
-      out.append(f"  const srd::Rng rs{si} = (A.drv_view == {v}u) ? srd::Rng{{D.root_offs[k], "
+      out.append(f"  const srd::Rng rs{si} = (drv_ == {v}u) ? srd::Rng{{D.root_offs[k], "
                  f"D.root_offs[k + 1]}} : srd::root_find(V.v[{v}], key);")
       out.append(f"  ok = ok && rs{si}.e > rs{si}.b;")
       reg[(v, pre)] = Rg(v, 1, f"rs{si}.b", f"rs{si}.e")
@@ -358,7 +367,7 @@ class PipelineGen:
     for si in range(len(slots)):
       out.append(f"  A.key_rng[uint64_t(k) * {2 * len(slots)} + {2 * si}] = rs{si}.b;")
       out.append(f"  A.key_rng[uint64_t(k) * {2 * len(slots)} + {2 * si + 1}] = rs{si}.e;")
-    out += ["  A.key_aux[k] = drv;", "  A.key_work[k] = work;", "}"]
+    out += ["  A.key_aux[k] = drv;", "  A.key_work[k] = work;", "  }", "}"]
     return "\n".join(out)
 
   # -------------------------------------------------------- level code ---
@@ -574,15 +583,27 @@ class PipelineGen:
   # ------------------------------------------------------- slice kernel ---
   def gen_run(self) -> str:
     self._in_run = True
-    o = ["template <int MODE>",
+    o = ["template <int MODE, bool DYN>",
          # eight waves per SIMD: the persistent grid launches 8 blocks per CU,
          # and with more than ~100 SGPRs a wave the eighth block could only
          # start when another finished (the compiler spills SGPRs to VGPR lanes)
          "__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) "
          "void run_k(SrdLaunchArgs A) {",
-         "  const SrdViews& V = A.views;",
-         "  const uint32_t lane = srd::lane_id();"]
-    deep = len(self.levels) >= 2
+         "  const SrdViews& V = DYN ? A.dyn->views : A.views;",
+         "  if (DYN && A.dyn->skip) return;",
+         "  const uint32_t nroot_ = DYN ? A.dyn->n_root : A.n_root;",
+         "  const uint32_t drv_ = DYN ? A.dyn->drv_view : A.drv_view;",
+         "  (void)drv_;",
+         "  const uint64_t tw_ = DYN ? A.dyn->total_work : A.total_work;",
+         "  const uint64_t ns_ = DYN ? A.dyn->n_slices : A.n_slices;"]
+    for di in range(len(self.dests)):
+      o.append(f"  const uint64_t ob{di}_ = DYN ? A.dyn->out_base[{di}] : A.out[{di}].base;")
+    o += ["  const uint32_t lane = srd::lane_id();"]
+    # levels walked by the wave from LDS frames: every level below the
+    # first (the first is one item per lane), or, under a scan root (whose
+    # rows are the items), every level
+    scan_root = isinstance(self.root, M.Scan)
+    deep = len(self.levels) >= 2 or (scan_root and len(self.levels) >= 1)
     self._lds = []
     if deep or self.ring:
       o += ["  const uint32_t wv = threadIdx.x / 64u;"]
@@ -626,11 +647,11 @@ class PipelineGen:
           "  uint32_t seg_room = 0;",
           "  bool seg_dead = false;  // staging exhausted: count only (host re-runs the write pass)",
           "  for (;;) {",
-          "    const uint64_t slice = srdj::next_slice(A);  // wave-uniform",
-          "    if (slice >= A.n_slices) break;",
+          "    const uint64_t slice = srdj::next_slice(A, ns_);  // wave-uniform",
+          "    if (slice >= ns_) break;",
           "    const uint64_t it0 = A.slice_start ? A.slice_start[slice] : slice * A.grain;",
           "    const uint64_t it1 = A.slice_start ? A.slice_start[slice + 1]"
-          " : min(it0 + A.grain, A.total_work);",
+          " : min(it0 + A.grain, tw_);",
           "    uint64_t opos = MODE == 1 ? A.offsets[slice] : 0;",
           "    uint32_t cnt = 0;",
           "    uint32_t seg_used = 0, seg_local = 0;",
@@ -639,9 +660,9 @@ class PipelineGen:
           "      const uint64_t item = base + lane;",
           "      const bool act = item < it1;",
           "      const uint64_t last_item = min(base + 63, it1 - 1);",
-          "      const uint32_t klo = kc == ~0u ? srdj::find_slot(A.work_start, A.n_root, base)",
-          "                                     : srdj::find_slot_from(A.work_start, kc, A.n_root, base);",
-          "      const uint32_t khi = srdj::find_slot_from(A.work_start, klo, A.n_root, last_item) + 1;",
+          "      const uint32_t klo = kc == ~0u ? srdj::find_slot(A.work_start, nroot_, base)",
+          "                                     : srdj::find_slot_from(A.work_start, kc, nroot_, base);",
+          "      const uint32_t khi = srdj::find_slot_from(A.work_start, klo, nroot_, last_item) + 1;",
           "      kc = khi - 1;",
           "      uint32_t k = klo;",
           "      if (act && khi - klo > 1) {",
@@ -664,7 +685,7 @@ class PipelineGen:
       o.append(f"{ind}bool h1 = act;")
     else:
       v0 = self.root.var
-      o.append(f"{ind}const uint32_t {self.var(v0)} = act ? V.v[A.drv_view].root_vals[k] : 0u;")
+      o.append(f"{ind}const uint32_t {self.var(v0)} = act ? V.v[drv_].root_vals[k] : 0u;")
       bound.add(v0)
       for si, (v, pre) in enumerate(slots):
         o.append(f"{ind}const uint32_t rs{si}b = act ? A.key_rng[uint64_t(k) * {2 * ns} + {2 * si}] : 0u;")
@@ -695,7 +716,7 @@ class PipelineGen:
         raise CodegenError(f"{self.ep.rule}: filter over unbound variables")
       o.append(self.emit_code("h1", ind))
     else:
-      self.gen_nested(1, reg, bound, done, o, ind, "h1")
+      self.gen_nested(0 if scan_root else 1, reg, bound, done, o, ind, "h1")
     o += ["    }",
           "    if (MODE == 2) {",
           "      srdj::seg_close(A, slice, seg_start, seg_used, seg_local);",
@@ -887,26 +908,39 @@ class PipelineGen:
   def gen_kernels(self) -> str:
     out = [f"namespace {self.ns} {{", self.gen_root(), "", self.gen_run(), "",
            "void launch_root(const SrdLaunchArgs* a, hipStream_t s) {",
-           "  root_k<<<(a->n_root + 255u) / 256u, 256, 0, s>>>(*a);",
+           "  root_k<false><<<(a->n_root + 255u) / 256u, 256, 0, s>>>(*a);",
            "}",
            "void launch_count(const SrdLaunchArgs* a, hipStream_t s) {",
-           "  run_k<0><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "  run_k<0, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}",
            "void launch_write(const SrdLaunchArgs* a, hipStream_t s) {",
-           "  run_k<1><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "  run_k<1, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}",
            "void launch_fused(const SrdLaunchArgs* a, hipStream_t s) {",
-           "  run_k<2><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "  run_k<2, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}",
            "void launch_direct(const SrdLaunchArgs* a, hipStream_t s) {",
-           "  run_k<3><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "  run_k<3, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}",
            "void launch_probe(const SrdLaunchArgs* a, hipStream_t s) {",
-           "  run_k<5><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "  run_k<5, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}"] + ([
            "void launch_weigh(const SrdLaunchArgs* a, hipStream_t s) {",
-           "  run_k<4><<<a->grid_blocks, 256, 0, s>>>(*a);",
-           "}"] if len(self.levels) >= 2 else []) + [
+           "  run_k<4, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}"] if len(self.levels) >= 2 else []) + ([
+           "// device-sized launches (a->dyn holds views and sizes; fixed grids)",
+           "void launch_root_dyn(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  root_k<true><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}",
+           "void launch_count_dyn(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<0, true><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}",
+           "void launch_write_dyn(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<1, true><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}"] + ([
+           "void launch_direct_dyn(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<3, true><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}"] if self.direct_ok() else []) if self.dyn else []) + [
            f"}}  // namespace {self.ns}", ""]
     return "\n".join(out)
 
@@ -934,7 +968,11 @@ class PipelineGen:
             f"&{self.ns}::launch_root, &{self.ns}::launch_count, &{self.ns}::launch_write, "
             f"&{self.ns}::launch_fused, &{self.ns}::launch_direct, "
             f"{'&' + self.ns + '::launch_weigh' if len(self.levels) >= 2 else 'nullptr'}, "
-            f"{1 if self.sorted_output() else 0}, &{self.ns}::launch_probe" + "}")
+            f"{1 if self.sorted_output() else 0}, &{self.ns}::launch_probe, "
+            + (f"&{self.ns}::launch_root_dyn, &{self.ns}::launch_count_dyn, "
+               f"&{self.ns}::launch_write_dyn, "
+               + (f"&{self.ns}::launch_direct_dyn" if self.direct_ok() else "nullptr")
+               if self.dyn else "nullptr, nullptr, nullptr, nullptr") + "}")
 
 
 @dataclass
@@ -1072,7 +1110,14 @@ int srdatalog_export_instances(const char* rel_name, uint32_t* out, uint64_t cap
 def generate_program_source(name: str, decls: list[RelDecl], prog: M.Program) -> str:
   """Complete HIP translation unit of a program."""
   arity = {d.name: d.arity for d in decls}
-  gens = [PipelineGen(ep, arity, i) for i, ep in enumerate(prog.pipelines())]
+  rec = set()
+  for st in prog.steps:
+    if st.recursive and isinstance(st.body, M.FixpointPlan):
+      for op in st.body.ops:
+        for ep in (op.ops if isinstance(op, M.ParallelGroup) else [op]):
+          if isinstance(ep, M.ExecutePipeline):
+            rec.add(id(ep))
+  gens = [PipelineGen(ep, arity, i, dyn=id(ep) in rec) for i, ep in enumerate(prog.pipelines())]
   seen = set()
   for g in gens:
     if g.ep.rule in seen:

@@ -34,7 +34,7 @@ from dataclasses import dataclass, field
 
 from . import mir as M
 from .codegen import RelDecl
-from .dsl import Atom, Filter, Negation, Program, Rule
+from .dsl import Atom, Filter, Negation, Program, Rule, Split
 
 
 class CompileError(ValueError):
@@ -184,6 +184,9 @@ class Variant:
   join_vars: set[str] = field(default_factory=set)
   atoms: list[Access] = field(default_factory=list)
   negs: list[Access] = field(default_factory=list)
+  split_at: int = -1          # body index of the SPLIT marker (-1: none)
+  temp_vars: list[str] = field(default_factory=list)
+  temp_rel: str = ""
 
 
 def _clause_vars(c) -> list[str]:
@@ -307,11 +310,127 @@ def plan_variant(v: Variant) -> None:
     v.clause_order = _heuristic_clause_order(rule, v.delta)
     v.var_order = _var_order(rule, v.clause_order, jv, v.delta)
   v.join_vars = jv
+  v.split_at = detect_split(rule)
+  if v.split_at >= 0 and rule.name:
+    v.temp_vars = split_temp_vars(rule, v.split_at)
+    v.temp_rel = f"_temp_{rule.name}"
   for k, c in enumerate(rule.body):
     if isinstance(c, Atom):
       v.atoms.append(_access(c, v.versions[k], jv, v.var_order, k, False))
     elif isinstance(c, Negation):
       v.negs.append(_access(c.atom, M.FULL, jv, v.var_order, k, True))
+
+
+# ------------------------------------------------------------- splits ----
+def detect_split(rule: Rule) -> int:
+  """Body index of the SPLIT marker, -1 if none (reference: hir/plan.py:385)."""
+  return next((i for i, c in enumerate(rule.body) if isinstance(c, Split)), -1)
+
+
+def split_temp_vars(rule: Rule, split_at: int) -> list[str]:
+  """Variables crossing the split: bound by an atom or negation above it and
+  used below it (by an atom, a negation or a head).  Those the atoms below
+  also use come first, then the head-only ones, each group sorted -- the
+  column order of the temp relation (reference: hir/plan.py:393-436)."""
+  above: set[str] = set()
+  for c in rule.body[:split_at]:
+    if isinstance(c, (Atom, Negation)):
+      above.update(_clause_vars(c))
+  below_atoms: set[str] = set()
+  for c in rule.body[split_at + 1:]:
+    if isinstance(c, Atom):
+      below_atoms.update(c.args)
+  below = set(below_atoms)
+  for h in rule.heads:
+    below.update(h.args)
+  for c in rule.body[split_at + 1:]:
+    if isinstance(c, Negation):
+      below.update(c.atom.args)
+  first = [x for x in sorted(above) if x in below_atoms and x in below]
+  return first + [x for x in sorted(above) if x in below and x not in below_atoms]
+
+
+def split_temps(rules) -> list[tuple[str, int]]:
+  """(name, arity) of every split rule's temp relation, in rule order."""
+  out = []
+  for r in rules:
+    at = detect_split(r)
+    if at >= 0 and r.name:
+      name = f"_temp_{r.name}"
+      if all(name != n for n, _ in out):
+        out.append((name, len(split_temp_vars(r, at))))
+  return out
+
+
+def _split_above(v: Variant) -> list:
+  """First pipeline of a split rule: the one positive atom above the split
+  as a scan, the negations and filters above it, written into the temp.
+  Empty when more than one atom is above the split (the rule is then
+  compiled unsplit), as the reference does (hir/lower.py:288-350)."""
+  above = [a for a in v.atoms if a.clause < v.split_at]
+  if len(above) != 1:
+    return []
+  a = above[0]
+  ops: list = [M.Scan(list(a.order), a.rel, list(a.index), a.ver, [])]
+  for n in v.negs:
+    if n.clause < v.split_at:
+      ops.append(M.Negation(n.rel, M.FULL, list(n.index), list(n.order[:n.prefix_len])))
+  for c in v.rule.body[:v.split_at]:
+    if isinstance(c, Filter):
+      ops.append(M.Filter(list(c.vars), c.code))
+  ops.append(M.InsertInto(v.temp_rel, M.NEW, list(range(len(v.temp_vars))), list(v.temp_vars)))
+  return ops
+
+
+def _split_below(v: Variant, s: Stratum, temp_ver: str) -> list:
+  """Second pipeline: scan the temp, then one cartesian level over the
+  atoms below the split that bind head variables (each prefixed by the temp
+  variables it shares), the negations and filters below, the inserts
+  (reference: hir/lower.py:353-433).  Like the reference, an atom below the
+  split that binds no head variable is not joined."""
+  tv = list(v.temp_vars)
+  tset = set(tv)
+  ops: list = [M.Scan(tv, v.temp_rel, list(range(len(tv))), temp_ver, [])]
+  heads = {x for h in v.rule.heads for x in h.args}
+  cvars: list[str] = []
+  srcs, vfs = [], []
+  for a in v.atoms:
+    if a.clause <= v.split_at:
+      continue
+    mine = [x for x in a.order if x not in tset and x in heads and x not in cvars]
+    if not mine:
+      continue
+    srcs.append(M.ColumnSource(a.rel, list(a.index), a.ver, [x for x in a.order if x in tset],
+                               clause_idx=a.clause))
+    vfs.append(mine)
+    cvars += mine
+  if cvars:
+    ops.append(M.CartesianJoin(cvars, vfs, srcs))
+  for n in v.negs:
+    if n.clause > v.split_at:
+      ops.append(M.Negation(n.rel, M.FULL, list(n.index), list(n.order[:n.prefix_len])))
+  for c in v.rule.body[v.split_at + 1:]:
+    if isinstance(c, Filter):
+      ops.append(M.Filter(list(c.vars), c.code))
+  return ops + _inserts(v.rule, s)
+
+
+def _split_phase(v: Variant, s: Stratum, recursive: bool) -> list:
+  """[clear temp NEW (recursive)], pipeline A, create-flat-view, pipeline
+  B; empty when the rule is compiled unsplit."""
+  above = _split_above(v) if v.split_at >= 0 and v.temp_rel else []
+  if not above:
+    return []
+  name = v.rule.name or ""
+  if v.delta >= 0:
+    name = f"{name}_D{v.delta}"
+  a = M.ExecutePipeline(name + "_splitA", above)
+  a.clause_order = list(v.clause_order)
+  b = M.ExecutePipeline(name + "_splitB", _split_below(v, s, M.NEW if recursive else M.FULL))
+  b.clause_order = list(v.clause_order)
+  idx = list(range(len(v.temp_vars)))
+  ops = [M.ClearRelation(v.temp_rel, M.NEW)] if recursive else []
+  return ops + [a, M.CreateFlatView(v.temp_rel, idx, M.NEW), b]
 
 
 # ------------------------------------------------------ index selection ----
@@ -482,14 +601,22 @@ def lower(strata: list[Stratum], arity: dict[str, int]) -> list[M.Step]:
   steps: list[M.Step] = []
   for s in strata:
     if s.recursive:
-      pipes = [_pipeline(v, s) for v in s.rec_variants]
+      pipes, split_ops = [], []
+      for v in s.rec_variants:
+        ph = _split_phase(v, s, True)
+        if ph:
+          split_ops += ph
+        else:
+          pipes.append(_pipeline(v, s))
       hooks = [M.InjectCppHook(v.rule.name or "", v.rule.debug_code)
                for v in s.rec_variants if v.rule.debug_code]
       ops: list = []
+      # unsplit rules first (in parallel), then the split phases in order
       if len(pipes) > 1:
         ops.append(M.ParallelGroup(pipes))
       elif pipes:
         ops.append(pipes[0])
+      ops += split_ops
       ops += hooks
       full_reads: dict[str, set[tuple]] = {}
       for v in s.rec_variants:
@@ -506,15 +633,19 @@ def lower(strata: list[Stratum], arity: dict[str, int]) -> list[M.Step]:
           if rel in s.canonical:
             steps.append(M.Step(False, M.PostStratumReconstruct(rel, list(s.canonical[rel]))))
     else:
-      pipes, maint, modified = [], [], []
+      pipes, maint, modified, split_ops = [], [], [], []
       for v in s.base_variants:
-        pipes.append(_pipeline(v, s))
+        ph = _split_phase(v, s, False)
+        if ph:
+          split_ops += ph
+        else:
+          pipes.append(_pipeline(v, s))
         for h in v.rule.heads:
           if h.rel not in modified:
             modified.append(h.rel)
           if h.rel in s.required:
             maint += _simple_maintenance(h.rel, s.required[h.rel], s.canonical[h.rel])
-      ops = []
+      ops = list(split_ops)   # the split phases run first, then the other rules
       if len(pipes) > 1:
         ops.append(M.ParallelGroup(pipes))
       elif pipes:
@@ -603,7 +734,10 @@ def _reorder_sources(steps: list[M.Step]) -> None:
 
 # ---------------------------------------------------------------- entry ----
 def relation_decls(program: Program) -> list[RelDecl]:
-  return [RelDecl(r.name, r.arity, r.input_file, r.print_size) for r in program.relations]
+  decls = [RelDecl(r.name, r.arity, r.input_file, r.print_size) for r in program.relations]
+  # split rules' temp relations, declared after the program's own
+  # (reference: hir/split.py TempRelSynthesisPass)
+  return decls + [RelDecl(n, a) for n, a in split_temps(program.rules)]
 
 
 def rename_wildcards(rules: list[Rule]) -> list[Rule]:
@@ -641,10 +775,14 @@ def rename_wildcards(rules: list[Rule]) -> list[Rule]:
 
 def compile_to_mir(program: Program) -> M.Program:
   arity = {r.name: r.arity for r in program.relations}
+  for n, a in split_temps(program.rules):
+    arity[n] = a
   for r in program.rules:
     for c in r.body:
-      if not isinstance(c, (Atom, Negation, Filter)):
+      if not isinstance(c, (Atom, Negation, Filter, Split)):
         raise CompileError(f"rule {r.name}: unsupported body clause {type(c).__name__}")
+    if sum(1 for c in r.body if isinstance(c, Split)) > 1:
+      raise CompileError(f"rule {r.name}: more than one split marker")
   strata = stratify(rename_wildcards(list(program.rules)))
   for s in strata:
     if s.recursive:

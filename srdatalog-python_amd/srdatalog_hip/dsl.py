@@ -12,11 +12,11 @@ it run here unchanged::
     ])
 
 Covered: Var, Relation (input_file / print_size / index_type), atoms,
-``&`` conjunction, ``~atom`` negation, Filter, PlanEntry, Rule.named /
+``&`` conjunction, ``~atom`` negation, Filter, the SPLIT marker, PlanEntry, Rule.named /
 with_plan / with_plans / with_count / with_inject_cpp, Program (relations
 derived from the rules in first-occurrence order, as in the reference's
-_derive_relations, dsl.py:641).  Constants, Let, aggregates and split rules
-are not part of the hot path and are rejected by the compiler.
+_derive_relations, dsl.py:641).  Constants, Let and aggregates are not part
+of the hot path and are rejected by the compiler.
 """
 
 from __future__ import annotations
@@ -82,13 +82,27 @@ class Filter:
 
 
 @dataclass(frozen=True)
+class Split:
+  """Split marker: the clauses before it run as a first pipeline writing the
+  variables the rest needs into a temp relation ``_temp_<rule name>``; a
+  second pipeline scans the temp and joins the clauses after it (negation
+  pushdown).  At most one per rule body (reference: dsl.py Split / SPLIT)."""
+
+  def __and__(self, other) -> "Conjunction":
+    return Conjunction((self,)) & other
+
+
+SPLIT = Split()
+
+
+@dataclass(frozen=True)
 class Conjunction:
   clauses: tuple
 
   def __and__(self, other) -> "Conjunction":
     if isinstance(other, Conjunction):
       return Conjunction(self.clauses + other.clauses)
-    if isinstance(other, (Atom, Negation, Filter)):
+    if isinstance(other, (Atom, Negation, Filter, Split)):
       return Conjunction(self.clauses + (other,))
     raise TypeError(f"cannot conjoin {type(other).__name__}")
 

@@ -132,6 +132,16 @@ class RebuildIndexFromIndex:
 
 
 @dataclass
+class CreateFlatView:
+  """Expose the rows a split rule's first pipeline wrote into its temp
+  relation as an index the second pipeline scans
+  (src/srdatalog/mir/types.py CreateFlatView)."""
+  rel: str
+  index: list[int]
+  ver: str
+
+
+@dataclass
 class InjectCppHook:
   rule: str
   code: str
@@ -241,6 +251,8 @@ def _node(n, ind: int) -> str:
   if isinstance(n, RebuildIndexFromIndex):
     return (f"{p}(rebuild-index-from-index :source {_idx(n.rel, n.source)}"
             f" :target {_idx(n.rel, n.target)} :ver {n.ver})")
+  if isinstance(n, CreateFlatView):
+    return f"{p}(create-flat-view :index {_idx(n.rel, n.index)} :ver {n.ver})"
   if isinstance(n, InjectCppHook):
     return f'{p}(inject-cpp-hook :rule {n.rule} :code "...")'
   if isinstance(n, PostStratumReconstruct):
@@ -396,6 +408,9 @@ def _parse_node(x):
     rel, src = _index_ref(_kw(x, ":source"))
     _, tgt = _index_ref(_kw(x, ":target"))
     return RebuildIndexFromIndex(rel, src, tgt, str(_kw(x, ":ver")))
+  if head == "create-flat-view":
+    rel, idx = _index_ref(_kw(x, ":index"))
+    return CreateFlatView(rel, idx, str(_kw(x, ":ver")))
   if head == "inject-cpp-hook":
     return InjectCppHook(str(_kw(x, ":rule")), str(_kw(x, ":code", "")))
   if head == "post-stratum-reconstruct-intern-cols":

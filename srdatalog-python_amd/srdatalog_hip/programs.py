@@ -14,7 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 from . import build as B
 from . import native
-from .dsl import Filter, PlanEntry, Program, Relation, Var
+from .dsl import SPLIT, Filter, PlanEntry, Program, Relation, Var
 from .runtime import ProgramLib
 
 PROG_DIR = os.path.join(native.LIB_DIR, "programs")
@@ -177,6 +177,32 @@ def reach_multi_program() -> Program:
   ])
 
 
+def split_program(split: bool) -> Program:
+  """Reachability that skips blocked vertices, with the negation pushed
+  below a split (the reference's tests/fixtures/split_rec.nim):
+  R(x,y) <= Seed(x,y); R(x,z) <= R(x,y), !Filter(y), [split,] Edge(y,z).
+  Without the split marker the same rule compiles to one pipeline; both
+  must give the same relation and deltas."""
+  x, y, z = Var("x"), Var("y"), Var("z")
+  seed = Relation("Seed", 2, input_file="Seed.csv")
+  filt = Relation("Filter", 1, input_file="Filter.csv")
+  edge = Relation("Edge", 2, input_file="Edge.csv")
+  r = Relation("R", 2, print_size=True)
+  body = r(x, y) & ~filt(y) & SPLIT & edge(y, z) if split else r(x, y) & ~filt(y) & edge(y, z)
+  return Program(rules=[(r(x, y) <= seed(x, y)).named("RBase"), (r(x, z) <= body).named("RRec")])
+
+
+def split_nonrec_program() -> Program:
+  """Q(x,z) <= A(x,y), !B(y), split, C(x,z) (tests/fixtures/split_test.nim):
+  the split of a non-recursive rule."""
+  x, y, z = Var("x"), Var("y"), Var("z")
+  a = Relation("A", 2, input_file="A.csv")
+  b = Relation("B", 1, input_file="B.csv")
+  c = Relation("C", 2, input_file="C.csv")
+  q = Relation("Q", 2, print_size=True)
+  return Program(rules=[(q(x, z) <= a(x, y) & ~b(y) & SPLIT & c(x, z)).named("SplitTest")])
+
+
 PROGRAMS = {
   "tc": tc_program,
   "sg": sg_program,
@@ -189,6 +215,9 @@ PROGRAMS = {
   "neg2hop_count": lambda: neg2hop_program(True),
   "q7": q7_program,
   "q3": q3_program,
+  "split_rec": lambda: split_program(True),
+  "split_rec_plain": lambda: split_program(False),
+  "split_nonrec": split_nonrec_program,
 }
 
 

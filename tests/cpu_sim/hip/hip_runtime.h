@@ -34,6 +34,22 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
   return hipSuccess;
 }
 
+// Graph capture: the host-only build never captures (the tests run the
+// device-resident loop with SRDATALOG_DEVLOOP_GRAPH=0); these only compile.
+typedef void* hipGraph_t;
+typedef void* hipGraphExec_t;
+typedef void* hipGraphNode_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal,
+                            hipStreamCaptureModeRelaxed };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { std::abort(); }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { std::abort(); }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, hipGraphNode_t*, char*, size_t) {
+  std::abort();
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { std::abort(); }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+
 struct srd_dim3 { unsigned x = 0, y = 0, z = 0; };
 static srd_dim3 threadIdx, blockIdx, blockDim, gridDim;
 

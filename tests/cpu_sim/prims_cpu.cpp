@@ -7,6 +7,7 @@
 #include <stdexcept>
 #include <vector>
 
+#include "srd_devloop.h"
 #include "srd_runtime.h"
 
 namespace srd {
@@ -257,5 +258,126 @@ void timing_begin(const char*, hipStream_t) {}
 void timing_end(hipStream_t, double) {}
 std::vector<KernelTime> timing_collect() { return {}; }
 void timing_reset() {}
+
+
+// ---- device-sized primitives (counts read from "device" words, which are
+// host memory here)
+uint64_t scan_dev_words(uint64_t cap) { return cap + 1; }
+uint64_t mp_tiles_for(uint64_t rows) { return (rows + 4095) / 4096; }
+uint64_t roots_dev_tiles(uint64_t cap) { return std::max<uint64_t>((cap + 4095) / 4096, 1); }
+uint64_t devsort_scratch_bytes(uint64_t cap) { return 64 + cap; }
+void devsort_clear_lookback(void*, uint64_t, hipStream_t) {}
+
+void exclusive_scan_u32_dev(const uint32_t* in, uint64_t* out, uint64_t cap, const uint64_t* n_dev,
+                            uint64_t*, hipStream_t) {
+  const uint64_t n = n_dev ? std::min(cap, *n_dev) : cap;
+  uint64_t t = 0;
+  for (uint64_t i = 0; i < n; ++i) out[i] = t, t += in[i];
+  out[n] = t;
+}
+void exclusive_scan_u64_dev(const uint64_t* in, uint64_t* out, uint64_t cap, const uint64_t* n_dev,
+                            uint64_t*, hipStream_t) {
+  const uint64_t n = n_dev ? std::min(cap, *n_dev) : cap;
+  uint64_t t = 0;
+  for (uint64_t i = 0; i < n; ++i) out[i] = t, t += in[i];
+  out[n] = t;
+}
+
+static std::vector<Row> rows_cols(const uint32_t* const* in, uint32_t k, uint64_t n) {
+  std::vector<Row> rows(n, Row(k));
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint32_t c = 0; c < k; ++c) rows[i][c] = in[c][i];
+  return rows;
+}
+static std::vector<Row> rows_strided(const uint32_t* a, uint64_t sa, uint32_t k, uint64_t n) {
+  std::vector<Row> rows(n, Row(k));
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint32_t c = 0; c < k; ++c) rows[i][c] = a[c * sa + i];
+  return rows;
+}
+static void put_rows(const std::vector<Row>& rows, uint32_t k, uint32_t* out, uint64_t so) {
+  for (uint64_t i = 0; i < rows.size(); ++i)
+    for (uint32_t c = 0; c < k; ++c) out[c * so + i] = rows[i][c];
+}
+
+void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const uint64_t* n_dev,
+                     uint64_t n_host, uint32_t, void*, uint32_t* out_cols, uint64_t out_stride,
+                     uint32_t* root_vals, uint32_t* root_offs, uint64_t* counts_out, bool,
+                     hipStream_t, bool) {
+  uint64_t n = n_dev ? *n_dev : n_host;
+  uint64_t err = 0;
+  if (n > cap) n = cap, err = 2;
+  auto rows = rows_cols(in, k, n);
+  std::sort(rows.begin(), rows.end());
+  rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+  put_rows(rows, k, out_cols, out_stride);
+  uint64_t r = 0;
+  for (uint64_t i = 0; i < rows.size(); ++i)
+    if (i == 0 || rows[i][0] != rows[i - 1][0]) root_vals[r] = rows[i][0], root_offs[r] = uint32_t(i), ++r;
+  root_offs[r] = uint32_t(rows.size());
+  if (counts_out) counts_out[0] = rows.size(), counts_out[1] = r, counts_out[2] = err;
+}
+
+void set_difference_dev(uint32_t k, const uint32_t* a, uint64_t sa, const uint32_t* b, uint64_t sb,
+                        uint64_t cap_ab, const uint64_t* nab, uint32_t* out, uint64_t so,
+                        const SetDiffDevScratch&, uint64_t* total_out, hipStream_t) {
+  if (nab[0] + nab[1] > cap_ab) fail("set_difference_dev: rows above the capacity");
+  auto ra = rows_strided(a, sa, k, nab[0]), rb = rows_strided(b, sb, k, nab[1]);
+  std::vector<Row> d;
+  std::set_difference(ra.begin(), ra.end(), rb.begin(), rb.end(), std::back_inserter(d));
+  put_rows(d, k, out, so);
+  *total_out = d.size();
+}
+
+void merge_dev(uint32_t k, const uint32_t* a, uint64_t sa, const uint32_t* b, uint64_t sb,
+               uint64_t cap_ab, const uint64_t* nab, uint32_t* out, uint64_t so, uint64_t*,
+               hipStream_t) {
+  if (nab[0] + nab[1] > cap_ab) fail("merge_dev: rows above the capacity");
+  auto ra = rows_strided(a, sa, k, nab[0]), rb = rows_strided(b, sb, k, nab[1]);
+  std::vector<Row> m;
+  std::merge(ra.begin(), ra.end(), rb.begin(), rb.end(), std::back_inserter(m));
+  for (uint64_t i = 1; i < m.size(); ++i)
+    if (!(m[i - 1] < m[i])) fail("merge_dev: inputs not disjoint");
+  put_rows(m, k, out, so);
+}
+
+void build_roots_dev(const uint32_t* c0, uint64_t cap, const uint64_t* n_dev, uint32_t*, uint32_t*,
+                     uint64_t*, uint64_t*, uint32_t* root_vals, uint32_t* root_offs, uint64_t* r_out,
+                     hipStream_t) {
+  const uint64_t n = std::min(cap, *n_dev);
+  uint64_t r = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (i == 0 || c0[i] != c0[i - 1]) root_vals[r] = c0[i], root_offs[r] = uint32_t(i), ++r;
+  root_offs[r] = uint32_t(n);
+  *r_out = r;
+}
+
+// ---- device-resident iteration steps, run directly
+void dl_iter_begin(DevCtl* c, uint32_t nrels, hipStream_t) { dl_iter_begin_step(c, nrels); }
+void dl_bind(DevCtl* c, SrdDyn* d, const DlBind& b, hipStream_t) { dl_bind_step(c, d, b); }
+void dl_plan(DevCtl* c, SrdDyn* d, const uint64_t* ws, const DlOut& o, hipStream_t) {
+  dl_plan_step(c, d, ws, o);
+}
+void dl_out(DevCtl* c, SrdDyn* d, const uint64_t* off, const DlOut& o, hipStream_t) {
+  dl_out_step(c, d, off, o);
+}
+void dl_check(DevCtl* c, uint32_t nrels, uint64_t cap_full, hipStream_t) {
+  dl_check_step(c, nrels, cap_full);
+}
+void dl_sd_prep(DevCtl* c, uint32_t r, hipStream_t) { dl_sd_prep_step(c, r); }
+void dl_sd_fin(DevCtl* c, uint32_t r, hipStream_t) { dl_sd_fin_step(c, r); }
+void dl_mg_prep(DevCtl* c, uint32_t r, hipStream_t) { dl_mg_prep_step(c, r); }
+void dl_mg_fin(DevCtl* c, uint32_t r, hipStream_t) { dl_mg_fin_step(c, r); }
+void dl_iter_end(DevCtl* c, const DlEnd& e, hipStream_t) { dl_iter_end_step(c, e); }
+void dl_inst(const DevCtl* c, uint32_t r, const uint32_t* col0, const uint32_t* bounds,
+             uint32_t n_inst, uint64_t* out, hipStream_t) {
+  if (c->skip) return;
+  const uint64_t n = c->delta_n[r];
+  for (uint32_t g = 0; g < n_inst; ++g) {
+    const uint32_t* lo = std::lower_bound(col0, col0 + n, bounds[g]);
+    const uint32_t* hi = std::lower_bound(lo, col0 + n, bounds[g + 1]);
+    out[c->iter * n_inst + g] = uint64_t(hi - lo);
+  }
+}
 
 }  // namespace srd

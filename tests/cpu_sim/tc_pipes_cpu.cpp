@@ -141,6 +141,23 @@ void rec_weigh(const SrdLaunchArgs* a, hipStream_t) {
   }
 }
 
+// Device-sized launches: the sizes, views and output bases come from the
+// SrdDyn the engine's bind / plan / out steps filled.
+template <SrdLaunchFn F>
+void dyn_launch(const SrdLaunchArgs* a, hipStream_t s) {
+  const SrdDyn& d = *a->dyn;
+  if (d.skip) return;
+  SrdLaunchArgs b = *a;
+  b.views = d.views;
+  b.n_root = d.n_root;
+  b.drv_view = d.drv_view;
+  b.total_work = d.total_work;
+  b.n_slices = d.n_slices;
+  for (int k = 0; k < SRD_MAX_DESTS; ++k) b.out[k].base = d.out_base[k];
+  b.dyn = nullptr;
+  F(&b, s);
+}
+
 const SrdPipelineDesc k_pipes[] = {
   {"EdgeLoad", 1, {{"ArcInput", 2, {0, 1}, SRD_FULL}}, SRD_ROOT_SCAN, 1, {0}, 1,
    {{"Edge", 2, {0, 1}}}, 0, 4096, 1, &scan_root<false>, &scan_run<false, 0>, &scan_run<false, 1>,
@@ -150,7 +167,8 @@ const SrdPipelineDesc k_pipes[] = {
    &scan_run<true, 2>, &scan_run<true, 3>},
   {"TCRec_D0", 2, {{"Path", 2, {1, 0}, SRD_DELTA}, {"Edge", 2, {0, 1}, SRD_FULL}}, SRD_ROOT_JOIN, 2,
    {0, 1}, 1, {{"Path", 2, {1, 0}}}, 0, 7, 1, &rec_root, &rec_run<0>, &rec_run<1>, &rec_run<2>,
-   &rec_run<3>, &rec_weigh},
+   &rec_run<3>, &rec_weigh, 0, nullptr, &dyn_launch<&rec_root>, &dyn_launch<&rec_run<0>>,
+   &dyn_launch<&rec_run<1>>, &dyn_launch<&rec_run<3>>},
 };
 
 }  // namespace

@@ -3,7 +3,7 @@ DSL (same rules, names and plans as the reference's
 tests/test_integration_*.py / test_hir_mir_tc_e2e.py), paired with the MIR
 goldens those tests check (tests/golden/mir/)."""
 
-from srdatalog_hip.dsl import Filter, PlanEntry, Program, Relation, Var
+from srdatalog_hip.dsl import SPLIT, Filter, PlanEntry, Program, Relation, Var
 
 
 def tc() -> Program:
@@ -170,11 +170,31 @@ def lsqb_q6_bg_test() -> Program:
   return _q6(True, False, "PathBG", "TwoHopBG", ["p2", "p3", "p1", "t"], bg=True)
 
 
+def split_test() -> Program:
+  """Q(x, z) :- A(x, y), !B(y), split, C(x, z) (the reference's
+  tests/fixtures/split_test.nim): negation pushed below a split."""
+  X, Y, Z = Var("x"), Var("y"), Var("z")
+  a, b, c, q = Relation("A", 2), Relation("B", 1), Relation("C", 2), Relation("Q", 2)
+  return Program(rules=[(q(X, Z) <= a(X, Y) & ~b(Y) & SPLIT & c(X, Z)).named("SplitTest")])
+
+
+def split_rec() -> Program:
+  """R(x, y) :- Seed(x, y);  R(x, z) :- R(x, y), !Filter(y), split, Edge(y, z)
+  (the reference's tests/fixtures/split_rec.nim)."""
+  X, Y, Z = Var("x"), Var("y"), Var("z")
+  seed, filt = Relation("Seed", 2), Relation("Filter", 1)
+  edge, r = Relation("Edge", 2), Relation("R", 2)
+  return Program(rules=[
+    (r(X, Y) <= seed(X, Y)).named("RBase"),
+    (r(X, Z) <= r(X, Y) & ~filt(Y) & SPLIT & edge(Y, Z)).named("RRec"),
+  ])
+
+
 GOLDEN = {
   "tc": tc, "tc_2level_bg": tc_2level_bg, "sg": sg, "andersen": andersen,
   "lsqb_triangle_count": lsqb_triangle_count, "triangle": triangle,
   "lsqb_q9_neg2hop": lsqb_q9_neg2hop, "lsqb_q7_optional": lsqb_q7_optional,
   "lsqb_q3_triangle": lsqb_q3_triangle, "lsqb_q6_2hop": lsqb_q6_2hop,
   "lsqb_q6_count": lsqb_q6_count, "lsqb_q6_nosj": lsqb_q6_nosj,
-  "lsqb_q6_bg_test": lsqb_q6_bg_test,
+  "lsqb_q6_bg_test": lsqb_q6_bg_test, "split_test": split_test, "split_rec": split_rec,
 }

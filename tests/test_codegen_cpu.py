@@ -36,7 +36,23 @@ def test_generated_desc_carries_the_flags():
   src = codegen.generate_program_source("triangle", compiler.relation_decls(programs.PROGRAMS["triangle"]()),
                                         compiler.compile_to_mir(programs.PROGRAMS["triangle"]()))
   line = next(l for l in src.splitlines() if l.lstrip().startswith('{"TriangleJoin"'))
-  # weigh launcher, sorted_output = 1, probe launcher
-  assert line.rstrip().endswith("launch_weigh, 1, &srdp_1_TriangleJoin::launch_probe}")
+  # weigh launcher, sorted_output = 1, probe launcher; a non-recursive rule
+  # has no device-sized launches
+  assert line.rstrip().endswith("launch_weigh, 1, &srdp_1_TriangleJoin::launch_probe, "
+                                "nullptr, nullptr, nullptr, nullptr}")
   # want_bitmaps, want_root_pos
   assert '{"Knows", 2, {0, 1}, SRD_FULL, 1, 1}' in line
+
+
+def test_recursive_rules_get_device_sized_launches():
+  """Rules of recursive steps carry root/count/write (and, with one tuple
+  per item, direct) launches that read their sizes from the device."""
+  prog = programs.PROGRAMS["tc"]()
+  src = codegen.generate_program_source("tc", compiler.relation_decls(prog),
+                                        compiler.compile_to_mir(prog))
+  rec = next(l for l in src.splitlines() if l.lstrip().startswith('{"TCRec'))
+  assert rec.rstrip().rstrip(",").endswith("::launch_direct_dyn}")
+  assert "launch_root_dyn" in rec and "launch_count_dyn" in rec and "launch_write_dyn" in rec
+  base = next(l for l in src.splitlines() if l.lstrip().startswith('{"TCBase'))
+  assert base.rstrip().rstrip(",").endswith("nullptr, nullptr, nullptr, nullptr}")
+  assert "run_k<3, true>" in src and "root_k<true>" in src

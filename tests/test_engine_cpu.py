@@ -41,6 +41,16 @@ def eng():
   return lib
 
 
+@pytest.fixture(autouse=True)
+def host_loop_by_default(monkeypatch):
+  """The tests below drive the host-driven loop unless they ask for the
+  device-resident one; the host-only build never captures graphs."""
+  monkeypatch.setenv("SRDATALOG_DEVLOOP", "0")
+  monkeypatch.setenv("SRDATALOG_DEVLOOP_GRAPH", "0")
+  for k in ("SRDATALOG_DEVLOOP_CAPS", "SRDATALOG_DEVLOOP_BATCH"):
+    monkeypatch.delenv(k, raising=False)
+
+
 def tc_text():
   prog = programs.tc_program()
   return codegen.program_text(compiler.relation_decls(prog), compiler.compile_to_mir(prog))
@@ -154,6 +164,56 @@ def test_engine_weighted_slices(eng, target, mode, monkeypatch):
   if mode == "chunked":
     monkeypatch.setenv("SRDATALOG_NEW_CHUNK_ROWS", "300")
   e = ds.tc_edges(5, 150, 500)
+  want = oracle.run("tc", {"ArcInput": e})
+  path, deltas, size = run_tc(eng, e)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]
+
+
+# ------------------------------------------------ device-resident loop ----
+# The device-sized iteration (capacities, control block, bind / plan / out /
+# check steps, growth after an overflow) driven by the host-only build: the
+# steps and primitives run on the CPU in stream order, as the kernels would.
+@pytest.mark.parametrize("caps,batch", [(0, 4), (8, 1), (33, 3), (200, 16), (64, 2)])
+@pytest.mark.parametrize("mode", ["direct", "twopass"])
+@pytest.mark.parametrize("n,m,seed", [(12, 20, 0), (60, 150, 1), (300, 700, 2)])
+def test_devloop_tc_matches_oracle(eng, n, m, seed, mode, caps, batch, monkeypatch):
+  monkeypatch.setenv("SRDATALOG_DEVLOOP", "1")
+  monkeypatch.setenv("SRDATALOG_DEVLOOP_BATCH", str(batch))
+  if caps:
+    monkeypatch.setenv("SRDATALOG_DEVLOOP_CAPS", str(caps))
+  if mode == "twopass":
+    monkeypatch.setenv("SRDATALOG_JOIN_MODE", "twopass")
+  else:
+    monkeypatch.delenv("SRDATALOG_JOIN_MODE", raising=False)
+  e = ds.tc_edges(seed, n, m)
+  want = oracle.run("tc", {"ArcInput": e})
+  path, deltas, size = run_tc(eng, e)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]
+  assert size == want.size("Path")
+
+
+@pytest.mark.parametrize("max_iters", [1, 2, 5])
+@pytest.mark.parametrize("caps", [0, 16])
+def test_devloop_tc_max_iters(eng, max_iters, caps, monkeypatch):
+  monkeypatch.setenv("SRDATALOG_DEVLOOP", "1")
+  if caps:
+    monkeypatch.setenv("SRDATALOG_DEVLOOP_CAPS", str(caps))
+  e = ds.tc_edges(3, 80, 200)
+  want = oracle.run("tc", {"ArcInput": e}, max_iters=max_iters)
+  path, deltas, _ = run_tc(eng, e, max_iters=max_iters)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]
+
+
+def test_devloop_tc_empty_and_long_tail(eng, monkeypatch):
+  monkeypatch.setenv("SRDATALOG_DEVLOOP", "1")
+  monkeypatch.setenv("SRDATALOG_DEVLOOP_CAPS", "50")
+  path, deltas, size = run_tc(eng, np.zeros((0, 2), np.uint32))
+  assert path.shape[0] == 0 and size == 0
+  n = 300   # a chain: 298 iterations, more than one iteration record of 50
+  e = np.stack([np.arange(n - 1), np.arange(1, n)], 1).astype(np.uint32)
   want = oracle.run("tc", {"ArcInput": e})
   path, deltas, size = run_tc(eng, e)
   np.testing.assert_array_equal(path, want.relations["Path"])

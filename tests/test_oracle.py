@@ -250,3 +250,53 @@ def test_triangle_key_stride_sample():
   finally:
     oracle.set_key_stride(1)
   np.testing.assert_array_equal(part, full[full[:, 0] % 4 == 0])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_split_rec_brute_is_blocked_bfs(seed):
+  """The split rule's model R(x,z) <= R(x,y), !Filter(y), Edge(y,z) from
+  Seed: (x, z) is in R iff some seed (x, y) reaches z along edges leaving
+  only unblocked vertices, and it first appears in iteration d - 1, d the
+  fewest such edges; checked by a per-x breadth-first search."""
+  from collections import deque
+  rng = np.random.default_rng(seed)
+  n = 60
+  edge = ds.tc_edges(seed, n, 150)
+  sd = np.stack([rng.integers(0, n, 12), rng.integers(0, n, 12)], 1)
+  filt = rng.choice(n, size=8 * seed, replace=False).reshape(-1, 1)
+  rel, deltas = brute.split_rec(sd, filt, edge)
+  deltas = deltas["R"]
+  blocked = {int(v) for v in filt.ravel()}
+  succ = {}
+  for a, b in edge:
+    succ.setdefault(int(a), []).append(int(b))
+  dist_count, pairs = {}, set()
+  for x in sorted({int(a) for a, _ in sd}):
+    dist = {int(b): 0 for a, b in sd if int(a) == x}
+    q = deque(dist)
+    while q:
+      u = q.popleft()
+      if u in blocked:
+        continue
+      for w in succ.get(u, ()):
+        if w not in dist:
+          dist[w] = dist[u] + 1
+          q.append(w)
+    for z, d in dist.items():
+      pairs.add((x, z))
+      if d > 0:
+        dist_count[d] = dist_count.get(d, 0) + 1
+  assert rel["R"] == pairs
+  want = [dist_count.get(i + 1, 0) for i in range(len(deltas))]
+  assert deltas == want
+  assert deltas[-1] == 0 and all(d > 0 for d in deltas[:-1])
+
+
+def test_split_nonrec_brute_closed_form():
+  rng = np.random.default_rng(5)
+  a = rng.integers(0, 40, (200, 2))
+  b = rng.choice(40, 10, replace=False).reshape(-1, 1)
+  c = rng.integers(0, 40, (150, 2))
+  rel, _ = brute.split_nonrec(a, b, c)
+  want = {(int(x), int(z)) for x, y in a for xx, z in c if x == xx and int(y) not in set(b.ravel())}
+  assert rel["Q"] == want

@@ -211,7 +211,10 @@ def test_csv_load_all(progs, tmp_path):
 # bounds the heavy-key bitmaps (0: none; a small budget keeps only the
 # longest keys' bitmaps, so bitmap probes and searches mix in one wave);
 # SRDATALOG_ROOT_POS=0 looks nested-level keys up by searching the root
-# values instead of through the dense root map.
+# values instead of through the dense root map.  Every variant but the
+# default ones runs recursive steps in the host-driven loop
+# (SRDATALOG_DEVLOOP=0); devloop-small-caps runs them device-resident from
+# capacities of 500 rows, growing as it goes.
 MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_JOIN_MODE": "fused"}, "fused"),
          ({"SRDATALOG_JOIN_MODE": "fused", "SRDATALOG_STAGE_CAP": "3000"}, "fused-overflow"),
@@ -227,6 +230,7 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_BITMAP_BYTES": "0"}, "no-bitmaps"),
          ({"SRDATALOG_BITMAP_BYTES": "20000"}, "few-bitmaps"),
          ({"SRDATALOG_ROOT_POS": "0"}, "root-search"),
+         ({"SRDATALOG_DEVLOOP": "1", "SRDATALOG_DEVLOOP_CAPS": "500"}, "devloop-small-caps"),
          ({}, "default")]
 
 
@@ -235,8 +239,12 @@ def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
   for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NO_SEGMENTS",
             "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN", "SRDATALOG_WEIGH_MIN",
             "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH", "SRDATALOG_BITMAP_BYTES",
-            "SRDATALOG_ROOT_POS"):
+            "SRDATALOG_ROOT_POS", "SRDATALOG_DEVLOOP", "SRDATALOG_DEVLOOP_CAPS"):
     monkeypatch.delenv(k, raising=False)
+  # the join modes of recursive rules are those of the host-driven loop (the
+  # device-resident one has its own variants)
+  if label not in ("default",) and "SRDATALOG_DEVLOOP" not in env:
+    monkeypatch.setenv("SRDATALOG_DEVLOOP", "0")
   for k, v in env.items():
     monkeypatch.setenv(k, v)
   cases = [
