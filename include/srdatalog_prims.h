@@ -36,6 +36,25 @@ int srd_prim_sort_dedup(const uint32_t* rows, uint64_t n, uint32_t k, uint32_t* 
                         uint64_t* out_n, uint32_t* root_vals, uint32_t* root_offs,
                         uint64_t* out_r);
 
+/* srd_prim_sort_dedup for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It has no
+ * counterpart in the reference.  Unlike srd_prim_sort_dedup it takes DEVICE
+ * buffers, enqueues on `stream` (a hipStream_t; NULL = the null stream), never
+ * synchronises and calls no hipMalloc/hipFree: every intermediate lives in a
+ * static device array of 34 MiB per GPU, held until the process ends.
+ *   cols        k columns of n rows, column c at cols + c*stride
+ *   out_cols    sorted distinct rows, column c at out_cols + c*out_stride
+ *   root_vals   capacity n: distinct column-0 values
+ *   root_offs   capacity n+1: first row of each, root_offs[r] = distinct rows
+ *   counts      3 u64 written on the device: distinct rows, distinct column-0
+ *               values, error bits (nonzero: the output is not valid)
+ * Limits: 1 <= k <= 4, n <= 1,048,576 rows (larger n is rejected with an
+ * error), strides >= n.  Calls that overlap on one GPU share the array and
+ * race; calls in stream order are safe. */
+int srd_prim_sort_dedup_noalloc(const uint32_t* cols, uint64_t stride, uint64_t n, uint32_t k,
+                                uint32_t* out_cols, uint64_t out_stride, uint32_t* root_vals,
+                                uint32_t* root_offs, uint64_t* counts, void* stream);
+
 /* out = a - b for sorted duplicate-free inputs of arity k (capacity na*k). */
 int srd_prim_set_difference(const uint32_t* a, uint64_t na, const uint32_t* b, uint64_t nb,
                             uint32_t k, uint32_t* out, uint64_t* out_n);

@@ -108,6 +108,10 @@ def _bind_prims(lib) -> None:
   lib.srd_prim_bench.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
                                  ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), _u64p]
   lib.srd_pool_stats.argtypes = [_u64p, ctypes.c_uint32]
+  _vp = ctypes.c_void_p
+  lib.srd_prim_sort_dedup_noalloc.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint64,
+                                              ctypes.c_uint32, _vp, ctypes.c_uint64, _vp, _vp,
+                                              _vp, _vp]
   lib.srd_timing_enable.argtypes = [ctypes.c_int]
   lib.srd_timing_report.restype = ctypes.c_uint64
   lib.srd_timing_report.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
