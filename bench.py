@@ -64,6 +64,9 @@ def parse():
                       "an index holds at most 2^32-1 rows, so a request estimated beyond half "
                       "of that is refused; memory admission is the HBM budget)")
   p.add_argument("--serve-budget", type=float, default=200e9)
+  p.add_argument("--serve-wall-cap", type=float, default=90.0,
+                 help="serve: no batch starts after this many seconds of the served run; "
+                      "requests not started by then are reported as unserved")
   p.add_argument("--serve-plain-cap", type=float, default=30.0,
                  help="serve: wall seconds after which the plain serial baseline starts "
                       "no new request")
@@ -135,7 +138,9 @@ def bench_batch(args) -> dict:
   n_i log-uniform in [500, 200000], mean out-degree U[1.5, 4].  Instances
   whose estimated fixpoint exceeds the per-instance cap are rejected (and
   listed).  The admitted ones run (a) batched: one packed database per
-  program, one launch sequence; (b) plain: one shim session per instance.
+  program, one launch sequence, the two programs' batches concurrently
+  (one host thread and stream each); (b) plain: one shim session per
+  instance, one after the other.
   Both are timed end to end from host arrays to per-instance host results;
   every instance's tuples and per-iteration delta sizes must be identical."""
   import numpy as np
@@ -161,20 +166,25 @@ def bench_batch(args) -> dict:
     grains[k] = g
     log(f"batch: autotuned grain for {k}: {g} ({', '.join(f'{a}: {b * 1e3:.0f} ms' for a, b in times.items())})")
 
-  for k, lst in kinds.items():                       # warm-up (pool growth)
-    batch.run_batched(k, lst, grain=grains[k])
+  batch.run_batched_kinds(kinds, grains)              # warm-up (pool growth)
+  from srdatalog_hip import native
+  p0 = native.pool_stats()
   t = time.perf_counter()
-  got_b = {k: batch.run_batched(k, lst, grain=grains[k]) for k, lst in kinds.items()}
+  got_b = batch.run_batched_kinds(kinds, grains)
   t_b = time.perf_counter() - t
   log(f"batch: batched {len(ok)} instances in {t_b:.2f}s")
   t = time.perf_counter()
   got_p = {k: batch.run_plain(k, lst, grain=grains[k]) for k, lst in kinds.items()}
   t_p = time.perf_counter() - t
   log(f"batch: plain {len(ok)} instances in {t_p:.2f}s")
+  p1 = native.pool_stats()
+  log("batch: device pool during the timed runs: " +
+      ", ".join(f"{k} +{p1[k] - p0[k]}" for k in ("driver_allocs", "reserved", "trims")) +
+      f", peak {p1['peak'] / 1e9:.1f} GB")
 
   # where the time goes: kernel time (HIP events) against wall time, for one
-  # more batched run and for plain runs of a 16-instance sample
-  from srdatalog_hip import native
+  # more batched run (its kinds one after the other) and for plain runs of a
+  # 16-instance sample
   diag = {}
   for label, runner, sets in (("batched", batch.run_batched, kinds),
                               ("plain_sample16", batch.run_plain,
@@ -243,25 +253,45 @@ def bench_serve(args) -> dict:
       f"and {len(reqs)} requests in {time.time() - t0:.1f}s")
   plain_reqs = copy.deepcopy(reqs)
 
-  srv = serve.Server(graphs, budget_bytes=args.serve_budget, cap_tuples=args.serve_cap)
+  import threading
+  last = [time.time()]
+  plock = threading.Lock()
+
+  def progress(kind, t, completed):
+    with plock:
+      if time.time() - last[0] < 15.0:
+        return
+      last[0] = time.time()
+    log(f"serve: {completed} requests completed at t={t:.1f}s (last batch: {kind})")
+
+  srv = serve.Server(graphs, budget_bytes=args.serve_budget, cap_tuples=args.serve_cap,
+                     progress=progress)
   try:
-    wall_s = srv.run(reqs)
+    wall_s = srv.run(reqs, wall_cap=args.serve_wall_cap)
   finally:
     srv.close()
   served = serve.latency_stats(reqs)
   log(f"serve: served {served.get('completed')} in {wall_s:.1f}s, batches {srv.batches}")
 
   keep = [p for p, r in zip(plain_reqs, reqs) if not r.rejected]
-  wall_p = serve.run_plain(keep, graphs, wall_cap=args.serve_plain_cap)
+  by_id = {r.rid: r for r in reqs}
+  cmp = {"compared": 0, "mismatched": 0}
+
+  def check(p):
+    # compare with the served result while it is retained, then drop the
+    # plain one: host memory holds one plain result at a time
+    r = by_id[p.rid]
+    if not r.result_dropped:
+      cmp["compared"] += 1
+      cmp["mismatched"] += not serve.same_result(r, p)
+    if p.kind != "tri":
+      p.result = None
+
+  wall_p = serve.run_plain(keep, graphs, wall_cap=args.serve_plain_cap, on_result=check)
   plain = serve.latency_stats(keep)
   log(f"serve: plain served {plain.get('completed')} in {wall_p:.1f}s")
-
-  by_id = {r.rid: r for r in reqs}
-  compared = mismatched = 0
-  for p in keep:
-    if p.t_done >= 0:
-      compared += 1
-      mismatched += not serve.same_result(by_id[p.rid], p)
+  compared, mismatched = cmp["compared"], cmp["mismatched"]
+  dropped = sum(r.result_dropped for r in reqs)
   kinds = {}
   for k in serve.KIND_NAMES:
     kinds[k] = serve.latency_stats([r for r in reqs if r.kind == k])
@@ -278,10 +308,13 @@ def bench_serve(args) -> dict:
     "speedup_vs_plain": (round((served["completed"] / wall_s) / (plain["completed"] / wall_p), 3)
                          if plain.get("completed") else None),
     "requests": len(reqs), "completed": served["completed"], "rejected": len(rejected),
+    "unserved": sum(1 for r in reqs if not r.rejected and r.t_done < 0),
+    "served_wall_cap_s": args.serve_wall_cap,
     "rejected_examples": [{"rid": r.rid, "kind": r.kind, "graph_scale": graphs[r.graph].scale,
                            "reason": r.rejected} for r in rejected[:5]],
     "by_kind": kinds, "batches": srv.batches,
     "results_compared": compared, "results_identical": mismatched == 0,
+    "results_not_retained": dropped, "retained_result_gb": round(srv.kept_bytes / 1e9, 2),
     "served_wall_s": round(wall_s, 3), "plain_wall_s": round(wall_p, 3),
   }
 

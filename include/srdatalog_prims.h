@@ -83,14 +83,17 @@ int srd_prim_bench(int op, uint64_t na, uint64_t nb, uint32_t k, uint32_t reps,
                    double* ms_per_rep, uint64_t* out_n);
 
 /* Calibration: `reps` launches of srd_calib_copy_kernel, a plain streaming
- * copy of `bytes` (read once, written once); *ms_per_rep = mean duration.
- * Its profiler HBM counters give the scale for the other kernels' counters. */
+ * copy of `bytes` with 16-byte words (read once, written once); *ms_per_rep
+ * = mean duration.  Then `reps` launches of srd_calib_copy8_kernel, the same
+ * copy with 8-byte words.  Their profiler HBM counters give the scale for
+ * the other kernels' counters (and check it for 8-byte accesses). */
 int srd_calib_copy(uint64_t bytes, int reps, double* ms_per_rep);
 
 /* Device memory pool counters, out[0..cap) of: bytes in use, peak bytes in
  * use, budget (0 = none), allocations served, allocations that reached
  * hipMalloc, bytes obtained from hipMalloc, bytes cached on the free list,
- * times the free list was handed back.  The reference's RMM pool
+ * times cached blocks were handed back to the device (least recently
+ * freed first, as much as a failed hipMalloc needs).  The reference's RMM pool
  * (gpu/device_array_rmm_impl.h:102-160) has no counterpart query. */
 int srd_pool_stats(uint64_t* out, uint32_t cap);
 

@@ -97,10 +97,10 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
-// Block-wide exclusive scan for kBlock threads; returns this thread's
-// exclusive prefix and writes the block total to *total.  `wsum` must hold
-// kWavesPerBlock entries of shared memory.
-template <class T>
+// Block-wide exclusive scan for NW waves (the block's size); returns this
+// thread's exclusive prefix and writes the block total to *total.  `wsum`
+// must hold NW entries of shared memory.
+template <class T, int NW = kWavesPerBlock>
 __device__ __forceinline__ T block_excl_scan(T v, T* wsum, T* total) {
   const uint32_t l = lane_id();
   const uint32_t w = threadIdx.x / kWave;
@@ -109,7 +109,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T* wsum, T* total) {
   __syncthreads();
   T base = 0, tot = 0;
 #pragma unroll
-  for (int i = 0; i < kWavesPerBlock; ++i) {
+  for (int i = 0; i < NW; ++i) {
     T s = wsum[i];
     if (uint32_t(i) < w) base += s;
     tot += s;

@@ -233,6 +233,17 @@ struct Relation {
   bool counted = false;
 };
 
+// Device scratch of one pipeline run (root keys, slices, fused staging,
+// weighed slices, probe counter), grown on demand and kept across runs.
+struct PipeScratch {
+  DevBuf<uint32_t> key_rng, key_aux, counts_buf, work_ctr;
+  DevBuf<uint64_t> key_work, work_start, offsets;
+  DevBuf<uint32_t> stage, overflow;
+  DevBuf<uint64_t> recs, stage_ctr;
+  DevBuf<uint64_t> item_cost, item_pref, slice_start;
+  DevBuf<uint64_t> probe_buf;
+};
+
 }  // namespace srd
 
 struct srd_engine {
@@ -249,12 +260,15 @@ struct srd_engine {
   double run_seconds = 0;
   std::vector<std::pair<std::string, uint64_t>> counts;  // rule -> count (count-only)
   uint32_t grain_override = 0;
-  // per-pipeline scratch, grown on demand
-  srd::DevBuf<uint32_t> key_rng, key_aux, counts_buf, work_ctr;
-  srd::DevBuf<uint64_t> key_work, work_start, offsets;
-  // fused-mode staging, grown from the largest output each rule produced
-  srd::DevBuf<uint32_t> stage, overflow;
-  srd::DevBuf<uint64_t> recs, stage_ctr;
+  // per-pipeline scratch (one per member of the widest parallel group),
+  // grown on demand, and the streams, fences and pinned words of groups
+  std::vector<std::unique_ptr<srd::PipeScratch>> scratch;
+  std::vector<hipStream_t> group_s;
+  std::vector<hipEvent_t> fence_events;
+  uint64_t* run_pins = nullptr;
+  uint64_t run_pins_bytes = 0;
+  bool group_streams = true;       // SRDATALOG_GROUP_STREAMS=0: a group's members run in order
+  // fused-mode staging is sized from the largest output each rule produced
   std::map<std::string, uint64_t> stage_hint;
   bool two_pass = false;           // SRDATALOG_JOIN_MODE=twopass: count + write only
   bool no_direct = false;          // SRDATALOG_JOIN_MODE=fused: never the direct mode
@@ -275,7 +289,6 @@ struct srd_engine {
   uint64_t min_slices_per_wave = 4;  // SRDATALOG_MIN_SLICES_PER_WAVE: floor of the weighed slices
   uint32_t xcd_ranges = SRD_XCDS;  // SRDATALOG_XCD_RANGES: slice ranges of the join launches
   uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
-  srd::DevBuf<uint64_t> item_cost, item_pref, slice_start;
   // column-1 bitmaps of heavy keys for views nested levels search
   uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
   bool root_pos = true;                    // SRDATALOG_ROOT_POS=0: search root values instead
@@ -290,7 +303,6 @@ struct srd_engine {
   // caller launch of the probe mode
   bool count_probes = false;
   uint64_t join_searches = 0;
-  srd::DevBuf<uint64_t> probe_buf;
   bool trace = false;              // SRDATALOG_TRACE: one stderr line per op
   bool op_profile = false;         // SRDATALOG_OP_PROFILE: host time per op, printed per run
   std::map<std::string, std::pair<double, uint64_t>> op_times;
@@ -302,6 +314,14 @@ struct srd_engine {
   bool devloop_graph = true;       // SRDATALOG_DEVLOOP_GRAPH=0: enqueue without capturing
   uint64_t devloop_batch = 4;      // SRDATALOG_DEVLOOP_BATCH: iterations per control read
   uint64_t devloop_caps = 0;       // SRDATALOG_DEVLOOP_CAPS: initial capacities (tests)
+  // SRDATALOG_DEVLOOP_MAX_BYTES: a loop that would reserve more hands its
+  // state to the host loop (default: a quarter of the device memory)
+  uint64_t devloop_max_bytes = 0;
+  // SRDATALOG_DEVLOOP_MAX_ROWS: a step whose FULL or NEW outgrows this many
+  // rows continues on the host loop.  Its iterations then run long enough
+  // that the host loop's read-backs cost little, while growing the device
+  // loop's capacities costs a re-run of the iteration and fresh buffers.
+  uint64_t devloop_max_rows = uint64_t(1) << 24;
 };
 
 namespace srd {
@@ -692,24 +712,53 @@ static uint32_t stage_cols(const SrdPipelineDesc* d) {
   return kmax;
 }
 
+// One pipeline in flight.  A single pipeline runs its phases on the
+// engine's stream; the pipelines of a parallel group run each phase on
+// their own streams with their own scratch and meet only where the host
+// needs their sizes -- one wait per phase for the whole group instead of
+// one per pipeline and phase (the reference's ParallelGroup dispatch,
+// codegen/jit/orchestrator_jit.py:236-474, over a StreamPool,
+// runtime/generalized_datalog/gpu/runtime/stream_pool.h:18-89).
+struct PipeRun {
+  const SrdPipelineDesc* d = nullptr;
+  PipeScratch* x = nullptr;
+  hipStream_t s = nullptr;
+  SrdLaunchArgs a;
+  std::vector<uint32_t> view_max[SRD_MAX_VIEWS];
+  uint64_t* host = nullptr;   // 4 pinned words for this pipeline's read-backs
+  uint64_t total = 0;         // tuples emitted
+  uint64_t slices = 0;
+  uint64_t chunk_rows = 0;
+  uint64_t rec_cap = 0;
+  uint64_t weigh_items = 0;   // > 0: slices are being weighed
+  int mode = 0;               // 0 nothing to run, 1 direct, 2 count (+ write), 3 fused
+  bool ovf = false;
+};
+
+// Read-back of one u64 on the run's stream, collected after the phase's wait.
+static void pr_fetch(PipeRun& R, int slot, const void* dev, size_t bytes) {
+  SRD_CHECK(hipMemcpyAsync(R.host + slot, dev, bytes, hipMemcpyDeviceToHost, R.s));
+}
+
 // Size the fused mode's staging buffer for a launch of `slices` slices: the
 // expected output (the rule's largest so far, +25%) plus one segment per
 // wave, which may stay partly filled, clamped to stage_max_bytes.  A launch
 // that still overflows falls back to the write pass over the counts it
 // produced.  Returns false, leaving the two-pass mode to run, when the
 // buffer cannot be allocated.
-static bool stage_ready(srd_engine& e, const SrdPipelineDesc* d, uint64_t slices) {
+static bool stage_ready(srd_engine& e, PipeRun& R) {
+  const SrdPipelineDesc* d = R.d;
   const uint32_t kmax = stage_cols(d);
   uint64_t cap = std::max<uint64_t>(e.stage_hint[d->rule], uint64_t(1) << 24);
   // a wave keeps one segment open across its slices: at most one partly
   // filled segment per wave of the persistent grid
-  cap = std::max<uint64_t>(cap, std::min<uint64_t>(slices, 4 * 2048) * SRD_SEG + (uint64_t(1) << 20));
+  cap = std::max<uint64_t>(cap, std::min<uint64_t>(R.slices, 4 * 2048) * SRD_SEG + (uint64_t(1) << 20));
   cap = std::min<uint64_t>(cap, e.stage_max_bytes / (4ull * kmax));
   if (e.stage_cap_env) cap = std::max<uint64_t>(e.stage_cap_env, 1);
-  if (e.stage.n >= cap * kmax) return true;
-  e.stage.release();
+  if (R.x->stage.n >= cap * kmax) return true;
+  R.x->stage.release();
   try {
-    e.stage.alloc(cap * kmax, e.s);
+    R.x->stage.alloc(cap * kmax, R.s);
   } catch (const std::exception& ex) {
     if (e.trace) {
       std::fprintf(stderr, "[srd]   %s: no staging (%s): two-pass\n", d->rule, ex.what());
@@ -720,53 +769,12 @@ static bool stage_ready(srd_engine& e, const SrdPipelineDesc* d, uint64_t slices
   return true;
 }
 
-// Replace the uniform slices (grain items each) of a rule with >= 2 nested
-// levels by slices of about equal cost: the weigh launch writes each item's
-// level-2 candidate count + SRD_ITEM_COST, a scan prefixes the costs, and
-// slice s is cut at the item where the prefix crosses s/S of the total.  An
-// item is never split, so one item's candidates bound a slice's cost.
-static void weigh_slices(srd_engine& e, const SrdPipelineDesc* d, SrdLaunchArgs& a) {
-  const uint64_t n = a.total_work;
-  e.item_cost.reserve(n, e.s);
-  e.item_pref.reserve(n + 1, e.s);
-  a.item_cost = e.item_cost.p;
-  a.n_ranges = e.weigh_ranges;
-  a.grid_blocks = uint32_t(std::min<uint64_t>((a.n_slices + 3) / 4, 2048));
-  fill_u32(a.work_ctr, 0, SRD_XCDS * SRD_CTR_STRIDE, e.s);
-  std::string nm = std::string("join.weigh:") + d->rule;
-  timing_begin(nm.c_str(), e.s);
-  d->weigh(&a, e.s);
-  SRD_CHECK(hipGetLastError());
-  timing_end(e.s);
-  exclusive_scan_u64(a.item_cost, e.item_pref.p, n, e.ws, e.s);
-  const uint64_t w = read_u64(e, e.item_pref.p + n);
-  // up to e.slices_per_wave slices per persistent wave of a full grid (4 x 2048), never
-  // fewer than the uniform slicing had, never more than one per item
-  // and at least e.min_slices_per_wave per wave, so that a mid-sized join
-  // whose cost / weigh_target is below the wave count still spreads
-  const uint64_t cap = std::max<uint64_t>(a.n_slices, e.slices_per_wave * 4 * 2048);
-  const uint64_t floor_s = e.min_slices_per_wave * 4 * 2048;
-  uint64_t S = std::min<uint64_t>(
-      std::max<uint64_t>(std::max<uint64_t>(a.n_slices, w / e.weigh_target), floor_s), cap);
-  S = std::max<uint64_t>(std::min<uint64_t>(S, n), 1);
-  e.slice_start.reserve(S + 1, e.s);
-  slice_bounds(e.item_pref.p, n, w, S, e.slice_start.p, e.s);
-  a.slice_start = e.slice_start.p;
-  a.n_slices = S;
-  a.item_cost = nullptr;
-  a.n_ranges = e.xcd_ranges;
-  if (e.trace) {
-    std::fprintf(stderr, "[srd]   %s: weighed %llu items, cost %llu, %llu slices\n", d->rule,
-                 (unsigned long long)n, (unsigned long long)w, (unsigned long long)S);
-    std::fflush(stderr);
-  }
-}
-
-static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t iteration) {
-  SrdLaunchArgs a;
-  std::memset(&a, 0, sizeof(a));
-  std::vector<uint32_t> view_max[SRD_MAX_VIEWS];  // column maxima of the views read
-  // Resolve views (iteration 0 reads FULL wherever the rule reads DELTA).
+// Phase 1 (host, engine stream): resolve the views (iteration 0 reads FULL
+// wherever the rule reads DELTA), build the bitmaps and root maps they ask
+// for, pick the driver.
+static void pr_views(srd_engine& e, PipeRun& R, uint64_t iteration) {
+  const SrdPipelineDesc* d = R.d;
+  std::memset(&R.a, 0, sizeof(R.a));
   for (uint32_t v = 0; v < d->n_views; ++v) {
     const SrdViewSpec& vs = d->views[v];
     Relation& r = e.rels[rel_id(e, vs.rel)];
@@ -778,7 +786,7 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
       x = &delta_index(e, r, order);
     else
       x = &new_index(e, r, order);
-    view_max[v] = x->col_max;
+    R.view_max[v] = x->col_max;
     if (vs.want_bitmaps && x->arity == 2 && e.bitmap_bytes) {
       const HeavyBits* h = x->heavy.get();
       if (!h || h->built_for != x->cols.p || h->built_n != x->n) {
@@ -800,244 +808,496 @@ static void run_pipeline(srd_engine& e, const SrdPipelineDesc* d, uint64_t itera
         timing_end(e.s);
       }
     }
-    a.views.v[v] = x->view();
+    R.a.views.v[v] = x->view();
   }
-  a.n_ranges = e.xcd_ranges;
-  a.grain = e.grain_override ? e.grain_override : d->default_grain;
+  R.a.n_ranges = e.xcd_ranges;
+  R.a.grain = e.grain_override ? e.grain_override : d->default_grain;
   if (d->root_kind == SRD_ROOT_JOIN) {
     uint32_t drv = d->root_view[0];
     for (uint32_t i = 0; i < d->n_root_src; ++i) {
       uint32_t v = d->root_view[i];
-      if (a.views.v[v].r < a.views.v[drv].r) drv = v;
+      if (R.a.views.v[v].r < R.a.views.v[drv].r) drv = v;
     }
-    a.drv_view = drv;
-    a.n_root = a.views.v[drv].r;
+    R.a.drv_view = drv;
+    R.a.n_root = R.a.views.v[drv].r;
   } else {
-    a.drv_view = d->root_view[0];
-    a.n_root = a.views.v[a.drv_view].n ? 1 : 0;
+    R.a.drv_view = d->root_view[0];
+    R.a.n_root = R.a.views.v[R.a.drv_view].n ? 1 : 0;
   }
-  uint64_t total = 0;
-  if (a.n_root > 0) {
-    const uint64_t nk = a.n_root;
-    e.key_rng.reserve(nk * 2 * SRD_MAX_VIEWS, e.s);
-    e.key_aux.reserve(nk, e.s);
-    e.key_work.reserve(nk, e.s);
-    e.work_start.reserve(nk + 1, e.s);
-    e.work_ctr.reserve(SRD_XCDS * SRD_CTR_STRIDE, e.s);
-    if (e.count_probes) {
-      e.probe_buf.reserve(1, e.s);
-      fill_u64(e.probe_buf.p, 0, 1, e.s);
-      a.probe_ctr = reinterpret_cast<unsigned long long*>(e.probe_buf.p);
+}
+
+static void pr_launch(srd_engine& e, PipeRun& R, SrdLaunchFn fn, const char* phase) {
+  fill_u32(R.a.work_ctr, 0, SRD_XCDS * SRD_CTR_STRIDE, R.s);
+  std::string nm = std::string("join.") + phase + ":" + R.d->rule;
+  timing_begin(nm.c_str(), R.s);
+  fn(&R.a, R.s);
+  SRD_CHECK(hipGetLastError());
+  timing_end(R.s);
+}
+
+// Phase 2 (run stream): the root launch and the scan of the per-key work;
+// the work total is fetched.
+static void pr_root(srd_engine& e, PipeRun& R) {
+  const SrdPipelineDesc* d = R.d;
+  PipeScratch& x = *R.x;
+  const uint64_t nk = R.a.n_root;
+  x.key_rng.reserve(nk * 2 * SRD_MAX_VIEWS, R.s);
+  x.key_aux.reserve(nk, R.s);
+  x.key_work.reserve(nk, R.s);
+  x.work_start.reserve(nk + 1, R.s);
+  x.work_ctr.reserve(SRD_XCDS * SRD_CTR_STRIDE, R.s);
+  if (e.count_probes) {
+    x.probe_buf.reserve(1, R.s);
+    fill_u64(x.probe_buf.p, 0, 1, R.s);
+    R.a.probe_ctr = reinterpret_cast<unsigned long long*>(x.probe_buf.p);
+  }
+  R.a.key_rng = x.key_rng.p;
+  R.a.key_aux = x.key_aux.p;
+  R.a.key_work = x.key_work.p;
+  R.a.work_start = x.work_start.p;
+  R.a.work_ctr = x.work_ctr.p;
+  {
+    std::string nm = std::string("join.root:") + d->rule;
+    timing_begin(nm.c_str(), R.s);
+    d->root(&R.a, R.s);
+    SRD_CHECK(hipGetLastError());
+    timing_end(R.s);
+  }
+  exclusive_scan_u64(R.a.key_work, R.a.work_start, nk, e.ws, R.s);
+  pr_fetch(R, 0, R.a.work_start + nk, 8);
+}
+
+// Phase 3 (host + run stream): uniform slices, and for a rule with >= 2
+// nested levels the weigh launch -- every level-1 item's level-2 candidates
+// + SRD_ITEM_COST, prefix-summed -- whose total is fetched.
+static void pr_slices(srd_engine& e, PipeRun& R) {
+  const SrdPipelineDesc* d = R.d;
+  R.a.total_work = R.host[0];
+  R.a.n_slices = (R.a.total_work + R.a.grain - 1) / R.a.grain;
+  const bool direct = d->direct_ok && !d->count_only && !e.two_pass && !e.no_direct;
+  R.weigh_items = 0;
+  if (!direct && d->weigh && !e.no_weigh && R.a.total_work >= e.weigh_min &&
+      R.a.total_work <= (uint64_t(1) << 28)) {
+    PipeScratch& x = *R.x;
+    const uint64_t n = R.a.total_work;
+    x.item_cost.reserve(n, R.s);
+    x.item_pref.reserve(n + 1, R.s);
+    R.a.item_cost = x.item_cost.p;
+    R.a.n_ranges = e.weigh_ranges;
+    R.a.grid_blocks = uint32_t(std::min<uint64_t>((R.a.n_slices + 3) / 4, 2048));
+    pr_launch(e, R, d->weigh, "weigh");
+    exclusive_scan_u64(R.a.item_cost, x.item_pref.p, n, e.ws, R.s);
+    pr_fetch(R, 1, x.item_pref.p + n, 8);
+    R.weigh_items = n;
+  }
+}
+
+// Replace the uniform slices (grain items each) of a weighed rule by slices
+// of about equal cost: slice s is cut at the item where the cost prefix
+// crosses s/S of the total.  An item is never split, so one item's
+// candidates bound a slice's cost.
+static void pr_weighed(srd_engine& e, PipeRun& R) {
+  if (!R.weigh_items) return;
+  PipeScratch& x = *R.x;
+  const uint64_t n = R.weigh_items, w = R.host[1];
+  // up to e.slices_per_wave slices per persistent wave of a full grid (4 x 2048), never
+  // fewer than the uniform slicing had, never more than one per item
+  // and at least e.min_slices_per_wave per wave, so that a mid-sized join
+  // whose cost / weigh_target is below the wave count still spreads
+  const uint64_t cap = std::max<uint64_t>(R.a.n_slices, e.slices_per_wave * 4 * 2048);
+  const uint64_t floor_s = e.min_slices_per_wave * 4 * 2048;
+  uint64_t S = std::min<uint64_t>(
+      std::max<uint64_t>(std::max<uint64_t>(R.a.n_slices, w / e.weigh_target), floor_s), cap);
+  S = std::max<uint64_t>(std::min<uint64_t>(S, n), 1);
+  x.slice_start.reserve(S + 1, R.s);
+  slice_bounds(x.item_pref.p, n, w, S, x.slice_start.p, R.s);
+  R.a.slice_start = x.slice_start.p;
+  R.a.n_slices = S;
+  R.a.item_cost = nullptr;
+  R.a.n_ranges = e.xcd_ranges;
+  if (e.trace) {
+    std::fprintf(stderr, "[srd]   %s: weighed %llu items, cost %llu, %llu slices\n", R.d->rule,
+                 (unsigned long long)n, (unsigned long long)w, (unsigned long long)S);
+    std::fflush(stderr);
+  }
+}
+
+// Phase 4 (run stream): the traversal that sizes the output -- nothing for
+// the direct mode (one row per item), the count launch, or the fused launch
+// into staging -- with its total (and the staging overflow flag) fetched.
+static void pr_traverse(srd_engine& e, PipeRun& R) {
+  const SrdPipelineDesc* d = R.d;
+  PipeScratch& x = *R.x;
+  if (e.trace) {
+    std::fprintf(stderr, "[srd]   %s: keys %u drv %u work %llu slices %llu\n", d->rule, R.a.n_root,
+                 R.a.drv_view, (unsigned long long)R.a.total_work, (unsigned long long)R.a.n_slices);
+    std::fflush(stderr);
+  }
+  R.slices = R.a.n_slices;
+  R.mode = 0;
+  if (R.slices == 0) return;
+  x.counts_buf.reserve(R.slices, R.s);
+  x.offsets.reserve(R.slices + 1, R.s);
+  R.a.counts = x.counts_buf.p;
+  R.a.offsets = x.offsets.p;
+  R.a.grid_blocks = uint32_t(std::min<uint64_t>((R.slices + 3) / 4, 2048));
+  // Rows of the write (direct) pass, materialised at once when they fit
+  // one chunk, else in slice ranges whose rows are each sorted and
+  // deduplicated into a NEW part (a join's output before deduplication
+  // can be far larger than HBM).
+  uint32_t amax = 1;
+  for (uint32_t k = 0; k < d->n_dest; ++k) amax = std::max(amax, d->dest[k].arity);
+  R.chunk_rows = e.new_chunk_rows ? e.new_chunk_rows
+                                  : std::min<uint64_t>(e.stage_max_bytes / (4ull * amax), uint64_t(1) << 31);
+  const bool direct = d->direct_ok && !d->count_only && !e.two_pass && !e.no_direct;
+  if (direct) {
+    // one tuple per item: rows land at base + item, nothing to count
+    R.mode = 1;
+    R.total = R.a.total_work;
+    return;
+  }
+  if (d->count_only || e.two_pass || !stage_ready(e, R)) {
+    R.mode = 2;
+    pr_launch(e, R, d->count, "count");
+    exclusive_scan_u32(R.a.counts, R.a.offsets, R.slices, e.ws, R.s);
+    pr_fetch(R, 2, R.a.offsets + R.slices, 8);
+    return;
+  }
+  // fused: one traversal into staging segments, then gather.  Every slice
+  // may hold one partly filled segment, so the capacity wanted is the
+  // expected output plus a segment per slice; it is clamped to
+  // stage_max_bytes, and a launch that overflows falls back to the write
+  // pass over the counts it produced.
+  R.mode = 3;
+  const uint32_t kmax = stage_cols(d);
+  uint64_t cap = x.stage.n / kmax;
+  if (e.stage_cap_env) cap = std::min<uint64_t>(cap, e.stage_cap_env);
+  R.rec_cap = R.slices + cap / (SRD_SEG - 64) + 64;
+  x.recs.reserve(3 * R.rec_cap, R.s);
+  x.stage_ctr.reserve(2, R.s);
+  x.overflow.reserve(2 * SRD_CTR_STRIDE, R.s);  // flag and record counter on own lines
+  fill_u64(x.stage_ctr.p, 0, 2, R.s);
+  fill_u32(x.overflow.p, 0, 2 * SRD_CTR_STRIDE, R.s);
+  R.a.stage = x.stage.p;
+  R.a.stage_cap = cap;
+  R.a.stage_ctr = reinterpret_cast<unsigned long long*>(x.stage_ctr.p);
+  R.a.recs = x.recs.p;
+  R.a.rec_cap = R.rec_cap;
+  R.a.rec_ctr = x.overflow.p + SRD_CTR_STRIDE;
+  R.a.overflow = x.overflow.p;
+  pr_launch(e, R, d->fused, "fused");
+  exclusive_scan_u32(R.a.counts, R.a.offsets, R.slices, e.ws, R.s);
+  pr_fetch(R, 2, R.a.offsets + R.slices, 8);
+  pr_fetch(R, 3, x.overflow.p, 4);
+}
+
+// After phase 4's wait: the totals, and whether the rows fit one chunk and
+// (fused) staging.  Returns true when the rows are written by the simple
+// paths of phase 5 (the chunked and overflow paths run sequentially after).
+static bool pr_sized(srd_engine& e, PipeRun& R) {
+  const SrdPipelineDesc* d = R.d;
+  if (R.mode == 2 || R.mode == 3) R.total = R.host[2];
+  R.ovf = R.mode == 3 && uint32_t(R.host[3]) != 0;
+  if (R.mode == 3)
+    e.stage_hint[d->rule] = std::max<uint64_t>(
+        e.stage_hint[d->rule], R.total + R.total / 4 + std::min<uint64_t>(R.slices, 4 * 2048) * SRD_SEG);
+  if (R.mode == 0 || R.total == 0 || d->count_only) return false;
+  return R.total <= R.chunk_rows && !R.ovf;
+}
+
+// Reserve the NEW rows of the destinations (host, engine stream).
+static void pr_bind(srd_engine& e, PipeRun& R, uint64_t n_rows) {
+  const SrdPipelineDesc* d = R.d;
+  for (uint32_t k = 0; k < d->n_dest; ++k) {
+    const SrdDestSpec& ds = d->dest[k];
+    Relation& r = e.rels[rel_id(e, ds.rel)];
+    if (r.raw_n > 0 && r.raw_n + n_rows > R.chunk_rows) flush_raw(e, r);
+    // rows of one sorted-output pipeline into an empty NEW stay an index;
+    // bounds: the largest value of any column the join reads
+    r.raw_sorted = r.raw_n == 0 && d->sorted_output && d->n_dest == 1;
+    if (r.raw_sorted) {
+      uint32_t bound = 0;
+      for (uint32_t v = 0; v < d->n_views; ++v)
+        for (uint32_t m : R.view_max[v]) bound = std::max(bound, m);
+      r.raw_bound.assign(ds.arity, bound);
     }
-    a.key_rng = e.key_rng.p;
-    a.key_aux = e.key_aux.p;
-    a.key_work = e.key_work.p;
-    a.work_start = e.work_start.p;
-    a.work_ctr = e.work_ctr.p;
-    {
-      std::string nm = std::string("join.root:") + d->rule;
-      timing_begin(nm.c_str(), e.s);
-      d->root(&a, e.s);
-      SRD_CHECK(hipGetLastError());
-      timing_end(e.s);
-    }
-    exclusive_scan_u64(a.key_work, a.work_start, nk, e.ws, e.s);
-    a.total_work = read_u64(e, a.work_start + nk);
-    a.n_slices = (a.total_work + a.grain - 1) / a.grain;
-    const bool direct = d->direct_ok && !d->count_only && !e.two_pass && !e.no_direct;
-    if (!direct && d->weigh && !e.no_weigh && a.total_work >= e.weigh_min &&
-        a.total_work <= (uint64_t(1) << 28))
-      weigh_slices(e, d, a);
-    if (e.trace) {
-      std::fprintf(stderr, "[srd]   %s: keys %u drv %u work %llu slices %llu\n", d->rule, a.n_root,
-                   a.drv_view, (unsigned long long)a.total_work, (unsigned long long)a.n_slices);
-      std::fflush(stderr);
-    }
-    const uint64_t slices = a.n_slices;
-    if (slices > 0) {
-      e.counts_buf.reserve(slices, e.s);
-      e.offsets.reserve(slices + 1, e.s);
-      a.counts = e.counts_buf.p;
-      a.offsets = e.offsets.p;
-      a.grid_blocks = uint32_t(std::min<uint64_t>((slices + 3) / 4, 2048));
-      // Rows of the write (direct) pass, materialised at once when they fit
-      // one chunk, else in slice ranges whose rows are each sorted and
-      // deduplicated into a NEW part (a join's output before deduplication
-      // can be far larger than HBM).
-      uint32_t amax = 1;
-      for (uint32_t k = 0; k < d->n_dest; ++k) amax = std::max(amax, d->dest[k].arity);
-      const uint64_t chunk_rows =
-          e.new_chunk_rows ? e.new_chunk_rows
-                           : std::min<uint64_t>(e.stage_max_bytes / (4ull * amax), uint64_t(1) << 31);
-      auto bind_dests = [&](uint64_t n_rows) {
-        for (uint32_t k = 0; k < d->n_dest; ++k) {
-          const SrdDestSpec& ds = d->dest[k];
-          Relation& r = e.rels[rel_id(e, ds.rel)];
-          if (r.raw_n > 0 && r.raw_n + n_rows > chunk_rows) flush_raw(e, r);
-          // rows of one sorted-output pipeline into an empty NEW stay an index;
-          // bounds: the largest value of any column the join reads
-          r.raw_sorted = r.raw_n == 0 && d->sorted_output && d->n_dest == 1;
-          if (r.raw_sorted) {
-            uint32_t bound = 0;
-            for (uint32_t v = 0; v < d->n_views; ++v)
-              for (uint32_t m : view_max[v]) bound = std::max(bound, m);
-            r.raw_bound.assign(ds.arity, bound);
-          }
-          Order order(ds.cols, ds.cols + ds.arity);
-          uint64_t base = reserve_new(e, r, order, n_rows);
-          a.out[k].cols = r.raw.p;
-          a.out[k].stride = r.raw_cap;
-          a.out[k].base = base;
-        }
-      };
-      auto launch = [&](SrdLaunchFn fn, const char* phase) {
-        fill_u32(a.work_ctr, 0, SRD_XCDS * SRD_CTR_STRIDE, e.s);
-        std::string nm = std::string("join.") + phase + ":" + d->rule;
-        timing_begin(nm.c_str(), e.s);
-        fn(&a, e.s);
-        SRD_CHECK(hipGetLastError());
-        timing_end(e.s);
-      };
-      auto write_rows = [&](SrdLaunchFn fn, const char* phase, bool by_items) {
-        if (total <= chunk_rows) {
-          bind_dests(total);
-          launch(fn, phase);
-          return;
-        }
-        std::vector<uint64_t> off;
-        if (!by_items) {
-          off.resize(slices + 1);
-          SRD_CHECK(hipMemcpyAsync(off.data(), a.offsets, (slices + 1) * 8, hipMemcpyDeviceToHost, e.s));
-          SRD_CHECK(hipStreamSynchronize(e.s));
-        }
-        auto start_of = [&](uint64_t sl) {
-          return by_items ? std::min<uint64_t>(sl * a.grain, a.total_work) : off[sl];
-        };
-        std::vector<DevBuf<uint32_t>> bufs(d->n_dest);
-        uint64_t s0 = 0;
-        while (s0 < slices) {
-          uint64_t s1 = s0 + 1;  // at least one slice, even when it alone is larger
-          while (s1 < slices && start_of(s1 + 1) - start_of(s0) <= chunk_rows) ++s1;
-          const uint64_t rows = start_of(s1) - start_of(s0);
-          if (rows > 0) {
-            const uint64_t stride = round_stride(rows);
-            for (uint32_t k = 0; k < d->n_dest; ++k) {
-              bufs[k].reserve(stride * d->dest[k].arity, e.s);
-              a.out[k].cols = bufs[k].p;
-              a.out[k].stride = stride;
-              a.out[k].base = uint64_t(0) - start_of(s0);  // row = position - chunk start
-            }
-            fill_u32(a.work_ctr, 0, SRD_XCDS * SRD_CTR_STRIDE, e.s);
-            a.slice_lo = s0;
-            a.n_slices = s1;
-            std::string nm = std::string("join.") + phase + ":" + d->rule;
-            timing_begin(nm.c_str(), e.s);
-            fn(&a, e.s);
-            SRD_CHECK(hipGetLastError());
-            timing_end(e.s);
-            for (uint32_t k = 0; k < d->n_dest; ++k) {
-              const SrdDestSpec& ds = d->dest[k];
-              Relation& r = e.rels[rel_id(e, ds.rel)];
-              std::vector<const uint32_t*> in(ds.arity);
-              for (uint32_t c = 0; c < ds.arity; ++c) in[c] = bufs[k].p + uint64_t(c) * stride;
-              Index part;
-              build_index(in.data(), ds.arity, rows, part, e.ws, e.s);
-              add_new_part(e, r, Order(ds.cols, ds.cols + ds.arity), std::move(part));
-            }
-            if (e.trace) {
-              std::fprintf(stderr, "[srd]   %s: chunk of slices [%llu, %llu): %llu rows\n", d->rule,
-                           (unsigned long long)s0, (unsigned long long)s1, (unsigned long long)rows);
-              std::fflush(stderr);
-            }
-          }
-          s0 = s1;
-        }
-        a.slice_lo = 0;
-        a.n_slices = slices;
-        if (e.trace) {
-          std::fprintf(stderr, "[srd]   %s: %llu rows written in chunks of <= %llu\n", d->rule,
-                       (unsigned long long)total, (unsigned long long)chunk_rows);
-          std::fflush(stderr);
-        }
-      };
-      if (direct) {
-        // one tuple per item: rows land at base + item, nothing to count
-        total = a.total_work;
-        write_rows(d->direct, "direct", true);
-      } else if (d->count_only || e.two_pass || !stage_ready(e, d, slices)) {
-        launch(d->count, "count");
-        exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
-        total = read_u64(e, a.offsets + slices);
-        if (total > 0 && !d->count_only) write_rows(d->write, "write", false);
-      } else {
-        // fused: one traversal into staging segments, then gather.  Every
-        // slice may hold one partly filled segment, so the capacity wanted is
-        // the expected output plus a segment per slice; it is clamped to
-        // stage_max_bytes, and a launch that overflows falls back to the
-        // write pass over the counts it produced.
-        const uint32_t kmax = stage_cols(d);
-        uint64_t cap = e.stage.n / kmax;
-        if (e.stage_cap_env) cap = std::min<uint64_t>(cap, e.stage_cap_env);
-        const uint64_t rec_cap = slices + cap / (SRD_SEG - 64) + 64;
-        e.recs.reserve(3 * rec_cap, e.s);
-        e.stage_ctr.reserve(2, e.s);
-        e.overflow.reserve(2 * SRD_CTR_STRIDE, e.s);  // flag and record counter on own lines
-        fill_u64(e.stage_ctr.p, 0, 2, e.s);
-        fill_u32(e.overflow.p, 0, 2 * SRD_CTR_STRIDE, e.s);
-        a.stage = e.stage.p;
-        a.stage_cap = cap;
-        a.stage_ctr = reinterpret_cast<unsigned long long*>(e.stage_ctr.p);
-        a.recs = e.recs.p;
-        a.rec_cap = rec_cap;
-        a.rec_ctr = e.overflow.p + SRD_CTR_STRIDE;
-        a.overflow = e.overflow.p;
-        launch(d->fused, "fused");
-        exclusive_scan_u32(a.counts, a.offsets, slices, e.ws, e.s);
-        uint32_t* h = pinned_words(e.ws, 4);
-        SRD_CHECK(hipMemcpyAsync(h, a.offsets + slices, 8, hipMemcpyDeviceToHost, e.s));
-        SRD_CHECK(hipMemcpyAsync(h + 2, e.overflow.p, 4, hipMemcpyDeviceToHost, e.s));
-        SRD_CHECK(hipStreamSynchronize(e.s));
-        total = uint64_t(h[0]) | (uint64_t(h[1]) << 32);
-        const bool ovf = h[2] != 0;
-        e.stage_hint[d->rule] = std::max<uint64_t>(e.stage_hint[d->rule],
-                                                   total + total / 4 +
-                                                       std::min<uint64_t>(slices, 4 * 2048) * SRD_SEG);
-        if (total > 0) {
-          if (!ovf) {
-            bind_dests(total);
-            GatherDest gd[SRD_MAX_DESTS];
-            for (uint32_t k = 0; k < d->n_dest; ++k)
-              gd[k] = GatherDest{a.out[k].cols, a.out[k].stride, a.out[k].base, d->dest[k].arity};
-            gather_segments(a.recs, a.rec_ctr, rec_cap, a.stage, cap, a.offsets, gd, d->n_dest,
-                            total, e.s);
-          } else {
-            write_rows(d->write, "write", false);
-          }
-        }
-        if (e.trace) {
-          std::fprintf(stderr, "[srd]   %s: fused %llu tuples%s\n", d->rule,
-                       (unsigned long long)total, ovf ? " (staging overflow: write pass)" : "");
-          std::fflush(stderr);
-        }
+    Order order(ds.cols, ds.cols + ds.arity);
+    uint64_t base = reserve_new(e, r, order, n_rows);
+    R.a.out[k].cols = r.raw.p;
+    R.a.out[k].stride = r.raw_cap;
+    R.a.out[k].base = base;
+  }
+}
+
+// Phase 5 (run stream): write the bound rows -- the direct launch, or the
+// gather of the fused mode's staging.
+static void pr_write(srd_engine& e, PipeRun& R) {
+  const SrdPipelineDesc* d = R.d;
+  if (R.mode == 1) {
+    pr_launch(e, R, d->direct, "direct");
+  } else if (R.mode == 2) {
+    pr_launch(e, R, d->write, "write");
+  } else if (R.mode == 3) {
+    GatherDest gd[SRD_MAX_DESTS];
+    for (uint32_t k = 0; k < d->n_dest; ++k)
+      gd[k] = GatherDest{R.a.out[k].cols, R.a.out[k].stride, R.a.out[k].base, d->dest[k].arity};
+    gather_segments(R.a.recs, R.a.rec_ctr, R.rec_cap, R.a.stage, R.a.stage_cap, R.a.offsets, gd,
+                    d->n_dest, R.total, R.s);
+  }
+}
+
+// The rows of a run that exceeds one chunk (written in slice ranges, each
+// sorted and deduplicated into a NEW part) or whose fused launch overflowed
+// staging (the write pass over its counts).  Engine stream.
+static void pr_write_slow(srd_engine& e, PipeRun& R) {
+  const SrdPipelineDesc* d = R.d;
+  const bool by_items = R.mode == 1;
+  SrdLaunchFn fn = R.mode == 1 ? d->direct : d->write;
+  const char* phase = R.mode == 1 ? "direct" : "write";
+  const uint64_t slices = R.slices;
+  if (R.total <= R.chunk_rows) {
+    pr_bind(e, R, R.total);
+    pr_launch(e, R, fn, phase);
+    return;
+  }
+  std::vector<uint64_t> off;
+  if (!by_items) {
+    off.resize(slices + 1);
+    SRD_CHECK(hipMemcpyAsync(off.data(), R.a.offsets, (slices + 1) * 8, hipMemcpyDeviceToHost, R.s));
+    SRD_CHECK(hipStreamSynchronize(R.s));
+  }
+  auto start_of = [&](uint64_t sl) {
+    return by_items ? std::min<uint64_t>(sl * R.a.grain, R.a.total_work) : off[sl];
+  };
+  std::vector<DevBuf<uint32_t>> bufs(d->n_dest);
+  uint64_t s0 = 0;
+  while (s0 < slices) {
+    uint64_t s1 = s0 + 1;  // at least one slice, even when it alone is larger
+    while (s1 < slices && start_of(s1 + 1) - start_of(s0) <= R.chunk_rows) ++s1;
+    const uint64_t rows = start_of(s1) - start_of(s0);
+    if (rows > 0) {
+      const uint64_t stride = round_stride(rows);
+      for (uint32_t k = 0; k < d->n_dest; ++k) {
+        bufs[k].reserve(stride * d->dest[k].arity, R.s);
+        R.a.out[k].cols = bufs[k].p;
+        R.a.out[k].stride = stride;
+        R.a.out[k].base = uint64_t(0) - start_of(s0);  // row = position - chunk start
+      }
+      R.a.slice_lo = s0;
+      R.a.n_slices = s1;
+      pr_launch(e, R, fn, phase);
+      for (uint32_t k = 0; k < d->n_dest; ++k) {
+        const SrdDestSpec& ds = d->dest[k];
+        Relation& r = e.rels[rel_id(e, ds.rel)];
+        std::vector<const uint32_t*> in(ds.arity);
+        for (uint32_t c = 0; c < ds.arity; ++c) in[c] = bufs[k].p + uint64_t(c) * stride;
+        Index part;
+        build_index(in.data(), ds.arity, rows, part, e.ws, R.s);
+        add_new_part(e, r, Order(ds.cols, ds.cols + ds.arity), std::move(part));
       }
       if (e.trace) {
-        std::fprintf(stderr, "[srd]   %s: %llu tuples\n", d->rule, (unsigned long long)total);
+        std::fprintf(stderr, "[srd]   %s: chunk of slices [%llu, %llu): %llu rows\n", d->rule,
+                     (unsigned long long)s0, (unsigned long long)s1, (unsigned long long)rows);
         std::fflush(stderr);
       }
-      if (e.count_probes && d->probe) launch(d->probe, "probe");
     }
-    if (e.count_probes) e.join_searches += read_u64(e, e.probe_buf.p);
+    s0 = s1;
   }
-  e.join_probes += total;
+  R.a.slice_lo = 0;
+  R.a.n_slices = slices;
+  if (e.trace) {
+    std::fprintf(stderr, "[srd]   %s: %llu rows written in chunks of <= %llu\n", d->rule,
+                 (unsigned long long)R.total, (unsigned long long)R.chunk_rows);
+    std::fflush(stderr);
+  }
+}
+
+// Count-only bookkeeping, probes and trace of a finished run.
+static void pr_finish(srd_engine& e, PipeRun& R) {
+  const SrdPipelineDesc* d = R.d;
+  if (e.trace && R.mode != 0) {
+    std::fprintf(stderr, "[srd]   %s: %llu tuples%s\n", d->rule, (unsigned long long)R.total,
+                 R.ovf ? " (staging overflow: write pass)" : "");
+    std::fflush(stderr);
+  }
+  e.join_probes += R.total;
   if (d->count_only) {
-    e.counts.emplace_back(d->rule, total);
+    e.counts.emplace_back(d->rule, R.total);
     for (uint32_t k = 0; k < d->n_dest; ++k) {
       Relation& r = e.rels[rel_id(e, d->dest[k].rel)];
-      r.count += total;
+      r.count += R.total;
       r.counted = true;
     }
-    if (!e.quiet) std::cout << " >>>>>>>>>>>>>>>>> " << d->rule << " count: " << total << std::endl;
+    if (!e.quiet) std::cout << " >>>>>>>>>>>>>>>>> " << d->rule << " count: " << R.total << std::endl;
   }
+}
+
+// Wait for every run's stream (the phase's read-backs have landed).
+static void pr_wait(std::vector<PipeRun>& runs) {
+  for (auto& R : runs) SRD_CHECK(hipStreamSynchronize(R.s));
+}
+
+// Make every stream of `to` wait for the work enqueued so far on `from`.
+static void pr_fence(srd_engine& e, hipStream_t from, const std::vector<PipeRun>& runs, bool into_runs) {
+  if (runs.size() == 1 && runs[0].s == e.s) return;
+  if (into_runs) {
+    hipEvent_t ev = e.fence_events[0];
+    SRD_CHECK(hipEventRecord(ev, from));
+    for (auto& R : runs) SRD_CHECK(hipStreamWaitEvent(R.s, ev, 0));
+  } else {
+    for (size_t i = 0; i < runs.size(); ++i) {
+      hipEvent_t ev = e.fence_events[1 + i];
+      SRD_CHECK(hipEventRecord(ev, runs[i].s));
+      SRD_CHECK(hipStreamWaitEvent(from, ev, 0));
+    }
+  }
+}
+
+// Run pipelines -- one, or the members of a parallel group -- phase by
+// phase.  A group's members each get a stream and scratch of their own (a
+// member counting probes or needing the chunked or overflow path finishes
+// on the engine's stream, after the others).
+static void run_pipelines(srd_engine& e, const std::vector<const SrdPipelineDesc*>& ds,
+                          uint64_t iteration) {
+  const size_t P = ds.size();
+  const bool group = P > 1 && e.group_streams;
+  while (e.scratch.size() < P) e.scratch.emplace_back(new PipeScratch());
+  if (group) {
+    while (e.group_s.size() < P) e.group_s.push_back(stream_acquire());
+    while (e.fence_events.size() < P + 1) {
+      hipEvent_t ev;
+      SRD_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      e.fence_events.push_back(ev);
+    }
+  } else if (e.fence_events.empty()) {
+    hipEvent_t ev;
+    SRD_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    e.fence_events.push_back(ev);
+  }
+  // pinned words of the runs' read-backs (a block of their own: the sorts a
+  // run may start reallocate the workspace's words)
+  if (e.run_pins_bytes < 64 * P) {
+    pinned_release(e.run_pins, e.run_pins_bytes);
+    e.run_pins = nullptr;
+    e.run_pins_bytes = 0;
+    e.run_pins = static_cast<uint64_t*>(pinned_acquire(64 * std::max<size_t>(P, 8), &e.run_pins_bytes));
+  }
+  uint64_t* pins = e.run_pins;
+  if (!group) {
+    // members one after the other on the engine's stream, as a single pipeline
+    for (size_t i = 0; i < P; ++i) {
+      std::vector<PipeRun> one(1);
+      PipeRun& R = one[0];
+      R.d = ds[i];
+      R.x = e.scratch[0].get();
+      R.s = e.s;
+      R.host = pins;
+      pr_views(e, R, iteration);
+      if (R.a.n_root > 0) {
+        pr_root(e, R);
+        pr_wait(one);
+        pr_slices(e, R);
+        if (R.weigh_items) pr_wait(one);
+        pr_weighed(e, R);
+        pr_traverse(e, R);
+        if (R.mode >= 2) pr_wait(one);
+        if (pr_sized(e, R)) {
+          pr_bind(e, R, R.total);
+          pr_write(e, R);
+        } else if (R.mode != 0 && R.total > 0 && !R.d->count_only) {
+          pr_write_slow(e, R);
+        }
+        if (e.count_probes && R.slices > 0 && R.d->probe) pr_launch(e, R, R.d->probe, "probe");
+        if (e.count_probes) {
+          pr_fetch(R, 0, R.x->probe_buf.p, 8);
+          pr_wait(one);
+          e.join_searches += R.host[0];
+        }
+      }
+      pr_finish(e, R);
+    }
+    return;
+  }
+  std::vector<PipeRun> runs(P);
+  for (size_t i = 0; i < P; ++i) {
+    runs[i].d = ds[i];
+    runs[i].x = e.scratch[i].get();
+    runs[i].s = e.group_s[i];
+    runs[i].host = pins + 8 * i;
+    pr_views(e, runs[i], iteration);
+  }
+  // every member reads indexes built on the engine's stream
+  pr_fence(e, e.s, runs, true);
+  std::vector<PipeRun*> live;
+  for (auto& R : runs)
+    if (R.a.n_root > 0) {
+      pr_root(e, R);
+      live.push_back(&R);
+    }
+  pr_wait(runs);
+  bool weighed = false;
+  for (PipeRun* R : live) {
+    pr_slices(e, *R);
+    weighed = weighed || R->weigh_items;
+  }
+  if (weighed) pr_wait(runs);
+  for (PipeRun* R : live) {
+    pr_weighed(e, *R);
+    pr_traverse(e, *R);
+  }
+  pr_wait(runs);
+  // sizes known: reserve every member's NEW rows on the engine's stream (the
+  // members are idle), then write them concurrently
+  std::vector<PipeRun*> fast, slow;
+  for (PipeRun* R : live) {
+    if (pr_sized(e, *R)) fast.push_back(R);
+    else if (R->mode != 0 && R->total > 0 && !R->d->count_only) slow.push_back(R);
+  }
+  // The members' rows go to NEW one after the other.  A destination whose
+  // rows would exceed a chunk takes them sequentially (the chunked path
+  // flushes written rows, and these are not written yet).
+  std::map<int, uint64_t> need;
+  for (PipeRun* R : fast)
+    for (uint32_t k = 0; k < R->d->n_dest; ++k) need[rel_id(e, R->d->dest[k].rel)] += R->total;
+  std::vector<PipeRun*> bound;
+  for (PipeRun* R : fast) {
+    bool fits = true;
+    for (uint32_t k = 0; k < R->d->n_dest; ++k) {
+      const int ri = rel_id(e, R->d->dest[k].rel);
+      fits = fits && e.rels[ri].raw_n + need[ri] <= R->chunk_rows;
+    }
+    (fits ? bound : slow).push_back(R);
+  }
+  for (PipeRun* R : bound) pr_bind(e, *R, R->total);
+  // a later reservation may have moved a relation's NEW buffer (rows kept):
+  // point every member at the final one
+  for (PipeRun* R : bound)
+    for (uint32_t k = 0; k < R->d->n_dest; ++k) {
+      const Relation& r = e.rels[rel_id(e, R->d->dest[k].rel)];
+      R->a.out[k].cols = r.raw.p;
+      R->a.out[k].stride = r.raw_cap;
+    }
+  fast = bound;
+  pr_fence(e, e.s, runs, true);
+  for (PipeRun* R : fast) pr_write(e, *R);
+  if (e.count_probes)
+    for (PipeRun* R : live)
+      if (R->slices > 0 && R->d->probe) pr_launch(e, *R, R->d->probe, "probe");
+  pr_fence(e, e.s, runs, false);
+  // the chunked / overflow members, one after the other
+  for (PipeRun* R : slow) {
+    R->s = e.s;
+    pr_write_slow(e, *R);
+  }
+  if (e.count_probes) {
+    for (PipeRun* R : live) {
+      R->s = e.s;
+      pr_fetch(*R, 0, R->x->probe_buf.p, 8);
+    }
+    SRD_CHECK(hipStreamSynchronize(e.s));
+    for (PipeRun* R : live) e.join_searches += R->host[0];
+  }
+  for (auto& R : runs) pr_finish(e, R);
 }
 
 // =========================================================== MIR ops ====
@@ -1070,9 +1330,11 @@ static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no
   const auto t0 = std::chrono::steady_clock::now();
   exec_op_body(e, op, iteration, step_no);
   const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (op.kind == OpKind::Parallel) return;  // its children are counted
-  std::string key = std::string(op_name(op.kind)) + " " +
-                    (op.kind == OpKind::Pipeline ? op.rule : (op.rel >= 0 ? e.rels[op.rel].name : ""));
+  std::string key = std::string(op_name(op.kind)) + " ";
+  if (op.kind == OpKind::Pipeline) key += op.rule;
+  else if (op.kind == OpKind::Parallel)
+    for (auto& k : op.kids) key += k.rule + " ";
+  else if (op.rel >= 0) key += e.rels[op.rel].name;
   auto& slot = e.op_times[key];
   slot.first += dt;
   slot.second += 1;
@@ -1087,11 +1349,16 @@ static void exec_op_body(srd_engine& e, const Op& op, uint64_t iteration, int st
   }
   switch (op.kind) {
     case OpKind::Pipeline:
-      run_pipeline(e, op.pipe, iteration);
+      run_pipelines(e, {op.pipe}, iteration);
       break;
-    case OpKind::Parallel:
-      exec_ops(e, op.kids, iteration, step_no);
+    case OpKind::Parallel: {
+      std::vector<const SrdPipelineDesc*> ds;
+      for (auto& k : op.kids)
+        if (k.kind == OpKind::Pipeline) ds.push_back(k.pipe);
+      if (ds.size() == op.kids.size()) run_pipelines(e, ds, iteration);
+      else exec_ops(e, op.kids, iteration, step_no);
       break;
+    }
     case OpKind::RebuildIndex: {
       Relation& r = e.rels[op.rel];
       if (op.ver == SRD_NEW) {
@@ -1323,7 +1590,7 @@ struct DevRelState {
   DevBuf<uint32_t> raw, nsort, nroot_v, nroot_o, delta, droot_v, droot_o, full[2];
   DevBuf<uint8_t> sort_scratch;
   DevBuf<uint64_t> sd_split, sd_off, sd_scan, mg_split, rt_off, rt_scan, deltas, inst;
-  DevBuf<uint32_t> sd_tmp, sd_cnt, rt_pos, rt_cnt;
+  DevBuf<uint32_t> sd_cnt, rt_cnt;
   uint32_t bound = 0;   // upper bound of every column value (for col_max)
 };
 
@@ -1500,12 +1767,13 @@ static void devloop_alloc(srd_engine& e, const Step& st, DevLoop& L, const DevCa
     R.full[0].alloc(k * cf, s);
     R.full[1].alloc(k * cf, s);
     R.sd_split.alloc(sd_tiles + 1, s);
-    R.sd_tmp.alloc(k * cn, s);
+    // the set difference stages its kept rows in the raw NEW columns and the
+    // root build its key positions in the sorted NEW's root values: both are
+    // dead once NEW is sorted
     R.sd_cnt.alloc(sd_tiles, s);
     R.sd_off.alloc(sd_tiles + 1, s);
     R.sd_scan.alloc(scan_dev_words(sd_tiles), s);
     R.mg_split.alloc(mg_tiles + 1, s);
-    R.rt_pos.alloc(cn, s);
     R.rt_cnt.alloc(rt_tiles, s);
     R.rt_off.alloc(rt_tiles + 1, s);
     R.rt_scan.alloc(scan_dev_words(rt_tiles), s);
@@ -1641,11 +1909,11 @@ static void devloop_enqueue(srd_engine& e, const Step& st, DevLoop& L, int parit
         const uint32_t j = slot_of(op->rel);
         DevRelState& R = L.rels[j];
         dl_sd_prep(c, j, s);
-        SetDiffDevScratch x{R.sd_split.p, R.sd_tmp.p, cn, R.sd_cnt.p, R.sd_off.p, R.sd_scan.p};
+        SetDiffDevScratch x{R.sd_split.p, R.raw.p, cn, R.sd_cnt.p, R.sd_off.p, R.sd_scan.p};
         set_difference_dev(R.k, R.nsort.p, cn, R.full[parity].p, cf, cn + cf, c->sd_nab[j], R.delta.p,
                            cn, x, &c->sd_total[j], s);
         dl_sd_fin(c, j, s);
-        build_roots_dev(R.delta.p, cn, &c->delta_n[j], R.rt_pos.p, R.rt_cnt.p, R.rt_off.p,
+        build_roots_dev(R.delta.p, cn, &c->delta_n[j], R.nroot_v.p, R.rt_cnt.p, R.rt_off.p,
                         R.rt_scan.p, R.droot_v.p, R.droot_o.p, &c->delta_r[j], s);
         if (e.n_inst) {
           dl_inst(c, j, R.delta.p, e.inst_bounds.p, e.n_inst, R.inst.p, s);
@@ -1673,6 +1941,7 @@ static void devloop_enqueue(srd_engine& e, const Step& st, DevLoop& L, int parit
 
 static void devloop_capture(srd_engine& e, const Step& st, DevLoop& L) {
   L.drop_graphs();
+  std::lock_guard<std::recursive_mutex> cg(capture_mutex());
   for (int p = 0; p < 2; ++p) {
     hipGraph_t g = nullptr;
     SRD_CHECK(hipStreamBeginCapture(e.s, hipStreamCaptureModeRelaxed));
@@ -1693,6 +1962,20 @@ static void devloop_capture(srd_engine& e, const Step& st, DevLoop& L) {
 }  // namespace srd
 
 namespace srd {
+
+// Device bytes of a loop at capacities cp (what devloop_alloc reserves).
+static uint64_t devloop_bytes(const DevLoop& L, const DevCaps& cp) {
+  const uint64_t cn = round_stride(cp.new_rows), cf = round_stride(cp.full_rows);
+  uint64_t b = 0;
+  for (auto& R : L.rels) {
+    b += uint64_t(R.k) * 4 * (3 * cn + 2 * cf);   // raw (also set-difference scratch), sorted NEW, DELTA, FULL x2
+    b += 4 * 4 * cn;                              // root arrays of sorted NEW (also root scratch) and DELTA
+    b += devsort_scratch_bytes(cn);
+    b += cp.iters * 8 * 2;
+  }
+  b += L.pipes.size() * (cp.keys * (4 * 2 * SRD_MAX_VIEWS + 4 + 8 + 8) + cp.slices * 12);
+  return b;
+}
 
 // Copy the loop state (FULL in full[parity], DELTA and its root arrays,
 // the recorded per-iteration sizes) from `from` into the fresh buffers of `to`.
@@ -1793,7 +2076,30 @@ static uint64_t run_step_device(srd_engine& e, const Step& st, int step_no, uint
   // DELTA starts as FULL: both must hold the base step's FULL
   cp.full_rows = std::max<uint64_t>(cp.full_rows, std::max<uint64_t>(full_max, 1));
   cp.new_rows = std::max<uint64_t>(cp.new_rows, std::max<uint64_t>(full_max, 1));
-  devloop_alloc(e, st, *L, cp);
+  if (full_max > e.devloop_max_rows) {
+    if (e.trace) {
+      std::fprintf(stderr, "[srd]   device loop: FULL above %llu rows: host loop\n",
+                   (unsigned long long)e.devloop_max_rows);
+      std::fflush(stderr);
+    }
+    return 0;
+  }
+  if (devloop_bytes(*L, cp) > e.devloop_max_bytes) {
+    if (e.trace) {
+      std::fprintf(stderr, "[srd]   device loop: initial capacities above the memory budget: host loop\n");
+      std::fflush(stderr);
+    }
+    return 0;
+  }
+  try {
+    devloop_alloc(e, st, *L, cp);
+  } catch (const std::exception& ex) {
+    if (e.trace) {
+      std::fprintf(stderr, "[srd]   device loop: %s: host loop\n", ex.what());
+      std::fflush(stderr);
+    }
+    return 0;
+  }
 
   // DELTA := FULL (iteration 0 reads FULL where the rules read DELTA)
   DevCtl h;
@@ -1846,11 +2152,27 @@ static uint64_t run_step_device(srd_engine& e, const Step& st, int step_no, uint
     }
     if (h.err) fail("device loop: a device-planned sort reported an error");
     if (h.overflow) {
+      if (h.need_new > e.devloop_max_rows || h.need_full > e.devloop_max_rows) {
+        if (e.trace) {
+          std::fprintf(stderr, "[srd]   device loop: NEW or FULL above %llu rows: host loop from "
+                       "iteration %llu\n", (unsigned long long)e.devloop_max_rows,
+                       (unsigned long long)h.iter);
+          std::fflush(stderr);
+        }
+        break;
+      }
       DevCaps nc{L->cap_new, L->cap_full, L->cap_keys, L->cap_slices, L->iter_cap};
       auto grow = [](uint64_t cur, uint64_t need) {
         return std::max<uint64_t>(2 * cur, need + need / 4);
       };
-      if (h.overflow & kOvfNew) nc.new_rows = grow(nc.new_rows, h.need_new);
+      if (h.overflow & kOvfNew) {
+        nc.new_rows = grow(nc.new_rows, h.need_new);
+        // FULL u NEW follows: grow FULL with NEW instead of re-running the
+        // iteration once more when the union outgrows it
+        uint64_t fmax = 0;
+        for (size_t j = 0; j < L->rels.size(); ++j) fmax = std::max(fmax, h.full_n[j]);
+        nc.full_rows = std::max(nc.full_rows, fmax + h.need_new);
+      }
       if (h.overflow & kOvfFull) nc.full_rows = grow(nc.full_rows, h.need_full);
       if (h.overflow & kOvfKeys) nc.keys = grow(nc.keys, h.need_keys);
       if (h.overflow & kOvfSlices) nc.slices = grow(nc.slices, h.need_slices);
@@ -1861,6 +2183,14 @@ static uint64_t run_step_device(srd_engine& e, const Step& st, int step_no, uint
                      (unsigned long long)nc.full_rows, (unsigned long long)nc.keys,
                      (unsigned long long)nc.slices, (unsigned long long)nc.iters);
         std::fflush(stderr);
+      }
+      if (devloop_bytes(*L, nc) > e.devloop_max_bytes) {
+        if (e.trace) {
+          std::fprintf(stderr, "[srd]   device loop: growth above the memory budget: host loop from "
+                       "iteration %llu\n", (unsigned long long)h.iter);
+          std::fflush(stderr);
+        }
+        break;
       }
       auto L2 = std::make_unique<DevLoop>();
       for (auto& R : L->rels) {
@@ -1881,7 +2211,16 @@ static uint64_t run_step_device(srd_engine& e, const Step& st, int step_no, uint
         std::memcpy(Q.vrel, P.vrel, sizeof(Q.vrel));
         L2->pipes.push_back(std::move(Q));
       }
-      devloop_alloc(e, st, *L2, nc);
+      try {
+        devloop_alloc(e, st, *L2, nc);
+      } catch (const std::exception& ex) {
+        if (e.trace) {
+          std::fprintf(stderr, "[srd]   device loop: %s: host loop from iteration %llu\n", ex.what(),
+                       (unsigned long long)h.iter);
+          std::fflush(stderr);
+        }
+        break;
+      }
       devloop_copy_state(e, *L, parity, *L2, h);
       L2->ctl_buf = L->ctl_buf;
       L2->ctl = L->ctl;
@@ -1956,7 +2295,7 @@ static uint64_t run_step_device(srd_engine& e, const Step& st, int step_no, uint
     clear_new(r);
   }
   e.join_probes += h.join_rows;
-  *converged = h.done != 0;
+  *converged = h.done != 0 || h.iter >= max_iters;
   return h.iter;
 }
 
@@ -2158,12 +2497,23 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     if (const char* c = std::getenv("SRDATALOG_STAGE_MAX_BYTES"))
       e->stage_max_bytes = std::max<uint64_t>(std::strtoull(c, nullptr, 10), uint64_t(1) << 20);
     if (const char* c = std::getenv("SRDATALOG_DEVLOOP")) e->devloop = !(*c == '0' && c[1] == 0);
+    if (const char* c = std::getenv("SRDATALOG_GROUP_STREAMS"))
+      e->group_streams = !(*c == '0' && c[1] == 0);
     if (const char* c = std::getenv("SRDATALOG_DEVLOOP_GRAPH"))
       e->devloop_graph = !(*c == '0' && c[1] == 0);
     if (const char* c = std::getenv("SRDATALOG_DEVLOOP_BATCH"))
       e->devloop_batch = std::max<uint64_t>(std::strtoull(c, nullptr, 10), 1);
     if (const char* c = std::getenv("SRDATALOG_DEVLOOP_CAPS"))
       e->devloop_caps = std::strtoull(c, nullptr, 10);
+    if (const char* c = std::getenv("SRDATALOG_DEVLOOP_MAX_ROWS"))
+      e->devloop_max_rows = std::strtoull(c, nullptr, 10);
+    {
+      size_t fr = 0, tot = 0;
+      std::lock_guard<std::recursive_mutex> cg(capture_mutex());
+      e->devloop_max_bytes = hipMemGetInfo(&fr, &tot) == hipSuccess ? uint64_t(tot) / 4 : uint64_t(32) << 30;
+      if (const char* c = std::getenv("SRDATALOG_DEVLOOP_MAX_BYTES"))
+        e->devloop_max_bytes = std::strtoull(c, nullptr, 10);
+    }
     if (const char* m = std::getenv("SRDATALOG_JOIN_MODE")) {
       e->two_pass = std::string(m) == "twopass";
       e->no_direct = std::string(m) == "fused";
@@ -2180,6 +2530,7 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
 void srd_engine_destroy(srd_engine* e) {
   if (!e) return;
   hipStream_t s = e->s;
+  std::vector<hipStream_t> gs = e->group_s;
   {
     std::unique_ptr<srd_engine> hold(e);
     for (auto& r : e->rels) {
@@ -2188,8 +2539,12 @@ void srd_engine_destroy(srd_engine* e) {
       clear_new(r);
       r.dev_input.release();
     }
+    e->scratch.clear();
+    for (hipEvent_t ev : e->fence_events) (void)hipEventDestroy(ev);
+    pinned_release(e->run_pins, e->run_pins_bytes);
   }
-  // the stream (and the blocks freed on it) serve the next engine
+  // the streams (and the blocks freed on them) serve the next engine
+  for (hipStream_t g : gs) stream_release(g);
   stream_release(s);
 }
 

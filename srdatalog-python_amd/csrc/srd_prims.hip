@@ -43,156 +43,6 @@ void check(hipError_t e, const char* what) {
   if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// ================================================================ pool ====
-// A caching allocator over hipMalloc, in the manner of the reference's RMM
-// pool (gpu/device_array_rmm_impl.h:102-160, memory is never handed back
-// while the process runs).  Freed blocks go to a free list tagged with the
-// stream they were freed on; a later allocation on the same stream reuses
-// one without synchronising (stream order makes the reuse safe), blocks of a
-// stream that has been retired (synchronised) serve any stream.  Steady-state
-// runs therefore never call into the driver's allocator: the stream-ordered
-// hipMallocAsync pool this replaced blocked the host for 0.1-5 s per run once
-// many blocks had passed through it.  Only when hipMalloc fails are the
-// cached blocks released (after a device synchronisation) and the request
-// retried.
-namespace {
-std::mutex g_pool_mu;
-PoolStats g_stats;
-bool g_pool_ready = false;
-int g_device = 0;
-
-struct FreeBlock {
-  void* p;
-  hipStream_t s;
-  bool any_stream;  // its stream was synchronised when retired
-};
-std::multimap<uint64_t, FreeBlock> g_free;    // block bytes -> block
-std::map<void*, uint64_t> g_live;             // live pointer -> block bytes
-uint64_t g_cached = 0;                        // bytes held in g_free
-
-uint64_t env_bytes(const char* a, const char* b) {
-  const char* v = std::getenv(a);
-  if (!v) v = std::getenv(b);
-  if (!v || !*v) return 0;
-  return std::strtoull(v, nullptr, 10);
-}
-
-// Size classes: powers of two up to 1 MiB, 2 MiB multiples beyond.
-uint64_t block_bytes(uint64_t bytes) {
-  if (bytes <= (uint64_t(1) << 20)) {
-    uint64_t b = 512;
-    while (b < bytes) b <<= 1;
-    return b;
-  }
-  const uint64_t g = uint64_t(2) << 20;
-  return (bytes + g - 1) / g * g;
-}
-
-// Largest cached block a request of `b` bytes may take (bounds the waste of
-// grow-only buffers handed blocks far above their size).
-uint64_t fit_limit(uint64_t b) {
-  if (b <= (uint64_t(1) << 20)) return b;
-  return b + std::max<uint64_t>(b / 4, uint64_t(8) << 20);
-}
-
-// Take a cached block for `b` bytes on stream s (lock held), or nullptr.
-void* take_cached(uint64_t b, hipStream_t s, uint64_t* got) {
-  const uint64_t lim = fit_limit(b);
-  for (auto it = g_free.lower_bound(b); it != g_free.end() && it->first <= lim; ++it) {
-    if (it->second.s == s || it->second.any_stream) {
-      void* p = it->second.p;
-      *got = it->first;
-      g_cached -= it->first;
-      g_free.erase(it);
-      return p;
-    }
-  }
-  return nullptr;
-}
-}  // namespace
-
-void pool_init(int device) {
-  std::lock_guard<std::mutex> g(g_pool_mu);
-  if (g_pool_ready) return;
-  SRD_CHECK(hipSetDevice(device));
-  g_device = device;
-  g_stats.budget = env_bytes("SRDATALOG_POOL_MAX_SIZE", "SRDATALOG_RMM_POOL_MAX_SIZE");
-  g_pool_ready = true;
-}
-
-void* pool_alloc(uint64_t bytes, hipStream_t s) {
-  if (!g_pool_ready) pool_init(0);
-  const uint64_t b = block_bytes(std::max<uint64_t>(bytes, 1));
-  std::lock_guard<std::mutex> g(g_pool_mu);
-  if (g_stats.budget && g_stats.in_use + bytes > g_stats.budget)
-    fail("srdatalog pool: allocation of " + std::to_string(bytes) + " bytes exceeds the budget of " +
-         std::to_string(g_stats.budget) + " bytes (" + std::to_string(g_stats.in_use) + " in use)");
-  uint64_t got = 0;
-  void* p = take_cached(b, s, &got);
-  if (!p) {
-    got = b;
-    hipError_t e = hipMalloc(&p, b);
-    if (e != hipSuccess) {
-      // Hand every cached block back to the device once the streams that
-      // freed them are idle, then try once more.
-      (void)hipGetLastError();
-      (void)hipDeviceSynchronize();
-      for (auto& kv : g_free) (void)hipFree(kv.second.p);
-      g_free.clear();
-      g_cached = 0;
-      g_stats.trims++;
-      p = nullptr;
-      e = hipMalloc(&p, b);
-    }
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      fail("srdatalog pool: hipMalloc(" + std::to_string(b) + ") failed: " + hipGetErrorString(e));
-    }
-    g_stats.reserved += b;
-    g_stats.driver_allocs++;
-  }
-  g_live[p] = got;
-  g_stats.in_use += bytes;
-  g_stats.peak = std::max(g_stats.peak, g_stats.in_use);
-  g_stats.allocs++;
-  return p;
-}
-
-void pool_free(void* p, uint64_t bytes, hipStream_t s) {
-  if (!p) return;
-  std::lock_guard<std::mutex> g(g_pool_mu);
-  auto it = g_live.find(p);
-  if (it == g_live.end()) return;  // not from this pool
-  g_free.emplace(it->second, FreeBlock{p, s, false});
-  g_cached += it->second;
-  g_live.erase(it);
-  g_stats.in_use -= bytes;
-}
-
-void pool_retire_stream(hipStream_t s) {
-  (void)hipStreamSynchronize(s);
-  std::lock_guard<std::mutex> g(g_pool_mu);
-  for (auto& kv : g_free)
-    if (kv.second.s == s) kv.second.any_stream = true;
-}
-
-PoolStats pool_stats() {
-  std::lock_guard<std::mutex> g(g_pool_mu);
-  PoolStats st = g_stats;
-  st.cached = g_cached;
-  return st;
-}
-
-void pool_set_budget(uint64_t bytes) {
-  std::lock_guard<std::mutex> g(g_pool_mu);
-  g_stats.budget = bytes;
-}
-
-void pool_reset_peak() {
-  std::lock_guard<std::mutex> g(g_pool_mu);
-  g_stats.peak = g_stats.in_use;
-}
-
 // ------------------------------------------------- streams, pinned memory --
 // Engines come and go (one per session, several per serving process); their
 // streams and pinned host buffers are kept for the next one instead of being
@@ -237,7 +87,10 @@ void* pinned_acquire(uint64_t bytes, uint64_t* got) {
     }
   }
   void* p = nullptr;
-  SRD_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  {
+    std::lock_guard<std::recursive_mutex> cg(capture_mutex());
+    SRD_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  }
   *got = bytes;
   return p;
 }
@@ -472,6 +325,17 @@ __global__ __launch_bounds__(kBlock) void srd_calib_copy_kernel(const uint4* __r
     dst[i] = src[i];
 }
 
+// The same copy with 8-byte words, the width of the sort's keys and of the
+// merge-path and join loads: its counters check that the scale of the
+// 16-byte copy holds for narrower accesses.
+__global__ __launch_bounds__(kBlock) void srd_calib_copy8_kernel(const uint64_t* __restrict__ src,
+                                                                 uint64_t* __restrict__ dst,
+                                                                 uint64_t n8) {
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n8;
+       i += uint64_t(gridDim.x) * kBlock)
+    dst[i] = src[i];
+}
+
 void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep) {
   const uint64_t n16 = (bytes + 15) / 16;
   DevBuf<uint4> a(n16, s), b(n16, s);
@@ -487,6 +351,11 @@ void calib_copy(uint64_t bytes, int reps, hipStream_t s, double* ms_per_rep) {
   float ms = 0;
   SRD_CHECK(hipEventElapsedTime(&ms, e0, e1));
   *ms_per_rep = reps ? ms / reps : 0;
+  for (int r = 0; r < reps; ++r)
+    srd_calib_copy8_kernel<<<grid, kBlock, 0, s>>>(reinterpret_cast<const uint64_t*>(a.p),
+                                                   reinterpret_cast<uint64_t*>(b.p), 2 * n16);
+  SRD_CHECK(hipGetLastError());
+  SRD_CHECK(hipStreamSynchronize(s));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
 }
@@ -1128,16 +997,23 @@ __device__ __forceinline__ uint64_t wave_lookback(uint64_t* base, uint64_t strid
 #define SRD_OS_ITEMS1 16
 #endif
 constexpr int os_items(int W) { return W == 1 ? SRD_OS_ITEMS1 : 8; }
+// Threads of a onesweep block: kBlock or 2 * kBlock.  The first kRadix
+// threads own one digit each for the tile's histogram and look-back.
+constexpr uint64_t os_tile(int W, int B) { return uint64_t(B) * os_items(W); }
+// Fewest keys of any onesweep tile: sizes look-back tables for every launch.
+constexpr uint64_t kOsMinTile = os_tile(2, kBlock);
 
-template <int W>
-__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+template <int W, int B>
+__global__ __launch_bounds__(B) void onesweep_kernel(
     const uint64_t* __restrict__ in_lo, const uint64_t* __restrict__ in_hi,
     uint64_t* __restrict__ out_lo, uint64_t* __restrict__ out_hi, uint32_t n, uint32_t shift,
     const uint32_t* __restrict__ bin_start, uint64_t* __restrict__ lookback,
     uint32_t* __restrict__ tile_ctr, uint32_t epoch, uint32_t* __restrict__ err, bool ticket,
     const DevPlan* __restrict__ dp, uint32_t pass_no) {
   constexpr int ITEMS = os_items(W);
-  constexpr uint32_t TILE = kBlock * ITEMS;
+  constexpr uint32_t TILE = B * ITEMS;
+  constexpr int WAVES = B / kWave;
+  static_assert(B % kRadix == 0, "every digit needs an owner thread");
   if (dp) {
     // device-planned sort: a skipped pass leaves the keys where they are;
     // an active one reads the buffer the schedule names
@@ -1155,10 +1031,10 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     }
   }
   __shared__ uint64_t s_keys[W][TILE];
-  __shared__ uint32_t s_wcnt[kWavesPerBlock][kRadix];
+  __shared__ uint32_t s_wcnt[WAVES][kRadix];
   __shared__ uint32_t s_local[kRadix];
   __shared__ uint64_t s_global[kRadix];
-  __shared__ uint32_t s_wsum[kWavesPerBlock];
+  __shared__ uint32_t s_wsum[WAVES];
 
   __shared__ uint32_t s_tile;
   const uint32_t tid = threadIdx.x, w = tid / kWave, l = lane_id();
@@ -1169,7 +1045,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   // runs instead (tile_ctr, zeroed by pack_hist_kernel), so a tile only ever
   // waits on tiles that are already running; the ticket's same-address
   // atomic costs ~10 ns per tile, which is why it is not always taken.
-  for (uint32_t i = tid; i < kWavesPerBlock * kRadix; i += kBlock) (&s_wcnt[0][0])[i] = 0;
+  for (uint32_t i = tid; i < WAVES * kRadix; i += B) (&s_wcnt[0][0])[i] = 0;
   if (ticket && tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
   __syncthreads();
   const uint32_t tile = ticket ? s_tile : blockIdx.x;
@@ -1194,18 +1070,23 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
   __syncthreads();
 
-  // Thread d owns digit d: per-wave exclusive offsets and the tile total.
-  const uint32_t d = tid;
+  // Thread d < kRadix owns digit d: per-wave exclusive offsets and the tile
+  // total.
+  const bool owner = B == kRadix || tid < kRadix;
+  const uint32_t d = owner ? tid : 0;
   uint32_t tot = 0;
+  if (owner) {
 #pragma unroll
-  for (int j = 0; j < kWavesPerBlock; ++j) {
-    uint32_t c = s_wcnt[j][d];
-    s_wcnt[j][d] = tot;
-    tot += c;
+    for (int j = 0; j < WAVES; ++j) {
+      uint32_t c = s_wcnt[j][d];
+      s_wcnt[j][d] = tot;
+      tot += c;
+    }
   }
   uint64_t* my_lb = lookback + uint64_t(tile) * kRadix + d;
   uint64_t excl = 0;
-  if (tile == 0) {
+  if (!owner) {
+  } else if (tile == 0) {
     lb_store(my_lb, lb_pack(epoch, kLbPrefix, tot));
   } else {
     lb_store(my_lb, lb_pack(epoch, kLbAgg, tot));
@@ -1246,9 +1127,11 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
   // Local start of digit d inside the LDS-sorted tile.
   uint32_t wtotal;
-  uint32_t lstart = block_excl_scan<uint32_t>(tot, s_wsum, &wtotal);
-  s_local[d] = lstart;
-  s_global[d] = uint64_t(bin_start[d]) + excl - lstart;
+  uint32_t lstart = block_excl_scan<uint32_t, WAVES>(tot, s_wsum, &wtotal);
+  if (owner) {
+    s_local[d] = lstart;
+    s_global[d] = uint64_t(bin_start[d]) + excl - lstart;
+  }
   __syncthreads();
 
 #pragma unroll
@@ -1263,7 +1146,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    uint32_t p = i * kBlock + tid;
+    uint32_t p = i * B + tid;
     if (p < count) {
       uint64_t lo = s_keys[0][p];
       uint64_t hi = (W == 2) ? s_keys[W - 1][p] : 0;
@@ -1274,6 +1157,53 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
       }
     }
   }
+}
+
+// Threads per onesweep block for a sort of n keys (SRDATALOG_OS_BLOCK =
+// 256, 512 or 1024 forces one).  Larger blocks sort larger tiles: fewer
+// tiles to chain through the look-back and longer runs of one digit in each
+// tile's scattered write, at the same 16 waves per CU (four 256-thread
+// blocks, two of 512 with 77 KB of LDS each, or one of 1024 with 147 KB).
+// Measured per pass (k = 2): 576 / 494 / 455 us at 64 M keys and 40 / 35 /
+// 34 us at 4 M for 256 / 512 / 1024; below 4 M keys the 1024-thread
+// tiles are too few to fill 256 CUs (30 us against 26 us for 512 at 1 M).
+static int os_block_for(uint64_t n) {
+  static const int forced = [] {
+    const char* e = getenv("SRDATALOG_OS_BLOCK");
+    const int v = e ? atoi(e) : 0;
+    return (v == kBlock || v == 2 * kBlock || v == 4 * kBlock) ? v : 0;
+  }();
+  if (forced) return forced;
+  return n >= (uint64_t(1) << 22) ? 4 * kBlock : 2 * kBlock;
+}
+
+static uint64_t os_tiles(int W, int B, uint64_t n) {
+  const uint64_t t = os_tile(W, B);
+  return (n + t - 1) / t;
+}
+
+// One onesweep pass over `tiles` tiles of B threads (os_tiles(W, B, n)).
+static void onesweep_launch(int W, int B, uint64_t tiles, const uint64_t* ilo, const uint64_t* ihi,
+                            uint64_t* olo, uint64_t* ohi, uint32_t n, uint32_t shift,
+                            const uint32_t* starts, uint64_t* lookback, uint32_t* ctr,
+                            uint32_t epoch, uint32_t* err, bool ticket, const DevPlan* dp,
+                            uint32_t pass_no, hipStream_t s) {
+#define SRD_OS_LAUNCH(WW, BB)                                                                 \
+  onesweep_kernel<WW, BB><<<uint32_t(tiles), BB, 0, s>>>(ilo, ihi, olo, ohi, n, shift, starts, \
+                                                          lookback, ctr, epoch, err, ticket, dp, \
+                                                          pass_no)
+  const int b = B;
+  if (W == 1) {
+    if (b == kBlock) SRD_OS_LAUNCH(1, kBlock);
+    else if (b == 2 * kBlock) SRD_OS_LAUNCH(1, 2 * kBlock);
+    else SRD_OS_LAUNCH(1, 4 * kBlock);
+  } else {
+    if (b == kBlock) SRD_OS_LAUNCH(2, kBlock);
+    else if (b == 2 * kBlock) SRD_OS_LAUNCH(2, 2 * kBlock);
+    else SRD_OS_LAUNCH(2, 4 * kBlock);
+  }
+#undef SRD_OS_LAUNCH
+  SRD_CHECK(hipGetLastError());
 }
 
 __global__ void bin_scan_kernel(const uint32_t* __restrict__ hist, uint32_t passes,
@@ -1527,8 +1457,8 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
       for (int d = 0; d < kRadix; ++d)
         if (hp[p * kRadix + d] == n || already_sorted) trivial[p] = true;
 
-    const uint64_t tile = uint64_t(kBlock) * os_items(W);
-    const uint64_t tiles = (n + tile - 1) / tile;
+    const int os_b = os_block_for(n);
+    const uint64_t tiles = os_tiles(W, os_b, n);
     uint64_t lb_words = tiles * kRadix;
     if (lb_words > ws.lookback.n) {
       ws.lookback.alloc(lb_words + lb_words / 4, s);
@@ -1546,17 +1476,9 @@ void build_index(const uint32_t* const* in, uint32_t k, uint64_t n, Index& out, 
       const uint64_t* ihi = W == 2 ? ws.keys[cur][1].p : nullptr;
       uint64_t* olo = ws.keys[cur ^ 1][0].p;
       uint64_t* ohi = W == 2 ? ws.keys[cur ^ 1][1].p : nullptr;
-      if (W == 1)
-        onesweep_kernel<1><<<uint32_t(tiles), kBlock, 0, s>>>(ilo, ihi, olo, ohi, uint32_t(n),
-                                                               p * kRadixBits, starts + p * kRadix,
-                                                               ws.lookback.p, ctr + p, ws.epoch, err,
-                                                               ws.ticket_order, nullptr, 0);
-      else
-        onesweep_kernel<2><<<uint32_t(tiles), kBlock, 0, s>>>(ilo, ihi, olo, ohi, uint32_t(n),
-                                                               p * kRadixBits, starts + p * kRadix,
-                                                               ws.lookback.p, ctr + p, ws.epoch, err,
-                                                               ws.ticket_order, nullptr, 0);
-      SRD_CHECK(hipGetLastError());
+      onesweep_launch(W, os_b, tiles, ilo, ihi, olo, ohi, uint32_t(n), p * kRadixBits,
+                      starts + p * kRadix, ws.lookback.p, ctr + p, ws.epoch, err,
+                      ws.ticket_order, nullptr, 0, s);
       cur ^= 1;
     }
   }
@@ -1647,7 +1569,7 @@ struct DevSortScratch {
   uint32_t* tickets;        // kMaxPasses onesweep tile tickets
   uint32_t* epoch_ctr;      // last look-back epoch handed out (device-resident, so
                             // replays of a captured sort never reuse one)
-  uint64_t* lookback;       // os_tiles * kRadix
+  uint64_t* lookback;       // lb_tiles * kRadix
   uint64_t lookback_words;
   uint32_t* tile_u;         // dedup tiles
   uint32_t* tile_r;
@@ -1688,14 +1610,13 @@ static void scan_noalloc(const T* in, uint64_t* out, uint64_t n, uint64_t* tmp, 
 // out), so one scratch serves every key width without moving its look-back
 // words: a look-back word is only ever written as one.
 uint64_t devsort_scratch_bytes(uint64_t cap) {
-  const uint64_t os_tiles = (cap + uint64_t(kBlock) * os_items(2) - 1) / (uint64_t(kBlock) * os_items(2));
-  const uint64_t os_tiles1 = (cap + uint64_t(kBlock) * os_items(1) - 1) / (uint64_t(kBlock) * os_items(1));
+  const uint64_t lb_tiles = (cap + kOsMinTile - 1) / kOsMinTile;
   const uint64_t dt = (cap + kTile - 1) / kTile;
   uint64_t b = align256(sizeof(DevPlan));
   b += 4 * align256(cap * 8);
   b += 2 * align256(uint64_t(kMaxPasses) * kRadix * 4);
   b += align256((kMaxCols + 1) * 4) + align256(kMaxPasses * 4) + align256(4);
-  b += align256(std::max(os_tiles, os_tiles1) * kRadix * 8);
+  b += align256(lb_tiles * kRadix * 8);
   b += 2 * align256(dt * 4) + 2 * align256((dt + 1) * 8);
   b += align256(std::max<uint64_t>(scan_noalloc_words(dt), 1) * 8);
   return b;
@@ -1708,8 +1629,7 @@ static DevSortScratch carve_devsort(void* base, uint64_t cap) {
     p += align256(bytes);
     return q;
   };
-  const uint64_t os_tiles = (cap + uint64_t(kBlock) * os_items(2) - 1) / (uint64_t(kBlock) * os_items(2));
-  const uint64_t os_tiles1 = (cap + uint64_t(kBlock) * os_items(1) - 1) / (uint64_t(kBlock) * os_items(1));
+  const uint64_t lb_tiles = (cap + kOsMinTile - 1) / kOsMinTile;
   const uint64_t dt = (cap + kTile - 1) / kTile;
   DevSortScratch x{};
   x.plan = reinterpret_cast<DevPlan*>(take(sizeof(DevPlan)));
@@ -1721,7 +1641,7 @@ static DevSortScratch carve_devsort(void* base, uint64_t cap) {
   x.colmax = reinterpret_cast<uint32_t*>(take((kMaxCols + 1) * 4));
   x.tickets = reinterpret_cast<uint32_t*>(take(kMaxPasses * 4));
   x.epoch_ctr = reinterpret_cast<uint32_t*>(take(4));
-  x.lookback_words = std::max(os_tiles, os_tiles1) * kRadix;
+  x.lookback_words = lb_tiles * kRadix;
   x.lookback = reinterpret_cast<uint64_t*>(take(x.lookback_words * 8));
   x.tile_u = reinterpret_cast<uint32_t*>(take(dt * 4));
   x.tile_r = reinterpret_cast<uint32_t*>(take(dt * 4));
@@ -2048,19 +1968,14 @@ void build_index_dev(const uint32_t* const* in, uint32_t k, uint64_t cap, const 
   // every pass the widest layout could need is launched; the plan skips
   // the ones the data does not
   const uint32_t max_passes = (32u * k + kRadixBits - 1) / kRadixBits;
-  const uint64_t tile = uint64_t(kBlock) * os_items(int(w));
-  const uint64_t tiles = (cap + tile - 1) / tile;
+  const int os_b = os_block_for(cap);
+  const uint64_t tiles = os_tiles(int(w), os_b, cap);
   for (uint32_t p = 0; p < std::min<uint32_t>(max_passes, uint32_t(w) * 64 / kRadixBits); ++p) {
     TimeScope t(tname("sort.onesweep"), s);
-    if (w == 1)
-      onesweep_kernel<1><<<uint32_t(tiles), kBlock, 0, s>>>(
-          x.keys[0][0], nullptr, x.keys[1][0], nullptr, uint32_t(cap), p * kRadixBits,
-          x.starts + p * kRadix, x.lookback, x.tickets + p, 0u, &x.plan->err, ticket, x.plan, p);
-    else
-      onesweep_kernel<2><<<uint32_t(tiles), kBlock, 0, s>>>(
-          x.keys[0][0], x.keys[0][1], x.keys[1][0], x.keys[1][1], uint32_t(cap), p * kRadixBits,
-          x.starts + p * kRadix, x.lookback, x.tickets + p, 0u, &x.plan->err, ticket, x.plan, p);
-    SRD_CHECK(hipGetLastError());
+    onesweep_launch(int(w), os_b, tiles, x.keys[0][0], w == 2 ? x.keys[0][1] : nullptr, x.keys[1][0],
+                    w == 2 ? x.keys[1][1] : nullptr, uint32_t(cap), p * kRadixBits,
+                    x.starts + p * kRadix, x.lookback, x.tickets + p, 0u, &x.plan->err, ticket,
+                    x.plan, p, s);
   }
   const uint64_t dt = (cap + kTile - 1) / kTile;
   {

@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -42,7 +43,7 @@ struct PoolStats {
   uint64_t driver_allocs = 0;  // of which went to hipMalloc
   uint64_t reserved = 0;       // bytes obtained from hipMalloc so far
   uint64_t cached = 0;         // bytes held on the free list
-  uint64_t trims = 0;          // times the free list was handed back
+  uint64_t trims = 0;          // times cached memory was handed back to the driver
 };
 
 void pool_init(int device);
@@ -60,6 +61,14 @@ hipStream_t stream_acquire();
 void stream_release(hipStream_t s);  // retires s in the pool first
 void* pinned_acquire(uint64_t bytes, uint64_t* got);
 void pinned_release(void* p, uint64_t bytes);
+
+// Held while a stream capture is open (HIP graph capture of a device-
+// resident iteration) and around the calls that would invalidate another
+// thread's open capture: hipDeviceSynchronize / hipFree when the pool hands
+// memory back, hipHostMalloc, hipMemGetInfo.  Engines of one process run on
+// several host threads (serving), so one thread's capture must not overlap
+// those calls in another.
+std::recursive_mutex& capture_mutex();
 
 // Runs in flight: run_enter() returns true when no other run is in flight.
 // Only such a run may take inter-tile look-back order from the block index;

@@ -228,6 +228,31 @@ def run_batched(kind: str, instances: list[Instance], grain: int = 0) -> list[Re
           for i in range(len(instances))]
 
 
+def run_batched_kinds(groups: dict[str, list[Instance]], grains: dict[str, int] | None = None
+                      ) -> dict[str, list[Result]]:
+  """run_batched for every program kind at once, one host thread and one
+  engine (with its own stream) per kind, so that one kind's export over
+  PCIe overlaps the other's kernels.  The foreign calls release the GIL."""
+  import threading
+  out: dict[str, list[Result]] = {}
+  errs: list[BaseException] = []
+
+  def one(kind: str, lst: list[Instance]) -> None:
+    try:
+      out[kind] = run_batched(kind, lst, grain=(grains or {}).get(kind, 0))
+    except BaseException as ex:  # re-raised below
+      errs.append(ex)
+
+  ths = [threading.Thread(target=one, args=(k, v)) for k, v in groups.items() if v]
+  for th in ths:
+    th.start()
+  for th in ths:
+    th.join()
+  if errs:
+    raise errs[0]
+  return {k: out.get(k, []) for k in groups}
+
+
 def autotune_grain(kind: str, instances: list[Instance],
                    candidates=(256, 1024, 4096, 16384)) -> tuple[int, dict]:
   """Pick the join work-slice size with the lowest batched wall time."""

@@ -23,7 +23,8 @@ LIB_DIR = os.path.join(PKG_DIR, "_lib")
 RUNTIME_LIB = os.path.join(LIB_DIR, "libsrdatalog_hip.so")
 
 OFFLOAD_ARCH = "gfx950"
-RUNTIME_SOURCES = ["srd_prims.hip", "srd_engine.hip", "srd_prims_capi.cpp", "srd_noalloc.hip"]
+RUNTIME_SOURCES = ["srd_prims.hip", "srd_engine.hip", "srd_prims_capi.cpp", "srd_noalloc.hip",
+                   "srd_pool.cpp"]
 
 
 def hipcc() -> str:

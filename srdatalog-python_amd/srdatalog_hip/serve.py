@@ -100,6 +100,7 @@ class Request:
   est: float = 0.0
   sub: np.ndarray | None = None     # extracted subgraph (tri, tc)
   result: object = None
+  result_dropped: bool = False      # delivered, then not retained (keep budget)
   t_done: float = -1.0
   rejected: str = ""
   prepared: bool = False
@@ -253,8 +254,18 @@ class Server:
   the call of run())."""
 
   def __init__(self, graphs: list[Graph], budget_bytes: float = 200e9, cap_tuples: float = 5e7,
-               max_batch: int = 256, batch_tuples: float = 2e8):
+               max_batch: int = 256, batch_tuples: float = 2e8, progress=None,
+               keep_bytes: float = 48e9):
     self.graphs = graphs
+    # progress(kind, seconds since the stream started, requests completed
+    # so far): called by a worker after each batch it completes
+    self.progress = progress
+    self.completed = 0
+    # Host bytes of results retained on the requests after delivery; a
+    # batch whose results would exceed it is delivered and not retained
+    # (its requests keep result None and result_dropped True).
+    self.keep_bytes = keep_bytes
+    self.kept_bytes = 0
     self.budget = budget_bytes
     self.cap = cap_tuples
     self.max_batch = max_batch
@@ -298,12 +309,14 @@ class Server:
         pos += 1
     return take, pos
 
-  def _worker(self, kind: str, queue: list[Request], t0: float) -> None:
+  def _worker(self, kind: str, queue: list[Request], t0: float, wall_cap: float | None) -> None:
     pos = 0
     runner = self.runners[kind]
     try:
       while pos < len(queue):
         now = time.perf_counter() - t0
+        if wall_cap is not None and now > wall_cap:
+          break                      # the rest stays unserved (t_done < 0)
         if queue[pos].t > now:
           time.sleep(min(queue[pos].t - now, 0.002))
           continue
@@ -317,19 +330,32 @@ class Server:
           continue
         runner.run(take)
         done = time.perf_counter() - t0
+        nbytes = sum(r.result.nbytes for r in take if isinstance(r.result, np.ndarray))
         with self.lock:
+          keep = self.kept_bytes + nbytes <= self.keep_bytes
+          if keep:
+            self.kept_bytes += nbytes
           for r in take:
             r.t_done = done
             self.in_flight -= r.est * BYTES_PER_TUPLE
+            if not keep and isinstance(r.result, np.ndarray):
+              r.result = None
+              r.result_dropped = True
           self.batches[kind] += 1
+          self.completed += len(take)
+          completed = self.completed
+        if self.progress is not None:
+          self.progress(kind, done, completed)
     except Exception as ex:  # surfaced by run()
       self.errors.append(f"{kind}: {ex!r}")
 
-  def run(self, reqs: list[Request]) -> float:
-    """Serve the stream; returns the wall seconds until the last completion."""
+  def run(self, reqs: list[Request], wall_cap: float | None = None) -> float:
+    """Serve the stream; returns the wall seconds until the last completion.
+    With wall_cap, no batch starts after wall_cap seconds: requests not
+    started by then stay unserved (t_done < 0)."""
     queues = {k: [r for r in reqs if r.kind == k] for k in KIND_NAMES}
     t0 = time.perf_counter()
-    ths = [threading.Thread(target=self._worker, args=(k, q, t0), daemon=True)
+    ths = [threading.Thread(target=self._worker, args=(k, q, t0, wall_cap), daemon=True)
            for k, q in queues.items()]
     for th in ths:
       th.start()
@@ -341,9 +367,11 @@ class Server:
 
 
 # ------------------------------------------------------------ plain serial --
-def run_plain(reqs: list[Request], graphs: list[Graph], wall_cap: float | None = None) -> float:
+def run_plain(reqs: list[Request], graphs: list[Graph], wall_cap: float | None = None,
+              on_result=None) -> float:
   """Serve the stream one request at a time, one shim session each; stops
-  starting new requests after wall_cap seconds.  Returns the wall seconds."""
+  starting new requests after wall_cap seconds.  Returns the wall seconds.
+  on_result(r), if given, is called with each request as it completes."""
   t0 = time.perf_counter()
   for r in reqs:
     now = time.perf_counter() - t0
@@ -378,6 +406,8 @@ def run_plain(reqs: list[Request], graphs: list[Graph], wall_cap: float | None =
     finally:
       lib.shutdown()
     r.t_done = time.perf_counter() - t0
+    if on_result is not None:
+      on_result(r)
   return time.perf_counter() - t0
 
 

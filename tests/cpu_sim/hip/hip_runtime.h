@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 
 #define __device__
 #define __host__
@@ -29,6 +30,51 @@ inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)1; return 0; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+// Everything runs in program order on the host: events and waits are no-ops.
+enum { hipEventDisableTiming = 2 };
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { *e = (void*)1; return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+// Simulated device memory (the pool's host-only tests): hipMalloc fails
+// once `capacity` bytes (0 = no limit) are handed out.  Memory is host
+// malloc'd and never touched by the pool.
+struct SimDevice {
+  size_t capacity = 0, used = 0;
+  long mallocs = 0, frees = 0, syncs = 0;
+  std::map<void*, size_t> blocks;
+};
+// Defined by the pool test build (pool_sim.cpp): one instance per loaded
+// copy of that library.  Other host-only builds never allocate through it.
+SimDevice& sim_device();
+enum { hipErrorOutOfMemory = 2 };
+inline hipError_t hipMalloc(void** p, size_t n) {
+  SimDevice& d = sim_device();
+  if (d.capacity && d.used + n > d.capacity) return hipErrorOutOfMemory;
+  *p = std::malloc(n ? n : 1);
+  d.used += n;
+  d.blocks[*p] = n;
+  d.mallocs++;
+  return hipSuccess;
+}
+inline hipError_t hipFree(void* p) {
+  SimDevice& d = sim_device();
+  auto it = d.blocks.find(p);
+  if (it == d.blocks.end()) std::abort();  // not a block hipMalloc handed out
+  d.used -= it->second;
+  d.blocks.erase(it);
+  d.frees++;
+  std::free(p);
+  return hipSuccess;
+}
+inline hipError_t hipDeviceSynchronize() {
+  sim_device().syncs++;
+  return hipSuccess;
+}
+inline hipError_t hipMemGetInfo(size_t* fr, size_t* tot) {
+  *fr = *tot = size_t(1) << 34;
+  return hipSuccess;
+}
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
   if (n) std::memmove(d, s, n);
   return hipSuccess;

@@ -37,6 +37,10 @@ void* pinned_acquire(uint64_t bytes, uint64_t* got) {
   return std::malloc(bytes ? bytes : 1);
 }
 void pinned_release(void* p, uint64_t) { std::free(p); }
+std::recursive_mutex& capture_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
 static int g_running = 0;
 bool run_enter() { return g_running++ == 0; }
 void run_exit() { --g_running; }

@@ -8,7 +8,9 @@ enqueues the device-sized iteration without capturing it,
 SRDATALOG_DEVLOOP_CAPS=N starts every capacity at N (tiny values make the
 loop abandon and re-run iterations while it grows its buffers), and
 SRDATALOG_DEVLOOP_BATCH sets how many iterations are launched between two
-reads of the control block."""
+reads of the control block, and SRDATALOG_DEVLOOP_MAX_BYTES and
+SRDATALOG_DEVLOOP_MAX_ROWS bound the loop's buffers and its FULL / NEW rows
+(beyond them, the loop hands its state to the host loop)."""
 
 import numpy as np
 import pytest
@@ -19,7 +21,8 @@ from srdatalog_hip import datasets as ds
 pytestmark = pytest.mark.gpu
 
 ENV_KEYS = ("SRDATALOG_DEVLOOP", "SRDATALOG_DEVLOOP_GRAPH", "SRDATALOG_DEVLOOP_CAPS",
-            "SRDATALOG_DEVLOOP_BATCH", "SRDATALOG_JOIN_MODE", "SRDATALOG_GRAIN")
+            "SRDATALOG_DEVLOOP_BATCH", "SRDATALOG_DEVLOOP_MAX_BYTES", "SRDATALOG_DEVLOOP_MAX_ROWS",
+            "SRDATALOG_JOIN_MODE", "SRDATALOG_GRAIN")
 
 VARIANTS = [({}, "default"),
             ({"SRDATALOG_DEVLOOP_GRAPH": "0"}, "eager"),
@@ -28,6 +31,10 @@ VARIANTS = [({}, "default"),
             ({"SRDATALOG_DEVLOOP_BATCH": "1"}, "batch-1"),
             ({"SRDATALOG_DEVLOOP_BATCH": "7", "SRDATALOG_GRAIN": "64"}, "batch-7-grain-64"),
             ({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
+            ({"SRDATALOG_DEVLOOP_CAPS": "64", "SRDATALOG_DEVLOOP_MAX_BYTES": "3000000"},
+             "budget-handback"),
+            ({"SRDATALOG_DEVLOOP_CAPS": "64", "SRDATALOG_DEVLOOP_MAX_ROWS": "40000"},
+             "rows-handback"),
             ({"SRDATALOG_DEVLOOP": "0"}, "host-loop")]
 
 

@@ -6,13 +6,13 @@ oracle without a GPU."""
 
 import ctypes
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
 import oracle
-from srdatalog_hip import codegen, compiler, programs
+from conftest import build_cpu_sim
+from srdatalog_hip import codegen, compiler, mir, programs
 from srdatalog_hip import datasets as ds
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_sim")
@@ -22,8 +22,7 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 @pytest.fixture(scope="module")
 def eng():
-  subprocess.run(["make", "-s", "-C", HERE], check=True)   # rebuilds when a source changed
-  lib = ctypes.CDLL(LIB)
+  lib = ctypes.CDLL(build_cpu_sim("libengine_cpu.so"))
   lib.cpu_tc_engine.restype = ctypes.c_void_p
   lib.cpu_tc_engine.argtypes = [ctypes.c_char_p]
   lib.srd_engine_ok.argtypes = [ctypes.c_void_p]
@@ -47,7 +46,8 @@ def host_loop_by_default(monkeypatch):
   device-resident one; the host-only build never captures graphs."""
   monkeypatch.setenv("SRDATALOG_DEVLOOP", "0")
   monkeypatch.setenv("SRDATALOG_DEVLOOP_GRAPH", "0")
-  for k in ("SRDATALOG_DEVLOOP_CAPS", "SRDATALOG_DEVLOOP_BATCH"):
+  for k in ("SRDATALOG_DEVLOOP_CAPS", "SRDATALOG_DEVLOOP_BATCH", "SRDATALOG_DEVLOOP_MAX_BYTES",
+            "SRDATALOG_DEVLOOP_MAX_ROWS"):
     monkeypatch.delenv(k, raising=False)
 
 
@@ -218,3 +218,87 @@ def test_devloop_tc_empty_and_long_tail(eng, monkeypatch):
   path, deltas, size = run_tc(eng, e)
   np.testing.assert_array_equal(path, want.relations["Path"])
   assert deltas == want.deltas["Path"]
+
+
+# ------------------------------------------------------ parallel groups ----
+def tc_group_text(copies: int) -> str:
+  """The TC plan with its recursive rule run `copies` times as the members
+  of one parallel group: every member writes the same rows into Path's NEW,
+  which deduplicates them, so the fixpoint and deltas are TC's."""
+  prog = programs.tc_program()
+  mp = compiler.compile_to_mir(prog)
+  for st in mp.steps:
+    if st.recursive:
+      ops = st.body.ops
+      ops[0] = mir.ParallelGroup([ops[0]] * copies)
+  return codegen.program_text(compiler.relation_decls(prog), mp)
+
+
+def run_tc_text(lib, text, edges):
+  e = lib.cpu_tc_engine(text.encode())
+  assert lib.srd_engine_ok(e) == 1
+  lib.srd_engine_set_quiet(e, 1)
+  edges = np.ascontiguousarray(edges, dtype=np.uint32)
+  lib.srd_engine_load_rows(e, b"ArcInput", edges.ctypes.data_as(_u32p), edges.shape[0], 2)
+  assert lib.srd_engine_run(e, 0) == 0
+  n = ctypes.c_uint64()
+  lib.srd_engine_export(e, b"Path", None, 0, ctypes.byref(n))
+  out = np.empty((max(n.value, 1), 2), np.uint32)
+  lib.srd_engine_export(e, b"Path", out.ctypes.data_as(_u32p), n.value, ctypes.byref(n))
+  cnt = lib.srd_engine_delta_sizes(e, b"Path", None, 0)
+  buf = (ctypes.c_ulonglong * max(cnt, 1))()
+  lib.srd_engine_delta_sizes(e, b"Path", buf, cnt)
+  lib.srd_engine_destroy(e)
+  return out[:n.value], [int(buf[i]) for i in range(cnt)]
+
+
+@pytest.mark.parametrize("copies", [2, 3])
+@pytest.mark.parametrize("env", [{}, {"SRDATALOG_GROUP_STREAMS": "0"},
+                                 {"SRDATALOG_JOIN_MODE": "twopass"},
+                                 {"SRDATALOG_JOIN_MODE": "fused"},
+                                 {"SRDATALOG_JOIN_MODE": "fused", "SRDATALOG_STAGE_CAP": "7"},
+                                 {"SRDATALOG_NEW_CHUNK_ROWS": "13", "SRDATALOG_GRAIN": "4"}],
+                         ids=["streams", "in-order", "twopass", "fused", "fused-overflow", "chunked"])
+def test_parallel_group_members(eng, copies, env, monkeypatch):
+  for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NEW_CHUNK_ROWS",
+            "SRDATALOG_GRAIN", "SRDATALOG_GROUP_STREAMS"):
+    monkeypatch.delenv(k, raising=False)
+  for k, v in env.items():
+    monkeypatch.setenv(k, v)
+  e = ds.tc_edges(6, 200, 500)
+  want = oracle.run("tc", {"ArcInput": e})
+  path, deltas = run_tc_text(eng, tc_group_text(copies), e)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]
+
+
+@pytest.mark.parametrize("budget", [1000, 60000, 250000, 400000])
+def test_devloop_hands_back_to_host_loop(eng, budget, monkeypatch):
+  """A device-resident loop whose buffers would exceed its memory budget
+  hands its state to the host loop (at the start or mid-run, after the
+  iterations it completed); the fixpoint and deltas are unchanged."""
+  monkeypatch.setenv("SRDATALOG_DEVLOOP", "1")
+  monkeypatch.setenv("SRDATALOG_DEVLOOP_CAPS", "16")
+  monkeypatch.setenv("SRDATALOG_DEVLOOP_MAX_BYTES", str(budget))
+  e = ds.tc_edges(2, 300, 700)
+  want = oracle.run("tc", {"ArcInput": e})
+  path, deltas, size = run_tc(eng, e)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]
+  assert size == want.size("Path")
+
+
+@pytest.mark.parametrize("rows", [10, 3000, 20000, 60000])
+def test_devloop_row_limit_hands_back(eng, rows, monkeypatch):
+  """A step whose FULL starts above SRDATALOG_DEVLOOP_MAX_ROWS runs on the
+  host loop; one whose NEW or FULL outgrows it mid-run continues there from
+  the last completed iteration; the fixpoint and deltas are unchanged."""
+  monkeypatch.setenv("SRDATALOG_DEVLOOP", "1")
+  monkeypatch.setenv("SRDATALOG_DEVLOOP_CAPS", "16")
+  monkeypatch.setenv("SRDATALOG_DEVLOOP_MAX_ROWS", str(rows))
+  e = ds.tc_edges(2, 300, 700)
+  want = oracle.run("tc", {"ArcInput": e})
+  path, deltas, size = run_tc(eng, e)
+  np.testing.assert_array_equal(path, want.relations["Path"])
+  assert deltas == want.deltas["Path"]
+  assert size == want.size("Path")

@@ -214,7 +214,10 @@ def test_csv_load_all(progs, tmp_path):
 # values instead of through the dense root map.  Every variant but the
 # default ones runs recursive steps in the host-driven loop
 # (SRDATALOG_DEVLOOP=0); devloop-small-caps runs them device-resident from
-# capacities of 500 rows, growing as it goes.
+# capacities of 500 rows, growing as it goes.  The members of a parallel
+# group (Andersen's five rules) run concurrently on their own streams;
+# group-in-order runs them one after the other, group-chunked-overflow
+# sends some members down the sequential chunked and overflow paths.
 MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_JOIN_MODE": "fused"}, "fused"),
          ({"SRDATALOG_JOIN_MODE": "fused", "SRDATALOG_STAGE_CAP": "3000"}, "fused-overflow"),
@@ -231,6 +234,9 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_BITMAP_BYTES": "20000"}, "few-bitmaps"),
          ({"SRDATALOG_ROOT_POS": "0"}, "root-search"),
          ({"SRDATALOG_DEVLOOP": "1", "SRDATALOG_DEVLOOP_CAPS": "500"}, "devloop-small-caps"),
+         ({"SRDATALOG_GROUP_STREAMS": "0"}, "group-in-order"),
+         ({"SRDATALOG_NEW_CHUNK_ROWS": "3000", "SRDATALOG_JOIN_MODE": "fused",
+           "SRDATALOG_STAGE_CAP": "5000"}, "group-chunked-overflow"),
          ({}, "default")]
 
 
@@ -239,7 +245,8 @@ def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
   for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NO_SEGMENTS",
             "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN", "SRDATALOG_WEIGH_MIN",
             "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH", "SRDATALOG_BITMAP_BYTES",
-            "SRDATALOG_ROOT_POS", "SRDATALOG_DEVLOOP", "SRDATALOG_DEVLOOP_CAPS"):
+            "SRDATALOG_ROOT_POS", "SRDATALOG_DEVLOOP", "SRDATALOG_DEVLOOP_CAPS",
+            "SRDATALOG_GROUP_STREAMS"):
     monkeypatch.delenv(k, raising=False)
   # the join modes of recursive rules are those of the host-driven loop (the
   # device-resident one has its own variants)

@@ -8,10 +8,11 @@ the range, long runs of duplicates)."""
 import bisect
 import ctypes
 import os
-import subprocess
 
 import numpy as np
 import pytest
+
+from conftest import build_cpu_sim
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_sim")
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -19,8 +20,7 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 @pytest.fixture(scope="module")
 def lib():
-  subprocess.run(["make", "-s", "-C", HERE, "libsearch_host.so"], check=True)
-  lib = ctypes.CDLL(os.path.join(HERE, "libsearch_host.so"))
+  lib = ctypes.CDLL(build_cpu_sim("libsearch_host.so"))
   for f in (lib.host_bound_la, lib.host_bound):
     f.restype = ctypes.c_uint32
     f.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
