@@ -360,6 +360,29 @@ def bench_fixpoint(args) -> dict:
       deltas = lib.delta_sizes(rel)
     lib.shutdown()
   sec = float(np.median(times))
+  # where the time goes: one more run with every launch timed (HIP events)
+  from srdatalog_hip import native
+  if lib.init() != 0:
+    raise RuntimeError("srdatalog_init failed")
+  for r, rows in inputs.items():
+    lib.load_rows(r, rows)
+  lib.prepare()
+  native.timing_reset()
+  native.timing_enable(True)
+  t = time.perf_counter()
+  lib.run(0)
+  wall = time.perf_counter() - t
+  native.timing_enable(False)
+  ks = native.timing_report()
+  native.timing_reset()
+  lib.shutdown()
+  kern = {"wall_s": round(wall, 4), "kernel_ms": round(sum(k["ms"] for k in ks), 2),
+          "launches": int(sum(k["launches"] for k in ks)),
+          "top": [{"name": k["name"], "ms": round(k["ms"], 2), "launches": k["launches"]}
+                  for k in sorted(ks, key=lambda z: -z["ms"])[:8]]}
+  log(f"{prog}: timed run wall {wall:.3f}s, kernels {kern['kernel_ms']:.1f} ms in "
+      f"{kern['launches']} launches (device-resident iterations replayed from graphs are "
+      "not timed per launch)")
   parity = None
   if not args.no_parity and (args.workload == "tc" or args.andersen_div >= 30):
     import oracle
@@ -370,6 +393,7 @@ def bench_fixpoint(args) -> dict:
           "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3), "n_gpus": 1,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
           "data": "synthetic (seeded PCG64), generated in-process",
+          "kernels": kern,
           "config": {"workload": label, "tuples": int(got.shape[0]), "iterations": len(deltas),
                      "deltas": deltas, "parity_vs_oracle": parity}}
 

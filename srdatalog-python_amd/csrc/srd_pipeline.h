@@ -140,6 +140,16 @@ struct SrdDyn {
 #define SRD_CTR_STRIDE 32u     // one 128-byte line per counter: atomics on one line serialise
 
 #define SRD_ITEM_COST 4u       // level-1 cost of an item, in level-2 candidates
+// Two-source cartesian products at a nested last level with at least this
+// many tuples, and a source of at least SRD_CART_TILE_DIM rows, are walked in
+// register tiles (one 64-row tile per lane register, the other broadcast a
+// row at a time) instead of the flattened per-tuple decode.
+#ifndef SRD_CART_TILE_MIN
+#define SRD_CART_TILE_MIN 1024u
+#endif
+#ifndef SRD_CART_TILE_DIM
+#define SRD_CART_TILE_DIM 48u
+#endif
 #define SRD_BM_MIN 512u        // default shortest range given a heavy-key bitmap
 
 #define SRD_SEG 2048u          // staging segment size (tuples)

@@ -28,6 +28,8 @@ expansion, filters and insert-into, with a wave64-native work layout.
 
 from __future__ import annotations
 
+import os
+
 import re
 from dataclasses import dataclass, field
 
@@ -75,8 +77,12 @@ class PipelineGen:
   """Emits the kernels of one pipeline."""
 
   def __init__(self, ep: M.ExecutePipeline, arity: dict[str, int], uid: int,
-               coop_search: bool = False, ring: bool = True, dyn: bool = False):
+               coop_search: bool = False, ring: bool = True, dyn: bool = False,
+               cart_tiles: bool = True):
     self.ep = ep
+    # large two-source cartesian products at a nested last level are walked
+    # in register tiles instead of the flattened per-item decode
+    self.cart_tiles = cart_tiles
     # also emit the device-sized launches (pipelines of recursive steps)
     self.dyn = dyn and not ep.count
     # run-narrowed cooperative searches (srdj::seg_lower_bound) instead of one
@@ -826,6 +832,15 @@ class PipelineGen:
       o.append(f"{ind}  if (act) A.item_cost[item] = sz{t} + SRD_ITEM_COST;")
       o.append(f"{ind}  continue;")
       o.append(f"{ind}}}")
+    tiled = (self.cart_tiles and isinstance(lv, M.CartesianJoin) and li == len(self.levels) - 1
+             and len(rngs) == 2 and all(len(vs) == 1 for vs in lv.var_from_source))
+    if tiled:
+      # a large product leaves the flattened walk for the register-tiled one
+      # below (its weigh cost above still counts it)
+      sizes = [self.size_expr(r, True) for r in rngs]
+      o.append(f"{ind}const bool big{t} = {hvar} && sz{t} >= SRD_CART_TILE_MIN && "
+               f"max(uint32_t({sizes[0]}), uint32_t({sizes[1]})) >= SRD_CART_TILE_DIM;")
+      o.append(f"{ind}if (big{t}) sz{t} = 0;")
     o += bm_code
     o.append(f"{ind}const uint64_t inc{t} = srd::wave_incl_scan<uint64_t>(sz{t});")
     o.append(f"{ind}const uint64_t tot{t} = srdj::uniform_from(inc{t}, 63);")
@@ -902,6 +917,74 @@ class PipelineGen:
     self.gen_nested(li + 1, reg2, bound2, done2, o, i3, hn)
     o.append(f"{i2}}}")
     o.append(f"{i2}__builtin_amdgcn_wave_barrier();")
+    o.append(f"{ind}}}")
+    if tiled:
+      self.gen_cart_tiles(lv, li, rngs, per_lane, keep, done, o, ind)
+
+  def gen_cart_tiles(self, lv: M.CartesianJoin, li: int, rngs: list[Rg], per_lane: list, keep: list,
+                     done: set[int], o: list[str], ind: str) -> None:
+    """Register-tiled walk of the large two-source cartesian products of a
+    nested last level (the reference stages 128-row tiles of both sources
+    in shared memory, ws_infrastructure.h:499-604).  The owners, one at a
+    time, wave-uniform: the lanes hold 64 rows of the longer source, a
+    64-row tile of the other is loaded one row per lane and broadcast a row
+    at a time by readlane, so every step emits up to 64 tuples with no
+    per-tuple index arithmetic.  The tuples are the product's, in another
+    order than the flattened walk's; count and write passes run the same
+    code, so their slice counts and offsets agree."""
+    t = f"{li}"
+    i1 = ind + "  "
+    i2 = i1 + "  "
+    o.append(f"{ind}uint64_t bm{t} = srd::ballot(big{t});")
+    o.append(f"{ind}while (bm{t}) {{")
+    o.append(f"{i1}const uint32_t ow{t} = uint32_t(__builtin_ctzll(bm{t}));")
+    o.append(f"{i1}bm{t} &= bm{t} - 1;")
+    # the owner's frame, read by every lane and marked wave-uniform so the
+    # tile loops run on scalar counters
+    rfl = "__builtin_amdgcn_readfirstlane"
+    for fi, v in enumerate(keep):
+      o.append(f"{i1}const uint32_t {self.var(v)} = {rfl}(fr_s{t}[wv][{fi}][ow{t}]);")
+    be = []
+    for j, r in enumerate(rngs):
+      if per_lane[j] is None:
+        be.append((r.b, r.e))
+      else:
+        bi, ei = per_lane[j]
+        o.append(f"{i1}const uint32_t tb{t}_{j} = {rfl}(fr_s{t}[wv][{bi}][ow{t}]);")
+        o.append(f"{i1}const uint32_t te{t}_{j} = {rfl}(fr_s{t}[wv][{ei}][ow{t}]);")
+        be.append((f"tb{t}_{j}", f"te{t}_{j}"))
+    cols = [self.col_expr(r.view, r.depth) for r in rngs]
+    o.append(f"{i1}const uint32_t n{t}_0 = {be[0][1]} - {be[0][0]}, n{t}_1 = {be[1][1]} - {be[1][0]};")
+    o.append(f"{i1}const bool l1{t} = n{t}_1 >= n{t}_0;  // the lanes walk the longer source")
+    o.append(f"{i1}const uint32_t nl{t} = l1{t} ? n{t}_1 : n{t}_0, nr{t} = l1{t} ? n{t}_0 : n{t}_1;")
+    o.append(f"{i1}const uint32_t* cl{t} = l1{t} ? {cols[1]} + {be[1][0]} : {cols[0]} + {be[0][0]};")
+    o.append(f"{i1}const uint32_t* cr{t} = l1{t} ? {cols[0]} + {be[0][0]} : {cols[1]} + {be[1][0]};")
+    o.append(f"{i1}for (uint32_t tl{t} = 0; tl{t} < nl{t}; tl{t} += 64) {{")
+    o.append(f"{i2}const bool inl{t} = tl{t} + lane < nl{t};")
+    o.append(f"{i2}const uint32_t wl{t} = inl{t} ? cl{t}[tl{t} + lane] : 0u;")
+    o.append(f"{i2}for (uint32_t tr{t} = 0; tr{t} < nr{t}; tr{t} += 64) {{")
+    i3 = i2 + "  "
+    o.append(f"{i3}const uint32_t cn{t} = min(64u, nr{t} - tr{t});")
+    o.append(f"{i3}const uint32_t wr{t} = lane < cn{t} ? cr{t}[tr{t} + lane] : 0u;")
+    o.append(f"{i3}for (uint32_t a{t} = 0; a{t} < cn{t}; ++a{t}) {{")
+    i4 = i3 + "  "
+    o.append(f"{i4}const uint32_t vr{t} = __builtin_amdgcn_readlane(wr{t}, a{t});")
+    (va,), (vb,) = lv.var_from_source
+    o.append(f"{i4}const uint32_t {self.var(va)} = l1{t} ? vr{t} : wl{t};")
+    o.append(f"{i4}const uint32_t {self.var(vb)} = l1{t} ? wl{t} : vr{t};")
+    hn = f"ht{t}"
+    o.append(f"{i4}bool {hn} = inl{t};")
+    bound2 = set(keep) | {va, vb}
+    done2 = set(done)
+    for i, f in self.filters_ready(bound2, done2):
+      o.append(f"{i4}{hn} = {hn} && {self.filter_code(f)};")
+      done2.add(i)
+    if len(done2) != len(self.filters):
+      raise CodegenError(f"{self.ep.rule}: filter over unbound variables")
+    o.append(self.emit_code(hn, i4))
+    o.append(f"{i3}}}")
+    o.append(f"{i2}}}")
+    o.append(f"{i1}}}")
     o.append(f"{ind}}}")
 
   # --------------------------------------------------------- assembly ----
@@ -1117,7 +1200,11 @@ def generate_program_source(name: str, decls: list[RelDecl], prog: M.Program) ->
         for ep in (op.ops if isinstance(op, M.ParallelGroup) else [op]):
           if isinstance(ep, M.ExecutePipeline):
             rec.add(id(ep))
-  gens = [PipelineGen(ep, arity, i, dyn=id(ep) in rec) for i, ep in enumerate(prog.pipelines())]
+  # SRDATALOG_CART_TILES=0 (at generation time): every cartesian level takes
+  # the flattened per-tuple walk
+  cart = os.environ.get("SRDATALOG_CART_TILES", "1") != "0"
+  gens = [PipelineGen(ep, arity, i, dyn=id(ep) in rec, cart_tiles=cart)
+          for i, ep in enumerate(prog.pipelines())]
   seen = set()
   for g in gens:
     if g.ep.rule in seen:
