@@ -69,6 +69,25 @@ __device__ __forceinline__ uint64_t read_lane_u64(uint64_t v, int lane) {
   return (uint64_t(hi) << 32) | lo;
 }
 
+// Adds, for every valid lane, 1 to h[d] (d this lane's bin), with one LDS
+// atomic per run of equal bins in consecutive lanes instead of one per lane:
+// keys a join wrote side by side share their high digits, and 64 lanes
+// incrementing one LDS word serialise.  The whole wave must call it, and the
+// valid lanes must be a prefix of the wave (the tail of a range).
+__device__ __forceinline__ void hist_add_runs(uint32_t* h, uint32_t d, bool valid) {
+  const uint32_t l = lane_id();
+  const uint32_t prev = __shfl_up(d, 1, kWave);  // lane 0 reads its own
+  const uint64_t vm = ballot(valid);
+  const bool start = valid && (l == 0 || prev != d);
+  const uint64_t sm = ballot(start);
+  if (start) {
+    // the run ends at the next start or at the first invalid lane
+    const uint64_t ends = (sm | ~vm) & ~((uint64_t(2) << l) - 1);
+    const uint32_t e = ends ? uint32_t(__builtin_ctzll(ends)) : uint32_t(kWave);
+    atomicAdd(&h[d], e - l);
+  }
+}
+
 template <class T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
   const uint32_t l = lane_id();

@@ -895,23 +895,29 @@ __global__ __launch_bounds__(kBlock) void pack_hist_kernel(PackSpec spec, uint64
   for (uint32_t i = threadIdx.x; i < spec.passes * kRadix; i += kBlock) h[i] = 0;
   if (blockIdx.x == 0 && threadIdx.x < kMaxPasses) tile_ctr[threadIdx.x] = 0;
   __syncthreads();
-  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
-       i += uint64_t(gridDim.x) * kBlock) {
+  // wave-uniform steps (hist_add_runs needs the whole wave)
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  for (uint64_t b = blockIdx.x * uint64_t(kBlock) + (threadIdx.x & ~uint32_t(kWave - 1)); b < n;
+       b += stride) {
+    const uint64_t i = b + lane_id();
+    const bool valid = i < n;
     uint64_t lo = 0, hi = 0;
-    for (uint32_t c = 0; c < spec.k; ++c) {
-      uint64_t v = spec.in[c][i];
-      uint32_t s = spec.shift[c];
-      if (s >= 64) {
-        hi |= v << (s - 64);
-      } else {
-        lo |= v << s;
-        if (W == 2 && s > 32) hi |= v >> (64 - s);
+    if (valid) {
+      for (uint32_t c = 0; c < spec.k; ++c) {
+        uint64_t v = spec.in[c][i];
+        uint32_t s = spec.shift[c];
+        if (s >= 64) {
+          hi |= v << (s - 64);
+        } else {
+          lo |= v << s;
+          if (W == 2 && s > 32) hi |= v >> (64 - s);
+        }
       }
+      lo_out[i] = lo;
+      if (W == 2) hi_out[i] = hi;
     }
-    lo_out[i] = lo;
-    if (W == 2) hi_out[i] = hi;
     for (uint32_t p = 0; p < spec.passes; ++p)
-      atomicAdd(&h[p * kRadix + digit_of<W>(lo, hi, p * kRadixBits)], 1u);
+      hist_add_runs(&h[p * kRadix], digit_of<W>(lo, hi, p * kRadixBits), valid);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < spec.passes * kRadix; i += kBlock)
@@ -1760,23 +1766,29 @@ __global__ __launch_bounds__(kBlock) void devsort_pack_kernel(PackSpec spec,
   for (uint32_t c = 0; c < spec.k; ++c) shift[c] = plan->shift[c];
   for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0;
   __syncthreads();
-  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n;
-       i += uint64_t(gridDim.x) * kBlock) {
+  // wave-uniform steps (hist_add_runs needs the whole wave)
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  for (uint64_t b = blockIdx.x * uint64_t(kBlock) + (threadIdx.x & ~uint32_t(kWave - 1)); b < n;
+       b += stride) {
+    const uint64_t i = b + lane_id();
+    const bool valid = i < n;
     uint64_t lo = 0, hi = 0;
-    for (uint32_t c = 0; c < spec.k; ++c) {
-      const uint64_t v = spec.in[c][i];
-      const uint32_t sh = shift[c];
-      if (sh >= 64) {
-        hi |= v << (sh - 64);
-      } else {
-        lo |= v << sh;
-        if (W == 2 && sh > 32) hi |= v >> (64 - sh);
+    if (valid) {
+      for (uint32_t c = 0; c < spec.k; ++c) {
+        const uint64_t v = spec.in[c][i];
+        const uint32_t sh = shift[c];
+        if (sh >= 64) {
+          hi |= v << (sh - 64);
+        } else {
+          lo |= v << sh;
+          if (W == 2 && sh > 32) hi |= v >> (64 - sh);
+        }
       }
+      lo_out[i] = lo;
+      if (W == 2) hi_out[i] = hi;
     }
-    lo_out[i] = lo;
-    if (W == 2) hi_out[i] = hi;
     for (uint32_t p = 0; p < passes; ++p)
-      atomicAdd(&h[p * kRadix + digit_of<W>(lo, hi, p * kRadixBits)], 1u);
+      hist_add_runs(&h[p * kRadix], digit_of<W>(lo, hi, p * kRadixBits), valid);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock)

@@ -106,4 +106,5 @@ template <class T> inline T __shfl_up(T, int, int) { std::abort(); }
 template <class T> inline T __shfl_xor(T, int, int) { std::abort(); }
 template <class T> inline T __shfl(T, int, int) { std::abort(); }
 inline void __syncthreads() { std::abort(); }
+template <class T> inline T atomicAdd(T*, T) { std::abort(); }
 inline unsigned __builtin_amdgcn_readlane(unsigned, int) { std::abort(); }
