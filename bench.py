@@ -394,6 +394,9 @@ def bench_fixpoint(args) -> dict:
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
           "data": "synthetic (seeded PCG64), generated in-process",
           "kernels": kern,
+          # every timed session of the process, in order: later sessions are
+          # not slower than the first
+          "session_s": [round(x, 4) for x in times],
           "config": {"workload": label, "tuples": int(got.shape[0]), "iterations": len(deltas),
                      "deltas": deltas, "parity_vs_oracle": parity}}
 
