@@ -64,6 +64,9 @@ def parse():
                       "an index holds at most 2^32-1 rows, so a request estimated beyond half "
                       "of that is refused; memory admission is the HBM budget)")
   p.add_argument("--serve-budget", type=float, default=200e9)
+  p.add_argument("--serve-tc-iterative", action="store_true",
+                 help="serve: TC requests join the running fixpoint between its iterations "
+                      "instead of waiting for the running batch of their kind")
   p.add_argument("--serve-wall-cap", type=float, default=90.0,
                  help="serve: no batch starts after this many seconds of the served run; "
                       "requests not started by then are reported as unserved")
@@ -265,7 +268,7 @@ def bench_serve(args) -> dict:
     log(f"serve: {completed} requests completed at t={t:.1f}s (last batch: {kind})")
 
   srv = serve.Server(graphs, budget_bytes=args.serve_budget, cap_tuples=args.serve_cap,
-                     progress=progress)
+                     progress=progress, iterative=args.serve_tc_iterative)
   try:
     wall_s = srv.run(reqs, wall_cap=args.serve_wall_cap)
   finally:
@@ -313,6 +316,7 @@ def bench_serve(args) -> dict:
     "rejected_examples": [{"rid": r.rid, "kind": r.kind, "graph_scale": graphs[r.graph].scale,
                            "reason": r.rejected} for r in rejected[:5]],
     "by_kind": kinds, "batches": srv.batches,
+    "tc_admission": "iteration" if args.serve_tc_iterative else "batch",
     "results_compared": compared, "results_identical": mismatched == 0,
     "results_not_retained": dropped, "retained_result_gb": round(srv.kept_bytes / 1e9, 2),
     "served_wall_s": round(wall_s, 3), "plain_wall_s": round(wall_p, 3),

@@ -112,6 +112,47 @@ long long srdatalog_instance_delta_sizes(const char* rel_name, unsigned long lon
 int srdatalog_export_instances(const char* rel_name, uint32_t* out, uint64_t cap_rows,
                                uint64_t* n_rows, uint64_t* cuts);
 
+/* ---- Engine handles and resumable runs (iteration-granular serving) ----
+ * No counterpart in the reference, which runs one fixpoint per process call.
+ * srdatalog_engine_new() creates another engine of this program (NULL on
+ * error; release with srdatalog_engine_free).  For a program whose last step
+ * is its only recursive one:
+ *   run_begin        runs every earlier step once (the recursive step at
+ *                    iteration 0); inputs come from engine_load_rows;
+ *   run_iterations   advances the recursive step by at most k iterations;
+ *                    *done = iterations run, *converged = 1 once an
+ *                    iteration left every DELTA empty;
+ *   absorb(dst, src) unites the state src reached after its own run_begin
+ *                    into dst's FULL (and into DELTA for the recursive
+ *                    step's relations), as if those rows had been in dst's
+ *                    base results; both are engines of this program;
+ *   range_counts     rows of rel's FULL (delta = 0) or DELTA (delta = 1)
+ *                    whose values lie in each of m ranges [lohi[2g],
+ *                    lohi[2g+1]), ascending and disjoint, into out[g];
+ *   export_range     rel's FULL rows with column 0 in [lo, hi), row-major
+ *                    in relation column order (*n_rows set when out is NULL);
+ *   erase_ranges     drops the rows in each of m ranges (as range_counts)
+ *                    from every relation, in one pass over each index.
+ * Instances must occupy disjoint value ranges with every value of an
+ * instance's rows inside its range (as a packed batch's do); an instance's
+ * rows and per-iteration |DELTA| are then those of a run of it alone,
+ * shifted by the iteration it was absorbed at.  0 ok, 1 null, 2 error
+ * (range_counts / export_range: 2 unknown relation, 3 error). */
+void* srdatalog_engine_new(void);
+void srdatalog_engine_free(void* h);
+void srdatalog_engine_set_quiet(void* h, int quiet);
+int srdatalog_engine_load_rows(void* h, const char* rel_name, const uint32_t* rows, uint64_t n,
+                               uint32_t arity);
+int srdatalog_engine_run_begin(void* h);
+int srdatalog_engine_run_iterations(void* h, unsigned long long k, unsigned long long* done,
+                                    int* converged);
+int srdatalog_engine_absorb(void* dst, void* src);
+int srdatalog_engine_range_counts(void* h, const char* rel_name, int delta, const uint32_t* lohi,
+                                  uint32_t m, unsigned long long* out);
+int srdatalog_engine_export_range(void* h, const char* rel_name, uint32_t lo, uint32_t hi,
+                                  uint32_t* out, uint64_t cap_rows, uint64_t* n_rows);
+int srdatalog_engine_erase_ranges(void* h, const uint32_t* lohi, uint32_t m);
+
 #ifdef __cplusplus
 }
 #endif

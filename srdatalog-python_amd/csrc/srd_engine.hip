@@ -257,6 +257,11 @@ struct srd_engine {
   bool quiet = false;
   bool ran = false;
   uint64_t iterations = 0;         // iterations of the last recursive step
+  // resumable run (srd_engine_run_begin): the program's last step, its only
+  // recursive one, advanced a few iterations per call
+  bool resumable = false;
+  size_t res_step = 0;
+  uint64_t res_it = 0;
   double run_seconds = 0;
   std::vector<std::pair<std::string, uint64_t>> counts;  // rule -> count (count-only)
   uint32_t grain_override = 0;
@@ -2374,6 +2379,222 @@ static void run_program(srd_engine& e, uint64_t max_iters) {
       }
 }
 
+// ======================================================= resumable runs ====
+// Iteration-granular serving.  A program whose last step is its only
+// recursive one runs its earlier steps once (run_begin) and then a few
+// iterations per call (run_iters).  Between two calls, absorb() unites into
+// it the state another engine of the same program reached after its own
+// run_begin -- the base-step results of new instances -- and erase_ranges()
+// drops the rows of finished instances.  Instances occupy disjoint value
+// ranges, every column of an instance's rows lies in its range, so in any
+// lexicographic index an instance's rows are contiguous and never join with
+// another instance's: absorbed rows enter FULL and DELTA of the running
+// step as they would enter a fresh run at its iteration 0 (which reads FULL
+// for DELTA), and each instance's rows and per-iteration deltas are those of
+// a run of that instance alone, shifted by the iteration it was absorbed at.
+
+static uint64_t delta_total(srd_engine& e, const Step& st) {
+  uint64_t tot = 0;
+  for (int ri : st.canonical_rels) {
+    Relation& r = e.rels[ri];
+    auto d = r.delta.find(r.canonical);
+    if (d != r.delta.end()) tot += d->second.n;
+  }
+  return tot;
+}
+
+// A step that computes nothing (post-stratum reconstructs, hooks, checks).
+static bool step_is_noop(const Step& st) {
+  for (const Op& op : st.ops)
+    if (op.kind != OpKind::Reconstruct && op.kind != OpKind::Hook && op.kind != OpKind::CheckSize)
+      return false;
+  return true;
+}
+
+static void run_begin(srd_engine& e) {
+  int rs = -1;
+  for (size_t si = 0; si < e.steps.size(); ++si)
+    if (e.steps[si].recursive) {
+      if (rs >= 0) fail("srdatalog_run_begin: the program has more than one recursive step");
+      rs = int(si);
+    }
+  bool last = rs >= 0;
+  for (size_t si = size_t(std::max(rs, 0)) + 1; last && si < e.steps.size(); ++si)
+    last = step_is_noop(e.steps[si]);
+  if (!last)
+    fail("srdatalog_run_begin: the program's only recursive step must be its last computing step");
+  reset_state(e);
+  for (auto& r : e.rels) upload_input(e, r);
+  for (int si = 0; si < rs; ++si) exec_ops(e, e.steps[si].ops, 0, si);
+  SRD_CHECK(hipStreamSynchronize(e.s));
+  e.res_step = size_t(rs);
+  e.res_it = 0;
+  e.resumable = true;
+  e.iterations = 0;
+  e.ran = true;
+}
+
+static uint64_t run_iters(srd_engine& e, uint64_t k, bool* converged) {
+  if (!e.resumable) fail("srdatalog_run_iterations: no run was begun");
+  const Step& st = e.steps[e.res_step];
+  uint64_t done = 0;
+  *converged = false;
+  for (; done < k; ++done) {
+    if (e.res_it > 0 && delta_total(e, st) == 0) {
+      *converged = true;
+      break;
+    }
+    exec_ops(e, st.ops, e.res_it, int(e.res_step));
+    e.res_it += 1;
+  }
+  if (!*converged && e.res_it > 0 && delta_total(e, st) == 0) *converged = true;
+  SRD_CHECK(hipStreamSynchronize(e.s));
+  e.iterations = e.res_it;
+  return done;
+}
+
+static bool in_step(const Step& st, int ri) {
+  return std::find(st.canonical_rels.begin(), st.canonical_rels.end(), ri) != st.canonical_rels.end();
+}
+
+static void absorb(srd_engine& d, srd_engine& s) {
+  if (&d == &s) fail("srdatalog_absorb: an engine cannot absorb itself");
+  if (!d.resumable || !s.resumable || s.res_it != 0)
+    fail("srdatalog_absorb: both engines need a begun run, the absorbed one at iteration 0");
+  if (d.rels.size() != s.rels.size()) fail("srdatalog_absorb: engines of different programs");
+  SRD_CHECK(hipStreamSynchronize(s.s));
+  const Step& st = d.steps[d.res_step];
+  for (size_t i = 0; i < d.rels.size(); ++i) {
+    Relation& rs = s.rels[i];
+    Relation& rd = d.rels[i];
+    if (rs.name != rd.name || rs.arity != rd.arity)
+      fail("srdatalog_absorb: engines of different programs");
+    if (!rs.canonical_set || (rs.full.empty() && !rs.has_input)) continue;
+    const Index& add = full_index(s, rs, rs.canonical);
+    if (add.n == 0) continue;
+    Versioned v;
+    const Index& cur = full_index(d, rd, rd.canonical);
+    if (cur.n == 0) copy_index(add, v.idx, d.s);
+    else merge_disjoint(cur, add, v.idx, d.ws, d.s);
+    rd.full.clear();
+    rd.segs.clear();
+    rd.full_version += 1;
+    rd.delta_seq = rd.full_version;
+    v.version = rd.full_version;
+    rd.full[rd.canonical] = std::move(v);
+    if (in_step(st, int(i))) {
+      auto it = rd.delta.find(rd.canonical);
+      Index nd;
+      if (it == rd.delta.end() || it->second.n == 0) copy_index(add, nd, d.s);
+      else merge_disjoint(it->second, add, nd, d.ws, d.s);
+      rd.delta.clear();
+      rd.delta[rd.canonical] = std::move(nd);
+    }
+  }
+  SRD_CHECK(hipStreamSynchronize(d.s));
+}
+
+// Rows of x (sorted on its first column) with first-column value in each of
+// the m ranges [lohi[2g], lohi[2g+1]) (ascending, disjoint): first row and
+// count of each.
+static std::vector<std::pair<uint64_t, uint64_t>> value_ranges(srd_engine& e, const Index& x,
+                                                               const uint32_t* lohi, uint32_t m) {
+  std::vector<std::pair<uint64_t, uint64_t>> out(m, {0, 0});
+  if (x.n == 0 || m == 0) return out;
+  // bounds 0, lo0, hi0, lo1, ...: slot 0 counts the rows before the first
+  // range, then ranges and gaps alternate
+  std::vector<uint32_t> b(2ull * m + 1);
+  b[0] = 0;
+  for (uint64_t i = 0; i < 2ull * m; ++i) b[i + 1] = lohi[i];
+  DevBuf<uint32_t> db(b.size(), e.s);
+  DevBuf<uint64_t> per(2ull * m, e.s);
+  SRD_CHECK(hipMemcpyAsync(db.p, b.data(), b.size() * 4, hipMemcpyHostToDevice, e.s));
+  instance_counts(x.cols.p, x.n, db.p, 2 * m, per.p, e.s);
+  std::vector<uint64_t> h(2ull * m);
+  SRD_CHECK(hipMemcpyAsync(h.data(), per.p, h.size() * 8, hipMemcpyDeviceToHost, e.s));
+  SRD_CHECK(hipStreamSynchronize(e.s));
+  uint64_t pos = 0;
+  for (uint32_t g = 0; g < m; ++g) {
+    pos += h[2ull * g];
+    out[g] = {pos, h[2ull * g + 1]};
+    pos += h[2ull * g + 1];
+  }
+  return out;
+}
+
+static std::pair<uint64_t, uint64_t> value_range(srd_engine& e, const Index& x, uint32_t lo,
+                                                 uint32_t hi) {
+  if (lo >= hi) return {0, 0};
+  const uint32_t lh[2] = {lo, hi};
+  return value_ranges(e, x, lh, 1)[0];
+}
+
+// x without the row ranges rm (ascending, disjoint; first row, count).
+static void erase_rows(srd_engine& e, const Index& x,
+                       const std::vector<std::pair<uint64_t, uint64_t>>& rm, Index& out) {
+  uint64_t gone = 0;
+  for (auto& [a, c] : rm) gone += c;
+  const uint64_t n2 = x.n - gone;
+  if (n2 == 0) {
+    out = empty_index(x.arity);
+    return;
+  }
+  DevBuf<uint32_t> cols(uint64_t(x.arity) * n2, e.s);
+  std::vector<const uint32_t*> in(x.arity);
+  for (uint32_t c = 0; c < x.arity; ++c) {
+    const uint32_t* src = x.cols.p + uint64_t(c) * x.stride;
+    uint32_t* dst = cols.p + uint64_t(c) * n2;
+    uint64_t from = 0, to = 0;
+    for (size_t i = 0; i <= rm.size(); ++i) {
+      const uint64_t end = i < rm.size() ? rm[i].first : x.n;
+      if (end > from)
+        SRD_CHECK(hipMemcpyAsync(dst + to, src + from, (end - from) * 4, hipMemcpyDeviceToDevice, e.s));
+      to += end - from;
+      if (i < rm.size()) from = rm[i].first + rm[i].second;
+    }
+    in[c] = dst;
+  }
+  build_index(in.data(), x.arity, n2, out, e.ws, e.s, nullptr, &cols, n2, /*presorted=*/true);
+}
+
+static void erase_ranges(srd_engine& e, const uint32_t* lohi, uint32_t m) {
+  if (!e.resumable) fail("srdatalog_erase_ranges: no run was begun");
+  for (uint64_t i = 1; i < 2ull * m; ++i)
+    if (lohi[i] < lohi[i - 1]) fail("srdatalog_erase_ranges: ranges must be ascending and disjoint");
+  if (m == 0) return;
+  auto any = [](const std::vector<std::pair<uint64_t, uint64_t>>& v) {
+    for (auto& p : v)
+      if (p.second) return true;
+    return false;
+  };
+  for (auto& r : e.rels) {
+    if (!r.canonical_set || (r.full.empty() && !r.has_input)) continue;
+    const Index& f = full_index(e, r, r.canonical);
+    auto rm = value_ranges(e, f, lohi, m);
+    if (any(rm)) {
+      Versioned v;
+      erase_rows(e, f, rm, v.idx);
+      r.full.clear();
+      r.segs.clear();
+      r.full_version += 1;
+      r.delta_seq = r.full_version;
+      v.version = r.full_version;
+      r.full[r.canonical] = std::move(v);
+    }
+    auto it = r.delta.find(r.canonical);
+    if (it != r.delta.end()) {
+      auto drm = value_ranges(e, it->second, lohi, m);
+      if (any(drm)) {
+        Index nd;
+        erase_rows(e, it->second, drm, nd);
+        r.delta.clear();
+        r.delta[r.canonical] = std::move(nd);
+      }
+    }
+  }
+  SRD_CHECK(hipStreamSynchronize(e.s));
+}
+
 // =============================================================== CSV ====
 static void load_csv_file(Relation& r, const std::string& path) {
   int fd = open(path.c_str(), O_RDONLY);
@@ -2716,6 +2937,84 @@ int srd_engine_shutdown(srd_engine* e) {
   }
   e->ran = false;
   return 0;
+}
+
+int srd_engine_run_begin(srd_engine* e) {
+  if (!e) return 1;
+  return eguard(e, 2, [&] { run_begin(*e); });
+}
+
+int srd_engine_run_iterations(srd_engine* e, unsigned long long k, unsigned long long* done,
+                              int* converged) {
+  if (!e) return 1;
+  return eguard(e, 2, [&] {
+    bool c = false;
+    const uint64_t d = run_iters(*e, k, &c);
+    if (done) *done = d;
+    if (converged) *converged = c ? 1 : 0;
+  });
+}
+
+int srd_engine_absorb(srd_engine* dst, srd_engine* src) {
+  if (!dst || !src) return 1;
+  return eguard(dst, 2, [&] { absorb(*dst, *src); });
+}
+
+int srd_engine_range_counts(srd_engine* e, const char* rel, int delta, const uint32_t* lohi,
+                            uint32_t m, unsigned long long* out) {
+  if (!e || !rel || (m && (!lohi || !out))) return 1;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return 2;
+  Relation& r = e->rels[it->second];
+  return eguard(e, 3, [&] {
+    for (uint32_t g = 0; g < m; ++g) out[g] = 0;
+    if (m == 0) return;
+    for (uint64_t i = 1; i < 2ull * m; ++i)
+      if (lohi[i] < lohi[i - 1]) fail("srdatalog_range_counts: ranges must be ascending and disjoint");
+    const Index* x = nullptr;
+    if (delta) {
+      auto d = r.delta.find(r.canonical);
+      if (d != r.delta.end()) x = &d->second;
+    } else if (r.canonical_set && (!r.full.empty() || r.has_input)) {
+      x = &full_index(*e, r, r.canonical);
+    }
+    if (!x || x->n == 0) return;
+    // instance_counts over the 2m bounds: even slots are the ranges, odd the gaps
+    DevBuf<uint32_t> b(2ull * m, e->s);
+    DevBuf<uint64_t> per(2ull * m - 1, e->s);
+    SRD_CHECK(hipMemcpyAsync(b.p, lohi, 8ull * m, hipMemcpyHostToDevice, e->s));
+    instance_counts(x->cols.p, x->n, b.p, 2 * m - 1, per.p, e->s);
+    std::vector<uint64_t> h(2ull * m - 1);
+    SRD_CHECK(hipMemcpyAsync(h.data(), per.p, h.size() * 8, hipMemcpyDeviceToHost, e->s));
+    SRD_CHECK(hipStreamSynchronize(e->s));
+    for (uint32_t g = 0; g < m; ++g) out[g] = h[2ull * g];
+  });
+}
+
+int srd_engine_export_range(srd_engine* e, const char* rel, uint32_t lo, uint32_t hi,
+                            uint32_t* out, uint64_t cap_rows, uint64_t* n_rows) {
+  if (!e || !rel || !n_rows) return 1;
+  auto it = e->rel_by_name.find(rel);
+  if (it == e->rel_by_name.end()) return 2;
+  Relation& r = e->rels[it->second];
+  return eguard(e, 3, [&] {
+    *n_rows = 0;
+    const Index* x = export_index(*e, r);
+    if (!x) return;
+    auto [a, cnt] = value_range(*e, *x, lo, hi);
+    *n_rows = cnt;
+    if (!out || cnt == 0) return;
+    if (cap_rows < cnt) fail("srdatalog_export_range: buffer too small");
+    Index v = *x;
+    v.cols.p += a;
+    v.n = cnt;
+    export_rows(v, out, nullptr, 0, e->ws, e->s);
+  });
+}
+
+int srd_engine_erase_ranges(srd_engine* e, const uint32_t* lohi, uint32_t m) {
+  if (!e || (m && !lohi)) return 1;
+  return eguard(e, 2, [&] { erase_ranges(*e, lohi, m); });
 }
 
 int srd_engine_set_instances(srd_engine* e, const uint32_t* bounds, uint64_t n_bounds) {

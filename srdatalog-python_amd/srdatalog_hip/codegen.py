@@ -1186,6 +1186,68 @@ int srdatalog_export_instances(const char* rel_name, uint32_t* out, uint64_t cap
   return srd_engine_export_instances(g_eng, rel_name, out, cap_rows, n_rows, cuts);
 }
 
+// ---- engine handles: resumable runs for iteration-granular serving ----
+void* srdatalog_engine_new(void) {
+  try {
+    srd_engine* e = srd_engine_create(k_program, k_pipes, sizeof(k_pipes) / sizeof(k_pipes[0]));
+    if (e && !srd_engine_ok(e)) {
+      srd_engine_destroy(e);
+      return nullptr;
+    }
+    return e;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void srdatalog_engine_free(void* h) {
+  if (h) srd_engine_destroy(static_cast<srd_engine*>(h));
+}
+
+void srdatalog_engine_set_quiet(void* h, int quiet) {
+  if (h) srd_engine_set_quiet(static_cast<srd_engine*>(h), quiet);
+}
+
+int srdatalog_engine_load_rows(void* h, const char* rel_name, const uint32_t* rows, uint64_t n,
+                               uint32_t arity) {
+  if (!h || !rel_name) return 1;
+  return srd_engine_load_rows(static_cast<srd_engine*>(h), rel_name, rows, n, arity);
+}
+
+int srdatalog_engine_run_begin(void* h) {
+  if (!h) return 1;
+  return srd_engine_run_begin(static_cast<srd_engine*>(h));
+}
+
+int srdatalog_engine_run_iterations(void* h, unsigned long long k, unsigned long long* done,
+                                    int* converged) {
+  if (!h) return 1;
+  return srd_engine_run_iterations(static_cast<srd_engine*>(h), k, done, converged);
+}
+
+int srdatalog_engine_absorb(void* dst, void* src) {
+  if (!dst || !src) return 1;
+  return srd_engine_absorb(static_cast<srd_engine*>(dst), static_cast<srd_engine*>(src));
+}
+
+int srdatalog_engine_range_counts(void* h, const char* rel_name, int delta, const uint32_t* lohi,
+                                  uint32_t m, unsigned long long* out) {
+  if (!h) return 1;
+  return srd_engine_range_counts(static_cast<srd_engine*>(h), rel_name, delta, lohi, m, out);
+}
+
+int srdatalog_engine_export_range(void* h, const char* rel_name, uint32_t lo, uint32_t hi,
+                                  uint32_t* out, uint64_t cap_rows, uint64_t* n_rows) {
+  if (!h) return 1;
+  return srd_engine_export_range(static_cast<srd_engine*>(h), rel_name, lo, hi, out, cap_rows,
+                                 n_rows);
+}
+
+int srdatalog_engine_erase_ranges(void* h, const uint32_t* lohi, uint32_t m) {
+  if (!h) return 1;
+  return srd_engine_erase_ranges(static_cast<srd_engine*>(h), lohi, m);
+}
+
 }  // extern "C"
 """
 

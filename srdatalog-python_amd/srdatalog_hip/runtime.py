@@ -22,6 +22,29 @@ class SrdatalogError(RuntimeError):
   pass
 
 
+def _bind_handles(lib) -> None:
+  vp, c_int, u32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32
+  ull_p = ctypes.POINTER(ctypes.c_ulonglong)
+  lib.srdatalog_engine_new.restype = vp
+  lib.srdatalog_engine_new.argtypes = []
+  lib.srdatalog_engine_free.argtypes = [vp]
+  lib.srdatalog_engine_free.restype = None
+  lib.srdatalog_engine_set_quiet.argtypes = [vp, c_int]
+  lib.srdatalog_engine_set_quiet.restype = None
+  lib.srdatalog_engine_load_rows.argtypes = [vp, ctypes.c_char_p, _u32p, ctypes.c_uint64, u32]
+  lib.srdatalog_engine_run_begin.argtypes = [vp]
+  lib.srdatalog_engine_run_iterations.argtypes = [vp, ctypes.c_ulonglong, ull_p,
+                                                  ctypes.POINTER(c_int)]
+  lib.srdatalog_engine_absorb.argtypes = [vp, vp]
+  lib.srdatalog_engine_range_counts.argtypes = [vp, ctypes.c_char_p, c_int, _u32p, u32, ull_p]
+  lib.srdatalog_engine_export_range.argtypes = [vp, ctypes.c_char_p, u32, u32, _u32p,
+                                                ctypes.c_uint64, _u64p]
+  lib.srdatalog_engine_erase_ranges.argtypes = [vp, _u32p, u32]
+  for f in ("load_rows", "run_begin", "run_iterations", "absorb", "range_counts", "export_range",
+            "erase_ranges"):
+    getattr(lib, f"srdatalog_engine_{f}").restype = c_int
+
+
 class ProgramLib:
   def __init__(self, so_path: str):
     native.load_runtime()
@@ -62,6 +85,8 @@ class ProgramLib:
     lib.srdatalog_export_instances.argtypes = [ctypes.c_char_p, _u32p, ctypes.c_uint64, _u64p,
                                                _u64p]
     lib.srdatalog_export_instances.restype = ctypes.c_int
+    if hasattr(lib, "srdatalog_engine_new"):
+      _bind_handles(lib)
     self.lib = lib
     self.path = so_path
 
@@ -172,3 +197,73 @@ class ProgramLib:
     if n.value == 0:
       cuts[:] = 0
     return out[:n.value], cuts.astype(np.int64)
+
+
+class Engine:
+  """Another engine of a loaded program (srdatalog_engine_new), with the
+  resumable-run calls of iteration-granular serving (include/srdatalog.h):
+  run_begin, run_iterations, absorb, range_counts, export_range,
+  erase_ranges.  Not thread-safe; one thread drives an engine."""
+
+  def __init__(self, prog: ProgramLib, quiet: bool = True):
+    if not hasattr(prog.lib, "srdatalog_engine_new"):
+      raise SrdatalogError(f"{prog.path} exports no engine handles; rebuild it")
+    self.lib = prog.lib
+    self.h = self.lib.srdatalog_engine_new()
+    if not self.h:
+      raise SrdatalogError("srdatalog_engine_new failed")
+    if quiet:
+      self.lib.srdatalog_engine_set_quiet(self.h, 1)
+
+  def close(self) -> None:
+    if self.h:
+      self.lib.srdatalog_engine_free(self.h)
+      self.h = None
+
+  def _ok(self, rc: int, what: str) -> None:
+    if rc != 0:
+      raise SrdatalogError(f"srdatalog_engine_{what} returned {rc}")
+
+  def load_rows(self, rel: str, rows: np.ndarray) -> None:
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    if rows.ndim == 1:
+      rows = rows.reshape(-1, 1)
+    self._ok(self.lib.srdatalog_engine_load_rows(self.h, rel.encode(), rows.ctypes.data_as(_u32p),
+                                                 rows.shape[0], rows.shape[1]), "load_rows")
+
+  def run_begin(self) -> None:
+    self._ok(self.lib.srdatalog_engine_run_begin(self.h), "run_begin")
+
+  def run_iterations(self, k: int = 1) -> tuple[int, bool]:
+    done, conv = ctypes.c_ulonglong(), ctypes.c_int()
+    self._ok(self.lib.srdatalog_engine_run_iterations(self.h, k, ctypes.byref(done),
+                                                      ctypes.byref(conv)), "run_iterations")
+    return int(done.value), bool(conv.value)
+
+  def absorb(self, src: "Engine") -> None:
+    self._ok(self.lib.srdatalog_engine_absorb(self.h, src.h), "absorb")
+
+  def range_counts(self, rel: str, ranges, delta: bool = True) -> np.ndarray:
+    lohi = np.ascontiguousarray(np.asarray(ranges, dtype=np.uint32).reshape(-1))
+    m = lohi.size // 2
+    out = np.zeros(max(m, 1), dtype=np.uint64)
+    self._ok(self.lib.srdatalog_engine_range_counts(
+        self.h, rel.encode(), 1 if delta else 0, lohi.ctypes.data_as(_u32p), m,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong))), "range_counts")
+    return out[:m].astype(np.int64)
+
+  def export_range(self, rel: str, arity: int, lo: int, hi: int) -> np.ndarray:
+    n = ctypes.c_uint64()
+    self._ok(self.lib.srdatalog_engine_export_range(self.h, rel.encode(), lo, hi, None, 0,
+                                                    ctypes.byref(n)), "export_range")
+    out = np.empty((max(n.value, 1), arity), dtype=np.uint32)
+    if n.value:
+      self._ok(self.lib.srdatalog_engine_export_range(
+          self.h, rel.encode(), lo, hi, out.ctypes.data_as(_u32p), n.value, ctypes.byref(n)),
+          "export_range")
+    return out[:n.value]
+
+  def erase_ranges(self, ranges) -> None:
+    lohi = np.ascontiguousarray(np.asarray(ranges, dtype=np.uint32).reshape(-1))
+    self._ok(self.lib.srdatalog_engine_erase_ranges(self.h, lohi.ctypes.data_as(_u32p),
+                                                    lohi.size // 2), "erase_ranges")

@@ -21,6 +21,22 @@ disjoint id ranges (srdatalog_hip.batch).  Requests that arrive meanwhile
 wait for the next batch of their kind; each result is split out of the
 batch's output and is, by construction, the request's result alone.
 
+TC requests can be admitted at iteration granularity instead
+(``iterative=True``): one resumable fixpoint keeps running (include/srdatalog.h,
+engine handles), and between two of its iterations the requests that
+arrived are packed into fresh id ranges, their base step is run on a
+second engine and absorbed into the running one, and the requests whose
+rows produced an empty delta are read out and erased.  A request thus
+starts at the next iteration boundary rather than after the running batch,
+and leaves as soon as its own fixpoint is reached; its rows never meet
+another request's (disjoint ranges), so its result and per-iteration deltas
+are those of a run of it alone.  The running fixpoint advances on the
+host-driven loop (its state changes between iterations), whereas a batch
+small enough runs its iterations device-resident from HIP graphs; on the
+overloaded 200 req/s stream that cost TC throughput (629 against 1,062
+TC requests completed in the 90 s window) while other kinds' latency
+improved, so batch admission stays the default.
+
 Admission: every request's output is estimated (reach: the graph's vertex
 count; tri: the listing bound m^1.5; tc: sampled BFS reach) and a request
 is admitted into a batch only while the summed estimate of in-flight
@@ -40,7 +56,7 @@ import time
 
 import numpy as np
 
-from . import batch, datasets, programs
+from . import batch, datasets, programs, runtime
 
 BYTES_PER_TUPLE = batch.BYTES_PER_TUPLE
 KIND_NAMES = ("reach", "tri", "tc")
@@ -248,6 +264,73 @@ class PackedRunner:
     self.lib.shutdown()
 
 
+class IterativeTCRunner:
+  """TC requests admitted between iterations of one running fixpoint."""
+
+  # fresh id ranges are taken from here on; the running engine starts over
+  # (when idle) before they could reach the u32 limit
+  ID_LIMIT = 1 << 30
+
+  def __init__(self):
+    self.prog = programs.load("tc")
+    self.main = runtime.Engine(self.prog)
+    self.stage = runtime.Engine(self.prog)
+    self.active: list[tuple[int, int, Request]] = []   # (lo, hi, request), ascending lo
+    self._restart()
+
+  # finished requests' rows stay in the engine until this many have
+  # accumulated (or nothing is running): one compaction pass removes them all
+  ERASE_ROWS = 1 << 24
+
+  def _restart(self) -> None:
+    self.main.load_rows("ArcInput", np.zeros((0, 2), np.uint32))
+    self.main.run_begin()
+    self.next_id = 0
+    self.dead: list[tuple[int, int]] = []
+    self.dead_rows = 0
+
+  def admit(self, reqs: list[Request]) -> None:
+    if not self.active and self.next_id > self.ID_LIMIT:
+      self._restart()
+    parts = []
+    for r in reqs:
+      k = int(r.verts.size)
+      lo, hi = self.next_id, self.next_id + max(k, 1)
+      self.next_id = hi
+      self.active.append((lo, hi, r))
+      if r.sub.shape[0]:
+        parts.append(r.sub.astype(np.uint32) + np.uint32(lo))
+    rows = np.concatenate(parts) if parts else np.zeros((0, 2), np.uint32)
+    self.stage.load_rows("ArcInput", rows)
+    self.stage.run_begin()
+    self.main.absorb(self.stage)
+
+  def step(self) -> list[Request]:
+    """One iteration; returns the requests that finished with it."""
+    if not self.active:
+      return []
+    self.main.run_iterations(1)
+    cnt = self.main.range_counts("Path", [(lo, hi) for lo, hi, _ in self.active], delta=True)
+    done, keep = [], []
+    for (lo, hi, r), c in zip(self.active, cnt):
+      if c:
+        keep.append((lo, hi, r))
+        continue
+      r.result = self.main.export_range("Path", 2, lo, hi) - np.uint32(lo)
+      self.dead.append((lo, hi))
+      self.dead_rows += int(r.result.shape[0]) + int(r.sub.shape[0])
+      done.append(r)
+    self.active = keep
+    if self.dead and (self.dead_rows >= self.ERASE_ROWS or not self.active):
+      self.main.erase_ranges(sorted(self.dead))
+      self.dead, self.dead_rows = [], 0
+    return done
+
+  def close(self) -> None:
+    self.main.close()
+    self.stage.close()
+
+
 # ----------------------------------------------------------------- server ---
 class Server:
   """Continuous batching over one request stream (arrival times relative to
@@ -255,7 +338,7 @@ class Server:
 
   def __init__(self, graphs: list[Graph], budget_bytes: float = 200e9, cap_tuples: float = 5e7,
                max_batch: int = 256, batch_tuples: float = 2e8, progress=None,
-               keep_bytes: float = 48e9):
+               keep_bytes: float = 48e9, iterative: bool = False):
     self.graphs = graphs
     # progress(kind, seconds since the stream started, requests completed
     # so far): called by a worker after each batch it completes
@@ -273,8 +356,10 @@ class Server:
     # deduplication, several times the estimate) must stay far below the
     # 2^32 rows an index holds
     self.batch_tuples = batch_tuples
+    # iterative: TC requests join the running fixpoint between iterations
+    self.iterative = iterative
     self.runners = {"reach": ReachRunner(graphs), "tri": PackedRunner("tri"),
-                    "tc": PackedRunner("tc")}
+                    "tc": IterativeTCRunner() if iterative else PackedRunner("tc")}
     self.lock = threading.Lock()
     self.in_flight = 0.0
     self.errors: list[str] = []
@@ -284,11 +369,14 @@ class Server:
     for r in self.runners.values():
       r.close()
 
-  def _take(self, queue: list[Request], pos: int, now: float) -> tuple[list[Request], int]:
+  def _take(self, queue: list[Request], pos: int, now: float,
+            held: float = 0.0) -> tuple[list[Request], int]:
     """Admit arrived, prepared requests in arrival order while the in-flight
-    estimate fits the budget."""
+    estimate fits the budget.  `held`: estimated tuples of requests already
+    running in the engine the new ones join (iterative TC), counted against
+    the per-batch tuple cap with them."""
     take = []
-    batch_est = 0.0
+    batch_est = held
     with self.lock:
       while pos < len(queue) and queue[pos].t <= now and len(take) < self.max_batch:
         r = queue[pos]
@@ -301,7 +389,7 @@ class Server:
         need = r.est * BYTES_PER_TUPLE
         if self.in_flight > 0 and self.in_flight + need > self.budget:
           break                      # wait for memory to free up
-        if take and batch_est + r.est > self.batch_tuples:
+        if (take or held > 0) and batch_est + r.est > self.batch_tuples:
           break                      # the rest goes into the next batch
         self.in_flight += need
         batch_est += r.est
@@ -309,7 +397,60 @@ class Server:
         pos += 1
     return take, pos
 
+  def _finish(self, kind: str, done: list[Request], t0: float) -> None:
+    """Completion of `done` at the current time: memory, counts, progress."""
+    t = time.perf_counter() - t0
+    nbytes = sum(r.result.nbytes for r in done if isinstance(r.result, np.ndarray))
+    with self.lock:
+      keep = self.kept_bytes + nbytes <= self.keep_bytes
+      if keep:
+        self.kept_bytes += nbytes
+      for r in done:
+        r.t_done = t
+        self.in_flight -= r.est * BYTES_PER_TUPLE
+        if not keep and isinstance(r.result, np.ndarray):
+          r.result = None
+          r.result_dropped = True
+      self.completed += len(done)
+      completed = self.completed
+    if self.progress is not None:
+      self.progress(kind, t, completed)
+
+  def _iter_worker(self, kind: str, queue: list[Request], t0: float,
+                   wall_cap: float | None) -> None:
+    """Iteration-granular admission: between two iterations of the running
+    fixpoint, absorb the requests that arrived (budget and in-flight tuple
+    cap permitting) and release the ones that finished."""
+    pos = 0
+    runner = self.runners[kind]
+    try:
+      while pos < len(queue) or runner.active:
+        now = time.perf_counter() - t0
+        open_ = wall_cap is None or now <= wall_cap
+        if open_ and pos < len(queue) and queue[pos].t <= now:
+          j = pos
+          while j < len(queue) and queue[j].t <= now and j - pos < self.max_batch:
+            prepare([queue[j]], self.graphs)
+            j += 1
+          take, pos = self._take(queue, pos, now, held=sum(r.est for _, _, r in runner.active))
+          if take:
+            runner.admit(take)
+            with self.lock:
+              self.batches[kind] += 1
+        if runner.active:
+          done = runner.step()
+          if done:
+            self._finish(kind, done, t0)
+        elif not open_:
+          break                      # the rest stays unserved (t_done < 0)
+        elif pos < len(queue):
+          time.sleep(min(max(queue[pos].t - now, 0.0), 0.002))
+    except Exception as ex:  # surfaced by run()
+      self.errors.append(f"{kind}: {ex!r}")
+
   def _worker(self, kind: str, queue: list[Request], t0: float, wall_cap: float | None) -> None:
+    if kind == "tc" and self.iterative:
+      return self._iter_worker(kind, queue, t0, wall_cap)
     pos = 0
     runner = self.runners[kind]
     try:
@@ -329,23 +470,9 @@ class Server:
           time.sleep(0.0005)
           continue
         runner.run(take)
-        done = time.perf_counter() - t0
-        nbytes = sum(r.result.nbytes for r in take if isinstance(r.result, np.ndarray))
         with self.lock:
-          keep = self.kept_bytes + nbytes <= self.keep_bytes
-          if keep:
-            self.kept_bytes += nbytes
-          for r in take:
-            r.t_done = done
-            self.in_flight -= r.est * BYTES_PER_TUPLE
-            if not keep and isinstance(r.result, np.ndarray):
-              r.result = None
-              r.result_dropped = True
           self.batches[kind] += 1
-          self.completed += len(take)
-          completed = self.completed
-        if self.progress is not None:
-          self.progress(kind, done, completed)
+        self._finish(kind, take, t0)
     except Exception as ex:  # surfaced by run()
       self.errors.append(f"{kind}: {ex!r}")
 

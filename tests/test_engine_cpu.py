@@ -302,3 +302,134 @@ def test_devloop_row_limit_hands_back(eng, rows, monkeypatch):
   np.testing.assert_array_equal(path, want.relations["Path"])
   assert deltas == want.deltas["Path"]
   assert size == want.size("Path")
+
+
+# ---- resumable runs (iteration-granular serving) --------------------------
+class Resumable:
+  """srd_engine_run_begin / run_iterations / absorb / range_counts /
+  export_range / erase_ranges over the host-only TC engine."""
+
+  def __init__(self, lib):
+    self.lib = lib
+    ull_p = ctypes.POINTER(ctypes.c_ulonglong)
+    vp = ctypes.c_void_p
+    lib.srd_engine_run_begin.argtypes = [vp]
+    lib.srd_engine_run_iterations.argtypes = [vp, ctypes.c_ulonglong, ull_p, ctypes.POINTER(ctypes.c_int)]
+    lib.srd_engine_absorb.argtypes = [vp, vp]
+    lib.srd_engine_range_counts.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _u32p, ctypes.c_uint32,
+                                            ull_p]
+    lib.srd_engine_export_range.argtypes = [vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32,
+                                            _u32p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    lib.srd_engine_erase_ranges.argtypes = [vp, _u32p, ctypes.c_uint32]
+
+  def begin(self, edges):
+    e = self.lib.cpu_tc_engine(tc_text().encode())
+    assert self.lib.srd_engine_ok(e) == 1
+    self.lib.srd_engine_set_quiet(e, 1)
+    edges = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+    self.lib.srd_engine_load_rows(e, b"ArcInput", edges.ctypes.data_as(_u32p), edges.shape[0], 2)
+    assert self.lib.srd_engine_run_begin(e) == 0
+    return e
+
+  def step(self, e, k=1):
+    done, conv = ctypes.c_ulonglong(), ctypes.c_int()
+    assert self.lib.srd_engine_run_iterations(e, k, ctypes.byref(done), ctypes.byref(conv)) == 0
+    return done.value, bool(conv.value)
+
+  def counts(self, e, ranges):
+    lohi = np.ascontiguousarray(np.asarray(ranges, np.uint32).reshape(-1))
+    out = (ctypes.c_ulonglong * max(len(ranges), 1))()
+    assert self.lib.srd_engine_range_counts(e, b"Path", 1, lohi.ctypes.data_as(_u32p), len(ranges),
+                                            out) == 0
+    return [int(out[i]) for i in range(len(ranges))]
+
+  def export(self, e, lo, hi):
+    n = ctypes.c_uint64()
+    assert self.lib.srd_engine_export_range(e, b"Path", lo, hi, None, 0, ctypes.byref(n)) == 0
+    out = np.empty((max(n.value, 1), 2), np.uint32)
+    assert self.lib.srd_engine_export_range(e, b"Path", lo, hi, out.ctypes.data_as(_u32p), n.value,
+                                            ctypes.byref(n)) == 0
+    return out[:n.value]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_resumable_instances_match_alone(eng, seed):
+  """Instances absorbed into a running TC fixpoint at different iterations
+  (one before the first iteration, the others later, one of them empty)
+  each end with the tuples and per-iteration |DELTA| of a run of that
+  instance alone; finished instances are read out and erased while the
+  others keep running."""
+  R = Resumable(eng)
+  rng = np.random.default_rng(seed)
+  sizes = [(40, 90), (25, 60), (0, 0), (70, 150), (12, 30)]
+  inst = []
+  base = 5
+  for n, m in sizes:
+    ed = ds.tc_edges(int(rng.integers(1 << 20)), max(n, 1), m) if m else np.zeros((0, 2), np.uint32)
+    inst.append((base, base + max(n, 1), ed))
+    base += max(n, 1) + int(rng.integers(0, 7))          # gaps between ranges
+  arrive = [0, 0, 2, 3, 5]                                # iteration each instance is absorbed at
+  main = R.begin(np.zeros((0, 2), np.uint32))
+  active, deltas, done = {}, {}, {}
+  pending = []
+
+  def erase(rr):
+    lohi = np.array(sorted(rr), np.uint32).reshape(-1)
+    assert eng.srd_engine_erase_ranges(main, lohi.ctypes.data_as(_u32p), len(rr)) == 0
+
+  it = 0
+  while len(done) < len(inst):
+    for j, a in enumerate(arrive):
+      if a == it and j not in active and j not in done:
+        lo, hi, ed = inst[j]
+        st = R.begin(ed + np.uint32(lo))
+        assert eng.srd_engine_absorb(main, st) == 0
+        eng.srd_engine_destroy(st)
+        active[j] = True
+        deltas[j] = []
+    if active:
+      R.step(main, 1)
+      order = sorted(active)
+      cnt = R.counts(main, [(inst[j][0], inst[j][1]) for j in order])
+      for j, c in zip(order, cnt):
+        deltas[j].append(c)
+        if c == 0:
+          lo, hi, _ = inst[j]
+          done[j] = R.export(main, lo, hi) - np.uint32(lo)
+          pending.append((lo, hi))
+          del active[j]
+      # seed 1 defers erasing until two finished instances wait, so that
+      # finished rows sit beside running ones and several ranges go at once
+      if pending and (len(pending) >= 2 or seed != 1):
+        erase(pending)
+        pending = []
+    it += 1
+    assert it < 200
+  if pending:
+    erase(pending)
+  for j, (lo, hi, ed) in enumerate(inst):
+    want = oracle.run("tc", {"ArcInput": ed})
+    np.testing.assert_array_equal(done[j], want.relations["Path"])
+    assert deltas[j] == want.deltas["Path"], j
+  # everything erased: the engine holds no Path rows any more
+  assert R.export(main, 0, 1 << 31).shape[0] == 0
+  eng.srd_engine_destroy(main)
+
+
+def test_resumable_rejects_misuse(eng):
+  R = Resumable(eng)
+  e = eng.cpu_tc_engine(tc_text().encode())
+  eng.srd_engine_set_quiet(e, 1)
+  done, conv = ctypes.c_ulonglong(), ctypes.c_int()
+  assert eng.srd_engine_run_iterations(e, 1, ctypes.byref(done), ctypes.byref(conv)) != 0
+  main = R.begin(np.array([[1, 2]], np.uint32))
+  assert eng.srd_engine_absorb(main, main) != 0
+  R.step(main, 1)
+  late = R.begin(np.array([[7, 8]], np.uint32))
+  R.step(late, 1)
+  assert eng.srd_engine_absorb(main, late) != 0           # absorbed engine past iteration 0
+  bad = (ctypes.c_uint32 * 4)(5, 9, 3, 4)
+  out = (ctypes.c_ulonglong * 2)()
+  assert eng.srd_engine_range_counts(main, b"Path", 1, ctypes.cast(bad, _u32p), 2, out) != 0
+  for x in (e, main, late):
+    eng.srd_engine_destroy(x)

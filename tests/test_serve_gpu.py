@@ -28,12 +28,13 @@ def _oracle_result(r, g):
   return oracle.run("tc", {"ArcInput": r.sub}).relations["Path"]
 
 
-def test_served_equals_alone_and_oracle(graphs):
+@pytest.mark.parametrize("iterative", [True, False], ids=["tc-iterative", "tc-batched"])
+def test_served_equals_alone_and_oracle(graphs, iterative):
   reqs = serve.make_requests(4, graphs, rate=300.0, seconds=0.4, tri_log2=(6, 10),
                              tc_range=(50, 400))
   assert len(reqs) > 40
   plain = copy.deepcopy(reqs)
-  srv = serve.Server(graphs, max_batch=64)
+  srv = serve.Server(graphs, max_batch=64, iterative=iterative)
   try:
     srv.run(reqs)
   finally:
@@ -64,3 +65,31 @@ def test_served_rejects_over_cap(graphs):
       assert r.rejected and r.t_done < 0
     else:
       assert not r.rejected and r.t_done >= r.t
+
+
+def test_iterative_tc_joins_running_fixpoint(graphs):
+  """TC requests absorbed into the running fixpoint between its iterations
+  -- the second and third after its first iteration, the last two into an
+  engine whose instances had all finished -- each end with the oracle's
+  closure of their own subgraph; finished ones are read out and erased."""
+  reqs = [r for r in serve.make_requests(9, graphs, rate=400.0, seconds=1.0, tc_range=(80, 600))
+          if r.kind == "tc"][:5]
+  assert len(reqs) == 5
+  serve.prepare(reqs, graphs)
+  runner = serve.IterativeTCRunner()
+  try:
+    runner.admit(reqs[:1])
+    runner.step()
+    runner.admit(reqs[1:3])
+    assert len(runner.active) >= 2                   # joined a fixpoint still running
+    finished = []
+    while runner.active:
+      finished += runner.step()
+    runner.admit(reqs[3:])
+    while runner.active:
+      finished += runner.step()
+  finally:
+    runner.close()
+  assert sorted(r.rid for r in finished) == sorted(r.rid for r in reqs)
+  for r in reqs:
+    np.testing.assert_array_equal(r.result, _oracle_result(r, graphs[r.graph]))
