@@ -129,15 +129,19 @@ int srdatalog_export_instances(const char* rel_name, uint32_t* out, uint64_t cap
  *   range_counts     rows of rel's FULL (delta = 0) or DELTA (delta = 1)
  *                    whose values lie in each of m ranges [lohi[2g],
  *                    lohi[2g+1]), ascending and disjoint, into out[g];
- *   export_range     rel's FULL rows with column 0 in [lo, hi), row-major
- *                    in relation column order (*n_rows set when out is NULL);
+ *   export_ranges    rel's FULL rows in each of m ranges (as range_counts),
+ *                    row-major in relation column order, the ranges one
+ *                    after the other: range g owns rows [cuts[g], cuts[g+1])
+ *                    (cuts: m+1 entries, may be NULL); with rebase != 0
+ *                    every value of range g's rows is reduced by lohi[2g];
+ *                    *n_rows is set, and cuts filled, also when out is NULL;
  *   erase_ranges     drops the rows in each of m ranges (as range_counts)
  *                    from every relation, in one pass over each index.
  * Instances must occupy disjoint value ranges with every value of an
  * instance's rows inside its range (as a packed batch's do); an instance's
  * rows and per-iteration |DELTA| are then those of a run of it alone,
  * shifted by the iteration it was absorbed at.  0 ok, 1 null, 2 error
- * (range_counts / export_range: 2 unknown relation, 3 error). */
+ * (range_counts / export_ranges: 2 unknown relation, 3 error). */
 void* srdatalog_engine_new(void);
 void srdatalog_engine_free(void* h);
 void srdatalog_engine_set_quiet(void* h, int quiet);
@@ -149,8 +153,9 @@ int srdatalog_engine_run_iterations(void* h, unsigned long long k, unsigned long
 int srdatalog_engine_absorb(void* dst, void* src);
 int srdatalog_engine_range_counts(void* h, const char* rel_name, int delta, const uint32_t* lohi,
                                   uint32_t m, unsigned long long* out);
-int srdatalog_engine_export_range(void* h, const char* rel_name, uint32_t lo, uint32_t hi,
-                                  uint32_t* out, uint64_t cap_rows, uint64_t* n_rows);
+int srdatalog_engine_export_ranges(void* h, const char* rel_name, const uint32_t* lohi, uint32_t m,
+                                   int rebase, uint32_t* out, uint64_t cap_rows, uint64_t* n_rows,
+                                   uint64_t* cuts);
 int srdatalog_engine_erase_ranges(void* h, const uint32_t* lohi, uint32_t m);
 
 #ifdef __cplusplus

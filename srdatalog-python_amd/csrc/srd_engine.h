@@ -41,8 +41,9 @@ int srd_engine_run_iterations(struct srd_engine* e, unsigned long long k, unsign
 int srd_engine_absorb(struct srd_engine* dst, struct srd_engine* src);
 int srd_engine_range_counts(struct srd_engine* e, const char* rel, int delta, const uint32_t* lohi,
                             uint32_t m, unsigned long long* out);
-int srd_engine_export_range(struct srd_engine* e, const char* rel, uint32_t lo, uint32_t hi,
-                            uint32_t* out, uint64_t cap_rows, uint64_t* n_rows);
+int srd_engine_export_ranges(struct srd_engine* e, const char* rel, const uint32_t* lohi, uint32_t m,
+                             int rebase, uint32_t* out, uint64_t cap_rows, uint64_t* n_rows,
+                             uint64_t* cuts);
 int srd_engine_erase_ranges(struct srd_engine* e, const uint32_t* lohi, uint32_t m);
 
 #ifdef __cplusplus

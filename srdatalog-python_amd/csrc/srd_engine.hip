@@ -2426,6 +2426,9 @@ static void run_begin(srd_engine& e) {
   reset_state(e);
   for (auto& r : e.rels) upload_input(e, r);
   for (int si = 0; si < rs; ++si) exec_ops(e, e.steps[si].ops, 0, si);
+  // FULL of relations no join of the step reads stays log-structured, so an
+  // iteration (and an absorb) costs what it adds, not the size of FULL
+  for (int ri : e.steps[rs].segmented_rels) e.rels[ri].segmented = true;
   SRD_CHECK(hipStreamSynchronize(e.s));
   e.res_step = size_t(rs);
   e.res_it = 0;
@@ -2457,6 +2460,36 @@ static bool in_step(const Step& st, int ri) {
   return std::find(st.canonical_rels.begin(), st.canonical_rels.end(), ri) != st.canonical_rels.end();
 }
 
+// Merge the two youngest parts of a segmented FULL while the older is
+// within 2x of the younger (the oldest part is full[canonical]).
+static void merge_young_segments(srd_engine& e, Relation& r) {
+  auto it = r.full.find(r.canonical);
+  while (!r.segs.empty()) {
+    const size_t m = r.segs.size();
+    Index& older = m >= 2 ? r.segs[m - 2] : it->second.idx;
+    if (older.n > 2 * r.segs.back().n && m < 24) break;
+    Index mg;
+    merge_disjoint(older, r.segs.back(), mg, e.ws, e.s);
+    older = std::move(mg);
+    r.segs.pop_back();
+  }
+}
+
+// The disjoint sorted parts FULL consists of (canonical order): the oldest
+// and the segments of a segmented FULL, else the one canonical index.
+static std::vector<Index*> full_parts(srd_engine& e, Relation& r) {
+  std::vector<Index*> parts;
+  if (!r.canonical_set || (r.full.empty() && !r.has_input)) return parts;
+  auto it = r.full.find(r.canonical);
+  if (!r.segs.empty() && it != r.full.end()) {
+    parts.push_back(&it->second.idx);
+    for (auto& sg : r.segs) parts.push_back(&sg);
+  } else {
+    parts.push_back(const_cast<Index*>(&full_index(e, r, r.canonical)));
+  }
+  return parts;
+}
+
 static void absorb(srd_engine& d, srd_engine& s) {
   if (&d == &s) fail("srdatalog_absorb: an engine cannot absorb itself");
   if (!d.resumable || !s.resumable || s.res_it != 0)
@@ -2472,16 +2505,30 @@ static void absorb(srd_engine& d, srd_engine& s) {
     if (!rs.canonical_set || (rs.full.empty() && !rs.has_input)) continue;
     const Index& add = full_index(s, rs, rs.canonical);
     if (add.n == 0) continue;
-    Versioned v;
-    const Index& cur = full_index(d, rd, rd.canonical);
-    if (cur.n == 0) copy_index(add, v.idx, d.s);
-    else merge_disjoint(cur, add, v.idx, d.ws, d.s);
-    rd.full.clear();
-    rd.segs.clear();
-    rd.full_version += 1;
-    rd.delta_seq = rd.full_version;
-    v.version = rd.full_version;
-    rd.full[rd.canonical] = std::move(v);
+    if (rd.segmented && rd.full.count(rd.canonical)) {
+      // the youngest segment; the two youngest merge while within 2x
+      Index c;
+      copy_index(add, c, d.s);
+      rd.segs.push_back(std::move(c));
+      merge_young_segments(d, rd);
+      Versioned base = std::move(rd.full[rd.canonical]);
+      rd.full.clear();
+      rd.full_version += 1;
+      rd.delta_seq = rd.full_version;
+      base.version = rd.full_version;
+      rd.full[rd.canonical] = std::move(base);
+    } else {
+      Versioned v;
+      const Index& cur = full_index(d, rd, rd.canonical);
+      if (cur.n == 0) copy_index(add, v.idx, d.s);
+      else merge_disjoint(cur, add, v.idx, d.ws, d.s);
+      rd.full.clear();
+      rd.segs.clear();
+      rd.full_version += 1;
+      rd.delta_seq = rd.full_version;
+      v.version = rd.full_version;
+      rd.full[rd.canonical] = std::move(v);
+    }
     if (in_step(st, int(i))) {
       auto it = rd.delta.find(rd.canonical);
       Index nd;
@@ -2568,18 +2615,23 @@ static void erase_ranges(srd_engine& e, const uint32_t* lohi, uint32_t m) {
     return false;
   };
   for (auto& r : e.rels) {
-    if (!r.canonical_set || (r.full.empty() && !r.has_input)) continue;
-    const Index& f = full_index(e, r, r.canonical);
-    auto rm = value_ranges(e, f, lohi, m);
-    if (any(rm)) {
-      Versioned v;
-      erase_rows(e, f, rm, v.idx);
+    bool changed = false;
+    for (Index* part : full_parts(e, r)) {
+      auto rm = value_ranges(e, *part, lohi, m);
+      if (!any(rm)) continue;
+      Index kept;
+      erase_rows(e, *part, rm, kept);
+      *part = std::move(kept);
+      changed = true;
+    }
+    if (changed) {
+      // FULL in other column orders follows the canonical parts again
+      Versioned base = std::move(r.full[r.canonical]);
       r.full.clear();
-      r.segs.clear();
       r.full_version += 1;
       r.delta_seq = r.full_version;
-      v.version = r.full_version;
-      r.full[r.canonical] = std::move(v);
+      base.version = r.full_version;
+      r.full[r.canonical] = std::move(base);
     }
     auto it = r.delta.find(r.canonical);
     if (it != r.delta.end()) {
@@ -2971,44 +3023,78 @@ int srd_engine_range_counts(srd_engine* e, const char* rel, int delta, const uin
     if (m == 0) return;
     for (uint64_t i = 1; i < 2ull * m; ++i)
       if (lohi[i] < lohi[i - 1]) fail("srdatalog_range_counts: ranges must be ascending and disjoint");
-    const Index* x = nullptr;
+    std::vector<Index*> parts;
     if (delta) {
       auto d = r.delta.find(r.canonical);
-      if (d != r.delta.end()) x = &d->second;
-    } else if (r.canonical_set && (!r.full.empty() || r.has_input)) {
-      x = &full_index(*e, r, r.canonical);
+      if (d != r.delta.end()) parts.push_back(&d->second);
+    } else {
+      parts = full_parts(*e, r);
     }
-    if (!x || x->n == 0) return;
-    // instance_counts over the 2m bounds: even slots are the ranges, odd the gaps
-    DevBuf<uint32_t> b(2ull * m, e->s);
-    DevBuf<uint64_t> per(2ull * m - 1, e->s);
-    SRD_CHECK(hipMemcpyAsync(b.p, lohi, 8ull * m, hipMemcpyHostToDevice, e->s));
-    instance_counts(x->cols.p, x->n, b.p, 2 * m - 1, per.p, e->s);
-    std::vector<uint64_t> h(2ull * m - 1);
-    SRD_CHECK(hipMemcpyAsync(h.data(), per.p, h.size() * 8, hipMemcpyDeviceToHost, e->s));
-    SRD_CHECK(hipStreamSynchronize(e->s));
-    for (uint32_t g = 0; g < m; ++g) out[g] = h[2ull * g];
+    for (Index* x : parts) {
+      auto v = value_ranges(*e, *x, lohi, m);
+      for (uint32_t g = 0; g < m; ++g) out[g] += v[g].second;
+    }
   });
 }
 
-int srd_engine_export_range(srd_engine* e, const char* rel, uint32_t lo, uint32_t hi,
-                            uint32_t* out, uint64_t cap_rows, uint64_t* n_rows) {
-  if (!e || !rel || !n_rows) return 1;
+int srd_engine_export_ranges(srd_engine* e, const char* rel, const uint32_t* lohi, uint32_t m,
+                             int rebase, uint32_t* out, uint64_t cap_rows, uint64_t* n_rows,
+                             uint64_t* cuts) {
+  if (!e || !rel || !n_rows || (m && !lohi)) return 1;
   auto it = e->rel_by_name.find(rel);
   if (it == e->rel_by_name.end()) return 2;
   Relation& r = e->rels[it->second];
   return eguard(e, 3, [&] {
+    for (uint64_t i = 1; i < 2ull * m; ++i)
+      if (lohi[i] < lohi[i - 1]) fail("srdatalog_export_ranges: ranges must be ascending and disjoint");
     *n_rows = 0;
-    const Index* x = export_index(*e, r);
-    if (!x) return;
-    auto [a, cnt] = value_range(*e, *x, lo, hi);
-    *n_rows = cnt;
-    if (!out || cnt == 0) return;
-    if (cap_rows < cnt) fail("srdatalog_export_range: buffer too small");
-    Index v = *x;
-    v.cols.p += a;
-    v.n = cnt;
-    export_rows(v, out, nullptr, 0, e->ws, e->s);
+    if (cuts) std::fill(cuts, cuts + m + 1, 0);
+    if (m == 0) return;
+    // the ranges' rows of every part of FULL (canonical order), gathered,
+    // then sorted in relation column order: only these rows are touched,
+    // and the ranges come out one after the other
+    std::vector<std::pair<const Index*, std::vector<std::pair<uint64_t, uint64_t>>>> hits;
+    std::vector<uint64_t> per(m, 0);
+    uint64_t tot = 0;
+    for (Index* part : full_parts(*e, r)) {
+      auto v = value_ranges(*e, *part, lohi, m);
+      for (uint32_t g = 0; g < m; ++g) {
+        per[g] += v[g].second;
+        tot += v[g].second;
+      }
+      hits.push_back({part, std::move(v)});
+    }
+    *n_rows = tot;
+    if (cuts)
+      for (uint32_t g = 0; g < m; ++g) cuts[g + 1] = cuts[g] + per[g];
+    if (!out || tot == 0) return;
+    if (cap_rows < tot) fail("srdatalog_export_ranges: buffer too small");
+    const uint32_t k = r.arity;
+    DevBuf<uint32_t> cols(uint64_t(k) * tot, e->s);
+    uint64_t at = 0;
+    for (auto& [part, v] : hits)
+      for (auto& [a, cnt] : v) {
+        if (!cnt) continue;
+        for (uint32_t c = 0; c < k; ++c)
+          SRD_CHECK(hipMemcpyAsync(cols.p + uint64_t(c) * tot + at,
+                                   part->cols.p + uint64_t(c) * part->stride + a, cnt * 4,
+                                   hipMemcpyDeviceToDevice, e->s));
+        at += cnt;
+      }
+    // relation column j is canonical column c with canonical[c] == j
+    std::vector<const uint32_t*> in(k);
+    for (uint32_t c = 0; c < k; ++c) in[r.canonical[c]] = cols.p + uint64_t(c) * tot;
+    Index x;
+    build_index(in.data(), k, tot, x, e->ws, e->s);
+    if (!rebase) {
+      export_rows(x, out, nullptr, 0, e->ws, e->s);
+      return;
+    }
+    // rebased on the device: the 2m bounds lo0, hi0, lo1, ... make ranges
+    // and gaps alternate; a row of range g loses lo_g from every value
+    DevBuf<uint32_t> b(2ull * m, e->s);
+    SRD_CHECK(hipMemcpyAsync(b.p, lohi, 8ull * m, hipMemcpyHostToDevice, e->s));
+    export_rows(x, out, b.p, 2 * m, e->ws, e->s);
   });
 }
 

@@ -1236,11 +1236,12 @@ int srdatalog_engine_range_counts(void* h, const char* rel_name, int delta, cons
   return srd_engine_range_counts(static_cast<srd_engine*>(h), rel_name, delta, lohi, m, out);
 }
 
-int srdatalog_engine_export_range(void* h, const char* rel_name, uint32_t lo, uint32_t hi,
-                                  uint32_t* out, uint64_t cap_rows, uint64_t* n_rows) {
+int srdatalog_engine_export_ranges(void* h, const char* rel_name, const uint32_t* lohi, uint32_t m,
+                                   int rebase, uint32_t* out, uint64_t cap_rows, uint64_t* n_rows,
+                                   uint64_t* cuts) {
   if (!h) return 1;
-  return srd_engine_export_range(static_cast<srd_engine*>(h), rel_name, lo, hi, out, cap_rows,
-                                 n_rows);
+  return srd_engine_export_ranges(static_cast<srd_engine*>(h), rel_name, lohi, m, rebase, out,
+                                  cap_rows, n_rows, cuts);
 }
 
 int srdatalog_engine_erase_ranges(void* h, const uint32_t* lohi, uint32_t m) {

@@ -37,10 +37,10 @@ def _bind_handles(lib) -> None:
                                                   ctypes.POINTER(c_int)]
   lib.srdatalog_engine_absorb.argtypes = [vp, vp]
   lib.srdatalog_engine_range_counts.argtypes = [vp, ctypes.c_char_p, c_int, _u32p, u32, ull_p]
-  lib.srdatalog_engine_export_range.argtypes = [vp, ctypes.c_char_p, u32, u32, _u32p,
-                                                ctypes.c_uint64, _u64p]
+  lib.srdatalog_engine_export_ranges.argtypes = [vp, ctypes.c_char_p, _u32p, u32, c_int, _u32p,
+                                                 ctypes.c_uint64, _u64p, _u64p]
   lib.srdatalog_engine_erase_ranges.argtypes = [vp, _u32p, u32]
-  for f in ("load_rows", "run_begin", "run_iterations", "absorb", "range_counts", "export_range",
+  for f in ("load_rows", "run_begin", "run_iterations", "absorb", "range_counts", "export_ranges",
             "erase_ranges"):
     getattr(lib, f"srdatalog_engine_{f}").restype = c_int
 
@@ -202,7 +202,7 @@ class ProgramLib:
 class Engine:
   """Another engine of a loaded program (srdatalog_engine_new), with the
   resumable-run calls of iteration-granular serving (include/srdatalog.h):
-  run_begin, run_iterations, absorb, range_counts, export_range,
+  run_begin, run_iterations, absorb, range_counts, export_ranges,
   erase_ranges.  Not thread-safe; one thread drives an engine."""
 
   def __init__(self, prog: ProgramLib, quiet: bool = True):
@@ -252,16 +252,25 @@ class Engine:
         out.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong))), "range_counts")
     return out[:m].astype(np.int64)
 
-  def export_range(self, rel: str, arity: int, lo: int, hi: int) -> np.ndarray:
+  def export_ranges(self, rel: str, arity: int, ranges,
+                    rebase: bool = False) -> tuple[np.ndarray, np.ndarray]:
+    """FULL rows of rel in each range (ascending, disjoint), in relation
+    column order, and the m+1 row cuts: range g owns rows cuts[g]:cuts[g+1]
+    (with rebase, its values minus the range's first value)."""
+    lohi = np.ascontiguousarray(np.asarray(ranges, dtype=np.uint32).reshape(-1))
+    m = lohi.size // 2
     n = ctypes.c_uint64()
-    self._ok(self.lib.srdatalog_engine_export_range(self.h, rel.encode(), lo, hi, None, 0,
-                                                    ctypes.byref(n)), "export_range")
+    cuts = np.zeros(m + 1, dtype=np.uint64)
+    self._ok(self.lib.srdatalog_engine_export_ranges(
+        self.h, rel.encode(), lohi.ctypes.data_as(_u32p), m, int(rebase), None, 0,
+        ctypes.byref(n), cuts.ctypes.data_as(_u64p)), "export_ranges")
     out = np.empty((max(n.value, 1), arity), dtype=np.uint32)
     if n.value:
-      self._ok(self.lib.srdatalog_engine_export_range(
-          self.h, rel.encode(), lo, hi, out.ctypes.data_as(_u32p), n.value, ctypes.byref(n)),
-          "export_range")
-    return out[:n.value]
+      self._ok(self.lib.srdatalog_engine_export_ranges(
+          self.h, rel.encode(), lohi.ctypes.data_as(_u32p), m, int(rebase),
+          out.ctypes.data_as(_u32p), n.value, ctypes.byref(n), cuts.ctypes.data_as(_u64p)),
+          "export_ranges")
+    return out[:n.value], cuts.astype(np.int64)
 
   def erase_ranges(self, ranges) -> None:
     lohi = np.ascontiguousarray(np.asarray(ranges, dtype=np.uint32).reshape(-1))

@@ -26,16 +26,14 @@ TC requests can be admitted at iteration granularity instead
 engine handles), and between two of its iterations the requests that
 arrived are packed into fresh id ranges, their base step is run on a
 second engine and absorbed into the running one, and the requests whose
-rows produced an empty delta are read out and erased.  A request thus
+rows produced an empty delta are read out together and later erased.  A request thus
 starts at the next iteration boundary rather than after the running batch,
 and leaves as soon as its own fixpoint is reached; its rows never meet
 another request's (disjoint ranges), so its result and per-iteration deltas
-are those of a run of it alone.  The running fixpoint advances on the
-host-driven loop (its state changes between iterations), whereas a batch
-small enough runs its iterations device-resident from HIP graphs; on the
-overloaded 200 req/s stream that cost TC throughput (629 against 1,062
-TC requests completed in the 90 s window) while other kinds' latency
-improved, so batch admission stays the default.
+are those of a run of it alone.  On the 200 req/s stream, which TC
+requests overload, both admissions serve about the same (116.3 against
+118.2 queries/s, 899 against 1,062 TC requests in the 90 s window), so
+batch admission stays the default.
 
 Admission: every request's output is estimated (reach: the graph's vertex
 count; tri: the listing bound m^1.5; tc: sampled BFS reach) and a request
@@ -278,9 +276,12 @@ class IterativeTCRunner:
     self.active: list[tuple[int, int, Request]] = []   # (lo, hi, request), ascending lo
     self._restart()
 
-  # finished requests' rows stay in the engine until this many have
-  # accumulated (or nothing is running): one compaction pass removes them all
-  ERASE_ROWS = 1 << 24
+  # finished requests' rows stay in the engine until nothing runs or this
+  # many have accumulated (their memory): one compaction pass over FULL then
+  # removes them all.  Meanwhile they cost an iteration nothing measurable
+  # (60 requests: 0.50 s of iterations with them kept, while compacting
+  # every few iterations doubled the total)
+  ERASE_ROWS = 1 << 28
 
   def _restart(self) -> None:
     self.main.load_rows("ArcInput", np.zeros((0, 2), np.uint32))
@@ -311,17 +312,19 @@ class IterativeTCRunner:
       return []
     self.main.run_iterations(1)
     cnt = self.main.range_counts("Path", [(lo, hi) for lo, hi, _ in self.active], delta=True)
-    done, keep = [], []
+    done, keep, fin = [], [], []
     for (lo, hi, r), c in zip(self.active, cnt):
-      if c:
-        keep.append((lo, hi, r))
-        continue
-      r.result = self.main.export_range("Path", 2, lo, hi) - np.uint32(lo)
-      self.dead.append((lo, hi))
-      self.dead_rows += int(r.result.shape[0]) + int(r.sub.shape[0])
-      done.append(r)
+      (fin if c == 0 else keep).append((lo, hi, r))
+    if fin:
+      # one gather, sort and copy for every request finished this iteration
+      rows, cuts = self.main.export_ranges("Path", 2, [(lo, hi) for lo, hi, _ in fin], rebase=True)
+      for g, (lo, hi, r) in enumerate(fin):
+        r.result = rows[cuts[g]:cuts[g + 1]]
+        self.dead.append((lo, hi))
+        self.dead_rows += int(r.result.shape[0]) + int(r.sub.shape[0])
+        done.append(r)
     self.active = keep
-    if self.dead and (self.dead_rows >= self.ERASE_ROWS or not self.active):
+    if self.dead and (not self.active or self.dead_rows >= self.ERASE_ROWS):
       self.main.erase_ranges(sorted(self.dead))
       self.dead, self.dead_rows = [], 0
     return done

@@ -307,7 +307,7 @@ def test_devloop_row_limit_hands_back(eng, rows, monkeypatch):
 # ---- resumable runs (iteration-granular serving) --------------------------
 class Resumable:
   """srd_engine_run_begin / run_iterations / absorb / range_counts /
-  export_range / erase_ranges over the host-only TC engine."""
+  export_ranges / erase_ranges over the host-only TC engine."""
 
   def __init__(self, lib):
     self.lib = lib
@@ -318,8 +318,9 @@ class Resumable:
     lib.srd_engine_absorb.argtypes = [vp, vp]
     lib.srd_engine_range_counts.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _u32p, ctypes.c_uint32,
                                             ull_p]
-    lib.srd_engine_export_range.argtypes = [vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32,
-                                            _u32p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    lib.srd_engine_export_ranges.argtypes = [vp, ctypes.c_char_p, _u32p, ctypes.c_uint32, ctypes.c_int,
+                                             _u32p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_uint64)]
     lib.srd_engine_erase_ranges.argtypes = [vp, _u32p, ctypes.c_uint32]
 
   def begin(self, edges):
@@ -343,13 +344,20 @@ class Resumable:
                                             out) == 0
     return [int(out[i]) for i in range(len(ranges))]
 
-  def export(self, e, lo, hi):
+  def export(self, e, ranges, rebase=0):
+    """Rows of each range, one array per range (one export call)."""
+    lohi = np.ascontiguousarray(np.asarray(ranges, np.uint32).reshape(-1))
+    m = len(ranges)
     n = ctypes.c_uint64()
-    assert self.lib.srd_engine_export_range(e, b"Path", lo, hi, None, 0, ctypes.byref(n)) == 0
+    cuts = np.zeros(m + 1, np.uint64)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    assert self.lib.srd_engine_export_ranges(e, b"Path", lohi.ctypes.data_as(_u32p), m, rebase,
+                                             None, 0, ctypes.byref(n), cuts.ctypes.data_as(u64p)) == 0
     out = np.empty((max(n.value, 1), 2), np.uint32)
-    assert self.lib.srd_engine_export_range(e, b"Path", lo, hi, out.ctypes.data_as(_u32p), n.value,
-                                            ctypes.byref(n)) == 0
-    return out[:n.value]
+    assert self.lib.srd_engine_export_ranges(e, b"Path", lohi.ctypes.data_as(_u32p), m, rebase,
+                                             out.ctypes.data_as(_u32p), n.value, ctypes.byref(n),
+                                             cuts.ctypes.data_as(u64p)) == 0
+    return [out[int(cuts[g]):int(cuts[g + 1])] for g in range(m)]
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
@@ -391,12 +399,17 @@ def test_resumable_instances_match_alone(eng, seed):
       R.step(main, 1)
       order = sorted(active)
       cnt = R.counts(main, [(inst[j][0], inst[j][1]) for j in order])
+      fin = []
       for j, c in zip(order, cnt):
         deltas[j].append(c)
         if c == 0:
-          lo, hi, _ = inst[j]
-          done[j] = R.export(main, lo, hi) - np.uint32(lo)
-          pending.append((lo, hi))
+          fin.append(j)
+      if fin:
+        # seed 2 lets the engine rebase the rows, the others rebase here
+        rows = R.export(main, [(inst[j][0], inst[j][1]) for j in fin], rebase=int(seed == 2))
+        for j, rj in zip(fin, rows):
+          done[j] = rj if seed == 2 else rj - np.uint32(inst[j][0])
+          pending.append((inst[j][0], inst[j][1]))
           del active[j]
       # seed 1 defers erasing until two finished instances wait, so that
       # finished rows sit beside running ones and several ranges go at once
@@ -412,7 +425,7 @@ def test_resumable_instances_match_alone(eng, seed):
     np.testing.assert_array_equal(done[j], want.relations["Path"])
     assert deltas[j] == want.deltas["Path"], j
   # everything erased: the engine holds no Path rows any more
-  assert R.export(main, 0, 1 << 31).shape[0] == 0
+  assert R.export(main, [(0, 1 << 31)])[0].shape[0] == 0
   eng.srd_engine_destroy(main)
 
 
