@@ -1345,6 +1345,21 @@ static void exec_op(srd_engine& e, const Op& op, uint64_t iteration, int step_no
   slot.second += 1;
 }
 
+// Merge the two youngest parts of a segmented FULL while the older is
+// within 2x of the younger (the oldest part is full[canonical]).
+static void merge_young_segments(srd_engine& e, Relation& r) {
+  auto it = r.full.find(r.canonical);
+  while (!r.segs.empty()) {
+    const size_t m = r.segs.size();
+    Index& older = m >= 2 ? r.segs[m - 2] : it->second.idx;
+    if (older.n > 2 * r.segs.back().n && m < 24) break;
+    Index mg;
+    merge_disjoint(older, r.segs.back(), mg, e.ws, e.s);
+    older = std::move(mg);
+    r.segs.pop_back();
+  }
+}
+
 static void exec_op_body(srd_engine& e, const Op& op, uint64_t iteration, int step_no) {
   if (e.trace) {
     std::fprintf(stderr, "[srd] step %d iter %llu %s %s\n", step_no, (unsigned long long)iteration,
@@ -1461,15 +1476,7 @@ static void exec_op_body(srd_engine& e, const Op& op, uint64_t iteration, int st
           copy_index(dl, c, e.s);
           r.segs.push_back(std::move(c));
         }
-        while (!r.segs.empty()) {
-          const size_t m = r.segs.size();
-          Index& older = m >= 2 ? r.segs[m - 2] : it->second.idx;
-          if (older.n > 2 * r.segs.back().n && m < 24) break;
-          Index mg;
-          merge_disjoint(older, r.segs.back(), mg, e.ws, e.s);
-          older = std::move(mg);
-          r.segs.pop_back();
-        }
+        merge_young_segments(e, r);
         it->second.version = target;
         r.full_version = target;
         break;
@@ -2458,21 +2465,6 @@ static uint64_t run_iters(srd_engine& e, uint64_t k, bool* converged) {
 
 static bool in_step(const Step& st, int ri) {
   return std::find(st.canonical_rels.begin(), st.canonical_rels.end(), ri) != st.canonical_rels.end();
-}
-
-// Merge the two youngest parts of a segmented FULL while the older is
-// within 2x of the younger (the oldest part is full[canonical]).
-static void merge_young_segments(srd_engine& e, Relation& r) {
-  auto it = r.full.find(r.canonical);
-  while (!r.segs.empty()) {
-    const size_t m = r.segs.size();
-    Index& older = m >= 2 ? r.segs[m - 2] : it->second.idx;
-    if (older.n > 2 * r.segs.back().n && m < 24) break;
-    Index mg;
-    merge_disjoint(older, r.segs.back(), mg, e.ws, e.s);
-    older = std::move(mg);
-    r.segs.pop_back();
-  }
 }
 
 // The disjoint sorted parts FULL consists of (canonical order): the oldest
