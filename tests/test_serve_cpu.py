@@ -78,3 +78,97 @@ def test_prepare_estimates():
     assert r.prepared and r.est >= 0
     if r.kind == "reach":
       assert r.est == gs[r.graph].n
+
+
+# ---- scheduling of the server, with stand-in runners (no GPU) --------------
+class _FakePacked:
+  def __init__(self, kind, log):
+    self.kind, self.log = kind, log
+
+  def run(self, reqs):
+    self.log.append((self.kind, [r.rid for r in reqs]))
+    for r in reqs:
+      r.result = 1 if self.kind == "tri" else np.full((3, 2), r.rid, np.uint32)
+
+  def close(self):
+    pass
+
+
+class _FakeIterative:
+  """Requests finish after a number of steps set by their estimate; records
+  the estimated tuples held at every admission."""
+
+  def __init__(self, log):
+    self.active, self.log, self.held_seen = [], log, []
+
+  def admit(self, reqs):
+    self.active += [[r, 1 + int(r.est) % 3] for r in reqs]
+    self.held_seen.append(sum(r.est for r, _ in self.active))
+    self.log.append(("tc", [r.rid for r in reqs]))
+
+  def step(self):
+    done = []
+    for a in self.active:
+      a[1] -= 1
+      if a[1] == 0:
+        a[0].result = np.full((2, 2), a[0].rid, np.uint32)
+        done.append(a[0])
+    self.active = [a for a in self.active if a[1] > 0]
+    return done
+
+  def close(self):
+    pass
+
+
+def _fake_server(monkeypatch, iterative, **kw):
+  log = []
+  it = _FakeIterative(log)
+  monkeypatch.setattr(serve, "ReachRunner", lambda graphs: _FakePacked("reach", log))
+  monkeypatch.setattr(serve, "PackedRunner", lambda kind: _FakePacked(kind, log))
+  monkeypatch.setattr(serve, "IterativeTCRunner", lambda: it)
+  monkeypatch.setattr(serve, "prepare", _fake_prepare)
+  srv = serve.Server(_graphs(), iterative=iterative, **kw)
+  return srv, log, it
+
+
+def _fake_prepare(reqs, graphs, seed=0):
+  for r in reqs:
+    r.est = float(10 + r.rid % 7)
+    r.prepared = True
+
+
+def _stream(n=60, dt=0.002):
+  reqs = []
+  for i in range(n):
+    r = serve.Request(i, i * dt, ("reach", "tri", "tc")[i % 3], 0)
+    reqs.append(r)
+  return reqs
+
+
+def test_server_iterative_admission_respects_tuple_cap(monkeypatch):
+  """TC requests join the running engine while its held estimate plus theirs
+  stays within the batch tuple cap; every request completes once."""
+  srv, log, it = _fake_server(monkeypatch, True, batch_tuples=40.0)
+  reqs = _stream()
+  srv.run(reqs)
+  assert all(r.t_done >= r.t for r in reqs)
+  assert max(it.held_seen) <= 40.0
+  tc_ids = sorted(i for k, ids in log if k == "tc" for i in ids)
+  assert tc_ids == sorted(r.rid for r in reqs if r.kind == "tc")
+
+
+def test_server_wall_cap_and_keep_budget(monkeypatch):
+  """No batch starts after the wall cap (later requests stay unserved) and
+  results beyond the keep budget are delivered but not retained."""
+  srv, log, _ = _fake_server(monkeypatch, True, keep_bytes=100.0)
+  reqs = _stream(n=90, dt=0.01)
+  seen = []
+  srv.progress = lambda kind, t, done: seen.append(done)
+  srv.run(reqs, wall_cap=0.3)
+  served = [r for r in reqs if r.t_done >= 0]
+  assert served and len(served) < len(reqs)
+  assert all(r.t <= 0.3 + 0.05 for r in served)
+  dropped = [r for r in served if r.result_dropped]
+  assert dropped and all(r.result is None for r in dropped)
+  assert srv.kept_bytes <= 100.0
+  assert seen and seen[-1] == len(served)
