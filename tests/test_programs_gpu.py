@@ -271,6 +271,29 @@ def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
       lib.shutdown()
 
 
+# Key-major bitmaps for over a thousand keys (ranges of 16+ rows on an R-MAT
+# scale-14 graph), so bitmap slots run past the few keys of the smaller
+# graphs: every key long enough, and a budget that doubles the threshold
+# once (bitmap tests and searches in one wave).
+BITMAP_MANY = [({"SRDATALOG_BITMAP_MIN": "16"}, "keys-16"),
+               ({"SRDATALOG_BITMAP_MIN": "16", "SRDATALOG_BITMAP_BYTES": "4000000"}, "keys-16-budget")]
+
+
+@pytest.mark.parametrize("env,label", BITMAP_MANY, ids=[m[1] for m in BITMAP_MANY])
+def test_triangle_many_bitmaps(progs, env, label, monkeypatch):
+  for k in ("SRDATALOG_BITMAP_MIN", "SRDATALOG_BITMAP_BYTES"):
+    monkeypatch.delenv(k, raising=False)
+  for k, v in env.items():
+    monkeypatch.setenv(k, v)
+  inputs = {"KnowsInput": ds.triangle_edges(6, 14, 300_000)}
+  orc = oracle.run("triangle", inputs)
+  lib = run_prog(progs, "triangle", inputs)
+  try:
+    check_rel(lib, orc, "Triangle", 3)
+  finally:
+    lib.shutdown()
+
+
 # --------------------------------------------------------------- negation ---
 def _neg2hop_inputs(seed, n, m, tags, mi):
   rng = np.random.default_rng(seed)
