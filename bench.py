@@ -189,14 +189,18 @@ def bench_batch(args) -> dict:
   # more batched run (its kinds one after the other) and for plain runs of a
   # 16-instance sample
   diag = {}
+  pick = {k: list(range(0, len(v), max(1, len(v) // 8)))[:8] for k, v in kinds.items()}
+  sample = {k: [kinds[k][j] for j in pick[k]] for k in kinds}
+  late = {}
   for label, runner, sets in (("batched", batch.run_batched, kinds),
-                              ("plain_sample16", batch.run_plain,
-                               {k: v[:: max(1, len(v) // 8)][:8] for k, v in kinds.items()})):
+                              ("plain_sample16", batch.run_plain, sample)):
     native.timing_reset()
     native.timing_enable(True)
     t = time.perf_counter()
     for k, lst in sets.items():
-      runner(k, lst, grain=grains[k])
+      res = runner(k, lst, grain=grains[k])
+      if label == "plain_sample16":
+        late[k] = [r.session_s for r in res]
     wall = time.perf_counter() - t
     native.timing_enable(False)
     ks = native.timing_report()
@@ -208,6 +212,18 @@ def bench_batch(args) -> dict:
                            for k_ in sorted(ks, key=lambda z: -z["ms"])[:6]]}
     log(f"batch: {label}: wall {wall:.2f}s, kernels {diag[label]['kernel_ms']:.0f} ms in "
         f"{diag[label]['launches']} launches")
+
+  # the sampled instances' plain sessions in the timed plain run and again
+  # after every session of the process (kernel timing on): equal times mean
+  # the plain baseline does not slow down with the number of sessions
+  early = {k: [got_p[k][j].session_s for j in pick[k]] for k in kinds}
+  ratios = [b / a for k in kinds for a, b in zip(early[k], late[k]) if a > 0]
+  diag["plain_sessions"] = {
+    "early_s": {k: [round(x, 4) for x in v] for k, v in early.items()},
+    "late_s": {k: [round(x, 4) for x in v] for k, v in late.items()},
+    "median_late_over_early": round(float(np.median(ratios)), 3) if ratios else None}
+  log(f"batch: plain sessions of the sample, late/early median "
+      f"{diag['plain_sessions']['median_late_over_early']}")
 
   mismatch = 0
   tuples = 0

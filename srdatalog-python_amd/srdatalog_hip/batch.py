@@ -60,6 +60,7 @@ class Instance:
 class Result:
   rows: np.ndarray           # (t, arity) u32, sorted, instance-local ids
   deltas: list[int]          # per-iteration |DELTA| of the output relation
+  session_s: float = 0.0     # run_plain: wall time of the instance's session (init to shutdown)
 
 
 # ------------------------------------------------------------- estimates ----
@@ -188,6 +189,7 @@ def run_plain(kind: str, instances: list[Instance], grain: int = 0) -> list[Resu
   lib = programs.load(prog)
   out = []
   for inst in instances:
+    t0 = time.perf_counter()
     if lib.init() != 0:
       raise RuntimeError("srdatalog_init failed")
     try:
@@ -196,9 +198,11 @@ def run_plain(kind: str, instances: list[Instance], grain: int = 0) -> list[Resu
       lib.load_rows(rin, inst.edges)
       if lib.run(0) != 0:
         raise RuntimeError(f"srdatalog_run failed for a {kind} instance (n={inst.n})")
-      out.append(Result(lib.export(rout, arity), lib.delta_sizes(rout)))
+      res = Result(lib.export(rout, arity), lib.delta_sizes(rout))
     finally:
       lib.shutdown()
+    res.session_s = time.perf_counter() - t0
+    out.append(res)
   return out
 
 
