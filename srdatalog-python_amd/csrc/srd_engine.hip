@@ -293,6 +293,11 @@ struct srd_engine {
   uint64_t slices_per_wave = 48;   // SRDATALOG_SLICES_PER_WAVE: cap of the weighed slices
   uint64_t min_slices_per_wave = 4;  // SRDATALOG_MIN_SLICES_PER_WAVE: floor of the weighed slices
   uint32_t xcd_ranges = SRD_XCDS;  // SRDATALOG_XCD_RANGES: slice ranges of the join launches
+  // SRDATALOG_WINDOW_MIN: a nested column join narrows an item's driving
+  // range to the values the other sources can match (between the largest
+  // first and the smallest last value of their ranges) when the item has at
+  // least this many candidates, two searches per item (0: never)
+  uint32_t window_min = 64;
   uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
   // column-1 bitmaps of heavy keys for views nested levels search
   uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
@@ -816,6 +821,7 @@ static void pr_views(srd_engine& e, PipeRun& R, uint64_t iteration) {
     R.a.views.v[v] = x->view();
   }
   R.a.n_ranges = e.xcd_ranges;
+  R.a.window_min = e.window_min;
   R.a.grain = e.grain_override ? e.grain_override : d->default_grain;
   if (d->root_kind == SRD_ROOT_JOIN) {
     uint32_t drv = d->root_view[0];
@@ -1814,6 +1820,7 @@ static void devloop_alloc(srd_engine& e, const Step& st, DevLoop& L, const DevCa
     P.a.grain = e.grain_override ? e.grain_override : d->default_grain;
     P.a.grid_blocks = 2048;
     P.a.n_ranges = e.xcd_ranges;
+    P.a.window_min = e.window_min;
     P.a.key_rng = P.key_rng.p;
     P.a.key_aux = P.key_aux.p;
     P.a.key_work = P.key_work.p;
@@ -2746,6 +2753,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     if (const char* c = std::getenv("SRDATALOG_NO_WEIGH")) e->no_weigh = *c && *c != '0';
     if (const char* c = std::getenv("SRDATALOG_XCD_RANGES"))
       e->xcd_ranges = std::min<uint32_t>(std::max<uint32_t>(uint32_t(std::atoi(c)), 1), SRD_XCDS);
+    if (const char* c = std::getenv("SRDATALOG_WINDOW_MIN"))
+      e->window_min = uint32_t(std::strtoul(c, nullptr, 10));
     if (const char* c = std::getenv("SRDATALOG_WEIGH_RANGES"))
       e->weigh_ranges = std::min<uint32_t>(std::max<uint32_t>(uint32_t(std::atoi(c)), 1), SRD_XCDS);
     if (const char* c = std::getenv("SRDATALOG_BITMAP_MIN"))

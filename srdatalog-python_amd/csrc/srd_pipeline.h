@@ -107,7 +107,8 @@ struct SrdLaunchArgs {
   uint64_t* item_cost;         // weigh mode output, one entry per item
   uint64_t slice_lo;           // the launch takes slices [slice_lo, n_slices)
   uint32_t n_ranges;           // slice ranges, 1..SRD_XCDS (0 is taken as 1)
-  uint32_t pad2_;
+  uint32_t window_min;         // nested column joins narrow the driving range of items with
+                               // at least this many candidates (0: never)
   // probe runs only (else nullptr): the root launch and the probe mode add
   // their lower/upper-bound searches, root-map lookups and bitmap tests
   unsigned long long* probe_ctr;

@@ -210,6 +210,8 @@ def test_csv_load_all(progs, tmp_path):
 # SRDATALOG_NO_WEIGH=1 keeps uniform slices everywhere; SRDATALOG_BITMAP_BYTES
 # bounds the heavy-key bitmaps (0: none; a small budget keeps only the
 # longest keys' bitmaps, so bitmap probes and searches mix in one wave);
+# SRDATALOG_WINDOW_MIN=1 narrows the driving range of every nested item to
+# the other sources' value window (0: of none);
 # SRDATALOG_ROOT_POS=0 looks nested-level keys up by searching the root
 # values instead of through the dense root map.  Every variant but the
 # default ones runs recursive steps in the host-driven loop
@@ -233,6 +235,8 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_BITMAP_BYTES": "0"}, "no-bitmaps"),
          ({"SRDATALOG_BITMAP_BYTES": "20000"}, "few-bitmaps"),
          ({"SRDATALOG_ROOT_POS": "0"}, "root-search"),
+         ({"SRDATALOG_WINDOW_MIN": "1"}, "window-all"),
+         ({"SRDATALOG_WINDOW_MIN": "0"}, "no-window"),
          ({"SRDATALOG_DEVLOOP": "1", "SRDATALOG_DEVLOOP_CAPS": "500"}, "devloop-small-caps"),
          ({"SRDATALOG_GROUP_STREAMS": "0"}, "group-in-order"),
          ({"SRDATALOG_NEW_CHUNK_ROWS": "3000", "SRDATALOG_JOIN_MODE": "fused",
@@ -245,7 +249,7 @@ def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
   for k in ("SRDATALOG_JOIN_MODE", "SRDATALOG_STAGE_CAP", "SRDATALOG_NO_SEGMENTS",
             "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN", "SRDATALOG_WEIGH_MIN",
             "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH", "SRDATALOG_BITMAP_BYTES",
-            "SRDATALOG_ROOT_POS", "SRDATALOG_DEVLOOP", "SRDATALOG_DEVLOOP_CAPS",
+            "SRDATALOG_ROOT_POS", "SRDATALOG_WINDOW_MIN", "SRDATALOG_DEVLOOP", "SRDATALOG_DEVLOOP_CAPS",
             "SRDATALOG_GROUP_STREAMS"):
     monkeypatch.delenv(k, raising=False)
   # the join modes of recursive rules are those of the host-driven loop (the
