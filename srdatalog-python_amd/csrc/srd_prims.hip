@@ -476,6 +476,28 @@ __global__ __launch_bounds__(kBlock) void heavy_set_kernel(const uint32_t* __res
   }
 }
 
+// Heavy key i's bitmap built by block i in LDS (zeroed there, one LDS atomic
+// OR per row, written out whole once): no fill of the bitmaps beforehand and
+// no global atomics.  Taken when a bitmap fits kHeavyLdsWords words (column-1
+// values below 1,179,648); wider domains take heavy_set_kernel.
+constexpr uint32_t kHeavyLdsWords = 36864;  // 144 KiB of the CU's 160 KiB
+constexpr int kHeavyLdsBlock = 1024;
+__global__ __launch_bounds__(kHeavyLdsBlock) void heavy_set_lds_kernel(
+    const uint32_t* __restrict__ col1, const uint32_t* __restrict__ starts,
+    const uint32_t* __restrict__ ends, uint32_t words, uint32_t* __restrict__ bits) {
+  __shared__ uint32_t bm[kHeavyLdsWords];
+  for (uint32_t w = threadIdx.x; w < words; w += kHeavyLdsBlock) bm[w] = 0;
+  __syncthreads();
+  const uint32_t e = ends[blockIdx.x];
+  for (uint32_t row = starts[blockIdx.x] + threadIdx.x; row < e; row += kHeavyLdsBlock) {
+    const uint32_t v = col1[row];
+    atomicOr(&bm[v >> 5], 1u << (v & 31u));
+  }
+  __syncthreads();
+  uint32_t* out = bits + uint64_t(blockIdx.x) * words;
+  for (uint32_t w = threadIdx.x; w < words; w += kHeavyLdsBlock) out[w] = bm[w];
+}
+
 void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, Workspace& ws,
                          hipStream_t s) {
   if (!x.heavy) x.heavy = std::make_shared<HeavyBits>();
@@ -504,9 +526,14 @@ void build_heavy_bitmaps(const Index& x, uint32_t min_len, uint64_t max_bytes, W
           x.root_offs.p, x.r, flag.p, pos.p, h.starts.p, ends.p);
       SRD_CHECK(hipGetLastError());
       h.bits.alloc(uint64_t(n) * words, s);
-      fill_u32(h.bits.p, 0, uint64_t(n) * words, s);
-      heavy_set_kernel<<<n * kHeavyParts, kBlock, 0, s>>>(x.col(1), h.starts.p, ends.p, words,
-                                                           h.bits.p);
+      if (words <= kHeavyLdsWords) {
+        heavy_set_lds_kernel<<<n, kHeavyLdsBlock, 0, s>>>(x.col(1), h.starts.p, ends.p, words,
+                                                          h.bits.p);
+      } else {
+        fill_u32(h.bits.p, 0, uint64_t(n) * words, s);
+        heavy_set_kernel<<<n * kHeavyParts, kBlock, 0, s>>>(x.col(1), h.starts.p, ends.p, words,
+                                                             h.bits.p);
+      }
       SRD_CHECK(hipGetLastError());
       h.n = n;
       h.words = words;

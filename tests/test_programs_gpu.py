@@ -298,6 +298,22 @@ def test_triangle_many_bitmaps(progs, env, label, monkeypatch):
     lib.shutdown()
 
 
+def test_triangle_wide_value_bitmaps(progs):
+  """Bitmaps over column-1 values up to 2.5 M: a bitmap no longer fits the
+  LDS of the block that builds it (values below 1,179,648 do), so the keys'
+  bits are set by global atomics instead.  The scale-12 graph relabelled
+  x -> 600 x keeps its edges oriented and its 11 keys of 512+ rows."""
+  e = ds.triangle_edges(4, 12, 80_000).astype(np.uint64) * 600
+  inputs = {"KnowsInput": e.astype(np.uint32)}
+  assert int(inputs["KnowsInput"].max()) > 1_179_648
+  orc = oracle.run("triangle", inputs)
+  lib = run_prog(progs, "triangle", inputs)
+  try:
+    check_rel(lib, orc, "Triangle", 3)
+  finally:
+    lib.shutdown()
+
+
 # --------------------------------------------------------------- negation ---
 def _neg2hop_inputs(seed, n, m, tags, mi):
   rng = np.random.default_rng(seed)
