@@ -36,6 +36,7 @@
 #include <functional>
 #include <iostream>
 #include <map>
+#include <set>
 #include <memory>
 #include <string>
 #include <vector>
@@ -231,6 +232,17 @@ struct Relation {
   uint64_t export_version = ~uint64_t(0);
   uint64_t count = 0;               // count-only result
   bool counted = false;
+  // NEW as a bitmap over the relation's key space while this iteration's
+  // pipelines mark it (their mode 6): tuple (v_0 .. v_k-1) in dense_order is
+  // bit sum(v_c << dense_shift*(k-1-c)).  new_index, or a pipeline that
+  // writes rows instead, turns the set bits into a NEW part; the bitmap is
+  // kept (clear) for the next iteration.
+  DevBuf<uint32_t> dense_bits;
+  uint64_t dense_words = 0;
+  uint32_t dense_shift = 0;
+  uint32_t dense_bound = 0;
+  bool dense_on = false;
+  Order dense_order;
 };
 
 // Device scratch of one pipeline run (root keys, slices, fused staging,
@@ -298,6 +310,11 @@ struct srd_engine {
   // first and the smallest last value of their ranges) when the item has at
   // least this many candidates, two searches per item (0: never)
   uint32_t window_min = 64;
+  // SRDATALOG_DENSE_MAX_BITS: a destination whose key space (arity x the bit
+  // width of the largest value the engine holds) has at most this many keys
+  // takes its NEW as a bitmap the joins mark, instead of raw rows sorted to
+  // remove repeats (0: never).  2^33 keys are 1 GiB of HBM per relation.
+  uint64_t dense_max_bits = uint64_t(1) << 33;
   uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
   // column-1 bitmaps of heavy keys for views nested levels search
   uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
@@ -589,10 +606,12 @@ static const Index& delta_index(srd_engine& e, Relation& r, const Order& order) 
 }
 
 static void union_into(srd_engine& e, Index& a, const Index& b);
+static void dense_flush(srd_engine& e, Relation& r);
 
 static const Index& new_index(srd_engine& e, Relation& r, const Order& order) {
   auto it = r.newv.find(order);
   if (it != r.newv.end()) return it->second;
+  dense_flush(e, r);  // marked tuples join the NEW parts
   Index x;
   if (!r.new_parts.empty()) {
     // chunked pipelines' parts, united with the rows other pipelines wrote
@@ -657,6 +676,10 @@ static uint64_t reserve_new(srd_engine& e, Relation& r, const Order& order, uint
 }
 
 static void clear_new(Relation& r) {
+  if (r.dense_on) {  // marks nobody read: clear them for the next iteration
+    fill_u32(r.dense_bits.p, 0, r.dense_words, r.dense_bits.s);
+    r.dense_on = false;
+  }
   r.newv.clear();
   r.raw.release();
   r.raw_n = r.raw_cap = 0;
@@ -690,6 +713,24 @@ static void add_new_part(srd_engine& e, Relation& r, const Order& order, Index&&
     union_into(e, older, r.new_parts.back());
     r.new_parts.pop_back();
   }
+}
+
+// The marked tuples of a dense NEW become a NEW part (already in order and
+// without repeats: the index adopts them); the bitmap is left clear.
+static void dense_flush(srd_engine& e, Relation& r) {
+  if (!r.dense_on) return;
+  r.dense_on = false;
+  DevBuf<uint32_t> buf;
+  uint64_t stride = 0;
+  const uint64_t n =
+      dense_extract(r.dense_bits.p, r.dense_words, r.arity, r.dense_shift, buf, stride, e.ws, e.s);
+  if (n == 0) return;
+  std::vector<const uint32_t*> in(r.arity);
+  for (uint32_t c = 0; c < r.arity; ++c) in[c] = buf.p + uint64_t(c) * stride;
+  const std::vector<uint32_t> bound(r.arity, r.dense_bound);
+  Index part;
+  build_index(in.data(), r.arity, n, part, e.ws, e.s, &bound, &buf, stride, /*presorted=*/true);
+  add_new_part(e, r, r.dense_order, std::move(part));
 }
 
 // Turn the relation's raw NEW rows into a deduplicated NEW part, so that
@@ -741,7 +782,8 @@ struct PipeRun {
   uint64_t chunk_rows = 0;
   uint64_t rec_cap = 0;
   uint64_t weigh_items = 0;   // > 0: slices are being weighed
-  int mode = 0;               // 0 nothing to run, 1 direct, 2 count (+ write), 3 fused
+  bool dense = false;         // the destination's NEW is a bitmap: the traversal marks it
+  int mode = 0;               // 0 nothing to run, 1 direct, 2 count (+ write), 3 fused, 4 mark
   bool ovf = false;
 };
 
@@ -947,11 +989,18 @@ static void pr_traverse(srd_engine& e, PipeRun& R) {
   R.slices = R.a.n_slices;
   R.mode = 0;
   if (R.slices == 0) return;
+  R.a.grid_blocks = uint32_t(std::min<uint64_t>((R.slices + 3) / 4, 2048));
+  if (R.dense) {
+    // every derived tuple sets its bit in the destination's bitmap: nothing
+    // to count, size or write
+    R.mode = 4;
+    pr_launch(e, R, d->mark, "mark");
+    return;
+  }
   x.counts_buf.reserve(R.slices, R.s);
   x.offsets.reserve(R.slices + 1, R.s);
   R.a.counts = x.counts_buf.p;
   R.a.offsets = x.offsets.p;
-  R.a.grid_blocks = uint32_t(std::min<uint64_t>((R.slices + 3) / 4, 2048));
   // Rows of the write (direct) pass, materialised at once when they fit
   // one chunk, else in slice ranges whose rows are each sorted and
   // deduplicated into a NEW part (a join's output before deduplication
@@ -1022,6 +1071,7 @@ static void pr_bind(srd_engine& e, PipeRun& R, uint64_t n_rows) {
   for (uint32_t k = 0; k < d->n_dest; ++k) {
     const SrdDestSpec& ds = d->dest[k];
     Relation& r = e.rels[rel_id(e, ds.rel)];
+    dense_flush(e, r);  // rows marked by earlier pipelines become a NEW part
     if (r.raw_n > 0 && r.raw_n + n_rows > R.chunk_rows) flush_raw(e, r);
     // rows of one sorted-output pipeline into an empty NEW stay an index;
     // bounds: the largest value of any column the join reads
@@ -1164,6 +1214,97 @@ static void pr_fence(srd_engine& e, hipStream_t from, const std::vector<PipeRun>
   }
 }
 
+// Largest value of any FULL or DELTA index or input of the engine (index
+// maxima are upper bounds): no join of the current iteration reads a value
+// beyond it, save from a NEW view, whose maxima the callers add.
+static uint32_t value_bound(const srd_engine& e) {
+  uint32_t b = 0;
+  auto take = [&](const std::vector<uint32_t>& mx) {
+    for (uint32_t m : mx) b = std::max(b, m);
+  };
+  for (const auto& r : e.rels) {
+    for (const auto& kv : r.full) take(kv.second.idx.col_max);
+    for (const auto& sg : r.segs) take(sg.col_max);
+    for (const auto& kv : r.delta) take(kv.second.col_max);
+    take(r.input_max);
+  }
+  return b;
+}
+
+// After the runs' views are prepared and before any of them launches: the
+// destinations whose key space fits dense_max_bits take their NEW as a
+// bitmap that the runs writing them mark (mode 6).  A destination some run
+// writes rows into (a rule with several destinations, constants in its
+// head, count-only or sorted output) keeps raw rows for all of them.  The
+// key layout covers every value the runs can emit -- the engine's value
+// bound and their views' maxima -- and a bitmap marked earlier in the
+// iteration under a narrower layout is first turned into a NEW part.
+static void pr_dense(srd_engine& e, std::vector<PipeRun*>& runs) {
+  if (!e.dense_max_bits) return;
+  std::set<int> rows;
+  std::map<int, uint32_t> need;
+  for (PipeRun* R : runs) {
+    const SrdPipelineDesc* d = R->d;
+    const bool can = d->mark && d->n_dest == 1 && !d->count_only && !d->sorted_output;
+    for (uint32_t k = 0; k < d->n_dest; ++k) {
+      const int ri = rel_id(e, d->dest[k].rel);
+      if (!can) {
+        rows.insert(ri);
+        continue;
+      }
+      uint32_t b = 0;
+      for (uint32_t v = 0; v < d->n_views; ++v)
+        for (uint32_t m : R->view_max[v]) b = std::max(b, m);
+      need[ri] = std::max(need[ri], b);
+    }
+  }
+  if (need.empty()) return;
+  const uint32_t gb = value_bound(e);
+  for (auto& kv : need) {
+    if (rows.count(kv.first)) continue;
+    Relation& r = e.rels[kv.first];
+    const uint32_t bound = std::max(kv.second, gb);
+    const uint32_t sh = std::max<uint32_t>(32 - __builtin_clz(bound | 1u), 1);
+    const uint64_t bits = uint64_t(sh) * r.arity;
+    if (r.dense_on && r.dense_shift < sh) dense_flush(e, r);
+    if (!r.dense_on) {
+      if (bits > 40 || (uint64_t(1) << bits) > e.dense_max_bits) continue;
+      // rows already written this iteration stay (they join the parts);
+      // an index already built from them does not take more
+      if (!r.newv.empty()) continue;
+      const SrdDestSpec* ds = nullptr;
+      for (PipeRun* R : runs)
+        if (R->d->n_dest == 1 && rel_id(e, R->d->dest[0].rel) == kv.first) ds = &R->d->dest[0];
+      const Order order(ds->cols, ds->cols + ds->arity);
+      if (!r.new_parts.empty() && r.new_parts_order != order) continue;
+      if (r.raw_n > 0 && r.raw_order != order) continue;
+      const uint64_t words =
+          std::max<uint64_t>(((uint64_t(1) << bits) / 32 + kDenseBlockWords - 1) / kDenseBlockWords,
+                             1) * kDenseBlockWords;
+      if (r.dense_bits.n < words) {
+        r.dense_bits.alloc(words, e.s);
+        fill_u32(r.dense_bits.p, 0, words, e.s);
+      }
+      r.dense_words = words;
+      r.dense_shift = sh;
+      r.dense_bound = bound;
+      r.dense_order = order;
+      r.dense_on = true;
+      if (e.trace) {
+        std::fprintf(stderr, "[srd]   %s: NEW as a bitmap of 2^%llu keys (%u bits per column)\n",
+                     r.name.c_str(), (unsigned long long)bits, sh);
+        std::fflush(stderr);
+      }
+    }
+    for (PipeRun* R : runs)
+      if (R->d->mark && R->d->n_dest == 1 && rel_id(e, R->d->dest[0].rel) == kv.first) {
+        R->dense = true;
+        R->a.mark_bits = r.dense_bits.p;
+        R->a.mark_shift = r.dense_shift;
+      }
+  }
+}
+
 // Run pipelines -- one, or the members of a parallel group -- phase by
 // phase.  A group's members each get a stream and scratch of their own (a
 // member counting probes or needing the chunked or overflow path finishes
@@ -1204,6 +1345,10 @@ static void run_pipelines(srd_engine& e, const std::vector<const SrdPipelineDesc
       R.s = e.s;
       R.host = pins;
       pr_views(e, R, iteration);
+      {
+        std::vector<PipeRun*> me{&R};
+        pr_dense(e, me);
+      }
       if (R.a.n_root > 0) {
         pr_root(e, R);
         pr_wait(one);
@@ -1211,7 +1356,7 @@ static void run_pipelines(srd_engine& e, const std::vector<const SrdPipelineDesc
         if (R.weigh_items) pr_wait(one);
         pr_weighed(e, R);
         pr_traverse(e, R);
-        if (R.mode >= 2) pr_wait(one);
+        if (R.mode == 2 || R.mode == 3) pr_wait(one);
         if (pr_sized(e, R)) {
           pr_bind(e, R, R.total);
           pr_write(e, R);
@@ -1237,7 +1382,13 @@ static void run_pipelines(srd_engine& e, const std::vector<const SrdPipelineDesc
     runs[i].host = pins + 8 * i;
     pr_views(e, runs[i], iteration);
   }
-  // every member reads indexes built on the engine's stream
+  {
+    std::vector<PipeRun*> all;
+    for (auto& R : runs) all.push_back(&R);
+    pr_dense(e, all);
+  }
+  // every member reads indexes built on the engine's stream (and the
+  // bitmaps cleared there)
   pr_fence(e, e.s, runs, true);
   std::vector<PipeRun*> live;
   for (auto& R : runs)
@@ -2753,6 +2904,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     if (const char* c = std::getenv("SRDATALOG_NO_WEIGH")) e->no_weigh = *c && *c != '0';
     if (const char* c = std::getenv("SRDATALOG_XCD_RANGES"))
       e->xcd_ranges = std::min<uint32_t>(std::max<uint32_t>(uint32_t(std::atoi(c)), 1), SRD_XCDS);
+    if (const char* c = std::getenv("SRDATALOG_DENSE_MAX_BITS"))
+      e->dense_max_bits = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_WINDOW_MIN"))
       e->window_min = uint32_t(std::strtoul(c, nullptr, 10));
     if (const char* c = std::getenv("SRDATALOG_WEIGH_RANGES"))

@@ -112,6 +112,12 @@ struct SrdLaunchArgs {
   // probe runs only (else nullptr): the root launch and the probe mode add
   // their lower/upper-bound searches, root-map lookups and bitmap tests
   unsigned long long* probe_ctr;
+  // mark mode (6) only: the destination's NEW as a dense bitmap over its
+  // key space, a tuple (v0 .. vk-1) in index order at bit
+  // sum(v_c << mark_shift * (k-1-c))
+  uint32_t* mark_bits;
+  uint32_t mark_shift;
+  uint32_t mark_pad_;
   // device-sized launches (the *_dyn entry points) only: views, key and
   // slice counts and output bases are read from this device struct, which
   // kernels of the same iteration fill in, instead of the fields above
@@ -187,6 +193,10 @@ struct SrdPipelineDesc {
   SrdLaunchFn count_dyn;
   SrdLaunchFn write_dyn;
   SrdLaunchFn direct_dyn;
+  // mode 6: set the bit of every derived tuple in the destination's dense
+  // bitmap instead of writing rows; nullptr unless the rule has a single
+  // destination whose terms are all variables
+  SrdLaunchFn mark;
 };
 
 #ifdef __cplusplus

@@ -314,6 +314,98 @@ void gather_segments(const uint64_t* recs, const uint32_t* rec_ctr, uint64_t rec
   SRD_CHECK(hipGetLastError());
 }
 
+// ===================================================== dense bitmaps ====
+// Block b covers words [b * kDenseBlockWords, (b + 1) * kDenseBlockWords),
+// thread t of it the 32 consecutive words at t * 32, so a thread's bits, and
+// the block's, are consecutive keys.
+constexpr uint32_t kDenseThreadWords = uint32_t(kDenseBlockWords / kBlock);
+static_assert(kDenseThreadWords == 32, "eight 16-byte loads per thread");
+
+__device__ __forceinline__ void dense_load(const uint32_t* bits, uint64_t w0, uint32_t (&v)[32]) {
+  const uint4* p = reinterpret_cast<const uint4*>(bits + w0);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint4 q = p[i];
+    v[4 * i] = q.x;
+    v[4 * i + 1] = q.y;
+    v[4 * i + 2] = q.z;
+    v[4 * i + 3] = q.w;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void dense_count_kernel(const uint32_t* __restrict__ bits,
+                                                             uint32_t* __restrict__ blk) {
+  __shared__ uint32_t wsum[kWavesPerBlock];
+  uint32_t v[32];
+  dense_load(bits, blockIdx.x * kDenseBlockWords + uint64_t(threadIdx.x) * kDenseThreadWords, v);
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) c += __popc(v[i]);
+  uint32_t tot;
+  block_excl_scan<uint32_t>(c, wsum, &tot);
+  if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+// Writes the tuples of the block's set bits at its scanned offset (thread
+// by thread, in key order) and clears the words it read.
+__global__ __launch_bounds__(kBlock) void dense_emit_kernel(uint32_t* __restrict__ bits,
+                                                            const uint64_t* __restrict__ blk_off,
+                                                            uint32_t k, uint32_t shift,
+                                                            uint32_t* __restrict__ out,
+                                                            uint64_t stride) {
+  __shared__ uint32_t wsum[kWavesPerBlock];
+  const uint64_t w0 = blockIdx.x * kDenseBlockWords + uint64_t(threadIdx.x) * kDenseThreadWords;
+  uint32_t v[32];
+  dense_load(bits, w0, v);
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) c += __popc(v[i]);
+  uint32_t tot;
+  uint64_t pos = blk_off[blockIdx.x] + block_excl_scan<uint32_t>(c, wsum, &tot);
+  if (c == 0) return;
+  const uint64_t mask = (uint64_t(1) << shift) - 1;
+  for (int i = 0; i < 32; ++i) {
+    uint32_t x = v[i];
+    if (!x) continue;
+    bits[w0 + i] = 0;
+    while (x) {
+      const uint64_t key = ((w0 + uint64_t(i)) << 5) | uint64_t(__builtin_ctz(x));
+      x &= x - 1;
+      for (uint32_t col = 0; col < k; ++col)
+        out[uint64_t(col) * stride + pos] = uint32_t((key >> (shift * (k - 1 - col))) & mask);
+      ++pos;
+    }
+  }
+}
+
+uint64_t dense_extract(uint32_t* bits, uint64_t words, uint32_t k, uint32_t shift,
+                       DevBuf<uint32_t>& out, uint64_t& stride, Workspace& ws, hipStream_t s) {
+  if (words == 0 || words % kDenseBlockWords) fail("dense_extract: bitmap size not whole blocks");
+  if (k == 0 || k > 4 || shift == 0 || uint64_t(shift) * k > 63)
+    fail("dense_extract: key layout out of range");
+  const uint64_t nb = words / kDenseBlockWords;
+  if (nb >= (uint64_t(1) << 31)) fail("dense_extract: bitmap too large");
+  DevBuf<uint32_t> blk(nb, s);
+  DevBuf<uint64_t> off(nb + 1, s);
+  uint64_t n = 0;
+  {
+    TimeScope t("new.dense_count", s, double(words) * 4);
+    dense_count_kernel<<<uint32_t(nb), kBlock, 0, s>>>(bits, blk.p);
+    SRD_CHECK(hipGetLastError());
+    exclusive_scan_u32(blk.p, off.p, nb, ws, s);
+    uint32_t* hw = pinned_words(ws, 2);
+    SRD_CHECK(hipMemcpyAsync(hw, off.p + nb, 8, hipMemcpyDeviceToHost, s));
+    SRD_CHECK(hipStreamSynchronize(s));
+    n = uint64_t(hw[0]) | (uint64_t(hw[1]) << 32);
+  }
+  stride = round_stride(std::max<uint64_t>(n, 1));
+  out.alloc(stride * k, s);
+  TimeScope t("new.dense_emit", s, double(words) * 4 + double(n) * 4 * k);
+  dense_emit_kernel<<<uint32_t(nb), kBlock, 0, s>>>(bits, off.p, k, shift, out.p, stride);
+  SRD_CHECK(hipGetLastError());
+  return n;
+}
+
 // ========================================================= calibration ====
 // Plain streaming copy: reads and writes exactly n words, so profiler HBM
 // counters read on it give the bytes-per-counter-unit scale.

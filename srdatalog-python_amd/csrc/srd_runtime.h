@@ -304,6 +304,18 @@ void exclusive_scan_u32(const uint32_t* in, uint64_t* out, uint64_t n, Workspace
 void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace& ws,
                         hipStream_t s);
 
+// Dense-bitmap NEW (the joins' mark mode): bit `key` of the bitmap holds
+// tuple (v_0 .. v_k-1) with key = sum(v_c << shift*(k-1-c)).  words must be
+// a multiple of kDenseBlockWords (the allocation is rounded up to it and the
+// tail stays clear).
+constexpr uint64_t kDenseBlockWords = 8192;
+// The tuples of the set bits, in key order (the index order: sorted and
+// duplicate-free), written to `out` (allocated here, column c at
+// out.p + c * stride, stride returned); the bitmap is left clear.  Returns
+// the number of tuples.
+uint64_t dense_extract(uint32_t* bits, uint64_t words, uint32_t k, uint32_t shift,
+                       DevBuf<uint32_t>& out, uint64_t& stride, Workspace& ws, hipStream_t s);
+
 // Fill n words with value (a kernel, safe under stream capture).
 void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 void fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);

@@ -172,7 +172,9 @@ class PipelineGen:
 
     MODE 0 counts, 1 writes at the slice's scanned offset, 2 appends to the
     slice's staging segment, 3 writes at the item's own row (direct), 5
-    counts the searches only (probe runs)."""
+    counts the searches only (probe runs), 6 sets the tuple's bit in the
+    destination's dense bitmap (a plain load first: most derivations of a
+    deduplicating join are repeats, whose bit is already set)."""
     lines = [f"{ind}{{",
              f"{ind}  const uint64_t m_ = srd::ballot({cond});",
              f"{ind}  const uint32_t c_ = __popcll(m_);",
@@ -222,11 +224,31 @@ class PipelineGen:
           lines.append(f"{ind}      A.stage[{di * 8 + c}ull * A.stage_cap + o_] = {self.var(d.terms[relcol])};")
       lines += [f"{ind}    }}"]
     lines += [f"{ind}    seg_used += c_;",
-              f"{ind}  }}",
-              f"{ind}  opos += c_;",
+              f"{ind}  }}"]
+    if self.mark_ok():
+      d = self.dests[0]
+      k = len(d.index)
+      parts = [f"(uint64_t({self.var(d.terms[relcol])}) << (A.mark_shift * {k - 1 - c}u))"
+               for c, relcol in enumerate(d.index)]
+      lines += [f"{ind}  if (MODE == 6 && ({cond})) {{",
+                f"{ind}    const uint64_t key_ = {' | '.join(parts)};",
+                f"{ind}    uint32_t* w_ = A.mark_bits + (key_ >> 5);",
+                f"{ind}    const uint32_t b_ = 1u << uint32_t(key_ & 31u);",
+                f"{ind}    if (!(*w_ & b_)) atomicOr(w_, b_);",
+                f"{ind}  }}"]
+    lines += [f"{ind}  opos += c_;",
               f"{ind}  cnt += c_;",
               f"{ind}}}"]
     return "\n".join(lines)
+
+  def mark_ok(self) -> bool:
+    """Mode 6 applies: one destination whose terms are all variables (every
+    value then comes from a source column, so the engine can bound the key
+    space), written by a materialising rule."""
+    if len(self.dests) != 1 or self.ep.count:
+      return False
+    d = self.dests[0]
+    return 1 <= len(d.index) <= 4 and all(isinstance(d.terms[c], str) for c in d.index)
 
   def direct_ok(self) -> bool:
     """Every level-1 item yields exactly one tuple: no filters, and either a
@@ -1056,6 +1078,9 @@ class PipelineGen:
            "void launch_probe(const SrdLaunchArgs* a, hipStream_t s) {",
            "  run_k<5, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}"] + ([
+           "void launch_mark(const SrdLaunchArgs* a, hipStream_t s) {",
+           "  run_k<6, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
+           "}"] if self.mark_ok() else []) + ([
            "void launch_weigh(const SrdLaunchArgs* a, hipStream_t s) {",
            "  run_k<4, false><<<a->grid_blocks, 256, 0, s>>>(*a);",
            "}"] if len(self.levels) >= 2 else []) + ([
@@ -1103,7 +1128,8 @@ class PipelineGen:
             + (f"&{self.ns}::launch_root_dyn, &{self.ns}::launch_count_dyn, "
                f"&{self.ns}::launch_write_dyn, "
                + (f"&{self.ns}::launch_direct_dyn" if self.direct_ok() else "nullptr")
-               if self.dyn else "nullptr, nullptr, nullptr, nullptr") + "}")
+               if self.dyn else "nullptr, nullptr, nullptr, nullptr")
+            + (f", &{self.ns}::launch_mark" if self.mark_ok() else ", nullptr") + "}")
 
 
 @dataclass

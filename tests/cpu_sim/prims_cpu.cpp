@@ -146,6 +146,31 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace
   for (uint64_t i = 0; i < n; ++i) out[i] = t, t += in[i];
   out[n] = t;
 }
+uint64_t dense_extract(uint32_t* bits, uint64_t words, uint32_t k, uint32_t shift,
+                       DevBuf<uint32_t>& out, uint64_t& stride, Workspace&, hipStream_t s) {
+  if (words == 0 || words % kDenseBlockWords) fail("dense_extract: bitmap size not whole blocks");
+  if (k == 0 || k > 4 || shift == 0 || uint64_t(shift) * k > 63)
+    fail("dense_extract: key layout out of range");
+  uint64_t n = 0;
+  for (uint64_t w = 0; w < words; ++w) n += uint64_t(__builtin_popcount(bits[w]));
+  stride = round_stride(std::max<uint64_t>(n, 1));
+  out.alloc(stride * k, s);
+  const uint64_t mask = (uint64_t(1) << shift) - 1;
+  uint64_t pos = 0;
+  for (uint64_t w = 0; w < words; ++w) {
+    uint32_t x = bits[w];
+    bits[w] = 0;
+    while (x) {
+      const uint64_t key = (w << 5) | uint64_t(__builtin_ctz(x));
+      x &= x - 1;
+      for (uint32_t c = 0; c < k; ++c)
+        out.p[uint64_t(c) * stride + pos] = uint32_t((key >> (shift * (k - 1 - c))) & mask);
+      ++pos;
+    }
+  }
+  return n;
+}
+
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
 void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t) { std::fill(p, p + n, v); }
 

@@ -39,7 +39,7 @@ def test_generated_desc_carries_the_flags():
   # weigh launcher, sorted_output = 1, probe launcher; a non-recursive rule
   # has no device-sized launches
   assert line.rstrip().endswith("launch_weigh, 1, &srdp_1_TriangleJoin::launch_probe, "
-                                "nullptr, nullptr, nullptr, nullptr}")
+                                "nullptr, nullptr, nullptr, nullptr, &srdp_1_TriangleJoin::launch_mark}")
   # want_bitmaps, want_root_pos
   assert '{"Knows", 2, {0, 1}, SRD_FULL, 1, 1}' in line
 
@@ -51,8 +51,23 @@ def test_recursive_rules_get_device_sized_launches():
   src = codegen.generate_program_source("tc", compiler.relation_decls(prog),
                                         compiler.compile_to_mir(prog))
   rec = next(l for l in src.splitlines() if l.lstrip().startswith('{"TCRec'))
-  assert rec.rstrip().rstrip(",").endswith("::launch_direct_dyn}")
+  assert rec.rstrip().rstrip(",").endswith("::launch_mark}")
+  assert "::launch_direct_dyn, &srdp_" in rec
   assert "launch_root_dyn" in rec and "launch_count_dyn" in rec and "launch_write_dyn" in rec
   base = next(l for l in src.splitlines() if l.lstrip().startswith('{"TCBase'))
-  assert base.rstrip().rstrip(",").endswith("nullptr, nullptr, nullptr, nullptr}")
+  assert base.rstrip().rstrip(",").endswith("nullptr, nullptr, nullptr, nullptr, &srdp_1_TCBase::launch_mark}")
+
+
+def test_mark_mode_needs_one_all_variable_destination():
+  """Mode 6 (dense-bitmap NEW) is generated for rules with one destination
+  whose terms are all variables, keyed in the destination's index order."""
+  prog = programs.PROGRAMS["andersen"]()
+  gs = gens(prog)
+  assert all(g.mark_ok() for g in gs.values())
+  src = codegen.generate_program_source("andersen", compiler.relation_decls(prog),
+                                        compiler.compile_to_mir(prog))
+  assert "run_k<6, false>" in src and "A.mark_bits + (key_ >> 5)" in src
+  # count-only rules write nothing
+  cnt = gens(programs.PROGRAMS["triangle_count"]())
+  assert not cnt["TriangleJoin"].mark_ok()
   assert "run_k<3, true>" in src and "root_k<true>" in src

@@ -210,6 +210,12 @@ def test_csv_load_all(progs, tmp_path):
 # SRDATALOG_NO_WEIGH=1 keeps uniform slices everywhere; SRDATALOG_BITMAP_BYTES
 # bounds the heavy-key bitmaps (0: none; a small budget keeps only the
 # longest keys' bitmaps, so bitmap probes and searches mix in one wave);
+# Rules whose destination's key space is small take their NEW as a bitmap
+# the joins mark (default; SRDATALOG_DENSE_MAX_BITS=0: never), so every
+# other mode turns that off to keep its own row-writing path under test;
+# dense-host runs the default with the host-driven loop (TC and SG mark
+# there too) and dense-groups-in-order with the parallel group's members
+# marking one after the other.
 # SRDATALOG_WINDOW_MIN=1 narrows the driving range of every nested item to
 # the other sources' value window (0: of none);
 # SRDATALOG_ROOT_POS=0 looks nested-level keys up by searching the root
@@ -241,6 +247,8 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
          ({"SRDATALOG_GROUP_STREAMS": "0"}, "group-in-order"),
          ({"SRDATALOG_NEW_CHUNK_ROWS": "3000", "SRDATALOG_JOIN_MODE": "fused",
            "SRDATALOG_STAGE_CAP": "5000"}, "group-chunked-overflow"),
+         ({"SRDATALOG_DEVLOOP": "0"}, "dense-host"),
+         ({"SRDATALOG_DEVLOOP": "0", "SRDATALOG_GROUP_STREAMS": "0"}, "dense-groups-in-order"),
          ({}, "default")]
 
 
@@ -250,8 +258,10 @@ def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
             "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN", "SRDATALOG_WEIGH_MIN",
             "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH", "SRDATALOG_BITMAP_BYTES",
             "SRDATALOG_ROOT_POS", "SRDATALOG_WINDOW_MIN", "SRDATALOG_DEVLOOP", "SRDATALOG_DEVLOOP_CAPS",
-            "SRDATALOG_GROUP_STREAMS"):
+            "SRDATALOG_GROUP_STREAMS", "SRDATALOG_DENSE_MAX_BITS"):
     monkeypatch.delenv(k, raising=False)
+  if not label.startswith(("dense", "default")):
+    monkeypatch.setenv("SRDATALOG_DENSE_MAX_BITS", "0")
   # the join modes of recursive rules are those of the host-driven loop (the
   # device-resident one has its own variants)
   if label not in ("default",) and "SRDATALOG_DEVLOOP" not in env:
