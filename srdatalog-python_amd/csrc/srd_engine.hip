@@ -238,6 +238,7 @@ struct Relation {
   // writes rows instead, turns the set bits into a NEW part; the bitmap is
   // kept (clear) for the next iteration.
   DevBuf<uint32_t> dense_bits;
+  DevBuf<uint32_t> dense_sum;  // a bit per block of the bitmap holding a set bit
   uint64_t dense_words = 0;
   uint32_t dense_shift = 0;
   uint32_t dense_bound = 0;
@@ -313,8 +314,12 @@ struct srd_engine {
   // SRDATALOG_DENSE_MAX_BITS: a destination whose key space (arity x the bit
   // width of the largest value the engine holds) has at most this many keys
   // takes its NEW as a bitmap the joins mark, instead of raw rows sorted to
-  // remove repeats (0: never).  2^33 keys are 1 GiB of HBM per relation.
-  uint64_t dense_max_bits = uint64_t(1) << 33;
+  // remove repeats (0: never).  2^36 keys are 8 GiB of HBM per relation,
+  // cleared once per engine; extraction reads only the blocks of 2^18 keys
+  // that were marked.  All of an engine's bitmaps together stay within an
+  // eighth of the device's memory.
+  uint64_t dense_max_bits = uint64_t(1) << 36;
+  uint64_t dense_bytes = 0;        // bitmaps allocated so far
   uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
   // column-1 bitmaps of heavy keys for views nested levels search
   uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
@@ -678,6 +683,7 @@ static uint64_t reserve_new(srd_engine& e, Relation& r, const Order& order, uint
 static void clear_new(Relation& r) {
   if (r.dense_on) {  // marks nobody read: clear them for the next iteration
     fill_u32(r.dense_bits.p, 0, r.dense_words, r.dense_bits.s);
+    fill_u32(r.dense_sum.p, 0, dense_sum_words(r.dense_words), r.dense_sum.s);
     r.dense_on = false;
   }
   r.newv.clear();
@@ -722,8 +728,8 @@ static void dense_flush(srd_engine& e, Relation& r) {
   r.dense_on = false;
   DevBuf<uint32_t> buf;
   uint64_t stride = 0;
-  const uint64_t n =
-      dense_extract(r.dense_bits.p, r.dense_words, r.arity, r.dense_shift, buf, stride, e.ws, e.s);
+  const uint64_t n = dense_extract(r.dense_bits.p, r.dense_sum.p, r.dense_words, r.arity,
+                                   r.dense_shift, buf, stride, e.ws, e.s);
   if (n == 0) return;
   std::vector<const uint32_t*> in(r.arity);
   for (uint32_t c = 0; c < r.arity; ++c) in[c] = buf.p + uint64_t(c) * stride;
@@ -1282,8 +1288,17 @@ static void pr_dense(srd_engine& e, std::vector<PipeRun*>& runs) {
           std::max<uint64_t>(((uint64_t(1) << bits) / 32 + kDenseBlockWords - 1) / kDenseBlockWords,
                              1) * kDenseBlockWords;
       if (r.dense_bits.n < words) {
+        size_t free_b = 0, total_b = 0;
+        {
+          std::lock_guard<std::recursive_mutex> lk(capture_mutex());
+          SRD_CHECK(hipMemGetInfo(&free_b, &total_b));
+        }
+        if (e.dense_bytes - r.dense_bits.n * 4 + words * 4 > total_b / 8) continue;
+        e.dense_bytes = e.dense_bytes - r.dense_bits.n * 4 + words * 4;
         r.dense_bits.alloc(words, e.s);
         fill_u32(r.dense_bits.p, 0, words, e.s);
+        r.dense_sum.alloc(dense_sum_words(words), e.s);
+        fill_u32(r.dense_sum.p, 0, dense_sum_words(words), e.s);
       }
       r.dense_words = words;
       r.dense_shift = sh;
@@ -1300,6 +1315,7 @@ static void pr_dense(srd_engine& e, std::vector<PipeRun*>& runs) {
       if (R->d->mark && R->d->n_dest == 1 && rel_id(e, R->d->dest[0].rel) == kv.first) {
         R->dense = true;
         R->a.mark_bits = r.dense_bits.p;
+        R->a.mark_sum = r.dense_sum.p;
         R->a.mark_shift = r.dense_shift;
       }
   }
