@@ -333,11 +333,17 @@ __device__ __forceinline__ void dense_load(const uint32_t* bits, uint64_t w0, ui
   }
 }
 
+__device__ __forceinline__ bool dense_marked(const uint32_t* sum, uint32_t b) {
+  return (sum[b >> 5] >> (b & 31u)) & 1u;
+}
+
 __global__ __launch_bounds__(kBlock) void dense_count_kernel(const uint32_t* __restrict__ bits,
+                                                             const uint32_t* __restrict__ sum,
                                                              uint32_t* __restrict__ blk) {
   __shared__ uint32_t wsum[kWavesPerBlock];
-  uint32_t v[32];
-  dense_load(bits, blockIdx.x * kDenseBlockWords + uint64_t(threadIdx.x) * kDenseThreadWords, v);
+  uint32_t v[32] = {};
+  if (dense_marked(sum, blockIdx.x))
+    dense_load(bits, blockIdx.x * kDenseBlockWords + uint64_t(threadIdx.x) * kDenseThreadWords, v);
   uint32_t c = 0;
 #pragma unroll
   for (int i = 0; i < 32; ++i) c += __popc(v[i]);
@@ -349,14 +355,15 @@ __global__ __launch_bounds__(kBlock) void dense_count_kernel(const uint32_t* __r
 // Writes the tuples of the block's set bits at its scanned offset (thread
 // by thread, in key order) and clears the words it read.
 __global__ __launch_bounds__(kBlock) void dense_emit_kernel(uint32_t* __restrict__ bits,
+                                                            const uint32_t* __restrict__ sum,
                                                             const uint64_t* __restrict__ blk_off,
                                                             uint32_t k, uint32_t shift,
                                                             uint32_t* __restrict__ out,
                                                             uint64_t stride) {
   __shared__ uint32_t wsum[kWavesPerBlock];
   const uint64_t w0 = blockIdx.x * kDenseBlockWords + uint64_t(threadIdx.x) * kDenseThreadWords;
-  uint32_t v[32];
-  dense_load(bits, w0, v);
+  uint32_t v[32] = {};
+  if (dense_marked(sum, blockIdx.x)) dense_load(bits, w0, v);
   uint32_t c = 0;
 #pragma unroll
   for (int i = 0; i < 32; ++i) c += __popc(v[i]);
@@ -378,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void dense_emit_kernel(uint32_t* __restrict
   }
 }
 
-uint64_t dense_extract(uint32_t* bits, uint64_t words, uint32_t k, uint32_t shift,
+uint64_t dense_extract(uint32_t* bits, uint32_t* sum, uint64_t words, uint32_t k, uint32_t shift,
                        DevBuf<uint32_t>& out, uint64_t& stride, Workspace& ws, hipStream_t s) {
   if (words == 0 || words % kDenseBlockWords) fail("dense_extract: bitmap size not whole blocks");
   if (k == 0 || k > 4 || shift == 0 || uint64_t(shift) * k > 63)
@@ -390,7 +397,7 @@ uint64_t dense_extract(uint32_t* bits, uint64_t words, uint32_t k, uint32_t shif
   uint64_t n = 0;
   {
     TimeScope t("new.dense_count", s, double(words) * 4);
-    dense_count_kernel<<<uint32_t(nb), kBlock, 0, s>>>(bits, blk.p);
+    dense_count_kernel<<<uint32_t(nb), kBlock, 0, s>>>(bits, sum, blk.p);
     SRD_CHECK(hipGetLastError());
     exclusive_scan_u32(blk.p, off.p, nb, ws, s);
     uint32_t* hw = pinned_words(ws, 2);
@@ -401,8 +408,9 @@ uint64_t dense_extract(uint32_t* bits, uint64_t words, uint32_t k, uint32_t shif
   stride = round_stride(std::max<uint64_t>(n, 1));
   out.alloc(stride * k, s);
   TimeScope t("new.dense_emit", s, double(words) * 4 + double(n) * 4 * k);
-  dense_emit_kernel<<<uint32_t(nb), kBlock, 0, s>>>(bits, off.p, k, shift, out.p, stride);
+  dense_emit_kernel<<<uint32_t(nb), kBlock, 0, s>>>(bits, sum, off.p, k, shift, out.p, stride);
   SRD_CHECK(hipGetLastError());
+  fill_u32(sum, 0, dense_sum_words(words), s);
   return n;
 }
 
@@ -2866,6 +2874,7 @@ void merge_disjoint(const Index& a, const Index& b, Index& out, Workspace& ws, h
     return;
   }
   if (a.arity != b.arity) fail("merge_disjoint: arity mismatch");
+  if (a.n + b.n >= (uint64_t(1) << 32)) fail("merge_disjoint: more than 2^32-1 rows");
   switch (a.arity) {
     case 1: merge_k<1>(a, b, out, ws, s); break;
     case 2: merge_k<2>(a, b, out, ws, s); break;
