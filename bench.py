@@ -422,14 +422,14 @@ def pair_hash(rows) -> int:
   of a mixed 64-bit key per pair (golden-ratio multiply, xor-shift 31), in
   chunks so a large result needs little extra host memory."""
   import numpy as np
-  total = np.uint64(0)
+  total = 0
   for lo in range(0, rows.shape[0], 1 << 26):
     part = rows[lo:lo + (1 << 26)]
     key = (part[:, 0].astype(np.uint64) << np.uint64(32)) | part[:, 1].astype(np.uint64)
     h = key * np.uint64(0x9E3779B97F4A7C15)
     h ^= h >> np.uint64(31)
-    total = total + h.sum(dtype=np.uint64)
-  return int(total)
+    total = (total + int(h.sum(dtype=np.uint64))) % (1 << 64)
+  return total
 
 
 def fixpoint_breakdown(lib, inputs, prog, native) -> dict:

@@ -320,6 +320,7 @@ struct srd_engine {
   // eighth of the device's memory.
   uint64_t dense_max_bits = uint64_t(1) << 36;
   uint64_t dense_bytes = 0;        // bitmaps allocated so far
+  uint64_t mark_chunk_slices = 0;  // SRDATALOG_MARK_CHUNK_SLICES (0: one launch per traversal)
   uint32_t weigh_ranges = SRD_XCDS;  // SRDATALOG_WEIGH_RANGES: of the weigh launch
   // column-1 bitmaps of heavy keys for views nested levels search
   uint32_t bitmap_min = SRD_BM_MIN;        // SRDATALOG_BITMAP_MIN
@@ -998,9 +999,29 @@ static void pr_traverse(srd_engine& e, PipeRun& R) {
   R.a.grid_blocks = uint32_t(std::min<uint64_t>((R.slices + 3) / 4, 2048));
   if (R.dense) {
     // every derived tuple sets its bit in the destination's bitmap: nothing
-    // to count, size or write
+    // to count, size or write.  With mark_chunk_slices the launch is split
+    // into slice ranges, each waited for and traced when tracing (a
+    // traversal of minutes then reports as it goes).
     R.mode = 4;
-    pr_launch(e, R, d->mark, "mark");
+    const uint64_t step = e.mark_chunk_slices;
+    if (!step || R.slices <= step) {
+      pr_launch(e, R, d->mark, "mark");
+      return;
+    }
+    for (uint64_t s0 = 0; s0 < R.slices; s0 += step) {
+      R.a.slice_lo = s0;
+      R.a.n_slices = std::min(s0 + step, R.slices);
+      pr_launch(e, R, d->mark, "mark");
+      if (e.trace) {
+        SRD_CHECK(hipStreamSynchronize(R.s));
+        std::fprintf(stderr, "[srd]   %s: marked slices [%llu, %llu) of %llu\n", d->rule,
+                     (unsigned long long)s0, (unsigned long long)R.a.n_slices,
+                     (unsigned long long)R.slices);
+        std::fflush(stderr);
+      }
+    }
+    R.a.slice_lo = 0;
+    R.a.n_slices = R.slices;
     return;
   }
   x.counts_buf.reserve(R.slices, R.s);
@@ -2920,6 +2941,8 @@ srd_engine* srd_engine_create(const char* text, const SrdPipelineDesc* pipes, ui
     if (const char* c = std::getenv("SRDATALOG_NO_WEIGH")) e->no_weigh = *c && *c != '0';
     if (const char* c = std::getenv("SRDATALOG_XCD_RANGES"))
       e->xcd_ranges = std::min<uint32_t>(std::max<uint32_t>(uint32_t(std::atoi(c)), 1), SRD_XCDS);
+    if (const char* c = std::getenv("SRDATALOG_MARK_CHUNK_SLICES"))
+      e->mark_chunk_slices = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_DENSE_MAX_BITS"))
       e->dense_max_bits = std::strtoull(c, nullptr, 10);
     if (const char* c = std::getenv("SRDATALOG_WINDOW_MIN"))

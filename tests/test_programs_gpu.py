@@ -214,8 +214,9 @@ def test_csv_load_all(progs, tmp_path):
 # the joins mark (default; SRDATALOG_DENSE_MAX_BITS=0: never), so every
 # other mode turns that off to keep its own row-writing path under test;
 # dense-host runs the default with the host-driven loop (TC and SG mark
-# there too) and dense-groups-in-order with the parallel group's members
-# marking one after the other.
+# there too), dense-groups-in-order with the parallel group's members
+# marking one after the other, dense-chunked-marks with every marking
+# traversal split into launches of 3 slices.
 # SRDATALOG_WINDOW_MIN=1 narrows the driving range of every nested item to
 # the other sources' value window (0: of none);
 # SRDATALOG_ROOT_POS=0 looks nested-level keys up by searching the root
@@ -249,6 +250,7 @@ MODES = [({"SRDATALOG_JOIN_MODE": "twopass"}, "twopass"),
            "SRDATALOG_STAGE_CAP": "5000"}, "group-chunked-overflow"),
          ({"SRDATALOG_DEVLOOP": "0"}, "dense-host"),
          ({"SRDATALOG_DEVLOOP": "0", "SRDATALOG_GROUP_STREAMS": "0"}, "dense-groups-in-order"),
+         ({"SRDATALOG_DEVLOOP": "0", "SRDATALOG_MARK_CHUNK_SLICES": "3"}, "dense-chunked-marks"),
          ({}, "default")]
 
 
@@ -258,7 +260,7 @@ def test_modes_tc_sg_triangle_andersen(progs, env, label, monkeypatch):
             "SRDATALOG_NEW_CHUNK_ROWS", "SRDATALOG_GRAIN", "SRDATALOG_WEIGH_MIN",
             "SRDATALOG_WEIGH_TARGET", "SRDATALOG_NO_WEIGH", "SRDATALOG_BITMAP_BYTES",
             "SRDATALOG_ROOT_POS", "SRDATALOG_WINDOW_MIN", "SRDATALOG_DEVLOOP", "SRDATALOG_DEVLOOP_CAPS",
-            "SRDATALOG_GROUP_STREAMS", "SRDATALOG_DENSE_MAX_BITS"):
+            "SRDATALOG_GROUP_STREAMS", "SRDATALOG_DENSE_MAX_BITS", "SRDATALOG_MARK_CHUNK_SLICES"):
     monkeypatch.delenv(k, raising=False)
   if not label.startswith(("dense", "default")):
     monkeypatch.setenv("SRDATALOG_DENSE_MAX_BITS", "0")
