@@ -1332,6 +1332,9 @@ static void pr_dense(srd_engine& e, std::vector<PipeRun*>& runs) {
         std::fflush(stderr);
       }
     }
+    // a layout kept from earlier in the iteration is wide enough; its bound
+    // (the extracted part's column maxima) must still cover these runs
+    r.dense_bound = std::max(r.dense_bound, bound);
     for (PipeRun* R : runs)
       if (R->d->mark && R->d->n_dest == 1 && rel_id(e, R->d->dest[0].rel) == kv.first) {
         R->dense = true;
