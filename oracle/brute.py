@@ -149,3 +149,47 @@ def split_nonrec(a, b, c):
   keep = {int(x) for x, y in a if int(y) not in blocked}
   q = {(int(x), int(z)) for x, z in c if int(x) in keep}
   return {"Q": q}, {}
+
+
+def typelookup(sup, cls, decl, name, mod, main_cls, obj=0, abstract=1, static=2, clinit=3,
+               main_name=7):
+  """The type-lookup program of the package's tests (subtyping, method
+  lookup, class initialisation in DOOP's style) as set comprehensions under
+  stratified semantics: each recursive relation is iterated to its least
+  fixpoint after the relations it negates are complete.  Constants restrict
+  body columns, negated constants are part of the excluded prefix, head
+  constants are emitted as values (semantics of the reference's Const and
+  negation, src/srdatalog/dsl.py:63-100, hir/plan.py:251-285)."""
+  sup = {(int(a), int(b)) for a, b in sup}
+  cls = {int(c) for (c,) in cls}
+  decl = {(int(a), int(b)) for a, b in decl}
+  name = {(int(a), int(b)) for a, b in name}
+  mod = {(int(a), int(b)) for a, b in mod}
+  main_cls = {int(c) for (c,) in main_cls}
+
+  def closure(base, step):
+    full = set(base)
+    while True:
+      new = step(full) - full
+      if not new:
+        return full
+      full |= new
+
+  up = {}
+  for a, b in sup:
+    up.setdefault(a, set()).add(b)
+  sub = closure({(c, c) for c in cls} | sup | {(c, obj) for c in cls},
+                lambda f: {(s, t) for s, u in f for t in up.get(u, ())})
+  impl = {(t, n, m) for m, t in decl for m2, n in name if m2 == m and (abstract, m) not in mod}
+  own = {(t, n) for t, n, _ in impl}
+  look = closure(impl, lambda f: {(t, n, m) for t, s in sup for s2, n, m in f
+                                  if s2 == s and (t, n) not in own})
+  main = {m for t in main_cls for m, t2 in decl if t2 == t and (m, main_name) in name
+          and (static, m) in mod and m != clinit}
+  cls_of = {}
+  for m, t in decl:
+    cls_of.setdefault(m, set()).add(t)
+  init = closure({t for m in main for t in cls_of.get(m, ())},
+                 lambda f: {s for t in f for s in up.get(t, ())})
+  return {"SubtypeOf": sub, "Implemented": impl, "Lookup": look, "MainMethod": {(m,) for m in main},
+          "Initialized": {(t,) for t in init}}

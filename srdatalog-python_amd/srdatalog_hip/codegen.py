@@ -34,6 +34,7 @@ import re
 from dataclasses import dataclass, field
 
 from . import mir as M
+from .dsl import const_value, is_const
 
 VER_CODE = {M.FULL: "SRD_FULL", M.DELTA: "SRD_DELTA", M.NEW: "SRD_NEW"}
 
@@ -132,6 +133,8 @@ class PipelineGen:
 
   @staticmethod
   def var(name: str) -> str:
+    if is_const(name):   # a constant argument: its literal
+      return f"{const_value(name)}u"
     return "v_" + cident(name)
 
   def col_expr(self, v: int, depth: int) -> str:
@@ -155,7 +158,7 @@ class PipelineGen:
     out = []
     for i, f in enumerate(self.filters):
       vs = f.prefix if isinstance(f, M.Negation) else f.vars
-      if i not in done and all(v in bound for v in vs):
+      if i not in done and all(v in bound or is_const(v) for v in vs):
         out.append((i, f))
     return out
 
@@ -174,7 +177,8 @@ class PipelineGen:
     slice's staging segment, 3 writes at the item's own row (direct), 5
     counts the searches only (probe runs), 6 sets the tuple's bit in the
     destination's dense bitmap (a plain load first: most derivations of a
-    deduplicating join are repeats, whose bit is already set)."""
+    deduplicating join are repeats, whose bit is already set), and, for a
+    bit it sets, its block's bit in the summary the extraction reads."""
     lines = [f"{ind}{{",
              f"{ind}  const uint64_t m_ = srd::ballot({cond});",
              f"{ind}  const uint32_t c_ = __popcll(m_);",
@@ -234,7 +238,12 @@ class PipelineGen:
                 f"{ind}    const uint64_t key_ = {' | '.join(parts)};",
                 f"{ind}    uint32_t* w_ = A.mark_bits + (key_ >> 5);",
                 f"{ind}    const uint32_t b_ = 1u << uint32_t(key_ & 31u);",
-                f"{ind}    if (!(*w_ & b_)) atomicOr(w_, b_);",
+                f"{ind}    if (!(*w_ & b_)) {{",
+                f"{ind}      atomicOr(w_, b_);",
+                f"{ind}      uint32_t* s_ = A.mark_sum + (key_ >> 23);",
+                f"{ind}      const uint32_t sb_ = 1u << uint32_t((key_ >> 18) & 31u);",
+                f"{ind}      if (!(*s_ & sb_)) atomicOr(s_, sb_);",
+                f"{ind}    }}",
                 f"{ind}  }}"]
     lines += [f"{ind}  opos += c_;",
               f"{ind}  cnt += c_;",
@@ -248,7 +257,8 @@ class PipelineGen:
     if len(self.dests) != 1 or self.ep.count:
       return False
     d = self.dests[0]
-    return 1 <= len(d.index) <= 4 and all(isinstance(d.terms[c], str) for c in d.index)
+    return 1 <= len(d.index) <= 4 and all(isinstance(d.terms[c], str) and not is_const(d.terms[c])
+                                          for c in d.index)
 
   def direct_ok(self) -> bool:
     """Every level-1 item yields exactly one tuple: no filters, and either a

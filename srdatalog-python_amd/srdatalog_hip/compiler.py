@@ -34,7 +34,7 @@ from dataclasses import dataclass, field
 
 from . import mir as M
 from .codegen import RelDecl
-from .dsl import Atom, Filter, Negation, Program, Rule, Split
+from .dsl import Atom, Filter, Negation, Program, Rule, Split, const_value, is_const
 
 
 class CompileError(ValueError):
@@ -191,9 +191,9 @@ class Variant:
 
 def _clause_vars(c) -> list[str]:
   if isinstance(c, Atom):
-    return list(c.args)
+    return [v for v in c.args if not is_const(v)]
   if isinstance(c, Negation):
-    return list(c.atom.args)
+    return [v for v in c.atom.args if not is_const(v)]
   if isinstance(c, Filter):
     return list(c.vars)
   return []
@@ -206,14 +206,14 @@ def _needs(c) -> set[str]:
 
 
 def _binds(c) -> set[str]:
-  return set(c.args) if isinstance(c, Atom) else set()
+  return set(_clause_vars(c)) if isinstance(c, Atom) else set()
 
 
 def _join_vars(rule: Rule) -> set[str]:
   seen: dict[str, int] = {}
   for c in rule.body:
     if isinstance(c, Atom):
-      for v in c.args:
+      for v in _clause_vars(c):
         seen[v] = seen.get(v, 0) + 1
   return {v for v, n in seen.items() if n > 1}
 
@@ -285,15 +285,21 @@ def _clause_order_from_vars(rule: Rule, var_order: list[str], delta: int) -> lis
 
 def _access(atom: Atom, ver: str, jv: set[str], var_order: list[str], clause: int,
             negated: bool) -> Access:
-  cvars = set(atom.args)
+  cvars = {v for v in atom.args if not is_const(v)}
   order = [v for v in var_order if v in cvars] + sorted(cvars - set(var_order))
+  index = [atom.args.index(v) for v in order]
   if negated:
     prefix_len = sum(1 for v in order if not v.startswith("_"))
+    # constants (kept in negated atoms) lead the index and the prefix
+    # (reference: hir/plan.py:283-285)
+    consts = [c for c, v in enumerate(atom.args) if is_const(v)]
+    order = [atom.args[c] for c in consts] + order
+    index = consts + index
+    prefix_len += len(consts)
   else:
+    if any(is_const(v) for v in atom.args):
+      raise CompileError(f"{atom.rel}: a constant in a positive atom was not bound")
     prefix_len = sum(1 for v in order if v in jv)
-  index = []
-  for v in order:
-    index.append(atom.args.index(v))
   index += [c for c in range(len(atom.args)) if c not in index]
   return Access(atom.rel, ver, order, index, prefix_len, clause)
 
@@ -342,7 +348,7 @@ def split_temp_vars(rule: Rule, split_at: int) -> list[str]:
       below_atoms.update(c.args)
   below = set(below_atoms)
   for h in rule.heads:
-    below.update(h.args)
+    below.update(_clause_vars(h))
   for c in rule.body[split_at + 1:]:
     if isinstance(c, Negation):
       below.update(c.atom.args)
@@ -391,7 +397,7 @@ def _split_below(v: Variant, s: Stratum, temp_ver: str) -> list:
   tv = list(v.temp_vars)
   tset = set(tv)
   ops: list = [M.Scan(tv, v.temp_rel, list(range(len(tv))), temp_ver, [])]
-  heads = {x for h in v.rule.heads for x in h.args}
+  heads = {x for h in v.rule.heads for x in _clause_vars(h)}
   cvars: list[str] = []
   srcs, vfs = [], []
   for a in v.atoms:
@@ -740,6 +746,57 @@ def relation_decls(program: Program) -> list[RelDecl]:
   return decls + [RelDecl(n, a) for n, a in split_temps(program.rules)]
 
 
+def bind_constants(rules: list[Rule]) -> list[Rule]:
+  """A constant argument of a positive body atom becomes a fresh variable
+  `K<n>_<value>` with an equality filter right after the atom, so the join
+  binds the column like any other and the filter keeps the matching rows;
+  a user plan's var_order gets the fresh variable after the atom's last
+  ordered variable.  Constants in negated atoms (a prefix of the negation's
+  index) and in heads (emitted as literals) stay as they are."""
+  import dataclasses
+  counter = 0
+  out = []
+  for r in rules:
+    body: list = []
+    fresh_after: dict[str, list[str]] = {}   # user variable -> fresh ones to place after it
+    fresh_lead: list[str] = []
+    for c in r.body:
+      if not isinstance(c, Atom) or not any(is_const(v) for v in c.args):
+        body.append(c)
+        continue
+      args, filters = [], []
+      for v in c.args:
+        if is_const(v):
+          counter += 1
+          fv = f"K{counter}_{const_value(v)}"
+          args.append(fv)
+          filters.append(Filter((fv,), f"return {fv} == {const_value(v)}u;"))
+        else:
+          args.append(v)
+      body.append(dataclasses.replace(c, args=tuple(args)))
+      body += filters
+      users = [v for v in c.args if not is_const(v)]
+      fresh = [f.vars[0] for f in filters]
+      if users:
+        fresh_after.setdefault(users[-1], []).extend(fresh)
+      else:
+        fresh_lead.extend(fresh)
+    plans = []
+    for p in r.plans:
+      if p.var_order:
+        vo: list[str] = list(fresh_lead)
+        for v in p.var_order:
+          vo.append(v)
+          vo += fresh_after.get(v, [])
+        placed = set(vo)
+        for fs in fresh_after.values():   # atoms whose variables the plan leaves out
+          vo += [f for f in fs if f not in placed]
+        p = dataclasses.replace(p, var_order=tuple(vo))
+      plans.append(p)
+    out.append(dataclasses.replace(r, body=tuple(body), plans=tuple(plans)))
+  return out
+
+
 def rename_wildcards(rules: list[Rule]) -> list[Rule]:
   """Give every `_`-prefixed variable occurrence a fresh name `_gen<N>`, N
   counting from 1 over the program's rules in order, heads before body
@@ -783,7 +840,7 @@ def compile_to_mir(program: Program) -> M.Program:
         raise CompileError(f"rule {r.name}: unsupported body clause {type(c).__name__}")
     if sum(1 for c in r.body if isinstance(c, Split)) > 1:
       raise CompileError(f"rule {r.name}: more than one split marker")
-  strata = stratify(rename_wildcards(list(program.rules)))
+  strata = stratify(rename_wildcards(bind_constants(list(program.rules))))
   for s in strata:
     if s.recursive:
       for r in s.rules:

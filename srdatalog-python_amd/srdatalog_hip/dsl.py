@@ -11,18 +11,56 @@ it run here unchanged::
         (Path(x, z) <= Path(x, y) & Edge(y, z)).named("TCRec"),
     ])
 
-Covered: Var, Relation (input_file / print_size / index_type), atoms,
-``&`` conjunction, ``~atom`` negation, Filter, the SPLIT marker, PlanEntry, Rule.named /
-with_plan / with_plans / with_count / with_inject_cpp, Program (relations
-derived from the rules in first-occurrence order, as in the reference's
-_derive_relations, dsl.py:641).  Constants, Let and aggregates are not part
-of the hot path and are rejected by the compiler.
+Covered: Var, Const (and plain ints) as atom arguments, Relation
+(input_file / print_size / index_type), atoms, ``&`` conjunction, ``~atom``
+negation, Filter, the SPLIT marker, PlanEntry, Rule.named / with_plan /
+with_plans / with_count / with_inject_cpp, Program (relations derived from
+the rules in first-occurrence order, as in the reference's
+_derive_relations, dsl.py:641).  Let and aggregates are not part of the hot
+path and are rejected by the compiler.
+
+A constant argument is kept in the atom as the name ``#<value>`` (no
+variable name starts with ``#``); the compiler binds it.
 """
 
 from __future__ import annotations
 
 import dataclasses
 from dataclasses import dataclass, field
+
+
+CONST_MARK = "#"
+
+
+def is_const(name: str) -> bool:
+  """Whether an atom argument name is a constant (``#<value>``)."""
+  return name.startswith(CONST_MARK)
+
+
+def const_value(name: str) -> int:
+  return int(name[len(CONST_MARK):])
+
+
+class Const:
+  """A constant atom argument (the reference's dsl.Const: a value baked in
+  at program construction, e.g. a dataset id read from a meta file).  Values
+  are u32 column values; cpp_expr is accepted for the same signature and
+  must spell the same integer when given."""
+
+  __slots__ = ("value", "cpp_expr")
+
+  def __init__(self, value, cpp_expr: str | None = None):
+    if isinstance(value, bool) or not isinstance(value, int):
+      raise TypeError(f"Const: integer values only, got {value!r}")
+    if not 0 <= value < (1 << 32):
+      raise ValueError(f"Const: {value} is not a u32 column value")
+    if cpp_expr is not None and cpp_expr.rstrip("uU") != str(value):
+      raise ValueError(f"Const: cpp_expr {cpp_expr!r} does not spell {value}")
+    self.value = value
+    self.cpp_expr = cpp_expr if cpp_expr is not None else str(value)
+
+  def __repr__(self) -> str:
+    return f"Const({self.value})"
 
 
 class Var:
@@ -183,9 +221,13 @@ class Relation:
       raise ValueError(f"{self.name} expects arity {self.arity}, got {len(args)}")
     names = []
     for a in args:
-      if not isinstance(a, Var):
-        raise TypeError(f"{self.name}: only variables are supported as arguments, got {a!r}")
-      names.append(a.name)
+      if isinstance(a, Var):
+        names.append(a.name)
+      elif isinstance(a, Const) or (isinstance(a, int) and not isinstance(a, bool)):
+        c = a if isinstance(a, Const) else Const(a)
+        names.append(f"{CONST_MARK}{c.value}")
+      else:
+        raise TypeError(f"{self.name}: arguments are variables or integer constants, got {a!r}")
     return Atom(self.name, tuple(names), relation=self)
 
   def __repr__(self) -> str:

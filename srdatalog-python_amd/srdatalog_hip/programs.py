@@ -14,7 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 from . import build as B
 from . import native
-from .dsl import SPLIT, Filter, PlanEntry, Program, Relation, Var
+from .dsl import SPLIT, Const, Filter, PlanEntry, Program, Relation, Var
 from .runtime import ProgramLib
 
 PROG_DIR = os.path.join(native.LIB_DIR, "programs")
@@ -192,6 +192,46 @@ def split_program(split: bool) -> Program:
   return Program(rules=[(r(x, y) <= seed(x, y)).named("RBase"), (r(x, z) <= body).named("RRec")])
 
 
+# Constants of typelookup_program (ids a fact extractor would assign).
+TL_OBJECT, TL_ABSTRACT, TL_STATIC, TL_CLINIT, TL_MAIN = 0, 1, 2, 3, 7
+
+
+def typelookup_program() -> Program:
+  """Subtyping, virtual-method lookup and class initialisation in the
+  style of DOOP's type rules (the reference's examples/doop.py), written for
+  this package's tests: constants in rule heads (every class is a subtype of
+  the root type), in body atoms (the main method's name and its static
+  modifier) and in negations (methods not declared abstract), a negation
+  with a wildcard (no method of that name declared in the class), a
+  repeated head variable (reflexive subtyping), a filter, and three
+  recursive relations, one of them over a stratified negation."""
+  s, t, u, m, n = Var("s"), Var("t"), Var("u"), Var("m"), Var("n")
+  sup = Relation("DirectSuperclass", 2, input_file="DirectSuperclass.csv")
+  cls = Relation("ClassType", 1, input_file="ClassType.csv")
+  decl = Relation("MethodDecl", 2, input_file="MethodDecl.csv")          # (method, class)
+  name = Relation("MethodName", 2, input_file="MethodName.csv")          # (method, name)
+  mod = Relation("MethodModifier", 2, input_file="MethodModifier.csv")   # (modifier, method)
+  main_cls = Relation("MainClass", 1, input_file="MainClass.csv")
+  sub = Relation("SubtypeOf", 2, print_size=True)
+  impl = Relation("Implemented", 3, print_size=True)                     # (class, name, method)
+  look = Relation("Lookup", 3, print_size=True)
+  main = Relation("MainMethod", 1, print_size=True)
+  init = Relation("Initialized", 1, print_size=True)
+  return Program(rules=[
+    (sub(s, s) <= cls(s)).named("SubRefl"),
+    (sub(s, t) <= sup(s, t)).named("SubDirect"),
+    (sub(s, t) <= sub(s, u) & sup(u, t)).named("SubTrans"),
+    (sub(s, Const(TL_OBJECT)) <= cls(s)).named("SubRoot"),
+    (impl(t, n, m) <= decl(m, t) & name(m, n) & ~mod(Const(TL_ABSTRACT), m)).named("Implemented"),
+    (look(t, n, m) <= impl(t, n, m)).named("LookupOwn"),
+    (look(t, n, m) <= sup(t, s) & look(s, n, m) & ~impl(t, n, Var("_"))).named("LookupInherited"),
+    (main(m) <= main_cls(t) & decl(m, t) & name(m, Const(TL_MAIN)) & mod(Const(TL_STATIC), m)
+     & Filter(("m",), f"return m != {TL_CLINIT}u;")).named("MainMethod"),
+    (init(t) <= main(m) & decl(m, t)).named("InitMain"),
+    (init(s) <= init(t) & sup(t, s)).named("InitSuper"),
+  ])
+
+
 def split_nonrec_program() -> Program:
   """Q(x,z) <= A(x,y), !B(y), split, C(x,z) (tests/fixtures/split_test.nim):
   the split of a non-recursive rule."""
@@ -218,6 +258,7 @@ PROGRAMS = {
   "split_rec": lambda: split_program(True),
   "split_rec_plain": lambda: split_program(False),
   "split_nonrec": split_nonrec_program,
+  "typelookup": typelookup_program,
 }
 
 
