@@ -193,3 +193,43 @@ def typelookup(sup, cls, decl, name, mod, main_cls, obj=0, abstract=1, static=2,
                  lambda f: {s for t in f for s in up.get(t, ())})
   return {"SubtypeOf": sub, "Implemented": impl, "Lookup": look, "MainMethod": {(m,) for m in main},
           "Initialized": {(t,) for t in init}}
+
+
+def cspa(assign, deref):
+  """CSPA (the reference's examples/cspa.py): the stratum's base rules give
+  F_0, then the three mutually recursive relations are iterated together,
+  F_{i+1} = F_i u T(F_i) with every rule reading F_i, recording
+  |T(F_i) - F_i| per relation and iteration until all three are empty --
+  the delta sequences of the reference's semi-naive loop over one strongly
+  connected component (iteration 0 reads FULL for DELTA)."""
+  a = {(int(p), int(q)) for p, q in assign}
+  d = {(int(p), int(q)) for p, q in deref}
+  vf = set(a) | {(x, x) for x, _ in a} | {(x, x) for _, x in a}
+  ma = {(x, x) for _, x in a} | {(x, x) for x, _ in a}
+  va: set = set()
+
+  def by_first(rel):
+    out: dict = {}
+    for p, q in rel:
+      out.setdefault(p, set()).add(q)
+    return out
+
+  deltas = {"ValueFlow": [], "ValueAlias": [], "MemoryAlias": []}
+  a1, d1 = by_first(a), by_first(d)
+  while True:
+    vf1, ma1, va1 = by_first(vf), by_first(ma), by_first(va)
+    t_vf = {(x, y) for x, z in vf for y in vf1.get(z, ())}
+    t_vf |= {(x, y) for x, z in a for y in ma1.get(z, ())}
+    t_va = {(x, y) for z, xs in vf1.items() for x in xs for y in xs}
+    t_va |= {(x, y) for z, xs in vf1.items() for w in ma1.get(z, ()) for x in xs
+             for y in vf1.get(w, ())}
+    t_ma = {(x, w) for y, xs in d1.items() for z in va1.get(y, ()) for x in xs
+            for w in d1.get(z, ())}
+    new = (t_vf - vf, t_va - va, t_ma - ma)
+    deltas["ValueFlow"].append(len(new[0]))
+    deltas["ValueAlias"].append(len(new[1]))
+    deltas["MemoryAlias"].append(len(new[2]))
+    vf, va, ma = vf | new[0], va | new[1], ma | new[2]
+    if not any(new):
+      break
+  return {"ValueFlow": vf, "ValueAlias": va, "MemoryAlias": ma}, deltas

@@ -192,6 +192,37 @@ def split_program(split: bool) -> Program:
   return Program(rules=[(r(x, y) <= seed(x, y)).named("RBase"), (r(x, z) <= body).named("RRec")])
 
 
+def cspa_program() -> Program:
+  """Context-sensitive points-to analysis (examples/cspa.py; the rules and
+  plans of its CSPA_DB program): ValueFlow, ValueAlias and MemoryAlias are
+  mutually recursive, one strongly connected component of three relations,
+  with wildcard-reflexive base rules."""
+  w, x, y, z = Var("w"), Var("x"), Var("y"), Var("z")
+  assign_in = Relation("AssignInput", 2, input_file="Assign.csv")
+  deref_in = Relation("DerefInput", 2, input_file="Dereference.csv")
+  assign = Relation("Assign", 2)
+  deref = Relation("Deref", 2)
+  vf = Relation("ValueFlow", 2, print_size=True)
+  va = Relation("ValueAlias", 2, print_size=True)
+  ma = Relation("MemoryAlias", 2, print_size=True)
+  return Program(rules=[
+    (assign(x, y) <= assign_in(x, y)).named("LoadAssign"),
+    (deref(x, y) <= deref_in(x, y)).named("LoadDeref"),
+    (vf(y, x) <= assign(y, x)).named("VF_Assign"),
+    (vf(x, x) <= assign(x, Var("_"))).named("VF_Refl1"),
+    (vf(x, x) <= assign(Var("_"), x)).named("VF_Refl2"),
+    (ma(x, x) <= assign(Var("_"), x)).named("MA_Refl1"),
+    (ma(x, x) <= assign(x, Var("_"))).named("MA_Refl2"),
+    (vf(x, y) <= vf(x, z) & vf(z, y)).named("VF_Trans").with_plan(var_order=["z", "x", "y"]),
+    (vf(x, y) <= assign(x, z) & ma(z, y)).named("VF_Assign_MA").with_plan(var_order=["z", "x", "y"]),
+    (va(x, y) <= vf(z, x) & vf(z, y)).named("VA_VF").with_plan(var_order=["z", "x", "y"]),
+    (va(x, y) <= vf(z, x) & ma(z, w) & vf(w, y)).named("VA_VF_MA_VF")
+    .with_plan(var_order=["z", "w", "x", "y"]),
+    (ma(x, w) <= deref(y, x) & va(y, z) & deref(z, w)).named("MA_Deref_VA")
+    .with_plan(var_order=["y", "z", "x", "w"]),
+  ])
+
+
 # Constants of typelookup_program (ids a fact extractor would assign).
 TL_OBJECT, TL_ABSTRACT, TL_STATIC, TL_CLINIT, TL_MAIN = 0, 1, 2, 3, 7
 
@@ -259,6 +290,7 @@ PROGRAMS = {
   "split_rec_plain": lambda: split_program(False),
   "split_nonrec": split_nonrec_program,
   "typelookup": typelookup_program,
+  "cspa": cspa_program,
 }
 
 

@@ -60,3 +60,20 @@ def test_const_values_are_u32():
     except (TypeError, ValueError):
       continue
     raise AssertionError(f"{bad!r} accepted")
+
+
+def test_brute_cspa_hand_case():
+  # assign 1 <- 2, deref: 1 points to 3 and 2 points to 4
+  out, deltas = brute.cspa([(1, 2)], [(1, 3), (2, 4)])
+  vf = {(1, 2), (1, 1), (2, 2)}
+  assert out["ValueFlow"] == vf
+  # value aliases: pairs with a common value-flow source (1 -> {1, 2}, 2 -> {2})
+  assert out["ValueAlias"] == {(1, 1), (1, 2), (2, 1), (2, 2)}
+  # memory aliases: reflexive on 1 and 2, and the dereferences of aliased 1, 2
+  assert out["MemoryAlias"] == {(1, 1), (2, 2), (3, 3), (3, 4), (4, 3), (4, 4)}
+  assert deltas["ValueFlow"][-1] == deltas["ValueAlias"][-1] == deltas["MemoryAlias"][-1] == 0
+  # the program compiles to one recursive stratum holding all three relations
+  from srdatalog_hip import compiler
+  strata = compiler.stratify(compiler.rename_wildcards(list(programs.PROGRAMS["cspa"]().rules)))
+  rec = [s for s in strata if s.recursive]
+  assert any({"ValueFlow", "ValueAlias", "MemoryAlias"} <= set(s.members) for s in rec)
