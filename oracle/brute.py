@@ -233,3 +233,50 @@ def cspa(assign, deref):
     if not any(new):
       break
   return {"ValueFlow": vf, "ValueAlias": va, "MemoryAlias": ma}, deltas
+
+
+def galen(p_in, q_in, r_in, c_in, u_in, s_in):
+  """Galen (the reference's examples/galen.py): F_0 = the loaded P and Q,
+  then OutP and OutQ iterated together, F_{i+1} = F_i u T(F_i), recording
+  |T(F_i) - F_i| per relation and iteration until both are empty."""
+  P = {(int(a), int(b)) for a, b in p_in}
+  Q = {(int(a), int(b), int(c)) for a, b, c in q_in}
+  R = [(int(a), int(b), int(c)) for a, b, c in r_in]
+  C = [(int(a), int(b), int(c)) for a, b, c in c_in]
+  U = [(int(a), int(b), int(c)) for a, b, c in u_in]
+  S = [(int(a), int(b)) for a, b in s_in]
+  deltas = {"OutP": [], "OutQ": []}
+  while True:
+    p_from, p_to = {}, {}
+    for a, b in P:
+      p_from.setdefault(a, set()).add(b)
+      p_to.setdefault(b, set()).add(a)
+    q_from = {}
+    q_by_x_r = {}
+    for a, b, c in Q:
+      q_from.setdefault(a, set()).add((b, c))
+      q_by_x_r.setdefault((a, b), set()).add(c)
+    q_by_r_y: dict = {}
+    for x, r, y in Q:
+      q_by_r_y.setdefault((r, y), set()).add(x)
+    t_p = {(x, z) for x, y in P for z in p_from.get(y, ())}                       # TC
+    t_p |= {(x, z) for w, r, z in U for y in p_to.get(w, ()) for x in q_by_r_y.get((r, y), ())}  # Join3a
+    t_p |= {(x, z) for y, w, z in C for x in p_to.get(w, ()) if (x, y) in P}      # Join3b
+    t_q = {(x, r, z) for x, y in P for r, z in q_from.get(y, ())}                 # PropQ
+    s_of: dict = {}
+    for r, q in S:
+      s_of.setdefault(r, set()).add(q)
+    t_q |= {(x, q, z) for x, r, z in Q for q in s_of.get(r, ())}                  # PropQS
+    r_of: dict = {}
+    for y, u, e in R:
+      r_of.setdefault(y, []).append((u, e))
+    t_q |= {(x, e, o) for x, y, z in Q for u, e in r_of.get(y, ())                 # Join3c
+            for o in q_by_x_r.get((z, u), ())}
+    new_p, new_q = t_p - P, t_q - Q
+    deltas["OutP"].append(len(new_p))
+    deltas["OutQ"].append(len(new_q))
+    P |= new_p
+    Q |= new_q
+    if not new_p and not new_q:
+      break
+  return {"OutP": P, "OutQ": Q}, deltas

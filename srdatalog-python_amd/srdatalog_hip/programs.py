@@ -223,6 +223,33 @@ def cspa_program() -> Program:
   ])
 
 
+def galen_program() -> Program:
+  """Galen ontology reasoning (examples/galen.py; the rules and plans of its
+  GalenDB program): OutP (pairs) and OutQ (triples) mutually recursive
+  through three-atom joins, some with plans for a given DELTA position."""
+  e, o, q, r, u, w, x, y, z = (Var(n) for n in ("e", "o", "q", "r", "u", "w", "x", "y", "z"))
+  p_in = Relation("PInput", 2, input_file="P.csv")
+  q_in = Relation("QInput", 3, input_file="Q.csv")
+  r_in = Relation("RInput", 3, input_file="R.csv")
+  c_in = Relation("CInput", 3, input_file="C.csv")
+  u_in = Relation("UInput", 3, input_file="U.csv")
+  s_in = Relation("SInput", 2, input_file="S.csv")
+  out_p = Relation("OutP", 2, print_size=True)
+  out_q = Relation("OutQ", 3, print_size=True)
+  return Program(rules=[
+    (out_p(x, z) <= p_in(x, z)).named("LoadP"),
+    (out_q(x, r, z) <= q_in(x, r, z)).named("LoadQ"),
+    (out_p(x, z) <= out_p(x, y) & out_p(y, z)).named("TC"),
+    (out_q(x, r, z) <= out_p(x, y) & out_q(y, r, z)).named("PropQ"),
+    (out_p(x, z) <= out_p(y, w) & u_in(w, r, z) & out_q(x, r, y)).named("Join3a")
+    .with_plan(delta=2, var_order=["y", "w", "r", "x", "z"]),
+    (out_p(x, z) <= c_in(y, w, z) & out_p(x, w) & out_p(x, y)).named("Join3b"),
+    (out_q(x, q, z) <= out_q(x, r, z) & s_in(r, q)).named("PropQS"),
+    (out_q(x, e, o) <= out_q(x, y, z) & r_in(y, u, e) & out_q(z, u, o)).named("Join3c")
+    .with_plan(delta=0, var_order=["z", "y", "u", "x", "e", "o"]),
+  ])
+
+
 # Constants of typelookup_program (ids a fact extractor would assign).
 TL_OBJECT, TL_ABSTRACT, TL_STATIC, TL_CLINIT, TL_MAIN = 0, 1, 2, 3, 7
 
@@ -291,6 +318,7 @@ PROGRAMS = {
   "split_nonrec": split_nonrec_program,
   "typelookup": typelookup_program,
   "cspa": cspa_program,
+  "galen": galen_program,
 }
 
 

@@ -77,3 +77,18 @@ def test_brute_cspa_hand_case():
   strata = compiler.stratify(compiler.rename_wildcards(list(programs.PROGRAMS["cspa"]().rules)))
   rec = [s for s in strata if s.recursive]
   assert any({"ValueFlow", "ValueAlias", "MemoryAlias"} <= set(s.members) for s in rec)
+
+
+def test_brute_galen_hand_case():
+  # P: 1->2, 2->3; Q: (3, 7, 4); R: (7, 8, 9) with Q(4, 8, 5) gives
+  # Q(x, 9, 5) for every x reaching 4 through Q(x, 7, 4); S: 7 -> 6
+  p = [(1, 2), (2, 3)]
+  q = [(3, 7, 4), (4, 8, 5)]
+  out, deltas = brute.galen(p, q, [(7, 8, 9)], [], [], [(7, 6)])
+  assert out["OutP"] == {(1, 2), (2, 3), (1, 3)}
+  # PropQ carries Q(3, ., .) back along P to 2 and 1; PropQS renames 7 -> 6;
+  # Join3c: Q(x, 7, 4) and Q(4, 8, 5) with R(7, 8, 9) give Q(x, 9, 5)
+  qx = {(3, 7, 4), (4, 8, 5), (3, 6, 4), (3, 9, 5)}
+  want = qx | {(a, r, c) for a in (1, 2) for (_, r, c) in qx if _ == 3}
+  assert out["OutQ"] == want
+  assert deltas["OutP"][-1] == deltas["OutQ"][-1] == 0
