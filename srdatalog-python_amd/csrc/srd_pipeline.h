@@ -114,8 +114,11 @@ struct SrdLaunchArgs {
   unsigned long long* probe_ctr;
   // mark mode (6) only: the destination's NEW as a dense bitmap over its
   // key space, a tuple (v0 .. vk-1) in index order at bit
-  // sum(v_c << mark_shift * (k-1-c))
+  // sum(v_c << mark_shift * (k-1-c)); mark_sum has a bit per block of
+  // 2^18 keys, set when a bit of the block is set, so extraction reads only
+  // marked blocks
   uint32_t* mark_bits;
+  uint32_t* mark_sum;
   uint32_t mark_shift;
   uint32_t mark_pad_;
   // device-sized launches (the *_dyn entry points) only: views, key and

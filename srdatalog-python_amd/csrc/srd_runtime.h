@@ -307,13 +307,16 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace
 // Dense-bitmap NEW (the joins' mark mode): bit `key` of the bitmap holds
 // tuple (v_0 .. v_k-1) with key = sum(v_c << shift*(k-1-c)).  words must be
 // a multiple of kDenseBlockWords (the allocation is rounded up to it and the
-// tail stays clear).
+// tail stays clear); summary bit b (of words / kDenseBlockWords, in
+// dense_sum_words u32) is set when block b (2^18 keys) holds a set bit.
 constexpr uint64_t kDenseBlockWords = 8192;
+inline uint64_t dense_sum_words(uint64_t words) { return (words / kDenseBlockWords + 31) / 32; }
 // The tuples of the set bits, in key order (the index order: sorted and
 // duplicate-free), written to `out` (allocated here, column c at
-// out.p + c * stride, stride returned); the bitmap is left clear.  Returns
-// the number of tuples.
-uint64_t dense_extract(uint32_t* bits, uint64_t words, uint32_t k, uint32_t shift,
+// out.p + c * stride, stride returned); only blocks marked in the summary
+// are read, and the bitmap and summary are left clear.  Returns the number
+// of tuples.
+uint64_t dense_extract(uint32_t* bits, uint32_t* sum, uint64_t words, uint32_t k, uint32_t shift,
                        DevBuf<uint32_t>& out, uint64_t& stride, Workspace& ws, hipStream_t s);
 
 // Fill n words with value (a kernel, safe under stream capture).

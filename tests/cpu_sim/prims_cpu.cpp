@@ -146,8 +146,9 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, Workspace
   for (uint64_t i = 0; i < n; ++i) out[i] = t, t += in[i];
   out[n] = t;
 }
-uint64_t dense_extract(uint32_t* bits, uint64_t words, uint32_t k, uint32_t shift,
+uint64_t dense_extract(uint32_t* bits, uint32_t* sum, uint64_t words, uint32_t k, uint32_t shift,
                        DevBuf<uint32_t>& out, uint64_t& stride, Workspace&, hipStream_t s) {
+  std::fill(sum, sum + dense_sum_words(words), 0u);
   if (words == 0 || words % kDenseBlockWords) fail("dense_extract: bitmap size not whole blocks");
   if (k == 0 || k > 4 || shift == 0 || uint64_t(shift) * k > 63)
     fail("dense_extract: key layout out of range");
