@@ -178,14 +178,21 @@ def bench_batch(args) -> dict:
   got_b = batch.run_batched_kinds(kinds, grains)
   t_b = time.perf_counter() - t
   log(f"batch: batched {len(ok)} instances in {t_b:.2f}s")
+  pm = native.pool_stats()
   t = time.perf_counter()
   got_p = {k: batch.run_plain(k, lst, grain=grains[k]) for k, lst in kinds.items()}
   t_p = time.perf_counter() - t
   log(f"batch: plain {len(ok)} instances in {t_p:.2f}s")
   p1 = native.pool_stats()
-  log("batch: device pool during the timed runs: " +
-      ", ".join(f"{k} +{p1[k] - p0[k]}" for k in ("driver_allocs", "reserved", "trims")) +
-      f", peak {p1['peak'] / 1e9:.1f} GB")
+  pool_delta = {}
+  for label, a, b in (("batched", p0, pm), ("plain", pm, p1)):
+    pool_delta[label] = {k: b[k] - a[k] for k in ("driver_allocs", "reserved", "trims",
+                                                  "cross_stream")}
+    log(f"batch: device pool during the timed {label} run: " +
+        ", ".join(f"{k} +{v}" for k, v in pool_delta[label].items()) +
+        f"; reserved {b['reserved'] / 1e9:.1f} GB, cached {b['cached'] / 1e9:.1f} GB")
+  log(f"batch: pool peak {p1['peak'] / 1e9:.1f} GB (reserved {p0['reserved'] / 1e9:.1f} GB, "
+      f"cached {p0['cached'] / 1e9:.1f} GB before the timed runs)")
 
   # where the time goes: kernel time (HIP events) against wall time, for one
   # more batched run (its kinds one after the other) and for plain runs of a
@@ -253,6 +260,7 @@ def bench_batch(args) -> dict:
     "mismatched_instances": mismatch,
     "grain": grains,
     "admission_s": round(admit_s, 2),
+    "pool_during_timed_runs": pool_delta,
     "time_split": diag,
   }
 

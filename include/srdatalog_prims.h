@@ -93,7 +93,9 @@ int srd_calib_copy(uint64_t bytes, int reps, double* ms_per_rep);
  * use, budget (0 = none), allocations served, allocations that reached
  * hipMalloc, bytes obtained from hipMalloc, bytes cached on the free list,
  * times cached blocks were handed back to the device (least recently
- * freed first, as much as a failed hipMalloc needs).  The reference's RMM pool
+ * freed first, as much as a failed hipMalloc needs), allocations served from
+ * a range another library stream freed (ordered after that stream's work by
+ * an event).  The reference's RMM pool
  * (gpu/device_array_rmm_impl.h:102-160) has no counterpart query. */
 int srd_pool_stats(uint64_t* out, uint32_t cap);
 

@@ -8,6 +8,7 @@
 #include <iterator>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -26,7 +27,13 @@ namespace srd {
 // free range is tagged with the stream it was freed on: a later allocation
 // on that stream reuses it without synchronising (stream order makes the
 // reuse safe); once the stream is retired (synchronised) the range serves
-// any stream and merges with other such ranges.  So the memory one run
+// any stream and merges with other such ranges.  Before going to the driver,
+// a request also takes a range freed on another stream the pool knows to
+// live until the process ends (the library's own, registered when created),
+// after making its stream wait for an event recorded on that stream: runs
+// on concurrent streams then share one cache instead of each growing its
+// own (unless a stream capture is open, which such a wait would join).
+// So the memory one run
 // obtained serves any mix of sizes in the next, and the reservation stays
 // near the peak in use.  Smaller requests are cached per power-of-two class.
 //
@@ -69,6 +76,7 @@ struct Range {
   uint64_t seq;
 };
 std::map<char*, Range> g_ranges;                // start -> free range (address order)
+std::set<hipStream_t> g_known;                  // streams that are never destroyed
 std::multimap<uint64_t, char*> g_by_size;       // bytes -> start of a free range
 
 struct Live {
@@ -160,6 +168,57 @@ void* take_range(uint64_t b, hipStream_t s) {
     return start;
   }
   return nullptr;
+}
+
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+    (void)hipGetLastError();
+    return true;
+  }
+  return st != hipStreamCaptureStatusNone;
+}
+
+// Best-fit free range for `b` bytes left by another registered stream; s
+// is made to wait for that stream's work enqueued so far, which includes
+// every use of the range (lock held; no other thread's capture open).
+void* take_range_cross(uint64_t b, hipStream_t s) {
+  for (auto it = g_by_size.lower_bound(b); it != g_by_size.end(); ++it) {
+    auto rit = g_ranges.find(it->second);
+    const Range r = rit->second;
+    if (r.any_stream || r.s == s || !g_known.count(r.s) || capturing(r.s)) continue;
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    const bool ok = hipEventRecord(ev, r.s) == hipSuccess && hipStreamWaitEvent(s, ev, 0) == hipSuccess;
+    (void)hipEventDestroy(ev);  // released once the recorded work completes
+    if (!ok) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    char* start = rit->first;
+    range_erase(rit);
+    if (r.bytes > b) {
+      Range rest = r;  // still tagged with r.s
+      rest.bytes = r.bytes - b;
+      range_insert(start + b, rest);
+    }
+    g_cached -= b;
+    g_stats.cross_stream++;
+    return start;
+  }
+  return nullptr;
+}
+
+// Whether no other thread may have a stream capture open and s is not
+// capturing: a capture holds capture_mutex() throughout, so a lock attempt
+// that succeeds rules out other threads (this thread's own capture is
+// checked per stream).  The caller releases `cg`.
+bool no_capture(hipStream_t s, std::unique_lock<std::recursive_mutex>& cg) {
+  cg = std::unique_lock<std::recursive_mutex>(capture_mutex(), std::try_to_lock);
+  return cg.owns_lock() && !capturing(s);
 }
 
 void* take_small(uint64_t b, hipStream_t s) {
@@ -308,6 +367,10 @@ void* pool_alloc(uint64_t bytes, hipStream_t s) {
     fail("srdatalog pool: allocation of " + std::to_string(bytes) + " bytes exceeds the budget of " +
          std::to_string(g_stats.budget) + " bytes (" + std::to_string(g_stats.in_use) + " in use)");
   void* p = small ? take_small(b, s) : take_range(b, s);
+  if (!p && !small && !g_known.empty()) {
+    std::unique_lock<std::recursive_mutex> cg;
+    if (no_capture(s, cg)) p = take_range_cross(b, s);
+  }
   if (!p) p = driver_alloc(b, small, s, g);
   g_live[p] = Live{b, small};
   g_stats.in_use += bytes;
@@ -349,6 +412,11 @@ void pool_retire_stream(hipStream_t s) {
     auto it = g_ranges.find(c);
     if (it != g_ranges.end()) coalesce(it);
   }
+}
+
+void pool_register_stream(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  g_known.insert(s);
 }
 
 PoolStats pool_stats() {

@@ -65,6 +65,7 @@ hipStream_t stream_acquire() {
   }
   hipStream_t s;
   SRD_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  pool_register_stream(s);  // never destroyed: released streams return to g_streams
   return s;
 }
 

@@ -261,9 +261,10 @@ int srd_calib_copy(uint64_t bytes, int reps, double* ms_per_rep) {
 int srd_pool_stats(uint64_t* out, uint32_t cap) {
   return guard([&] {
     const srd::PoolStats st = srd::pool_stats();
-    const uint64_t v[8] = {st.in_use, st.peak,     st.budget, st.allocs,
-                           st.driver_allocs, st.reserved, st.cached, st.trims};
-    for (uint32_t i = 0; i < cap && i < 8; ++i) out[i] = v[i];
+    const uint64_t v[9] = {st.in_use,        st.peak,     st.budget, st.allocs,
+                           st.driver_allocs, st.reserved, st.cached, st.trims,
+                           st.cross_stream};
+    for (uint32_t i = 0; i < cap && i < 9; ++i) out[i] = v[i];
   });
 }
 

@@ -44,6 +44,7 @@ struct PoolStats {
   uint64_t reserved = 0;       // bytes obtained from hipMalloc so far
   uint64_t cached = 0;         // bytes held on the free list
   uint64_t trims = 0;          // times cached memory was handed back to the driver
+  uint64_t cross_stream = 0;   // allocations served from another stream's free range
 };
 
 void pool_init(int device);
@@ -51,6 +52,10 @@ void* pool_alloc(uint64_t bytes, hipStream_t s);
 void pool_free(void* p, uint64_t bytes, hipStream_t s);
 // Synchronise s and let every block it freed serve any stream.
 void pool_retire_stream(hipStream_t s);
+// Declare s a stream that lives until the process ends: when an allocation
+// on another stream finds no free range of its own, a range s freed serves
+// it after that stream is made to wait for s's work so far (an event).
+void pool_register_stream(hipStream_t s);
 PoolStats pool_stats();
 void pool_set_budget(uint64_t bytes);
 void pool_reset_peak();

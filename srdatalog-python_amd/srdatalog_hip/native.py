@@ -119,7 +119,7 @@ def _bind_prims(lib) -> None:
 
 
 POOL_FIELDS = ("in_use", "peak", "budget", "allocs", "driver_allocs", "reserved", "cached",
-               "trims")
+               "trims", "cross_stream")
 
 
 def pool_stats() -> dict:

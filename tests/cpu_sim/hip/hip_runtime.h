@@ -30,11 +30,25 @@ inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)1; return 0; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
-// Everything runs in program order on the host: events and waits are no-ops.
+// Everything runs in program order on the host: events and waits do nothing
+// but report to sim_stream_event (op 0: a record on s, 1: s waits), which
+// only the pool test build defines.
+extern "C" __attribute__((weak)) void sim_stream_event(int op, hipStream_t s);
 enum { hipEventDisableTiming = 2 };
 inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { *e = (void*)1; return hipSuccess; }
-inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
-inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t s) {
+  if (sim_stream_event) sim_stream_event(0, s);
+  return hipSuccess;
+}
+inline hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t, unsigned) {
+  if (sim_stream_event) sim_stream_event(1, s);
+  return hipSuccess;
+}
+enum hipStreamCaptureStatus { hipStreamCaptureStatusNone = 0, hipStreamCaptureStatusActive = 1 };
+inline hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus* st) {
+  *st = hipStreamCaptureStatusNone;
+  return hipSuccess;
+}
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 // Simulated device memory (the pool's host-only tests): hipMalloc fails
 // once `capacity` bytes (0 = no limit) are handed out.  Memory is host

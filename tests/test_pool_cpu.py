@@ -2,7 +2,8 @@
 over a simulated hipMalloc with a capacity (tests/cpu_sim/pool_sim.cpp):
 live blocks never overlap and lie inside driver blocks, freed memory serves
 later requests of other sizes (ranges split and merge), a range freed on a
-stream serves another stream only after that stream is retired, and a
+stream serves another stream only after that stream is retired, or at once
+behind an event wait when both are registered library streams, and a
 failed hipMalloc hands back only cached memory (least recently freed first)
 before the request is retried.  Each test loads its own copy of the library
 so the pool starts empty."""
@@ -18,7 +19,8 @@ from conftest import build_cpu_sim
 
 GiB = 1 << 30
 MiB = 1 << 20
-STATS = ("in_use", "peak", "budget", "allocs", "driver_allocs", "reserved", "cached", "trims")
+STATS = ("in_use", "peak", "budget", "allocs", "driver_allocs", "reserved", "cached", "trims",
+         "cross_stream")
 
 
 class Pool:
@@ -29,6 +31,9 @@ class Pool:
     self.lib.sim_pool_alloc.argtypes = [u64, u64]
     self.lib.sim_pool_free.argtypes = [u64, u64, u64]
     self.lib.sim_pool_retire.argtypes = [u64]
+    self.lib.sim_pool_register.argtypes = [u64]
+    self.lib.sim_event_log.restype = u64
+    self.lib.sim_event_log.argtypes = [ctypes.POINTER(u64), u64]
     self.lib.sim_pool_budget.argtypes = [u64]
     self.lib.sim_pool_stats.argtypes = [ctypes.POINTER(u64)]
     self.lib.sim_device_capacity.argtypes = [u64]
@@ -52,8 +57,18 @@ class Pool:
   def retire(self, s):
     self.lib.sim_pool_retire(s)
 
+  def register(self, s):
+    self.lib.sim_pool_register(s)
+
+  def events(self):
+    """(op, stream) of every event record (op 0) and stream wait (op 1)."""
+    n = self.lib.sim_event_log(None, 0)
+    out = (ctypes.c_uint64 * max(n, 1))()
+    self.lib.sim_event_log(out, n)
+    return [(int(x) >> 32, int(x) & 0xffffffff) for x in out[:n]]
+
   def stats(self):
-    out = (ctypes.c_uint64 * 8)()
+    out = (ctypes.c_uint64 * len(STATS))()
     self.lib.sim_pool_stats(out)
     return dict(zip(STATS, out))
 
@@ -71,10 +86,16 @@ def pool(tmp_path):
   return Pool(dst)
 
 
-def test_random_workload_no_overlap(pool):
+@pytest.mark.parametrize("registered", [False, True], ids=["own-stream", "cross-stream"])
+def test_random_workload_no_overlap(pool, registered):
   """Random sizes (small classes and arena ranges) on three streams, frees in
   random order with retirements: blocks never overlap, in_use is exact, and
-  the driver memory stays within a small factor of the peak in use."""
+  the driver memory stays within a small factor of the peak in use; with
+  the streams registered, ranges also move between streams (every such
+  take is a record on the freeing stream followed by a wait)."""
+  if registered:
+    for s in (1, 2, 3):
+      pool.register(s)
   rng = np.random.default_rng(0)
   for step in range(3000):
     if pool.live and rng.random() < 0.45:
@@ -88,6 +109,11 @@ def test_random_workload_no_overlap(pool):
     assert pool.stats()["in_use"] == sum(pool.live.values())
   st, dev = pool.stats(), pool.device()
   assert dev["used"] <= max(4 * st["peak"], 2 * GiB)
+  ev = pool.events()
+  assert len(ev) == 2 * st["cross_stream"]
+  assert (st["cross_stream"] > 0) == registered
+  for (op_r, s_r), (op_w, s_w) in zip(ev[::2], ev[1::2]):
+    assert op_r == 0 and op_w == 1 and s_r != s_w
 
 
 def test_freed_ranges_serve_other_sizes(pool):
@@ -123,6 +149,32 @@ def test_stream_tag_until_retired(pool):
   pool.free(r, s=1)
   pool.retire(1)
   assert pool.alloc(512 * MiB, s=2) == p
+
+
+def test_cross_stream_reuse_of_registered_streams(pool):
+  """A range freed on a registered stream serves another stream that finds
+  no range of its own, after an event recorded on the freeing stream and a
+  wait on the requesting one (in that order); no hipMalloc is made.  The
+  rest of a split range stays with the freeing stream, and a range freed on
+  an unregistered stream is never taken across."""
+  pool.register(1)
+  pool.register(2)
+  p = pool.alloc(1024 * MiB, s=1)
+  pool.free(p, s=1)
+  mallocs = pool.device()["mallocs"]
+  q = pool.alloc(256 * MiB, s=2)
+  assert q == p
+  assert pool.events() == [(0, 1), (1, 2)]
+  assert pool.stats()["cross_stream"] == 1
+  r = pool.alloc(768 * MiB, s=1)       # the rest, still stream 1's: no wait
+  assert r == p + 256 * MiB
+  assert pool.events() == [(0, 1), (1, 2)]
+  assert pool.device()["mallocs"] == mallocs
+  x = pool.alloc(512 * MiB, s=3)       # stream 3 is not registered
+  pool.free(x, s=3)
+  y = pool.alloc(512 * MiB, s=2)
+  assert y != x
+  assert pool.stats()["cross_stream"] == 1
 
 
 def test_oom_hands_back_cached_memory(pool):
