@@ -99,6 +99,64 @@ def neg2hop(knows, interest):
   return {"Knows": kn, "HasInterest": hi, "Path": path}, {"Path": [len(path)]}
 
 
+def twohop(knows, interest):
+  """LSQB Q6 (examples/lsqb_q6_2hop.py): Path(p1,p2,p3,t) <= Knows(p1,p2),
+  Knows(p2,p3), HasInterest(p3,t), p1 != p3."""
+  kn = {(int(a), int(b)) for a, b in knows}
+  hi = {(int(a), int(b)) for a, b in interest}
+  succ, tags = {}, {}
+  for a, b in kn:
+    succ.setdefault(a, set()).add(b)
+  for p, t in hi:
+    tags.setdefault(p, set()).add(t)
+  path = {(p1, p2, p3, t) for (p1, p2) in kn for p3 in succ.get(p2, ())
+          if p1 != p3 for t in tags.get(p3, ())}
+  return {"Knows": kn, "HasInterest": hi, "Path": path}, {"Path": [len(path)]}
+
+
+def twohop_as_planned(knows, interest):
+  """What the plan order p1, p2, p3, t compiles LSQB Q6 to (the reference's
+  tests/fixtures/integration/lsqb_q6_2hop.mir.sexpr): p2 is joined on column
+  0 of Knows' (0 1) index and p1 drawn from the same index under prefix p2,
+  so Path(p1,p2,p3,t) <= Knows(p2,p1), Knows(p2,p3), HasInterest(p3,t),
+  p1 != p3."""
+  kn = {(int(a), int(b)) for a, b in knows}
+  succ, tags = {}, {}
+  for a, b in kn:
+    succ.setdefault(a, set()).add(b)
+  for p, t in interest:
+    tags.setdefault(int(p), set()).add(int(t))
+  return {(p1, p2, p3, t) for p2, s in succ.items() for p1 in s for p3 in s if p1 != p3
+          for t in tags.get(p3, ())}
+
+
+def twohop_as_planned_count_closed_form(knows, interest, n):
+  """|twohop_as_planned|: for each p2, every ordered pair of distinct
+  successors (p1, p3) times the tags of p3 = sum_p (deg+(p) - 1) (A t)[p]."""
+  import numpy as np
+  e = np.unique(np.asarray(knows, dtype=np.int64), axis=0)
+  h = np.unique(np.asarray(interest, dtype=np.int64), axis=0)
+  t = np.bincount(h[:, 0], minlength=n).astype(np.int64)
+  deg = np.bincount(e[:, 0], minlength=n).astype(np.int64)
+  at = np.bincount(e[:, 0], weights=t[e[:, 1]], minlength=n).astype(np.int64)
+  return int(((deg - 1) * at).sum())
+
+
+def twohop_count_closed_form(knows, interest, n):
+  """|Path| of LSQB Q6 without enumeration: with A the n x n 0/1 Knows
+  matrix and t[p] the number of tags of p, the sum over p1 != p3 of
+  (A A)[p1, p3] t[p3] = 1' A A t - sum_p (A A)[p, p] t[p] (sparse products)."""
+  import numpy as np
+  import scipy.sparse as sp
+  e = np.unique(np.asarray(knows, dtype=np.int64), axis=0)
+  a = sp.csr_matrix((np.ones(e.shape[0], np.int64), (e[:, 0], e[:, 1])), shape=(n, n))
+  h = np.unique(np.asarray(interest, dtype=np.int64), axis=0)
+  t = np.bincount(h[:, 0], minlength=n).astype(np.int64)
+  total = int((a @ (a @ t)).sum())
+  diag = np.asarray((a.multiply(a.T)).sum(axis=1)).ravel()   # (A A)[p, p] = sum_q A[p,q] A[q,p]
+  return total - int((diag * t).sum())
+
+
 def q7(has_tag, has_creator, reply_of, likes):
   """LSQB Q7 by enumerating each case's distinct head tuples."""
   ht = {(int(a), int(b)) for a, b in has_tag}

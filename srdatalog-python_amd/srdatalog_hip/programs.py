@@ -115,6 +115,36 @@ def neg2hop_program(count_only: bool) -> Program:
   ])
 
 
+def twohop_program(count_only: bool, p2_first: bool | None = None) -> Program:
+  """LSQB Q6, two-hop paths ending at a person with an interest
+  (examples/lsqb_q6_2hop.py: TwoHopPath over Knows, Knows, HasInterest with
+  p1 != p3, plan order p1, p2, p3, t).  p1 occurs in one atom only, so it is
+  no join variable, and that order keeps it ahead of p2 in Knows(p1, p2)'s
+  index (0 1): the compiled plan, the reference's own
+  (tests/fixtures/integration/lsqb_q6_2hop.mir.sexpr), joins p2 on that
+  column and reads p1 as a successor of p2.  The count variant takes the
+  reference's count test's order p2, p3, p1, t
+  (tests/test_integration_lsqb_q6_count.py), whose Knows(p1, p2) index is
+  (1 0): it counts the rule as written."""
+  P, P1, P2, P3, T, X, Y = (Var(n) for n in ("p", "p1", "p2", "p3", "t", "x", "y"))
+  k_in = Relation("KnowsInput", 2, input_file="Person_knows_Person.csv")
+  hi_in = Relation("HasInterestInput", 2, input_file="Person_hasInterest_Tag.csv")
+  knows, hi = Relation("Knows", 2), Relation("HasInterest", 2)
+  path = Relation("Path", 4, print_size=True)
+  rule = (path(P1, P2, P3, T) <= knows(P1, P2) & knows(P2, P3) & hi(P3, T)
+          & Filter(("p1", "p3"), "return p1 != p3;")) \
+      .named("TwoHopPath").with_plan(
+          var_order=["p2", "p3", "p1", "t"] if (count_only if p2_first is None else p2_first)
+          else ["p1", "p2", "p3", "t"])
+  if count_only:
+    rule = rule.with_count()
+  return Program(rules=[
+    (knows(X, Y) <= k_in(X, Y)).named("KnowsLoad"),
+    (hi(P, T) <= hi_in(P, T)).named("InterestLoad"),
+    rule,
+  ])
+
+
 def q3_program() -> Program:
   """LSQB Q3, directed triangles whose persons live in one country
   (examples/lsqb_q3_triangle.py): a seven-variable, six-level join."""
@@ -311,6 +341,9 @@ PROGRAMS = {
   "reach_multi": reach_multi_program,
   "neg2hop": lambda: neg2hop_program(False),
   "neg2hop_count": lambda: neg2hop_program(True),
+  "twohop": lambda: twohop_program(False),
+  "twohop_count": lambda: twohop_program(True),
+  "twohop_p2": lambda: twohop_program(False, p2_first=True),
   "q7": q7_program,
   "q3": q3_program,
   "split_rec": lambda: split_program(True),
